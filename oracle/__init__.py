@@ -1,0 +1,421 @@
+"""ctypes bindings for the CPU oracle (liboracle.so) and the reference build (_ref/libref.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker.  The product package
+(fft_method_test_amd) never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libref.so")
+
+SUPPORT_FULLSOLID = 0
+SUPPORT_INTERFACE = 1
+TRANSFORM_FFT = 0
+TRANSFORM_IDENTITY = 1
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+_u64p = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
+_i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+_u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+
+
+def build(ref: bool = True) -> None:
+    """Compile liboracle.so (and _ref/libref.so when the reference sources exist)."""
+    target = "all" if ref else "oracle"
+    subprocess.run(["make", "-s", "-C", HERE, target], check=True)
+
+
+class SolveInfo(C.Structure):
+    _fields_ = [("iterations", C.c_uint64), ("final_rel_residual", C.c_double),
+                ("converged", C.c_int), ("history_len", C.c_uint64)]
+
+
+class BrinkmanInfo(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("residual", C.c_double), ("converged", C.c_int)]
+
+
+_lib = None
+_ref = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_SO):
+            build(ref=False)
+        L = C.CDLL(ORACLE_SO)
+        L.orc_create.restype = C.c_void_p
+        L.orc_create.argtypes = [C.c_size_t] * 3 + [C.c_double] * 6 + [C.c_double, C.c_int, C.c_int]
+        L.orc_destroy.argtypes = [C.c_void_p]
+        L.orc_last_error.restype = C.c_char_p
+        L.orc_set_medium.argtypes = [C.c_void_p, _u8p, C.c_int, C.POINTER(C.c_uint64)]
+        L.orc_get_support.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_green_tensor.argtypes = [C.c_void_p, _dp]
+        L.orc_fft3d.argtypes = [C.c_size_t] * 3 + [_dp, C.c_int]
+        L.orc_green_apply.argtypes = [C.c_void_p, _dp, _dp]
+        L.orc_background_force.argtypes = [C.c_void_p, _dp, _dp]
+        L.orc_rhs.argtypes = [C.c_void_p, _dp, _dp]
+        L.orc_operator_apply.argtypes = [C.c_void_p, _dp, _dp]
+        L.orc_minres.argtypes = [C.c_void_p, _dp, C.c_void_p, C.c_double, C.c_uint64, _dp,
+                                 C.c_void_p, C.POINTER(SolveInfo)]
+        L.orc_minres_dense.argtypes = [_dp, C.c_uint64, _dp, C.c_void_p, C.c_double, C.c_uint64,
+                                       _dp, C.c_void_p, C.POINTER(SolveInfo)]
+        L.orc_recover.argtypes = [C.c_void_p, _dp, _dp, C.c_void_p, _dp]
+        L.orc_solve_direction.argtypes = [C.c_void_p, _dp, C.c_double, C.c_uint64, _dp,
+                                          C.POINTER(SolveInfo)]
+        L.orc_solve_tensor.argtypes = [C.c_void_p, C.c_double, C.c_uint64, _dp, C.POINTER(SolveInfo)]
+        L.orc_residual_norm.restype = C.c_double
+        L.orc_residual_norm.argtypes = [C.c_void_p, _dp, _dp]
+        L.orc_symmetry_gap.restype = C.c_double
+        L.orc_symmetry_gap.argtypes = [C.c_void_p, _dp, _dp]
+        L.orc_brinkman_solve.argtypes = [C.c_size_t] * 3 + [_dp, _dp, C.c_double, C.c_double, C.c_int,
+                                                             _dp, _dp, C.c_void_p,
+                                                             C.POINTER(BrinkmanInfo)]
+        _lib = L
+    return _lib
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_SO)
+
+
+def ref():
+    global _ref
+    if _ref is None:
+        R = C.CDLL(REF_SO)
+        R.ref_last_error.restype = C.c_char_p
+        R.ref_create.restype = C.c_void_p
+        R.ref_create.argtypes = [C.c_size_t] * 3 + [C.c_double, _u8p, C.c_int, C.c_double, C.c_int]
+        R.ref_destroy.argtypes = [C.c_void_p]
+        R.ref_num_active.restype = C.c_uint64
+        R.ref_num_active.argtypes = [C.c_void_p]
+        R.ref_get_support.argtypes = [C.c_void_p, _u64p, C.c_void_p]
+        R.ref_green_tensor.argtypes = [C.c_void_p, _dp]
+        R.ref_green_has_nan.argtypes = [C.c_void_p]
+        R.ref_background_force.argtypes = [C.c_void_p, _dp, _dp]
+        R.ref_green_apply.argtypes = [C.c_void_p, _dp, _dp]
+        R.ref_rhs.argtypes = [C.c_void_p, _dp, _dp]
+        R.ref_operator_apply.argtypes = [C.c_void_p, _dp, _dp]
+        R.ref_minres.argtypes = [C.c_void_p, _dp, C.c_void_p, C.c_double, C.c_uint64, _dp, C.c_void_p,
+                                 C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_double),
+                                 C.POINTER(C.c_int)]
+        R.ref_recover.argtypes = [C.c_void_p, _dp, _dp, C.c_void_p, _dp, _dp]
+        R.ref_set_column_k.argtypes = [C.c_double, C.c_int, _dp, _dp]
+        R.ref_residual_norm.restype = C.c_double
+        R.ref_residual_norm.argtypes = [C.c_void_p, _dp, _dp]
+        R.ref_symmetry_gap.restype = C.c_double
+        R.ref_symmetry_gap.argtypes = [C.c_void_p, _dp, _dp]
+        R.ref_brinkman_solve.argtypes = [C.c_size_t] * 3 + [_u32p, C.c_uint32, C.c_double, C.c_double,
+                                                             C.c_double, C.c_double, C.c_int, _dp, _dp,
+                                                             C.c_void_p, C.POINTER(C.c_int),
+                                                             C.POINTER(C.c_double), C.POINTER(C.c_int),
+                                                             C.POINTER(C.c_double)]
+        R.ref_brinkman_tensor.argtypes = [C.c_size_t] * 3 + [_u32p, C.c_uint32, C.c_double, C.c_double,
+                                                              C.c_double, C.c_double, C.c_int, C.c_double,
+                                                              C.c_double, _dp, C.POINTER(C.c_int * 3),
+                                                              C.POINTER(C.c_double * 3)]
+        R.ref_load_mask.argtypes = [C.c_char_p, C.c_int, _u64p, C.c_void_p, C.c_uint64]
+        R.ref_load_geometry.argtypes = [C.c_char_p, _u64p, C.c_void_p, C.c_uint64]
+        _ref = R
+    return _ref
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class Oracle:
+    """One oracle context: grid + Green tensor (+ medium and support after set_medium)."""
+
+    def __init__(self, shape, h=1.0, mu=1.0, p_radius=2, transform=TRANSFORM_FFT, lengths=(0.0, 0.0, 0.0)):
+        nx, ny, nz = shape
+        self.shape = (nx, ny, nz)
+        self.N = nx * ny * nz
+        self.mu = mu
+        self._h = lib().orc_create(nx, ny, nz, h, h, h, lengths[0], lengths[1], lengths[2], mu,
+                                   p_radius, transform)
+        if not self._h:
+            raise RuntimeError(lib().orc_last_error().decode())
+        self.nb = 0
+
+    def close(self):
+        if self._h:
+            lib().orc_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_medium(self, fluid_mask, support=SUPPORT_INTERFACE):
+        m = np.ascontiguousarray(np.asarray(fluid_mask, dtype=np.uint8).reshape(-1))
+        assert m.size == self.N
+        nb = C.c_uint64(0)
+        if lib().orc_set_medium(self._h, m, support, C.byref(nb)) != 0:
+            raise RuntimeError(lib().orc_last_error().decode())
+        self.nb = nb.value
+        return self.nb
+
+    def support(self):
+        act = np.zeros(max(self.nb, 1), dtype=np.uint64)
+        v2l = np.zeros(self.N, dtype=np.int64)
+        lib().orc_get_support(self._h, _ptr(act), _ptr(v2l))
+        return act[: self.nb], v2l
+
+    def green_tensor(self):
+        t = np.zeros(6 * self.N)
+        lib().orc_green_tensor(self._h, t)
+        return t.reshape(6, self.N)
+
+    def green_apply(self, f):
+        f = np.ascontiguousarray(f, dtype=np.float64).reshape(-1)
+        u = np.zeros(3 * self.N)
+        lib().orc_green_apply(self._h, f, u)
+        return u
+
+    def background_force(self, grad):
+        out = np.zeros(3 * self.N)
+        lib().orc_background_force(self._h, np.asarray(grad, dtype=np.float64), out)
+        return out
+
+    def rhs(self, grad):
+        b = np.zeros(3 * self.nb)
+        lib().orc_rhs(self._h, np.asarray(grad, dtype=np.float64), b)
+        return b
+
+    def apply(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros(3 * self.nb)
+        lib().orc_operator_apply(self._h, x, y)
+        return y
+
+    def minres(self, b, tol, maxit, x0=None, history=True):
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.zeros(b.size)
+        hist = np.zeros(maxit + 2) if history else None
+        info = SolveInfo()
+        x0a = None if x0 is None else np.ascontiguousarray(x0, dtype=np.float64)
+        rc = lib().orc_minres(self._h, b, _ptr(x0a), tol, maxit, x, _ptr(hist), C.byref(info))
+        if rc != 0:
+            raise RuntimeError(lib().orc_last_error().decode())
+        h = None if hist is None else hist[: info.history_len].copy()
+        return x, info, h
+
+    def recover(self, grad, x, want_field=False):
+        u = np.zeros(3 * self.N) if want_field else None
+        avg = np.zeros(3)
+        lib().orc_recover(self._h, np.asarray(grad, dtype=np.float64),
+                          np.ascontiguousarray(x, dtype=np.float64), _ptr(u), avg)
+        return avg, u
+
+    def solve_direction(self, grad, tol, maxit):
+        avg = np.zeros(3)
+        info = SolveInfo()
+        rc = lib().orc_solve_direction(self._h, np.asarray(grad, dtype=np.float64), tol, maxit, avg,
+                                       C.byref(info))
+        if rc != 0:
+            raise RuntimeError(lib().orc_last_error().decode())
+        return avg, info
+
+    def solve_tensor(self, tol, maxit):
+        K = np.zeros(9)
+        infos = (SolveInfo * 3)()
+        rc = lib().orc_solve_tensor(self._h, tol, maxit, K, infos)
+        if rc != 0:
+            raise RuntimeError(lib().orc_last_error().decode())
+        return K.reshape(3, 3), list(infos)
+
+    def residual_norm(self, x, b):
+        return lib().orc_residual_norm(self._h, np.ascontiguousarray(x, dtype=np.float64),
+                                       np.ascontiguousarray(b, dtype=np.float64))
+
+    def symmetry_gap(self, x, y):
+        return lib().orc_symmetry_gap(self._h, np.ascontiguousarray(x, dtype=np.float64),
+                                      np.ascontiguousarray(y, dtype=np.float64))
+
+
+def fft3d(data, sign=-1):
+    """3D C2C FFT of an (nz, ny, nx) complex array (unnormalised; sign -1 forward)."""
+    a = np.ascontiguousarray(data, dtype=np.complex128)
+    nz, ny, nx = a.shape
+    buf = a.view(np.float64).reshape(-1).copy()
+    lib().orc_fft3d(nx, ny, nz, buf, sign)
+    return buf.view(np.complex128).reshape(nz, ny, nx)
+
+
+def minres_dense(A, b, tol, maxit, x0=None):
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    x = np.zeros(b.size)
+    hist = np.zeros(maxit + 2)
+    info = SolveInfo()
+    x0a = None if x0 is None else np.ascontiguousarray(x0, dtype=np.float64)
+    lib().orc_minres_dense(A, b.size, b, _ptr(x0a), tol, maxit, x, _ptr(hist), C.byref(info))
+    return x, info, hist[: info.history_len].copy()
+
+
+def brinkman_coefficients(fluid_mask, penalty, viscosity=1.0):
+    """buildCoefficientFields (src/brinkman/FluidBrinkman.cpp:152-191) for one fluid id."""
+    m = np.asarray(fluid_mask).reshape(-1) != 0
+    alpha = np.where(m, 0.0, float(penalty))
+    visc = np.where(m, float(viscosity), 1.0)
+    return np.ascontiguousarray(alpha), np.ascontiguousarray(visc)
+
+
+def brinkman_solve(shape, fluid_mask, penalty, force, viscosity=1.0, relaxation=0.9, tol=1e-8,
+                   max_iterations=600, want_field=False):
+    nx, ny, nz = shape
+    alpha, visc = brinkman_coefficients(fluid_mask, penalty, viscosity)
+    U = np.zeros(3)
+    u = np.zeros(3 * nx * ny * nz) if want_field else None
+    info = BrinkmanInfo()
+    rc = lib().orc_brinkman_solve(nx, ny, nz, alpha, visc, relaxation, tol, max_iterations,
+                                  np.asarray(force, dtype=np.float64), U, _ptr(u), C.byref(info))
+    if rc != 0:
+        raise RuntimeError(lib().orc_last_error().decode())
+    return U, info, u
+
+
+def brinkman_tensor(shape, fluid_mask, penalty, viscosity=1.0, relaxation=0.9, tol=1e-8,
+                    max_iterations=600, forcing=1.0, voxel_size=1.0):
+    """computePermeabilityTensor (src/brinkman/FluidBrinkman.cpp:509-553), single fluid id."""
+    K = np.zeros(9)
+    infos = []
+    scale = voxel_size * voxel_size
+    for j in range(3):
+        force = np.zeros(3)
+        force[j] = forcing
+        U, info, _ = brinkman_solve(shape, fluid_mask, penalty, force, viscosity, relaxation, tol,
+                                    max_iterations)
+        for i in range(3):
+            K[i * 3 + j] = viscosity * U[i] / force[j] * scale
+        infos.append(info)
+    return K.reshape(3, 3), infos
+
+
+class Ref:
+    """The reference's own classes (oracle/_ref/libref.so), FFT-less fallback transform."""
+
+    def __init__(self, shape, fluid_mask, support=SUPPORT_INTERFACE, h=1.0, mu=1.0, p_radius=2):
+        nx, ny, nz = shape
+        self.shape = shape
+        self.N = nx * ny * nz
+        self.mu = mu
+        m = np.ascontiguousarray(np.asarray(fluid_mask, dtype=np.uint8).reshape(-1))
+        self._h = ref().ref_create(nx, ny, nz, h, m, support, mu, p_radius)
+        if not self._h:
+            raise RuntimeError(ref().ref_last_error().decode())
+        self.nb = int(ref().ref_num_active(self._h))
+
+    def close(self):
+        if self._h:
+            ref().ref_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def support(self):
+        act = np.zeros(max(self.nb, 1), dtype=np.uint64)
+        v2l = np.zeros(self.N, dtype=np.int64)
+        ref().ref_get_support(self._h, act, _ptr(v2l))
+        return act[: self.nb], v2l
+
+    def green_tensor(self):
+        t = np.zeros(6 * self.N)
+        ref().ref_green_tensor(self._h, t)
+        return t.reshape(6, self.N)
+
+    def background_force(self, grad):
+        out = np.zeros(3 * self.N)
+        ref().ref_background_force(self._h, np.asarray(grad, dtype=np.float64), out)
+        return out
+
+    def green_apply(self, f):
+        u = np.zeros(3 * self.N)
+        ref().ref_green_apply(self._h, np.ascontiguousarray(f, dtype=np.float64), u)
+        return u
+
+    def rhs(self, grad):
+        b = np.zeros(3 * self.nb)
+        ref().ref_rhs(self._h, np.asarray(grad, dtype=np.float64), b)
+        return b
+
+    def apply(self, x):
+        y = np.zeros(3 * self.nb)
+        ref().ref_operator_apply(self._h, np.ascontiguousarray(x, dtype=np.float64), y)
+        return y
+
+    def minres(self, b, tol, maxit, x0=None):
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.zeros(b.size)
+        hist = np.zeros(maxit + 2)
+        hl, it, fr, cv = C.c_uint64(), C.c_uint64(), C.c_double(), C.c_int()
+        x0a = None if x0 is None else np.ascontiguousarray(x0, dtype=np.float64)
+        rc = ref().ref_minres(self._h, b, _ptr(x0a), tol, maxit, x, _ptr(hist), C.byref(hl), C.byref(it),
+                              C.byref(fr), C.byref(cv))
+        if rc != 0:
+            raise RuntimeError(ref().ref_last_error().decode())
+        return x, dict(iterations=it.value, final_rel_residual=fr.value, converged=cv.value), hist[: hl.value]
+
+    def recover(self, grad, x, want_field=False):
+        u = np.zeros(3 * self.N) if want_field else None
+        avg = np.zeros(3)
+        avs = np.zeros(3)
+        ref().ref_recover(self._h, np.asarray(grad, dtype=np.float64), np.ascontiguousarray(x, dtype=np.float64),
+                          _ptr(u), avg, avs)
+        return avg, avs, u
+
+    def residual_norm(self, x, b):
+        return ref().ref_residual_norm(self._h, np.ascontiguousarray(x, dtype=np.float64),
+                                       np.ascontiguousarray(b, dtype=np.float64))
+
+    def symmetry_gap(self, x, y):
+        return ref().ref_symmetry_gap(self._h, np.ascontiguousarray(x, dtype=np.float64),
+                                      np.ascontiguousarray(y, dtype=np.float64))
+
+
+def ref_set_column_k(mu, col, avg, K):
+    ref().ref_set_column_k(mu, col, np.asarray(avg, dtype=np.float64), K)
+
+
+def ref_brinkman_solve(shape, ids, fluid_id, viscosity, penalty, force, relaxation=0.9, tol=1e-8,
+                       maxit=600, want_field=False):
+    nx, ny, nz = shape
+    U = np.zeros(3)
+    u = np.zeros(3 * nx * ny * nz) if want_field else None
+    it, res, cv, por = C.c_int(), C.c_double(), C.c_int(), C.c_double()
+    rc = ref().ref_brinkman_solve(nx, ny, nz, np.ascontiguousarray(ids, dtype=np.uint32), fluid_id, viscosity,
+                                  penalty, relaxation, tol, maxit, np.asarray(force, dtype=np.float64), U,
+                                  _ptr(u), C.byref(it), C.byref(res), C.byref(cv), C.byref(por))
+    if rc != 0:
+        raise RuntimeError(ref().ref_last_error().decode())
+    return U, dict(iterations=it.value, residual=res.value, converged=cv.value, porosity=por.value), u
+
+
+def ref_brinkman_tensor(shape, ids, fluid_id, viscosity, penalty, relaxation=0.9, tol=1e-8, maxit=600,
+                        forcing=1.0, voxel_size=1.0):
+    nx, ny, nz = shape
+    K = np.zeros(9)
+    its = (C.c_int * 3)()
+    res = (C.c_double * 3)()
+    rc = ref().ref_brinkman_tensor(nx, ny, nz, np.ascontiguousarray(ids, dtype=np.uint32), fluid_id, viscosity,
+                                   penalty, relaxation, tol, maxit, forcing, voxel_size, K, C.byref(its),
+                                   C.byref(res))
+    if rc != 0:
+        raise RuntimeError(ref().ref_last_error().decode())
+    return K.reshape(3, 3), list(its), list(res)
