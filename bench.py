@@ -1,0 +1,250 @@
+#!/usr/bin/env python3
+"""Benchmark: 256^3 permeability-tensor solves per second on one MI355X.
+
+Workload (the config the metric is quoted on): a periodic 256^3 Boolean model of
+overlapping radius-8 solid spheres at porosity 0.6 (seed 2001), full 3x3
+permeability tensor from the three unit load directions, Bignonnet MINRES to a
+relative recurrence residual of 1e-10.  One "step" = support detection from the
+device-resident mask + the three direction solves (RHS, MINRES, velocity
+recovery, domain average).  The Green tensor is precomputed once per grid and
+viscosity (reported separately as setup).
+
+Prints ONE JSON line.  --gpus N with N > 1 exits: this build runs on one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "256³ k-tensor solves/s, mixed+learned vs plain FP64 GPU, at ≤1e-6 rel k error"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def fp64_bytes_zgreen(Nh, s_c=16, s_r=8):
+    # fused z pass: read + write 3 complex components, read the 6 real Green components
+    return 2 * 3 * Nh * s_c + 6 * Nh * s_r
+
+
+def bytes_apply(N, Nh, L, s_c=16, s_r=8):
+    # x-fwd: support map (int32) + compact vector in, 3 half spectra out; y fwd/inv and z: in+out;
+    # z adds the Green tensor; x-inv: spectra + map in, compact vector out
+    xf = 4 * N + 8 * L + 3 * Nh * s_c
+    yp = 2 * 3 * Nh * s_c
+    zg = fp64_bytes_zgreen(Nh, s_c, s_r)
+    xi = 3 * Nh * s_c + 4 * N + 8 * L
+    return xf + 2 * yp + zg + xi
+
+
+def run_precision(args, P, _lib, mask_dev, precision):
+    import ctypes as C
+
+    import torch
+
+    n = args.n
+    grid = P.Grid3D(n, n, n)
+    t0 = time.perf_counter()
+    green = P.GreenOperator(grid, 1.0, 2, precision)
+    setup_s = time.perf_counter() - t0
+    nb = C.c_uint64()
+    K = np.zeros(9)
+    infos = (_lib.SolveInfo * 3)()
+
+    def step():
+        _lib.check(_lib.load().fftperm_set_medium_dev(green._h, C.c_void_p(mask_dev.data_ptr()),
+                                                       _lib.SUPPORT_INTERFACE, C.byref(nb)))
+        _lib.check(_lib.load().fftperm_solve_tensor(green._h, args.tol, args.maxit, K, C.cast(infos, C.c_void_p)))
+
+    for w in range(args.warmup):
+        step()
+        log(f"[bench] warmup {w + 1}/{args.warmup} precision={precision} iterations="
+            f"{[int(i.iterations) for i in infos]}")
+    _lib.check(_lib.load().fftperm_profile(green._h, 1))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        step()
+        log(f"[bench] step {s + 1}/{args.steps} precision={precision} iterations="
+            f"{[int(i.iterations) for i in infos]}")
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    prof = _lib.ProfileStats()
+    _lib.check(_lib.load().fftperm_profile_read(green._h, C.byref(prof)))
+    res = dict(
+        precision=precision, elapsed=elapsed, solves_per_s=args.steps / elapsed, ms_per_step=1e3 * elapsed / args.steps,
+        K=K.reshape(3, 3).copy(), iterations=[int(i.iterations) for i in infos],
+        sweeps=[int(i.refinement_sweeps) for i in infos], converged=[bool(i.converged) for i in infos],
+        final_rel=[float(i.final_rel_residual) for i in infos], setup_s=setup_s, nb=int(nb.value),
+        prof=dict(z_ms=list(prof.zgreen_ms), z_n=list(prof.zgreen_launches), a_ms=list(prof.apply_ms),
+                  a_n=list(prof.apply_launches), N=int(prof.N), Nh=int(prof.Nh)),
+    )
+    green.close()
+    return res
+
+
+def cpu_baseline(args, mask, applies_per_solve):
+    """Oracle (FP64 restatement of the reference path) on the host cores: one operator apply at the
+    same size, scaled by the applies one tensor solve needs."""
+    import oracle
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    n = args.n
+    o = oracle.Oracle((n, n, n))
+    o.set_medium(mask.reshape(-1))
+    x = np.random.default_rng(0).standard_normal(3 * o.nb)
+    t0 = time.perf_counter()
+    o.apply(x)
+    t_apply = time.perf_counter() - t0
+    o.close()
+    return {
+        "value": 1.0 / (t_apply * applies_per_solve),
+        "unit": "k-tensor solves/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"one FP64 VariationalOperator apply of the oracle at {n}^3 ({t_apply:.2f} s, 6 C2C 3D FFTs + "
+                   f"Green multiply, {threads} OpenMP threads), scaled by the {applies_per_solve} applies of one "
+                   f"tensor solve (RHS + MINRES iterations + recovery, 3 directions); Green-tensor setup excluded"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--radius", type=float, default=8.0)
+    ap.add_argument("--porosity", type=float, default=0.6)
+    ap.add_argument("--seed", type=int, default=2001)
+    ap.add_argument("--tol", type=float, default=1e-10)
+    ap.add_argument("--maxit", type=int, default=10000)
+    ap.add_argument("--precision", choices=["fp64", "mixed", "both"], default="both")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              f"(value is the single-GPU throughput)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from fft_method_test_amd import _lib, geometry, permeability as P
+
+    _lib.load()
+    log(f"[bench] device: {_lib.device_info()}")
+    t0 = time.perf_counter()
+    mask = geometry.boolean_spheres(args.n, args.radius, args.porosity, args.seed)
+    log(f"[bench] geometry {args.n}^3 porosity={mask.mean():.4f} ({time.perf_counter() - t0:.1f} s)")
+    mask_dev = torch.from_numpy(mask.reshape(-1).copy()).to("cuda:0")
+    torch.cuda.synchronize()
+
+    runs = {}
+    if args.precision in ("fp64", "both"):
+        runs["fp64"] = run_precision(args, P, _lib, mask_dev, _lib.PRECISION_FP64)
+    if args.precision in ("mixed", "both"):
+        runs["mixed"] = run_precision(args, P, _lib, mask_dev, _lib.PRECISION_MIXED)
+
+    k_err = None
+    if "fp64" in runs and "mixed" in runs:
+        Kf, Km = runs["fp64"]["K"], runs["mixed"]["K"]
+        k_err = float(np.linalg.norm(Km - Kf) / np.linalg.norm(Kf))
+    # the headline value: mixed if it meets the 1e-6 relative K error bar against FP64, else FP64
+    if "mixed" in runs and k_err is not None and k_err <= 1e-6:
+        head = runs["mixed"]
+        dtype = "f32 FFT/Green + f64 Krylov"
+    elif "fp64" in runs:
+        head = runs["fp64"]
+        dtype = "f64"
+    else:
+        head = runs["mixed"]
+        dtype = "f32 FFT/Green + f64 Krylov"
+
+    # roofline of the dominant kernel (fused z pass) from the live HIP-event timings
+    pi = 1 if head["precision"] == _lib.PRECISION_MIXED else 0
+    pr = head["prof"]
+    Nh, N = pr["Nh"], pr["N"]
+    sc, sr = (8, 4) if pi == 1 else (16, 8)
+    z_bytes = fp64_bytes_zgreen(Nh, sc, sr)
+    z_ms = pr["z_ms"][pi] / max(1, pr["z_n"][pi])
+    achieved = z_bytes / (z_ms * 1e-3) / 1e9
+    L = 3 * head["nb"]
+    a_bytes = bytes_apply(N, Nh, L, sc, sr)
+    a_ms = pr["a_ms"][pi] / max(1, pr["a_n"][pi])
+
+    out = {
+        "metric": METRIC,
+        "value": head["solves_per_s"],
+        "unit": "k-tensor solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": dtype,
+        "data": "synthetic",
+        "config": {
+            "workload": (f"{args.n}^3 periodic Boolean model of overlapping radius-{args.radius:g} spheres, porosity "
+                         f"{args.porosity:g} (seed {args.seed}); full 3x3 K from 3 load directions; Bignonnet MINRES "
+                         f"tol {args.tol:g}, p_radius 2, mu 1; step = support detection + 3 direction solves"),
+            "n": args.n,
+            "porosity_measured": float(mask.mean()),
+            "support_voxels": head["nb"],
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_zgreen (fused z FFT, symmetric Green multiply, z inverse FFT)",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "algorithmic_bytes_per_launch": z_bytes,
+            "avg_launch_ms": z_ms,
+            "launches": pr["z_n"][pi],
+        },
+        "apply": {
+            "avg_ms": a_ms,
+            "algorithmic_bytes": a_bytes,
+            "achieved_GBs": a_bytes / (a_ms * 1e-3) / 1e9,
+            "frac": a_bytes / (a_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        },
+        "runs": {
+            k: {
+                "solves_per_s": r["solves_per_s"], "ms_per_step": r["ms_per_step"], "iterations": r["iterations"],
+                "refinement_sweeps": r["sweeps"], "converged": r["converged"], "final_rel_residual": r["final_rel"],
+                "setup_s": r["setup_s"], "K": r["K"].tolist(),
+            } for k, r in runs.items()
+        },
+        "k_rel_error_mixed_vs_fp64": k_err,
+        "speedup_mixed_vs_fp64": (runs["mixed"]["solves_per_s"] / runs["fp64"]["solves_per_s"]
+                                  if "fp64" in runs and "mixed" in runs else None),
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        ref_run = runs.get("fp64", head)
+        applies = int(sum(it + 2 for it in ref_run["iterations"]))
+        try:
+            out["cpu_baseline"] = cpu_baseline(args, mask, applies)
+        except Exception as e:  # the baseline is reported, never required
+            log(f"[bench] cpu baseline failed: {e}")
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,1149 @@
+// api.hip -- C ABI of libfftperm.so: contexts, planning, and the solver drivers.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <climits>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/fftperm.h"
+#include "brinkman.hpp"
+#include "kernels.hpp"
+#include "minres.hpp"
+
+using namespace fftperm;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+struct Err : std::runtime_error {
+    int code;
+    Err(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHK(expr)                                                                                     \
+    do {                                                                                                 \
+        hipError_t e_ = (expr);                                                                          \
+        if (e_ != hipSuccess)                                                                            \
+            throw Err(FFTPERM_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));               \
+    } while (0)
+
+template <typename F>
+int guarded(F&& f) {
+    try {
+        f();
+        g_last_error.clear();
+        return FFTPERM_OK;
+    } catch (const Err& e) {
+        g_last_error = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        g_last_error = "out of host memory";
+        return FFTPERM_ERUNTIME;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return FFTPERM_ERUNTIME;
+    }
+}
+
+template <typename T>
+T* dalloc(size_t count) {
+    void* p = nullptr;
+    if (count == 0) count = 1;
+    HIPCHK(hipMalloc(&p, count * sizeof(T)));
+    return static_cast<T*>(p);
+}
+
+template <typename T>
+void dfree(T*& p) {
+    if (p) (void)hipFree(const_cast<void*>(static_cast<const void*>(p)));
+    p = nullptr;
+}
+
+// Radix plan for a line of length n: powers of two split into balanced radix-16/8/4/2
+// stages, then the odd prime factors 3, 5, 7, 11, 13.
+LinePlan make_plan(int n) {
+    LinePlan p;
+    std::memset(&p, 0, sizeof(p));
+    p.n = n;
+    int m = n, k = 0;
+    while (m % 2 == 0) {
+        m /= 2;
+        ++k;
+    }
+    std::vector<int> rad;
+    if (k > 0) {
+        const int ns = (k + 3) / 4;
+        for (int s = 0; s < ns; ++s) {
+            const int bits = k / ns + (s < k % ns ? 1 : 0);
+            rad.push_back(1 << bits);
+        }
+    }
+    for (int f : {3, 5, 7, 11, 13}) {
+        while (m % f == 0) {
+            rad.push_back(f);
+            m /= f;
+        }
+    }
+    if (m != 1) throw Err(FFTPERM_EUNSUPPORTED, "grid edge " + std::to_string(n) +
+                                                    " has a prime factor above 13 (unsupported FFT length)");
+    if (rad.size() > 8) throw Err(FFTPERM_EUNSUPPORTED, "grid edge " + std::to_string(n) + " needs too many stages");
+    p.nst = (int)rad.size();
+    p.maxbf = 1;
+    for (int s = 0; s < p.nst; ++s) {
+        p.rad[s] = rad[s];
+        p.maxbf = std::max(p.maxbf, n / rad[s]);
+    }
+    if (p.maxbf > 1024)
+        throw Err(FFTPERM_EUNSUPPORTED, "grid edge " + std::to_string(n) + " needs more than 1024 butterflies per stage");
+    return p;
+}
+
+template <typename R>
+cx<R>* make_twiddles(int n) {
+    std::vector<cx<R>> h(n);
+    const long double pi = 3.141592653589793238462643383279502884L;
+    for (int k = 0; k < n; ++k) {
+        const long double a = 2.0L * pi * (long double)k / (long double)n;
+        h[k].x = (R)cosl(a);
+        h[k].y = (R)(-sinl(a));
+    }
+    cx<R>* d = dalloc<cx<R>>(n);
+    HIPCHK(hipMemcpy(d, h.data(), n * sizeof(cx<R>), hipMemcpyHostToDevice));
+    return d;
+}
+
+constexpr double kPiRef = 3.14159265358979323846;
+
+// sinc and q_component of src/bignonnet/GreenOperator.cpp:13-23, evaluated with the same
+// double expressions, tabulated per axis for k in [0,n) and p in [-P,P].
+void green_axis_table(int n, double period, int P, std::vector<double>& s2, std::vector<double>& q) {
+    const int W = 2 * P + 1;
+    s2.assign((size_t)n * W, 0.0);
+    q.assign((size_t)n * W, 0.0);
+    for (int k = 0; k < n; ++k) {
+        for (int p = -P; p <= P; ++p) {
+            const long long m = (long long)k + (long long)p * (long long)n;
+            const double x = kPiRef * (double)m / (double)n;
+            const double ax = std::fabs(x);
+            const double s = (ax < 1e-12) ? 1.0 : std::sin(x) / x;
+            s2[(size_t)k * W + (p + P)] = s * s;
+            q[(size_t)k * W + (p + P)] = (2.0 * kPiRef * (double)m) / period;
+        }
+    }
+}
+
+}  // namespace
+
+struct fftperm_ctx {
+    Geometry3 g{};
+    double h[3]{};
+    double L[3]{};
+    double mu = 1.0;
+    int P = 2;
+    int precision = FFTPERM_PRECISION_FP64;
+    int device = 0;
+    hipStream_t s = nullptr;
+
+    FFTTables<double> t64{};
+    FFTTables<float> t32{};
+    double2* S64 = nullptr;
+    float2* S32 = nullptr;
+    double* G64 = nullptr;
+    float* G32 = nullptr;
+    double* gtab = nullptr;  // per-axis Green tables (6 arrays)
+    GreenAxisTables gt{};
+
+    // medium
+    bool has_medium = false;
+    uint64_t nb = 0, nf = 0;
+    uint8_t* mask = nullptr;
+    int32_t* code = nullptr;
+    int64_t* active = nullptr;
+    void* scan_scratch = nullptr;
+    unsigned long long* counts = nullptr;
+
+    // reductions
+    double* part_x = nullptr;  // 6 x xinv blocks
+    double* part = nullptr;    // 6 x kRedBlocks
+    double* small = nullptr;   // 16 scalars
+    long xblocks = 0;
+
+    // Krylov vectors (L each)
+    size_t Lcap = 0;
+    double *x = nullptr, *v_old = nullptr, *v = nullptr, *z = nullptr, *w_prev = nullptr, *w = nullptr;
+    double *b = nullptr, *tmp = nullptr, *r = nullptr, *dx = nullptr;
+    MinresState* st = nullptr;
+    double* hist = nullptr;
+    size_t hist_cap = 0;
+
+    // live kernel timing (fftperm_profile): HIP events on this context's stream around the
+    // fused z-pass and around the whole five-sweep apply
+    struct ProfRec {
+        hipEvent_t a, b, c, d;  // before x-fwd, before z-green, after z-green, after x-inv
+        long long tag;          // MINRES iteration index, or -1 (always a real apply)
+        int fp32;
+    };
+    bool prof = false;
+    std::vector<ProfRec> prof_pending;
+    std::vector<hipEvent_t> ev_free;
+    double prof_z_ms[2] = {0, 0}, prof_apply_ms[2] = {0, 0};
+    long long prof_z_n[2] = {0, 0}, prof_apply_n[2] = {0, 0};
+    long long cur_tag = -1;
+
+    hipEvent_t take_event() {
+        if (!ev_free.empty()) {
+            hipEvent_t e = ev_free.back();
+            ev_free.pop_back();
+            return e;
+        }
+        hipEvent_t e;
+        HIPCHK(hipEventCreate(&e));
+        return e;
+    }
+    // Accumulate recorded applies; MINRES applies launched after convergence (tag >= valid_below)
+    // were no-ops and are dropped.
+    void prof_resolve(long long valid_below) {
+        if (prof_pending.empty()) return;
+        HIPCHK(hipStreamSynchronize(s));
+        for (const ProfRec& r : prof_pending) {
+            if (r.tag < 0 || r.tag < valid_below) {
+                float zms = 0.f, ams = 0.f;
+                HIPCHK(hipEventElapsedTime(&zms, r.b, r.c));
+                HIPCHK(hipEventElapsedTime(&ams, r.a, r.d));
+                prof_z_ms[r.fp32] += zms;
+                prof_apply_ms[r.fp32] += ams;
+                prof_z_n[r.fp32] += 1;
+                prof_apply_n[r.fp32] += 1;
+            }
+            ev_free.push_back(r.a);
+            ev_free.push_back(r.b);
+            ev_free.push_back(r.c);
+            ev_free.push_back(r.d);
+        }
+        prof_pending.clear();
+    }
+
+    // lazily allocated hook buffers
+    double* field = nullptr;   // 3N
+    double2* full_a = nullptr; // 3N complex
+    double* t6 = nullptr;      // 6N
+
+    ~fftperm_ctx() {
+        if (s) (void)hipStreamSynchronize(s);
+        dfree(t64.twx); dfree(t64.twy); dfree(t64.twz);
+        dfree(t32.twx); dfree(t32.twy); dfree(t32.twz);
+        dfree(S64); dfree(S32); dfree(G64); dfree(G32); dfree(gtab);
+        dfree(mask); dfree(code); dfree(active); dfree(scan_scratch); dfree(counts);
+        dfree(part_x); dfree(part); dfree(small);
+        dfree(x); dfree(v_old); dfree(v); dfree(z); dfree(w_prev); dfree(w);
+        dfree(b); dfree(tmp); dfree(r); dfree(dx);
+        dfree(st); dfree(hist); dfree(field); dfree(full_a); dfree(t6);
+        for (const ProfRec& r : prof_pending) {
+            (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b);
+            (void)hipEventDestroy(r.c); (void)hipEventDestroy(r.d);
+        }
+        for (hipEvent_t e : ev_free) (void)hipEventDestroy(e);
+        if (s) (void)hipStreamDestroy(s);
+    }
+
+    // no copies
+    fftperm_ctx() = default;
+    fftperm_ctx(const fftperm_ctx&) = delete;
+    fftperm_ctx& operator=(const fftperm_ctx&) = delete;
+
+    long L3() const { return 3L * (long)nb; }
+
+    void sync() {
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(s));
+    }
+
+    void ensure_vectors() {
+        const size_t L = (size_t)std::max(1L, L3());
+        if (L <= Lcap) return;
+        dfree(x); dfree(v_old); dfree(v); dfree(z); dfree(w_prev); dfree(w);
+        dfree(b); dfree(tmp); dfree(r); dfree(dx);
+        x = dalloc<double>(L); v_old = dalloc<double>(L); v = dalloc<double>(L); z = dalloc<double>(L);
+        w_prev = dalloc<double>(L); w = dalloc<double>(L); b = dalloc<double>(L); tmp = dalloc<double>(L);
+        r = dalloc<double>(L); dx = dalloc<double>(L);
+        Lcap = L;
+    }
+
+    void ensure_hist(size_t n) {
+        if (n <= hist_cap) return;
+        dfree(hist);
+        hist = dalloc<double>(n);
+        hist_cap = n;
+    }
+
+    double* ensure_field() {
+        if (!field) field = dalloc<double>(3 * (size_t)g.N);
+        return field;
+    }
+
+    void require_medium() const {
+        if (!has_medium) throw Err(FFTPERM_ERUNTIME, "no medium set on this context");
+    }
+    void require_support() const {
+        require_medium();
+        if (nb == 0) throw Err(FFTPERM_ERUNTIME, "Support set is empty; cannot run variational solve");
+    }
+
+    template <typename R>
+    void chain(const XSource& src, const XSink& sink, const int* skip);
+
+    double host_scalar(const double* d) {
+        double h;
+        HIPCHK(hipMemcpyAsync(&h, d, sizeof(double), hipMemcpyDeviceToHost, s));
+        sync();
+        return h;
+    }
+};
+
+template <typename R, typename S_, typename G_, typename T_>
+static void run_chain(fftperm_ctx* c, S_* S, const G_* G, const T_& t, const XSource& src, const XSink& sink,
+                      const int* skip, int fp32) {
+    fftperm_ctx::ProfRec rec{};
+    if (c->prof) {
+        rec.a = c->take_event();
+        rec.b = c->take_event();
+        rec.c = c->take_event();
+        rec.d = c->take_event();
+        rec.tag = c->cur_tag;
+        rec.fp32 = fp32;
+        HIPCHK(hipEventRecord(rec.a, c->s));
+    }
+    launch_xfwd<R>(src, S, c->g, t, skip, c->s);
+    launch_ypass<R>(S, -1, c->g, t, skip, c->s);
+    if (c->prof) HIPCHK(hipEventRecord(rec.b, c->s));
+    launch_zgreen<R>(S, G, c->g, t, skip, c->s);
+    if (c->prof) HIPCHK(hipEventRecord(rec.c, c->s));
+    launch_ypass<R>(S, +1, c->g, t, skip, c->s);
+    launch_xinv<R>(S, sink, c->g, t, skip, c->s);
+    if (c->prof) {
+        HIPCHK(hipEventRecord(rec.d, c->s));
+        c->prof_pending.push_back(rec);
+    }
+}
+
+template <>
+void fftperm_ctx::chain<double>(const XSource& src, const XSink& sink, const int* skip) {
+    run_chain<double>(this, S64, G64, t64, src, sink, skip, 0);
+}
+
+template <>
+void fftperm_ctx::chain<float>(const XSource& src, const XSink& sink, const int* skip) {
+    run_chain<float>(this, S32, G32, t32, src, sink, skip, 1);
+}
+
+namespace {
+
+XSource src_none(const fftperm_ctx* c) {
+    XSource sr;
+    std::memset(&sr, 0, sizeof(sr));
+    sr.code = c->code;
+    return sr;
+}
+
+XSink sink_none(const fftperm_ctx* c) {
+    XSink sk;
+    std::memset(&sk, 0, sizeof(sk));
+    sk.code = c->code;
+    return sk;
+}
+
+void check_ctx(const fftperm_ctx* c) {
+    if (!c) throw Err(FFTPERM_EINVAL, "null context");
+    HIPCHK(hipSetDevice(c->device));
+}
+
+// f0 coefficients of BackgroundForce::build (src/bignonnet/BackgroundForce.cpp:34-35).
+void f0_coefficients(const fftperm_ctx* c, const double grad[3], double af[3], double as[3]) {
+    const double c_fluid = -1.0;
+    const double c_support = (c->nb == 0) ? 0.0 : -(double)c->nf / (double)c->nb;
+    for (int k = 0; k < 3; ++k) {
+        af[k] = c_fluid * grad[k];
+        as[k] = c_support * grad[k];
+    }
+}
+
+// Device operator apply y = A x (FP64 or FP32 transform chain), all pointers device.
+template <typename R>
+void device_apply(fftperm_ctx* c, const double* xin, double* yout) {
+    const long nb = (long)c->nb;
+    const double inv_nb = 1.0 / (double)nb;
+    launch_compact_sums(xin, nb, c->part, nullptr, c->s);
+    launch_finish_sum(c->part, kRedBlocks, 3, inv_nb, c->small, nullptr, c->s);
+    XSource sr = src_none(c);
+    sr.compact = xin;
+    sr.mean = c->small;
+    XSink sk = sink_none(c);
+    sk.compact = yout;
+    sk.partials = c->part_x;
+    c->chain<R>(sr, sk, nullptr);
+    launch_finish_sum(c->part_x, (int)c->xblocks, 6, inv_nb, c->small + 4, nullptr, c->s);
+    launch_sub_mean(yout, c->small + 4, yout, nb, 1.0, nullptr, c->s);
+}
+
+double device_norm(fftperm_ctx* c, const double* a, long n) {
+    launch_dot_partials(a, a, n, c->part, c->s);
+    launch_finish_sum(c->part, kRedBlocks, 1, 1.0, c->small + 16, nullptr, c->s);
+    const double d = c->host_scalar(c->small + 16);
+    return std::sqrt(std::max(0.0, d));
+}
+
+// b (device) = VariationalRHS::build(grad)  (src/bignonnet/VariationalRHS.cpp:15-31), FP64.
+void device_rhs(fftperm_ctx* c, const double grad[3], double* bout) {
+    const long nb = (long)c->nb;
+    XSource sr = src_none(c);
+    f0_coefficients(c, grad, sr.a_fluid, sr.a_support);
+    XSink sk = sink_none(c);
+    sk.compact = c->tmp;
+    sk.partials = c->part_x;
+    c->chain<double>(sr, sk, nullptr);
+    launch_finish_sum(c->part_x, (int)c->xblocks, 6, 1.0 / (double)nb, c->small + 4, nullptr, c->s);
+    launch_sub_mean(c->tmp, c->small + 4, bout, nb, -1.0, nullptr, c->s);
+}
+
+// One MINRES iteration on device; every kernel is a no-op once st->done is set.
+template <typename R>
+void minres_iteration(fftperm_ctx* c) {
+    const long nb = (long)c->nb;
+    const double inv_nb = 1.0 / (double)nb;
+    MinresState* st = c->st;
+    XSource sr = src_none(c);
+    sr.compact = c->v;
+    sr.mean = st->mean_in;
+    XSink sk = sink_none(c);
+    sk.compact = c->z;
+    sk.partials = c->part_x;
+    c->chain<R>(sr, sk, &st->done);
+    launch_mr_finish_mean_out(c->part_x, (int)c->xblocks, st, inv_nb, c->s);
+    launch_mr_sub_dot(c->z, c->v, st, nb, c->part, c->s);
+    launch_mr_finish_alpha(c->part, st, c->s);
+    launch_mr_axpy_norm(c->z, c->v, c->v_old, st, 3 * nb, c->part, c->s);
+    launch_mr_givens(c->part, st, c->hist, c->s);
+    launch_mr_update(c->x, c->v_old, c->v, c->z, c->w_prev, c->w, st, nb, c->part, c->s);
+    launch_mr_finish_mean_in(c->part, st, inv_nb, c->s);
+}
+
+struct MinresOut {
+    uint64_t iterations = 0;
+    double final_rel = 1.0;
+    int converged = 0;
+    uint64_t hist_len = 0;
+};
+
+// MinresSolver::solve (src/solver/MinresSolver.cpp:46-218) with the device operator.
+// bdev: rhs; x0dev: initial guess or null; result in c->x; history in c->hist[0..hist_len).
+template <typename R>
+MinresOut device_minres(fftperm_ctx* c, const double* bdev, const double* x0dev, double tol, uint64_t maxit) {
+    if (!(tol > 0.0)) throw Err(FFTPERM_EINVAL, "MinresSolver tolerance must be positive");
+    const long L = c->L3();
+    const long nb = (long)c->nb;
+    MinresOut out;
+    c->ensure_hist((size_t)maxit + 2);
+    if (x0dev)
+        HIPCHK(hipMemcpyAsync(c->x, x0dev, L * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+    else
+        HIPCHK(hipMemsetAsync(c->x, 0, L * sizeof(double), c->s));
+    const double rhs_raw_norm = device_norm(c, bdev, L);
+    if (rhs_raw_norm == 0.0) {
+        const double zero = 0.0;
+        HIPCHK(hipMemcpyAsync(c->hist, &zero, sizeof(double), hipMemcpyHostToDevice, c->s));
+        c->sync();
+        out.converged = 1;
+        out.final_rel = 0.0;
+        out.hist_len = 1;
+        return out;
+    }
+    const double rhs_norm = std::max(rhs_raw_norm, DBL_MIN);
+    // r0 = b - A x0 (A 0 = 0 exactly, so the empty guess gives r0 = b)
+    if (x0dev) {
+        device_apply<R>(c, c->x, c->z);
+        launch_axpby(1.0, bdev, -1.0, c->z, c->z, L, c->s);
+    } else {
+        HIPCHK(hipMemcpyAsync(c->z, bdev, L * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+    }
+    const double beta1 = device_norm(c, c->z, L);
+    const double rel0 = beta1 / rhs_norm;
+    HIPCHK(hipMemcpyAsync(c->hist, &rel0, sizeof(double), hipMemcpyHostToDevice, c->s));
+    out.final_rel = rel0;
+    out.hist_len = 1;
+    if (rel0 <= tol || beta1 == 0.0) {
+        c->sync();
+        out.converged = 1;
+        return out;
+    }
+    launch_scale_copy(c->z, beta1, c->v, L, c->s);
+    HIPCHK(hipMemsetAsync(c->v_old, 0, L * sizeof(double), c->s));
+    HIPCHK(hipMemsetAsync(c->w_prev, 0, L * sizeof(double), c->s));
+    HIPCHK(hipMemsetAsync(c->w, 0, L * sizeof(double), c->s));
+    MinresState h;
+    std::memset(&h, 0, sizeof(h));
+    h.beta = beta1;
+    h.cs_old = -1.0;
+    h.sn_old = 0.0;
+    h.cs = -1.0;
+    h.sn = 0.0;
+    h.phi_bar = beta1;
+    h.rhs_norm = rhs_norm;
+    h.tol = tol;
+    h.rel_res = rel0;
+    h.iterations = 0;
+    h.maxit = (long long)maxit;
+    h.done = (maxit == 0) ? 1 : 0;
+    HIPCHK(hipMemcpyAsync(c->st, &h, sizeof(h), hipMemcpyHostToDevice, c->s));
+    launch_compact_sums(c->v, nb, c->part, nullptr, c->s);
+    launch_finish_sum(c->part, kRedBlocks, 3, 1.0 / (double)nb, c->st->mean_in, nullptr, c->s);
+    const uint64_t chunk = (c->g.N >= (1L << 21)) ? 4 : 32;
+    uint64_t launched = 0;
+    int done = h.done;
+    while (!done && launched < maxit) {
+        const uint64_t k = std::min<uint64_t>(chunk, maxit - launched);
+        for (uint64_t i = 0; i < k; ++i) {
+            c->cur_tag = (long long)(launched + i);
+            minres_iteration<R>(c);
+        }
+        c->cur_tag = -1;
+        launched += k;
+        HIPCHK(hipMemcpyAsync(&done, &c->st->done, sizeof(int), hipMemcpyDeviceToHost, c->s));
+        c->sync();
+    }
+    HIPCHK(hipMemcpyAsync(&h, c->st, sizeof(h), hipMemcpyDeviceToHost, c->s));
+    c->sync();
+    c->prof_resolve(h.iterations);
+    out.iterations = (uint64_t)h.iterations;
+    out.final_rel = (h.iterations > 0) ? h.rel_res : rel0;
+    out.converged = h.converged || (out.final_rel <= tol);
+    out.hist_len = (uint64_t)h.iterations + 1;
+    return out;
+}
+
+// Mixed precision: FP64 iterative refinement around FP32-operator MINRES sweeps.
+MinresOut device_minres_mixed(fftperm_ctx* c, const double* bdev, const double* x0dev, double tol, uint64_t maxit,
+                              uint64_t* sweeps_out, std::vector<double>* hist_out) {
+    if (!(tol > 0.0)) throw Err(FFTPERM_EINVAL, "MinresSolver tolerance must be positive");
+    const long L = c->L3();
+    MinresOut out;
+    const double bnorm = device_norm(c, bdev, L);
+    double* xacc = c->dx;  // accumulated solution
+    if (x0dev) {
+        if (x0dev != xacc)
+            HIPCHK(hipMemcpyAsync(xacc, x0dev, L * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+    } else {
+        HIPCHK(hipMemsetAsync(xacc, 0, L * sizeof(double), c->s));
+    }
+    bool xacc_zero = (x0dev == nullptr);
+    if (bnorm == 0.0) {
+        HIPCHK(hipMemcpyAsync(c->x, xacc, L * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+        c->sync();
+        out.converged = 1;
+        out.final_rel = 0.0;
+        if (hist_out) hist_out->push_back(0.0);
+        out.hist_len = 1;
+        *sweeps_out = 0;
+        return out;
+    }
+    // r = b - A64 x
+    auto true_residual = [&]() {
+        if (!xacc_zero) {
+            device_apply<double>(c, xacc, c->tmp);
+            launch_axpby(1.0, bdev, -1.0, c->tmp, c->r, L, c->s);
+        } else {
+            HIPCHK(hipMemcpyAsync(c->r, bdev, L * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+        }
+        return device_norm(c, c->r, L) / bnorm;
+    };
+    double rel = true_residual();
+    if (hist_out) hist_out->push_back(rel);
+    uint64_t sweeps = 0;
+    const double inner_tol = 3e-6;
+    while (rel > tol && out.iterations < maxit && sweeps < 64) {
+        const double itol = std::max(inner_tol, 0.5 * tol / rel);
+        MinresOut in = device_minres<float>(c, c->r, nullptr, itol, maxit - out.iterations);
+        out.iterations += in.iterations;
+        ++sweeps;
+        // x += dx (MINRES result in c->x)
+        launch_axpby(1.0, xacc, 1.0, c->x, xacc, L, c->s);
+        xacc_zero = false;
+        rel = true_residual();
+        if (hist_out) hist_out->push_back(rel);
+        if (in.iterations == 0) break;
+    }
+    HIPCHK(hipMemcpyAsync(c->x, xacc, L * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+    c->sync();
+    out.final_rel = rel;
+    out.converged = rel <= tol ? 1 : 0;
+    out.hist_len = hist_out ? hist_out->size() : 0;
+    *sweeps_out = sweeps;
+    return out;
+}
+
+MinresOut run_minres(fftperm_ctx* c, const double* bdev, const double* x0dev, double tol, uint64_t maxit,
+                     uint64_t* sweeps, std::vector<double>* mixed_hist) {
+    *sweeps = 0;
+    if (c->precision == FFTPERM_PRECISION_MIXED) return device_minres_mixed(c, bdev, x0dev, tol, maxit, sweeps, mixed_hist);
+    return device_minres<double>(c, bdev, x0dev, tol, maxit);
+}
+
+// VelocityRecovery::recover + average_over_domain (src/bignonnet/VelocityRecovery.cpp:12-47), FP64.
+void device_recover(fftperm_ctx* c, const double grad[3], const double* xdev, double avg[3]) {
+    const long nb = (long)c->nb;
+    const double inv_nb = 1.0 / (double)nb;
+    double* field = c->ensure_field();
+    launch_compact_sums(xdev, nb, c->part, nullptr, c->s);
+    launch_finish_sum(c->part, kRedBlocks, 3, inv_nb, c->small, nullptr, c->s);
+    XSource sr = src_none(c);
+    f0_coefficients(c, grad, sr.a_fluid, sr.a_support);
+    sr.compact = xdev;
+    sr.mean = c->small;
+    XSink sk = sink_none(c);
+    sk.field = field;
+    sk.partials = c->part_x;
+    c->chain<double>(sr, sk, nullptr);
+    launch_finish_sum(c->part_x, (int)c->xblocks, 6, inv_nb, c->small + 4, nullptr, c->s);
+    double mb[3];
+    HIPCHK(hipMemcpyAsync(mb, c->small + 4, 3 * sizeof(double), hipMemcpyDeviceToHost, c->s));
+    c->sync();
+    launch_field_shift_sum(field, -mb[0], -mb[1], -mb[2], c->g.N, c->part, c->s);
+    launch_finish_sum(c->part, kRedBlocks, 3, 1.0 / (double)c->g.N, c->small + 12, nullptr, c->s);
+    HIPCHK(hipMemcpyAsync(avg, c->small + 12, 3 * sizeof(double), hipMemcpyDeviceToHost, c->s));
+    c->sync();
+    c->prof_resolve(LLONG_MAX);
+}
+
+void fill_info(fftperm_solve_info* info, const MinresOut& o, uint64_t sweeps) {
+    if (!info) return;
+    info->iterations = o.iterations;
+    info->final_rel_residual = o.final_rel;
+    info->converged = o.converged;
+    info->history_len = o.hist_len;
+    info->refinement_sweeps = sweeps;
+}
+
+// Upload an initial guess where the solver of the context's precision mode expects it:
+// FP64 MINRES copies it into x itself (tmp is free), the refinement accumulates in dx.
+const double* stage_guess(fftperm_ctx* c, const double* x0host) {
+    double* dst = (c->precision == FFTPERM_PRECISION_MIXED) ? c->dx : c->tmp;
+    HIPCHK(hipMemcpyAsync(dst, x0host, c->L3() * sizeof(double), hipMemcpyHostToDevice, c->s));
+    return dst;
+}
+
+void solve_direction_dev(fftperm_ctx* c, const double grad[3], double tol, uint64_t maxit, const double* x0host,
+                         double avg[3], double* uhost, fftperm_solve_info* info) {
+    c->require_support();
+    c->ensure_vectors();
+    device_rhs(c, grad, c->b);
+    const double* x0dev = x0host ? stage_guess(c, x0host) : nullptr;
+    uint64_t sweeps = 0;
+    MinresOut o = run_minres(c, c->b, x0dev, tol, maxit, &sweeps, nullptr);
+    device_recover(c, grad, c->x, avg);
+    if (uhost) {
+        HIPCHK(hipMemcpyAsync(uhost, c->field, 3 * c->g.N * sizeof(double), hipMemcpyDeviceToHost, c->s));
+        c->sync();
+    }
+    fill_info(info, o, sweeps);
+}
+
+void set_medium_impl(fftperm_ctx* c, const uint8_t* mask, bool host, int mode, uint64_t* nb_out) {
+    if (mode != FFTPERM_SUPPORT_INTERFACE && mode != FFTPERM_SUPPORT_FULLSOLID)
+        throw Err(FFTPERM_EINVAL, "unknown support mode");
+    if (!mask) throw Err(FFTPERM_EINVAL, "null mask");
+    const long N = c->g.N;
+    if (!c->mask) c->mask = dalloc<uint8_t>(N);
+    if (!c->code) c->code = dalloc<int32_t>(N);
+    if (!c->active) c->active = dalloc<int64_t>(N);
+    if (!c->scan_scratch) c->scan_scratch = dalloc<unsigned char>(scan_scratch_bytes(N));
+    if (!c->counts) c->counts = dalloc<unsigned long long>(2);
+    HIPCHK(hipMemcpyAsync(c->mask, mask, N, host ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice, c->s));
+    HIPCHK(hipMemsetAsync(c->counts, 0, 2 * sizeof(unsigned long long), c->s));
+    launch_support(c->mask, c->g.nx, c->g.ny, c->g.nz, mode == FFTPERM_SUPPORT_INTERFACE ? 1 : 0, c->code, c->active,
+                   c->counts, c->scan_scratch, c->s);
+    unsigned long long cnt[2];
+    HIPCHK(hipMemcpyAsync(cnt, c->counts, sizeof(cnt), hipMemcpyDeviceToHost, c->s));
+    c->sync();
+    c->nb = cnt[0];
+    c->nf = cnt[1];
+    c->has_medium = true;
+    if (nb_out) *nb_out = c->nb;
+}
+
+}  // namespace
+
+// ============================================================== C ABI
+extern "C" {
+
+const char* fftperm_last_error(void) { return g_last_error.c_str(); }
+
+int fftperm_device_info(char* buf, size_t len) {
+    return guarded([&] {
+        int dev = 0;
+        HIPCHK(hipGetDevice(&dev));
+        hipDeviceProp_t p;
+        HIPCHK(hipGetDeviceProperties(&p, dev));
+        if (buf && len) std::snprintf(buf, len, "%s (%s), %d CUs, %.1f GiB", p.name, p.gcnArchName,
+                                      p.multiProcessorCount, (double)p.totalGlobalMem / (1024.0 * 1024 * 1024));
+    });
+}
+
+int fftperm_create(const fftperm_grid* grid, double mu, int p_radius, int precision, fftperm_ctx** out) {
+    return guarded([&] {
+        if (!grid || !out) throw Err(FFTPERM_EINVAL, "null argument");
+        *out = nullptr;
+        if (grid->nx == 0 || grid->ny == 0 || grid->nz == 0)
+            throw Err(FFTPERM_EINVAL, "Grid dimensions must be positive");
+        if (!(mu > 0.0)) throw Err(FFTPERM_EINVAL, "GreenOperator viscosity must be positive");
+        if (precision != FFTPERM_PRECISION_FP64 && precision != FFTPERM_PRECISION_MIXED)
+            throw Err(FFTPERM_EINVAL, "unknown precision mode");
+        if (grid->nx > (1u << 20) || grid->ny > (1u << 20) || grid->nz > (1u << 20) ||
+            grid->nx * grid->ny * grid->nz > (uint64_t)INT32_MAX)
+            throw Err(FFTPERM_EUNSUPPORTED, "grid too large (N must fit in int32)");
+        auto c = std::make_unique<fftperm_ctx>();
+        HIPCHK(hipGetDevice(&c->device));
+        {
+            hipDeviceProp_t prop;
+            HIPCHK(hipGetDeviceProperties(&prop, c->device));
+            set_lds_limit(prop.sharedMemPerBlock);
+        }
+        HIPCHK(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
+        Geometry3& g = c->g;
+        g.nx = (int)grid->nx;
+        g.ny = (int)grid->ny;
+        g.nz = (int)grid->nz;
+        g.nxh = g.nx / 2 + 1;
+        g.N = (long)g.nx * g.ny * g.nz;
+        g.Nh = (long)g.nxh * g.ny * g.nz;
+        c->h[0] = grid->hx; c->h[1] = grid->hy; c->h[2] = grid->hz;
+        // Grid3D: l = h * n unless given (src/core/Grid3D.hpp:19-21)
+        c->L[0] = grid->lx > 0.0 ? grid->lx : grid->hx * (double)g.nx;
+        c->L[1] = grid->ly > 0.0 ? grid->ly : grid->hy * (double)g.ny;
+        c->L[2] = grid->lz > 0.0 ? grid->lz : grid->hz * (double)g.nz;
+        c->mu = mu;
+        c->P = p_radius < 0 ? 0 : p_radius;
+        c->precision = precision;
+        c->t64.px = make_plan(g.nx);
+        c->t64.py = make_plan(g.ny);
+        c->t64.pz = make_plan(g.nz);
+        for (const LinePlan* lp : {&c->t64.px, &c->t64.py, &c->t64.pz}) {
+            const PassCfg pc = cfg_zgreen(*lp, 1);
+            if (pc.lds > lds_limit())
+                throw Err(FFTPERM_EUNSUPPORTED, "grid edge " + std::to_string(lp->n) + " needs more LDS than the device has");
+        }
+        c->t64.twx = make_twiddles<double>(g.nx);
+        c->t64.twy = make_twiddles<double>(g.ny);
+        c->t64.twz = make_twiddles<double>(g.nz);
+        c->S64 = dalloc<double2>(3 * (size_t)g.Nh);
+        c->G64 = dalloc<double>(6 * (size_t)g.Nh);
+        if (precision == FFTPERM_PRECISION_MIXED) {
+            c->t32.px = c->t64.px;
+            c->t32.py = c->t64.py;
+            c->t32.pz = c->t64.pz;
+            c->t32.twx = make_twiddles<float>(g.nx);
+            c->t32.twy = make_twiddles<float>(g.ny);
+            c->t32.twz = make_twiddles<float>(g.nz);
+            c->S32 = dalloc<float2>(3 * (size_t)g.Nh);
+            c->G32 = dalloc<float>(6 * (size_t)g.Nh);
+        }
+        // per-axis Green tables
+        std::vector<double> s2x, qx, s2y, qy, s2z, qz;
+        green_axis_table(g.nx, c->L[0], c->P, s2x, qx);
+        green_axis_table(g.ny, c->L[1], c->P, s2y, qy);
+        green_axis_table(g.nz, c->L[2], c->P, s2z, qz);
+        const size_t W = 2 * (size_t)c->P + 1;
+        const size_t tot = (2 * (g.nx + g.ny + g.nz)) * W;
+        c->gtab = dalloc<double>(tot);
+        std::vector<double> host(tot);
+        size_t off = 0;
+        auto put = [&](const std::vector<double>& v) {
+            std::copy(v.begin(), v.end(), host.begin() + off);
+            const double* p = c->gtab + off;
+            off += v.size();
+            return p;
+        };
+        c->gt.s2x = put(s2x);
+        c->gt.qx = put(qx);
+        c->gt.s2y = put(s2y);
+        c->gt.qy = put(qy);
+        c->gt.s2z = put(s2z);
+        c->gt.qz = put(qz);
+        c->gt.P = c->P;
+        c->gt.mu = mu;
+        HIPCHK(hipMemcpy(c->gtab, host.data(), tot * sizeof(double), hipMemcpyHostToDevice));
+        launch_green_sym(c->gt, g.nx, g.ny, g.nz, c->G64, c->G32, c->s);
+        HIPCHK(hipGetLastError());
+        c->xblocks = xinv_blocks(g, c->t64.px);
+        c->part_x = dalloc<double>(6 * (size_t)c->xblocks);
+        c->part = dalloc<double>(6 * (size_t)kRedBlocks);
+        c->small = dalloc<double>(32);
+        c->st = dalloc<MinresState>(1);
+        c->sync();
+        *out = c.release();
+    });
+}
+
+int fftperm_destroy(fftperm_ctx* ctx) {
+    return guarded([&] {
+        if (ctx) {
+            HIPCHK(hipSetDevice(ctx->device));
+            delete ctx;
+        }
+    });
+}
+
+int fftperm_set_medium(fftperm_ctx* ctx, const uint8_t* fluid_mask, int support_mode, uint64_t* nb_out) {
+    return guarded([&] {
+        check_ctx(ctx);
+        set_medium_impl(ctx, fluid_mask, true, support_mode, nb_out);
+    });
+}
+
+int fftperm_set_medium_dev(fftperm_ctx* ctx, const uint8_t* fluid_mask_dev, int support_mode, uint64_t* nb_out) {
+    return guarded([&] {
+        check_ctx(ctx);
+        set_medium_impl(ctx, fluid_mask_dev, false, support_mode, nb_out);
+    });
+}
+
+int fftperm_porosity(fftperm_ctx* ctx, double* porosity) {
+    return guarded([&] {
+        check_ctx(ctx);
+        ctx->require_medium();
+        if (!porosity) throw Err(FFTPERM_EINVAL, "null output");
+        *porosity = (double)ctx->nf / (double)ctx->g.N;
+    });
+}
+
+int fftperm_get_support(fftperm_ctx* ctx, uint64_t* active, int64_t* voxel_to_local) {
+    return guarded([&] {
+        check_ctx(ctx);
+        ctx->require_medium();
+        if (active && ctx->nb)
+            HIPCHK(hipMemcpyAsync(active, ctx->active, ctx->nb * sizeof(int64_t), hipMemcpyDeviceToHost, ctx->s));
+        if (voxel_to_local) {
+            std::vector<int32_t> code(ctx->g.N);
+            HIPCHK(hipMemcpyAsync(code.data(), ctx->code, ctx->g.N * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->s));
+            ctx->sync();
+            for (long i = 0; i < ctx->g.N; ++i) voxel_to_local[i] = code[i] >= 0 ? (int64_t)code[i] : -1;
+        }
+        ctx->sync();
+    });
+}
+
+int fftperm_green_tensor(fftperm_ctx* ctx, double* t6) {
+    return guarded([&] {
+        check_ctx(ctx);
+        if (!t6) throw Err(FFTPERM_EINVAL, "null output");
+        if (!ctx->t6) ctx->t6 = dalloc<double>(6 * (size_t)ctx->g.N);
+        launch_green_full(ctx->gt, ctx->g.nx, ctx->g.ny, ctx->g.nz, ctx->t6, ctx->s);
+        HIPCHK(hipMemcpyAsync(t6, ctx->t6, 6 * ctx->g.N * sizeof(double), hipMemcpyDeviceToHost, ctx->s));
+        ctx->sync();
+    });
+}
+
+int fftperm_fft_forward(fftperm_ctx* ctx, const double* real3, double* spec3) {
+    return guarded([&] {
+        check_ctx(ctx);
+        if (!real3 || !spec3) throw Err(FFTPERM_EINVAL, "null argument");
+        const long N = ctx->g.N;
+        double* f = ctx->ensure_field();
+        if (!ctx->full_a) ctx->full_a = dalloc<double2>(3 * (size_t)N);
+        HIPCHK(hipMemcpyAsync(f, real3, 3 * N * sizeof(double), hipMemcpyHostToDevice, ctx->s));
+        XSource sr;
+        std::memset(&sr, 0, sizeof(sr));
+        sr.field = f;
+        launch_xfwd<double>(sr, ctx->S64, ctx->g, ctx->t64, nullptr, ctx->s);
+        launch_ypass<double>(ctx->S64, -1, ctx->g, ctx->t64, nullptr, ctx->s);
+        launch_zpass<double>(ctx->S64, -1, ctx->g, ctx->t64, nullptr, ctx->s);
+        launch_expand_full<double>(ctx->S64, ctx->full_a, ctx->g, ctx->s);
+        HIPCHK(hipMemcpyAsync(spec3, ctx->full_a, 3 * N * sizeof(double2), hipMemcpyDeviceToHost, ctx->s));
+        ctx->sync();
+    });
+}
+
+int fftperm_fft_inverse(fftperm_ctx* ctx, const double* spec3, double* real3) {
+    return guarded([&] {
+        check_ctx(ctx);
+        if (!real3 || !spec3) throw Err(FFTPERM_EINVAL, "null argument");
+        const long N = ctx->g.N;
+        double* f = ctx->ensure_field();
+        if (!ctx->full_a) ctx->full_a = dalloc<double2>(3 * (size_t)N);
+        if (!ctx->code) {  // the sink needs a support map; an all "-2" map is fine without a medium
+            ctx->code = dalloc<int32_t>(N);
+            HIPCHK(hipMemsetAsync(ctx->code, 0xFF, N * sizeof(int32_t), ctx->s));
+        }
+        HIPCHK(hipMemcpyAsync(ctx->full_a, spec3, 3 * N * sizeof(double2), hipMemcpyHostToDevice, ctx->s));
+        launch_hermitian_half<double>(ctx->full_a, ctx->S64, ctx->g, ctx->s);
+        launch_zpass<double>(ctx->S64, +1, ctx->g, ctx->t64, nullptr, ctx->s);
+        launch_ypass<double>(ctx->S64, +1, ctx->g, ctx->t64, nullptr, ctx->s);
+        XSink sk;
+        std::memset(&sk, 0, sizeof(sk));
+        sk.code = ctx->code;
+        sk.field = f;
+        launch_xinv<double>(ctx->S64, sk, ctx->g, ctx->t64, nullptr, ctx->s);
+        HIPCHK(hipMemcpyAsync(real3, f, 3 * N * sizeof(double), hipMemcpyDeviceToHost, ctx->s));
+        ctx->sync();
+    });
+}
+
+int fftperm_spectral_green(fftperm_ctx* ctx, const double* t6, const double* force_spec, double* vel_spec) {
+    return guarded([&] {
+        check_ctx(ctx);
+        if (!t6 || !force_spec || !vel_spec) throw Err(FFTPERM_EINVAL, "null argument");
+        const long N = ctx->g.N;
+        if (!ctx->t6) ctx->t6 = dalloc<double>(6 * (size_t)N);
+        if (!ctx->full_a) ctx->full_a = dalloc<double2>(3 * (size_t)N);
+        double2* fb = dalloc<double2>(3 * (size_t)N);
+        try {
+            HIPCHK(hipMemcpyAsync(ctx->t6, t6, 6 * N * sizeof(double), hipMemcpyHostToDevice, ctx->s));
+            HIPCHK(hipMemcpyAsync(ctx->full_a, force_spec, 3 * N * sizeof(double2), hipMemcpyHostToDevice, ctx->s));
+            launch_spectral_green_full(ctx->t6, ctx->full_a, fb, N, ctx->s);
+            HIPCHK(hipMemcpyAsync(vel_spec, fb, 3 * N * sizeof(double2), hipMemcpyDeviceToHost, ctx->s));
+            ctx->sync();
+        } catch (...) {
+            dfree(fb);
+            throw;
+        }
+        dfree(fb);
+    });
+}
+
+int fftperm_green_apply(fftperm_ctx* ctx, const double* fin, double* uout) {
+    return guarded([&] {
+        check_ctx(ctx);
+        if (!fin || !uout) throw Err(FFTPERM_EINVAL, "null argument");
+        const long N = ctx->g.N;
+        double* f = ctx->ensure_field();
+        if (!ctx->code) {
+            ctx->code = dalloc<int32_t>(N);
+            HIPCHK(hipMemsetAsync(ctx->code, 0xFF, N * sizeof(int32_t), ctx->s));
+        }
+        HIPCHK(hipMemcpyAsync(f, fin, 3 * N * sizeof(double), hipMemcpyHostToDevice, ctx->s));
+        XSource sr;
+        std::memset(&sr, 0, sizeof(sr));
+        sr.field = f;
+        XSink sk;
+        std::memset(&sk, 0, sizeof(sk));
+        sk.code = ctx->code;
+        sk.field = f;  // the x-inverse pass reads only the spectrum, so in-place output is safe
+        ctx->chain<double>(sr, sk, nullptr);
+        HIPCHK(hipMemcpyAsync(uout, f, 3 * N * sizeof(double), hipMemcpyDeviceToHost, ctx->s));
+        ctx->sync();
+    });
+}
+
+int fftperm_background_force(fftperm_ctx* ctx, const double grad[3], double* f0) {
+    return guarded([&] {
+        check_ctx(ctx);
+        ctx->require_medium();
+        if (!grad || !f0) throw Err(FFTPERM_EINVAL, "null argument");
+        double* f = ctx->ensure_field();
+        Coef3 af, as;
+        f0_coefficients(ctx, grad, af.v, as.v);
+        launch_background_force(ctx->code, ctx->g.N, af, as, f, ctx->s);
+        HIPCHK(hipMemcpyAsync(f0, f, 3 * ctx->g.N * sizeof(double), hipMemcpyDeviceToHost, ctx->s));
+        ctx->sync();
+    });
+}
+
+int fftperm_rhs(fftperm_ctx* ctx, const double grad[3], double* b) {
+    return guarded([&] {
+        check_ctx(ctx);
+        ctx->require_support();
+        if (!grad || !b) throw Err(FFTPERM_EINVAL, "null argument");
+        ctx->ensure_vectors();
+        device_rhs(ctx, grad, ctx->b);
+        HIPCHK(hipMemcpyAsync(b, ctx->b, ctx->L3() * sizeof(double), hipMemcpyDeviceToHost, ctx->s));
+        ctx->sync();
+    });
+}
+
+int fftperm_operator_apply(fftperm_ctx* ctx, const double* x, double* y) {
+    return guarded([&] {
+        check_ctx(ctx);
+        ctx->require_support();
+        if (!x || !y) throw Err(FFTPERM_EINVAL, "null argument");
+        ctx->ensure_vectors();
+        const long L = ctx->L3();
+        HIPCHK(hipMemcpyAsync(ctx->r, x, L * sizeof(double), hipMemcpyHostToDevice, ctx->s));
+        device_apply<double>(ctx, ctx->r, ctx->tmp);
+        HIPCHK(hipMemcpyAsync(y, ctx->tmp, L * sizeof(double), hipMemcpyDeviceToHost, ctx->s));
+        ctx->sync();
+    });
+}
+
+int fftperm_minres(fftperm_ctx* ctx, const double* b, const double* x0, double tol, uint64_t maxit, double* x,
+                   double* history, fftperm_solve_info* info) {
+    return guarded([&] {
+        check_ctx(ctx);
+        ctx->require_support();
+        if (!b || !x) throw Err(FFTPERM_EINVAL, "null argument");
+        ctx->ensure_vectors();
+        const long L = ctx->L3();
+        HIPCHK(hipMemcpyAsync(ctx->b, b, L * sizeof(double), hipMemcpyHostToDevice, ctx->s));
+        const double* x0dev = x0 ? stage_guess(ctx, x0) : nullptr;
+        uint64_t sweeps = 0;
+        std::vector<double> mh;
+        MinresOut o;
+        o = run_minres(ctx, ctx->b, x0dev, tol, maxit, &sweeps, &mh);
+        HIPCHK(hipMemcpyAsync(x, ctx->x, L * sizeof(double), hipMemcpyDeviceToHost, ctx->s));
+        if (history) {
+            if (ctx->precision == FFTPERM_PRECISION_MIXED) {
+                std::copy(mh.begin(), mh.end(), history);
+            } else {
+                HIPCHK(hipMemcpyAsync(history, ctx->hist, o.hist_len * sizeof(double), hipMemcpyDeviceToHost,
+                                      ctx->s));
+            }
+        }
+        ctx->sync();
+        fill_info(info, o, sweeps);
+    });
+}
+
+int fftperm_recover(fftperm_ctx* ctx, const double grad[3], const double* x, double* u, double avg[3]) {
+    return guarded([&] {
+        check_ctx(ctx);
+        ctx->require_support();
+        if (!grad || !x || !avg) throw Err(FFTPERM_EINVAL, "null argument");
+        ctx->ensure_vectors();
+        HIPCHK(hipMemcpyAsync(ctx->tmp, x, ctx->L3() * sizeof(double), hipMemcpyHostToDevice, ctx->s));
+        device_recover(ctx, grad, ctx->tmp, avg);
+        if (u) {
+            HIPCHK(hipMemcpyAsync(u, ctx->field, 3 * ctx->g.N * sizeof(double), hipMemcpyDeviceToHost, ctx->s));
+            ctx->sync();
+        }
+    });
+}
+
+int fftperm_solve_direction(fftperm_ctx* ctx, const double grad[3], double tol, uint64_t maxit, const double* x0,
+                            double avg[3], double* u, fftperm_solve_info* info) {
+    return guarded([&] {
+        check_ctx(ctx);
+        if (!grad || !avg) throw Err(FFTPERM_EINVAL, "null argument");
+        solve_direction_dev(ctx, grad, tol, maxit, x0, avg, u, info);
+    });
+}
+
+int fftperm_solve_tensor(fftperm_ctx* ctx, double tol, uint64_t maxit, double K[9], fftperm_solve_info* info) {
+    return guarded([&] {
+        check_ctx(ctx);
+        if (!K) throw Err(FFTPERM_EINVAL, "null argument");
+        for (int col = 0; col < 3; ++col) {
+            double g[3] = {0.0, 0.0, 0.0};
+            g[col] = 1.0;
+            double avg[3];
+            solve_direction_dev(ctx, g, tol, maxit, nullptr, avg, nullptr, info ? &info[col] : nullptr);
+            for (int r = 0; r < 3; ++r) {
+                const double colK = -ctx->mu * avg[r];
+                K[r * 3 + col] = -colK;
+            }
+        }
+    });
+}
+
+int fftperm_residual_norm(fftperm_ctx* ctx, const double* x, const double* b, double* out) {
+    return guarded([&] {
+        check_ctx(ctx);
+        ctx->require_support();
+        if (!x || !b || !out) throw Err(FFTPERM_EINVAL, "null argument");
+        ctx->ensure_vectors();
+        const long L = ctx->L3();
+        HIPCHK(hipMemcpyAsync(ctx->r, x, L * sizeof(double), hipMemcpyHostToDevice, ctx->s));
+        HIPCHK(hipMemcpyAsync(ctx->dx, b, L * sizeof(double), hipMemcpyHostToDevice, ctx->s));
+        device_apply<double>(ctx, ctx->r, ctx->tmp);
+        launch_axpby(1.0, ctx->tmp, -1.0, ctx->dx, ctx->tmp, L, ctx->s);
+        *out = device_norm(ctx, ctx->tmp, L);
+    });
+}
+
+int fftperm_symmetry_gap(fftperm_ctx* ctx, const double* x, const double* y, double* out) {
+    return guarded([&] {
+        check_ctx(ctx);
+        ctx->require_support();
+        if (!x || !y || !out) throw Err(FFTPERM_EINVAL, "null argument");
+        ctx->ensure_vectors();
+        const long L = ctx->L3();
+        double* dxv = ctx->r;
+        double* dyv = ctx->dx;
+        double* ax = ctx->tmp;
+        double* ay = ctx->z;
+        HIPCHK(hipMemcpyAsync(dxv, x, L * sizeof(double), hipMemcpyHostToDevice, ctx->s));
+        HIPCHK(hipMemcpyAsync(dyv, y, L * sizeof(double), hipMemcpyHostToDevice, ctx->s));
+        device_apply<double>(ctx, dxv, ax);
+        device_apply<double>(ctx, dyv, ay);
+        auto dot = [&](const double* a, const double* b2) {
+            launch_dot_partials(a, b2, L, ctx->part, ctx->s);
+            launch_finish_sum(ctx->part, kRedBlocks, 1, 1.0, ctx->small + 16, nullptr, ctx->s);
+            return ctx->host_scalar(ctx->small + 16);
+        };
+        const double xay = dot(dxv, ay);
+        const double axy = dot(ax, dyv);
+        const double nx = std::sqrt(std::max(0.0, dot(dxv, dxv)));
+        const double ny = std::sqrt(std::max(0.0, dot(dyv, dyv)));
+        const double denom = std::max({1.0, std::fabs(xay), std::fabs(axy), nx * ny});
+        *out = std::fabs(xay - axy) / denom;
+    });
+}
+
+int fftperm_profile(fftperm_ctx* ctx, int enable) {
+    return guarded([&] {
+        check_ctx(ctx);
+        ctx->prof_resolve(LLONG_MAX);
+        ctx->prof = enable != 0;
+        for (int i = 0; i < 2; ++i) {
+            ctx->prof_z_ms[i] = ctx->prof_apply_ms[i] = 0.0;
+            ctx->prof_z_n[i] = ctx->prof_apply_n[i] = 0;
+        }
+    });
+}
+
+int fftperm_profile_read(fftperm_ctx* ctx, fftperm_profile_stats* out) {
+    return guarded([&] {
+        check_ctx(ctx);
+        if (!out) throw Err(FFTPERM_EINVAL, "null output");
+        ctx->prof_resolve(LLONG_MAX);
+        for (int i = 0; i < 2; ++i) {
+            out->zgreen_ms[i] = ctx->prof_z_ms[i];
+            out->zgreen_launches[i] = (uint64_t)ctx->prof_z_n[i];
+            out->apply_ms[i] = ctx->prof_apply_ms[i];
+            out->apply_launches[i] = (uint64_t)ctx->prof_apply_n[i];
+        }
+        out->N = (uint64_t)ctx->g.N;
+        out->Nh = (uint64_t)ctx->g.Nh;
+        out->nb = ctx->nb;
+    });
+}
+
+int fftperm_brinkman_solve(uint64_t nx, uint64_t ny, uint64_t nz, const double* alpha, const double* viscosity,
+                           double relaxation, double tol, int max_iterations, const double force[3], int precision,
+                           double U[3], double* u, fftperm_brinkman_info* info) {
+    return guarded([&] {
+        if (!alpha || !viscosity || !force || !U) throw Err(FFTPERM_EINVAL, "null argument");
+        if (nx == 0 || ny == 0 || nz == 0) throw Err(FFTPERM_EINVAL, "Grid dimensions must be positive");
+        if (precision != FFTPERM_PRECISION_FP64 && precision != FFTPERM_PRECISION_MIXED)
+            throw Err(FFTPERM_EINVAL, "unknown precision mode");
+        BrinkmanResult res;
+        const int rc = brinkman_run((int)nx, (int)ny, (int)nz, alpha, viscosity, relaxation, tol, max_iterations,
+                                    force, precision == FFTPERM_PRECISION_MIXED, U, u, &res);
+        if (info) {
+            info->iterations = res.iterations;
+            info->residual = res.residual;
+            info->converged = res.converged;
+        }
+        if (rc == 2)
+            throw Err(FFTPERM_ERUNTIME, "Brinkman solver became unstable: non-finite residual at iteration " +
+                                            std::to_string(res.iterations));
+        if (rc == 3) throw Err(FFTPERM_ERUNTIME, "Brinkman solver produced non-finite macroscopic velocity");
+        if (rc != 0) throw Err(FFTPERM_EDEVICE, res.message);
+    });
+}
+
+}  // extern "C"

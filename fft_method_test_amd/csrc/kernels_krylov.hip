@@ -1,0 +1,342 @@
+// kernels_krylov.hip -- deterministic reductions, compact-vector kernels and MINRES steps.
+//
+// Every reduction writes one partial per block (fixed grid of kRedBlocks blocks with
+// grid-stride loops) and a single-block finish kernel sums the partials in a fixed
+// order, so results do not depend on scheduling.
+#include "kernels.hpp"
+#include "minres.hpp"
+
+#include <cmath>
+
+namespace fftperm {
+
+namespace {
+inline unsigned red_grid(long n) {
+    long b = (n + kRedThreads - 1) / kRedThreads;
+    if (b > kRedBlocks) b = kRedBlocks;
+    if (b < 1) b = 1;
+    return (unsigned)b;
+}
+}  // namespace
+
+// ------------------------------------------------------------ generic finish
+__global__ void k_finish_sum(const double* __restrict__ partials, int nparts, int nv, double scale,
+                             double* __restrict__ out, const int* skip) {
+    if (skip != nullptr && *skip) return;
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    for (int p = threadIdx.x; p < nparts; p += blockDim.x) {
+        for (int j = 0; j < nv; ++j) acc[j] += partials[(long)p * nv + j];
+    }
+    __shared__ double res[6];
+    block_sum<6>(acc, res);
+    if (threadIdx.x < (unsigned)nv) out[threadIdx.x] = res[threadIdx.x] * scale;
+}
+
+void launch_finish_sum(const double* partials, int nparts, int nv, double scale, double* out, const int* skip,
+                       hipStream_t st) {
+    hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(kRedThreads), 0, st, partials, nparts, nv, scale, out, skip);
+}
+
+__global__ void k_compact_sums(const double* __restrict__ x, long nb, double* __restrict__ partials,
+                               const int* skip) {
+    if (skip != nullptr && *skip) return;
+    double acc[3] = {0, 0, 0};
+    for (long l = (long)blockIdx.x * blockDim.x + threadIdx.x; l < nb; l += (long)gridDim.x * blockDim.x) {
+        acc[0] += x[3 * l];
+        acc[1] += x[3 * l + 1];
+        acc[2] += x[3 * l + 2];
+    }
+    block_sum<3>(acc, partials + 3L * blockIdx.x);
+}
+
+void launch_compact_sums(const double* x, long nb, double* partials, const int* skip, hipStream_t st) {
+    hipLaunchKernelGGL(k_compact_sums, dim3(kRedBlocks), dim3(kRedThreads), 0, st, x, nb, partials, skip);
+}
+
+__global__ void k_sub_mean(const double* __restrict__ x, const double* __restrict__ mean, double* __restrict__ y,
+                           long nb, double sign, const int* skip) {
+    if (skip != nullptr && *skip) return;
+    const double m0 = mean[0], m1 = mean[1], m2 = mean[2];
+    for (long l = (long)blockIdx.x * blockDim.x + threadIdx.x; l < nb; l += (long)gridDim.x * blockDim.x) {
+        y[3 * l] = sign * (x[3 * l] - m0);
+        y[3 * l + 1] = sign * (x[3 * l + 1] - m1);
+        y[3 * l + 2] = sign * (x[3 * l + 2] - m2);
+    }
+}
+
+void launch_sub_mean(const double* x, const double* mean, double* y, long nb, double sign, const int* skip,
+                     hipStream_t st) {
+    hipLaunchKernelGGL(k_sub_mean, dim3(red_grid(nb)), dim3(kRedThreads), 0, st, x, mean, y, nb, sign, skip);
+}
+
+__global__ void k_field_shift_sum(double* __restrict__ f, double s0, double s1, double s2, long N,
+                                  double* __restrict__ partials) {
+    double acc[3] = {0, 0, 0};
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
+        const double a = f[i] + s0;
+        const double b = f[N + i] + s1;
+        const double c = f[2 * N + i] + s2;
+        f[i] = a;
+        f[N + i] = b;
+        f[2 * N + i] = c;
+        acc[0] += a;
+        acc[1] += b;
+        acc[2] += c;
+    }
+    block_sum<3>(acc, partials + 3L * blockIdx.x);
+}
+
+void launch_field_shift_sum(double* f, double s0, double s1, double s2, long N, double* partials,
+                            hipStream_t st) {
+    hipLaunchKernelGGL(k_field_shift_sum, dim3(kRedBlocks), dim3(kRedThreads), 0, st, f, s0, s1, s2, N, partials);
+}
+
+__global__ void k_support_sums(const double* __restrict__ f, const int64_t* __restrict__ active, long nb, long N,
+                               double* __restrict__ partials) {
+    double acc[3] = {0, 0, 0};
+    for (long l = (long)blockIdx.x * blockDim.x + threadIdx.x; l < nb; l += (long)gridDim.x * blockDim.x) {
+        const long v = active[l];
+        acc[0] += f[v];
+        acc[1] += f[N + v];
+        acc[2] += f[2 * N + v];
+    }
+    block_sum<3>(acc, partials + 3L * blockIdx.x);
+}
+
+void launch_support_sums(const double* f, const int64_t* active, long nb, long N, double* partials,
+                         hipStream_t st) {
+    hipLaunchKernelGGL(k_support_sums, dim3(kRedBlocks), dim3(kRedThreads), 0, st, f, active, nb, N, partials);
+}
+
+// BackgroundForce::build (src/bignonnet/BackgroundForce.cpp:14-60): 0 + c*g on fluid and support.
+__global__ void k_background_force(const int32_t* __restrict__ code, long N, Coef3 af, Coef3 as,
+                                   double* __restrict__ field) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
+        const int c = code[i];
+        for (int k = 0; k < 3; ++k) {
+            double v = 0.0;
+            if (c == -1) v += af.v[k];
+            if (c >= 0) v += as.v[k];
+            field[k * N + i] = v;
+        }
+    }
+}
+
+void launch_background_force(const int32_t* code, long N, Coef3 af, Coef3 as, double* field, hipStream_t st) {
+    hipLaunchKernelGGL(k_background_force, dim3(kRedBlocks), dim3(kRedThreads), 0, st, code, N, af, as, field);
+}
+
+// ------------------------------------------------------------ generic vectors
+__global__ void k_dot_partials(const double* __restrict__ a, const double* __restrict__ b, long n,
+                               double* __restrict__ partials) {
+    double acc[1] = {0.0};
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        acc[0] += a[i] * b[i];
+    block_sum<1>(acc, partials + blockIdx.x);
+}
+
+void launch_dot_partials(const double* a, const double* b, long n, double* partials, hipStream_t s) {
+    hipLaunchKernelGGL(k_dot_partials, dim3(kRedBlocks), dim3(kRedThreads), 0, s, a, b, n, partials);
+}
+
+__global__ void k_axpby(double a, const double* __restrict__ x, double b, const double* __restrict__ y,
+                        double* __restrict__ out, long n) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        out[i] = a * x[i] + b * y[i];
+}
+
+void launch_axpby(double a, const double* x, double b, const double* y, double* out, long n, hipStream_t s) {
+    hipLaunchKernelGGL(k_axpby, dim3(red_grid(n)), dim3(kRedThreads), 0, s, a, x, b, y, out, n);
+}
+
+__global__ void k_scale_copy(const double* __restrict__ x, double d, double* __restrict__ out, long n) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        out[i] = x[i] / d;
+}
+
+void launch_scale_copy(const double* x, double scale_div, double* out, long n, hipStream_t s) {
+    hipLaunchKernelGGL(k_scale_copy, dim3(red_grid(n)), dim3(kRedThreads), 0, s, x, scale_div, out, n);
+}
+
+// ------------------------------------------------------------ MINRES steps
+__global__ void k_mr_finish_mean_out(const double* __restrict__ partials, int nparts, MinresState* st,
+                                     double inv_nb) {
+    if (st->done) return;
+    double acc[3] = {0, 0, 0};
+    for (int p = threadIdx.x; p < nparts; p += blockDim.x) {
+        acc[0] += partials[6L * p];
+        acc[1] += partials[6L * p + 1];
+        acc[2] += partials[6L * p + 2];
+    }
+    __shared__ double res[3];
+    block_sum<3>(acc, res);
+    if (threadIdx.x < 3) st->mean_out[threadIdx.x] = res[threadIdx.x] * inv_nb;
+}
+
+void launch_mr_finish_mean_out(const double* partials, int nparts, MinresState* st, double inv_nb,
+                               hipStream_t s) {
+    hipLaunchKernelGGL(k_mr_finish_mean_out, dim3(1), dim3(kRedThreads), 0, s, partials, nparts, st, inv_nb);
+}
+
+__global__ void k_mr_sub_dot(double* __restrict__ z, const double* __restrict__ v, MinresState* st, long nb,
+                             double* __restrict__ partials) {
+    if (st->done) return;
+    const double m0 = st->mean_out[0], m1 = st->mean_out[1], m2 = st->mean_out[2];
+    double acc[1] = {0.0};
+    for (long l = (long)blockIdx.x * blockDim.x + threadIdx.x; l < nb; l += (long)gridDim.x * blockDim.x) {
+        const double z0 = z[3 * l] - m0;
+        const double z1 = z[3 * l + 1] - m1;
+        const double z2 = z[3 * l + 2] - m2;
+        z[3 * l] = z0;
+        z[3 * l + 1] = z1;
+        z[3 * l + 2] = z2;
+        acc[0] += v[3 * l] * z0 + v[3 * l + 1] * z1 + v[3 * l + 2] * z2;
+    }
+    block_sum<1>(acc, partials + blockIdx.x);
+}
+
+void launch_mr_sub_dot(double* z, const double* v, MinresState* st, long nb, double* partials, hipStream_t s) {
+    hipLaunchKernelGGL(k_mr_sub_dot, dim3(kRedBlocks), dim3(kRedThreads), 0, s, z, v, st, nb, partials);
+}
+
+__global__ void k_mr_finish_alpha(const double* __restrict__ partials, MinresState* st) {
+    if (st->done) return;
+    double acc[1] = {0.0};
+    for (int p = threadIdx.x; p < kRedBlocks; p += blockDim.x) acc[0] += partials[p];
+    __shared__ double res[1];
+    block_sum<1>(acc, res);
+    if (threadIdx.x == 0) st->alpha = res[0];
+}
+
+void launch_mr_finish_alpha(const double* partials, MinresState* st, hipStream_t s) {
+    hipLaunchKernelGGL(k_mr_finish_alpha, dim3(1), dim3(kRedThreads), 0, s, partials, st);
+}
+
+__global__ void k_mr_axpy_norm(double* __restrict__ z, const double* __restrict__ v,
+                               const double* __restrict__ v_old, MinresState* st, long n,
+                               double* __restrict__ partials) {
+    if (st->done) return;
+    const double a = st->alpha, b = st->beta;
+    double acc[1] = {0.0};
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const double zi = z[i] - (a * v[i] + b * v_old[i]);
+        z[i] = zi;
+        acc[0] += zi * zi;
+    }
+    block_sum<1>(acc, partials + blockIdx.x);
+}
+
+void launch_mr_axpy_norm(double* z, const double* v, const double* v_old, MinresState* st, long n,
+                         double* partials, hipStream_t s) {
+    hipLaunchKernelGGL(k_mr_axpy_norm, dim3(kRedBlocks), dim3(kRedThreads), 0, s, z, v, v_old, st, n, partials);
+}
+
+// Steps 4-12 of the reference iteration (src/solver/MinresSolver.cpp:135-202) on scalars.
+__global__ void k_mr_givens(const double* __restrict__ partials, MinresState* st, double* __restrict__ history) {
+    if (st->done) return;
+    double acc[1] = {0.0};
+    for (int p = threadIdx.x; p < kRedBlocks; p += blockDim.x) acc[0] += partials[p];
+    __shared__ double res[1];
+    block_sum<1>(acc, res);
+    if (threadIdx.x != 0) return;
+    const double kBreakdownEps = 1e-14;
+    const double s = res[0];
+    const double beta_new = sqrt(s > 0.0 ? s : 0.0);
+    const double beta = st->beta;
+    const double alpha = st->alpha;
+    const double cs_old = st->cs_old, sn_old = st->sn_old, cs = st->cs, sn = st->sn;
+    const double eps = sn_old * beta;
+    const double delta_raw = -cs_old * beta;
+    const double delta = cs * delta_raw + sn * alpha;
+    const double phi = sn * delta_raw - cs * alpha;
+    const double rho = hypot(phi, beta_new);
+    double cs_new = 1.0, sn_new = 0.0;
+    if (rho > kBreakdownEps) {
+        cs_new = phi / rho;
+        sn_new = beta_new / rho;
+    }
+    st->beta_new = beta_new;
+    st->v_zero = (beta_new > kBreakdownEps) ? 0 : 1;
+    st->w_zero = (rho > kBreakdownEps) ? 0 : 1;
+    st->c_eps = eps;
+    st->c_delta = delta;
+    st->c_rho = rho;
+    st->c_xcoef = cs_new * st->phi_bar;
+    const double phi_bar = st->phi_bar * sn_new;
+    st->phi_bar = phi_bar;
+    st->cs_old = cs;
+    st->sn_old = sn;
+    st->cs = cs_new;
+    st->sn = sn_new;
+    st->beta = beta_new;
+    const double rel = fabs(phi_bar) / st->rhs_norm;
+    st->rel_res = rel;
+    const long long k = st->iterations;
+    if (history != nullptr) history[k + 1] = rel;
+    st->iterations = k + 1;
+    const int conv = (rel <= st->tol) ? 1 : 0;
+    st->converged = conv;
+    st->pending_done = (conv || beta_new < kBreakdownEps || (k + 1) >= st->maxit) ? 1 : 0;
+}
+
+void launch_mr_givens(const double* partials, MinresState* st, double* history, hipStream_t s) {
+    hipLaunchKernelGGL(k_mr_givens, dim3(1), dim3(kRedThreads), 0, s, partials, st, history);
+}
+
+__global__ void k_mr_update(double* __restrict__ x, double* __restrict__ v_old, double* __restrict__ v,
+                            const double* __restrict__ z, double* __restrict__ w_prev, double* __restrict__ w,
+                            MinresState* st, long nb, double* __restrict__ partials) {
+    if (st->done) return;
+    const double beta_new = st->beta_new;
+    const double eps = st->c_eps, delta = st->c_delta, rho = st->c_rho, xc = st->c_xcoef;
+    const int vz = st->v_zero, wz = st->w_zero;
+    double acc[3] = {0, 0, 0};
+    for (long l = (long)blockIdx.x * blockDim.x + threadIdx.x; l < nb; l += (long)gridDim.x * blockDim.x) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const long i = 3 * l + c;
+            const double vo = v[i];
+            const double vn = vz ? 0.0 : z[i] / beta_new;
+            const double wp = w_prev[i];
+            const double wc = w[i];
+            const double wn = wz ? 0.0 : (vo - eps * wp - delta * wc) / rho;
+            x[i] += xc * wn;
+            v_old[i] = vo;
+            v[i] = vn;
+            w_prev[i] = wc;
+            w[i] = wn;
+            acc[c] += vn;
+        }
+    }
+    block_sum<3>(acc, partials + 3L * blockIdx.x);
+}
+
+void launch_mr_update(double* x, double* v_old, double* v, const double* z, double* w_prev, double* w,
+                      MinresState* st, long nb, double* partials, hipStream_t s) {
+    hipLaunchKernelGGL(k_mr_update, dim3(kRedBlocks), dim3(kRedThreads), 0, s, x, v_old, v, z, w_prev, w, st, nb,
+                       partials);
+}
+
+__global__ void k_mr_finish_mean_in(const double* __restrict__ partials, MinresState* st, double inv_nb) {
+    if (st->done) return;
+    double acc[3] = {0, 0, 0};
+    for (int p = threadIdx.x; p < kRedBlocks; p += blockDim.x) {
+        acc[0] += partials[3 * p];
+        acc[1] += partials[3 * p + 1];
+        acc[2] += partials[3 * p + 2];
+    }
+    __shared__ double res[3];
+    block_sum<3>(acc, res);
+    if (threadIdx.x == 0) {
+        st->mean_in[0] = res[0] * inv_nb;
+        st->mean_in[1] = res[1] * inv_nb;
+        st->mean_in[2] = res[2] * inv_nb;
+        st->done = st->pending_done;
+    }
+}
+
+void launch_mr_finish_mean_in(const double* partials, MinresState* st, double inv_nb, hipStream_t s) {
+    hipLaunchKernelGGL(k_mr_finish_mean_in, dim3(1), dim3(kRedThreads), 0, s, partials, st, inv_nb);
+}
+
+}  // namespace fftperm

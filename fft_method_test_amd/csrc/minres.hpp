@@ -1,0 +1,56 @@
+// minres.hpp -- device-resident MINRES state and the launchers of its vector kernels.
+//
+// MinresSolver::solve (src/solver/MinresSolver.cpp:46-218) is split into kernels whose
+// scalar recurrences (alpha, beta, Givens rotation, phi_bar) are computed on the device
+// by single-block "finish" kernels, so an iteration needs no host synchronisation.
+// Every kernel of an iteration returns immediately once `done` is set: iterations
+// launched past convergence are no-ops, and the iteration count equals the reference's.
+#pragma once
+
+#include "common.hpp"
+
+namespace fftperm {
+
+struct MinresState {
+    // recurrence (names of src/solver/MinresSolver.cpp:112-121)
+    double beta, cs_old, sn_old, cs, sn, phi_bar;
+    double rhs_norm, tol;
+    double alpha;
+    // coefficients of the current iteration's vector update
+    double beta_new, c_eps, c_delta, c_rho, c_xcoef;
+    int v_zero, w_zero;
+    double rel_res;
+    double mean_in[3];   // support mean of v (input projection of the next apply)
+    double mean_out[3];  // support mean of the gathered apply output
+    long long iterations;
+    long long maxit;
+    int done;
+    int pending_done;
+    int converged;
+    int pad_;
+};
+
+// z = z - mean_out ; partials: sum v.z                     (Algorithm 1 step 17 + alpha)
+void launch_mr_sub_dot(double* z, const double* v, MinresState* st, long nb, double* partials, hipStream_t s);
+void launch_mr_finish_alpha(const double* partials, MinresState* st, hipStream_t s);
+// z -= alpha v + beta v_old ; partials: sum z^2
+void launch_mr_axpy_norm(double* z, const double* v, const double* v_old, MinresState* st, long n,
+                         double* partials, hipStream_t s);
+// beta_new, Givens rotation, phi_bar, relative residual estimate, stop flags, history
+void launch_mr_givens(const double* partials, MinresState* st, double* history, hipStream_t s);
+// v_old <- v, v <- z / beta_new, w_next, x += (cs_new phi_bar) w_next, w_prev <- w, w <- w_next ;
+// partials: per-component sums of the new v
+void launch_mr_update(double* x, double* v_old, double* v, const double* z, double* w_prev, double* w,
+                      MinresState* st, long nb, double* partials, hipStream_t s);
+// mean_in = sums / nb ; done = pending_done
+void launch_mr_finish_mean_in(const double* partials, MinresState* st, double inv_nb, hipStream_t s);
+// mean_out = support sums of the x-inverse partials / nb
+void launch_mr_finish_mean_out(const double* partials, int nparts, MinresState* st, double inv_nb,
+                               hipStream_t s);
+
+// Generic helpers
+void launch_dot_partials(const double* a, const double* b, long n, double* partials, hipStream_t s);
+void launch_axpby(double a, const double* x, double b, const double* y, double* out, long n, hipStream_t s);
+void launch_scale_copy(const double* x, double scale_div, double* out, long n, hipStream_t s);
+
+}  // namespace fftperm

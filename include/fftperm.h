@@ -1,0 +1,180 @@
+/*
+ * fftperm.h -- C ABI of the MI355X (gfx950) FFT permeability library (libfftperm.so).
+ *
+ * Drop-in boundary for the hot path of lcolabella/FFT_Method_Test: the Bignonnet
+ * variational Stokes solver (support map, energy-consistent Green operator applied
+ * by 3D FFTs, MINRES on the interface forces, velocity recovery, K assembly) and the
+ * Brinkman penalisation loop.  Plain C types only; no exception crosses this ABI.
+ *
+ * Conventions (as in the reference):
+ *  - grid flat index (k*ny + j)*nx + i                    (src/core/Grid3D.hpp:41-43)
+ *  - vector fields are SoA by component: [x(N) | y(N) | z(N)]  (src/core/Field3D.hpp:16-114)
+ *  - compact support vectors are AoS: [x0,y0,z0,x1,...], L = 3*NB
+ *                                                          (src/bignonnet/TrialForceField.cpp:31)
+ *  - complex spectra are interleaved (re, im) doubles, full spectrum of N per component
+ *  - fluid mask: 1 = fluid, 0 = solid                     (src/geometry/BinaryMedium.cpp:14-20)
+ * Pointers are host memory unless the name ends in _dev (device memory of the context's GPU).
+ *
+ * Return value: FFTPERM_OK (0) or an error code; fftperm_last_error() gives the message
+ * of the last failure on the calling thread.  The codes map the reference's exceptions:
+ * std::invalid_argument -> FFTPERM_EINVAL, std::out_of_range -> FFTPERM_ERANGE,
+ * std::runtime_error -> FFTPERM_ERUNTIME; HIP failures -> FFTPERM_EDEVICE.
+ *
+ * Threading: contexts are independent (own HIP stream and buffers); one context must
+ * not be used from two threads at once.
+ */
+#ifndef FFTPERM_H
+#define FFTPERM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    FFTPERM_OK = 0,
+    FFTPERM_EINVAL = 1,
+    FFTPERM_ERANGE = 2,
+    FFTPERM_ERUNTIME = 3,
+    FFTPERM_EDEVICE = 4,
+    FFTPERM_EUNSUPPORTED = 5
+};
+
+/* SupportMode (src/core/Types.hpp:26-29) */
+enum { FFTPERM_SUPPORT_FULLSOLID = 0, FFTPERM_SUPPORT_INTERFACE = 1 };
+/* FFTPERM_PRECISION_FP64: every step in FP64 (the reference with FFT_METHOD_PRECISION=DOUBLE).
+ * FFTPERM_PRECISION_MIXED: FFTs and Green multiply in FP32 inside MINRES; Lanczos vectors,
+ * inner products, Givens rotations, true residuals, the refinement update, RHS,
+ * velocity recovery and averages in FP64. */
+enum { FFTPERM_PRECISION_FP64 = 0, FFTPERM_PRECISION_MIXED = 1 };
+
+typedef struct fftperm_ctx fftperm_ctx;
+
+/* Grid3D(nx,ny,nz,hx,hy,hz,lx,ly,lz) (src/core/Grid3D.hpp:14-24): l <= 0 means h * n. */
+typedef struct {
+    uint64_t nx, ny, nz;
+    double hx, hy, hz;
+    double lx, ly, lz;
+} fftperm_grid;
+
+/* SolverResult (src/solver/SolverResult.hpp:10-16) without the solution vector. */
+typedef struct {
+    uint64_t iterations;        /* MINRES iterations (summed over refinement sweeps in MIXED) */
+    double final_rel_residual;  /* recurrence estimate |phi_bar| / ||b|| (FP64), or the true
+                                   relative residual of the last refinement sweep (MIXED) */
+    int converged;
+    uint64_t history_len;       /* entries written to the history array */
+    uint64_t refinement_sweeps; /* 0 in FP64 mode */
+} fftperm_solve_info;
+
+/* Last error message of the calling thread ("" if none). */
+const char* fftperm_last_error(void);
+
+/* Human-readable device description (IFFTBackend::device_info, src/fft/IFFTBackend.hpp:48-49). */
+int fftperm_device_info(char* buf, size_t len);
+
+/* GreenOperator construction (src/bignonnet/GreenOperator.cpp:27-52) + IFFTBackend::initialize
+ * (src/fft/IFFTBackend.hpp:16): plans the FFT passes and precomputes the symmetrised
+ * energy-consistent Green tensor once per context (the reference recomputes it per load case,
+ * src/app/PermeabilityRunner.cpp:186-198).  mu <= 0 -> FFTPERM_EINVAL.  Grid edges must factor
+ * into 2, 3, 5, 7, 11, 13 (FFTPERM_EUNSUPPORTED otherwise). */
+int fftperm_create(const fftperm_grid* grid, double mu, int p_radius, int precision, fftperm_ctx** out);
+int fftperm_destroy(fftperm_ctx* ctx);
+
+/* BinaryMedium + ForceSupport (src/geometry/BinaryMedium.cpp:7-12, src/bignonnet/ForceSupport.cpp:9-25):
+ * uploads the fluid mask and builds the support on the GPU.  An empty support is not an error
+ * here; solves reject it as the runner does (src/app/PermeabilityRunner.cpp:523-526). */
+int fftperm_set_medium(fftperm_ctx* ctx, const uint8_t* fluid_mask, int support_mode, uint64_t* nb_out);
+int fftperm_set_medium_dev(fftperm_ctx* ctx, const uint8_t* fluid_mask_dev, int support_mode, uint64_t* nb_out);
+
+/* BinaryMedium::porosity (src/geometry/BinaryMedium.cpp:34-43). */
+int fftperm_porosity(fftperm_ctx* ctx, double* porosity);
+
+/* ForceSupport::active_voxels (ascending) and voxel_to_local (-1 where the reference stores
+ * kInvalid = SIZE_MAX) (src/bignonnet/ForceSupport.hpp:22-29).  Either pointer may be NULL. */
+int fftperm_get_support(fftperm_ctx* ctx, uint64_t* active, int64_t* voxel_to_local);
+
+/* GreenOperator::tensor_components for every index: 6 arrays of N (xx,xy,xz,yy,yz,zz) over the
+ * unshifted frequencies (src/bignonnet/GreenOperator.cpp:95-204). */
+int fftperm_green_tensor(fftperm_ctx* ctx, double* t6);
+
+/* IFFTBackend::forward: real 3N -> full complex spectrum 3N (unnormalised, e^{-i}),
+ * (src/fft/IFFTBackend.hpp:18-19, src/fft/FFTWBackend.cpp:203-264). */
+int fftperm_fft_forward(fftperm_ctx* ctx, const double* real3, double* spec3);
+/* IFFTBackend::inverse: Re(backward C2C) / N (src/fft/IFFTBackend.hpp:21-22, FFTWBackend.cpp:266-329). */
+int fftperm_fft_inverse(fftperm_ctx* ctx, const double* spec3, double* real3);
+/* IFFTBackend::apply_spectral_green_tensor (src/fft/IFFTBackend.hpp:26-43): V = T F per frequency,
+ * T = 6 arrays of N, F and V full complex spectra of 3N. */
+int fftperm_spectral_green(fftperm_ctx* ctx, const double* t6, const double* force_spec, double* vel_spec);
+
+/* GreenOperator::apply (src/bignonnet/GreenOperator.cpp:54-65): u = Re IFFT(G FFT f), 3N -> 3N. */
+int fftperm_green_apply(fftperm_ctx* ctx, const double* f, double* u);
+/* BackgroundForce::build (src/bignonnet/BackgroundForce.cpp:14-60): f0 (3N). */
+int fftperm_background_force(fftperm_ctx* ctx, const double grad[3], double* f0);
+/* VariationalRHS::build (src/bignonnet/VariationalRHS.cpp:15-31): b (L). */
+int fftperm_rhs(fftperm_ctx* ctx, const double grad[3], double* b);
+/* VariationalOperator::apply (src/bignonnet/VariationalOperator.cpp:19-41): y = A x (L -> L).
+ * FP64 in every precision mode. */
+int fftperm_operator_apply(fftperm_ctx* ctx, const double* x, double* y);
+
+/* MinresSolver(tol, maxit).solve(op, b, x0) (src/solver/MinresSolver.cpp:46-218) on the variational
+ * operator.  x0 may be NULL (the reference's empty guess).  history (maxit + 1 entries) may be NULL.
+ * tol <= 0 -> FFTPERM_EINVAL.  In MIXED mode: iterative refinement with FP32-operator MINRES sweeps
+ * and FP64 true residuals until ||b - A x|| <= tol ||b|| (or maxit iterations in total). */
+int fftperm_minres(fftperm_ctx* ctx, const double* b, const double* x0, double tol, uint64_t maxit,
+                   double* x, double* history, fftperm_solve_info* info);
+
+/* VelocityRecovery::recover + average_over_domain (src/bignonnet/VelocityRecovery.cpp:12-47):
+ * u (3N, may be NULL) and its domain average avg[3]. */
+int fftperm_recover(fftperm_ctx* ctx, const double grad[3], const double* x, double* u, double avg[3]);
+
+/* solve_one_direction (src/app/PermeabilityRunner.cpp:173-242): RHS, MINRES (x0 empty unless given),
+ * recovery, domain average.  u (3N) may be NULL. */
+int fftperm_solve_direction(fftperm_ctx* ctx, const double grad[3], double tol, uint64_t maxit,
+                            const double* x0, double avg[3], double* u, fftperm_solve_info* info);
+
+/* PermeabilityRunner::run K assembly for gradient=all (src/app/PermeabilityRunner.cpp:740-777 with
+ * src/post/PermeabilityTensor.cpp:11-18): K (row-major 9) = +mu <v_r>^(c).  info[3] may be NULL. */
+int fftperm_solve_tensor(fftperm_ctx* ctx, double tol, uint64_t maxit, double K[9], fftperm_solve_info* info);
+
+/* Diagnostics::residual_norm / relative_symmetry_gap (src/post/Diagnostics.cpp:26-64), FP64. */
+int fftperm_residual_norm(fftperm_ctx* ctx, const double* x, const double* b, double* out);
+int fftperm_symmetry_gap(fftperm_ctx* ctx, const double* x, const double* y, double* out);
+
+/* Live kernel timing (no counterpart in the reference; a measurement hook).  While enabled, the
+ * context records HIP events on its own stream around every fused z-pass launch (z forward FFT,
+ * Green multiply, z backward FFT) and around every five-sweep Green apply; MINRES applies that ran
+ * after convergence (no-ops) are not counted.  fftperm_profile(ctx, 1) resets and enables. */
+typedef struct {
+    double zgreen_ms[2];         /* [0] FP64, [1] FP32 (mixed) : summed durations */
+    uint64_t zgreen_launches[2];
+    double apply_ms[2];
+    uint64_t apply_launches[2];
+    uint64_t N, Nh, nb;          /* grid size, half-spectrum size, support voxels */
+} fftperm_profile_stats;
+
+int fftperm_profile(fftperm_ctx* ctx, int enable);
+int fftperm_profile_read(fftperm_ctx* ctx, fftperm_profile_stats* out);
+
+/* Brinkman penalisation: common::fluid::solveBrinkman (src/brinkman/FluidBrinkman.cpp:282-507) for
+ * one forcing vector on an (nx,ny,nz) grid.  alpha / viscosity: per-voxel coefficient fields
+ * (buildCoefficientFields, :152-191: alpha = penalty in solid and 0 in fluid; viscosity = the fluid's
+ * in fluid and 1 in solid).  precision: FFTPERM_PRECISION_FP64 or FFTPERM_PRECISION_MIXED (FP32
+ * pressure-Poisson sweeps, FP64 velocity/pressure/residual).  U[3] = superficial velocity; u (3N) may
+ * be NULL.  Non-finite residual -> FFTPERM_ERUNTIME, as the reference throws (:473-477). */
+typedef struct {
+    int iterations;
+    double residual;
+    int converged;
+} fftperm_brinkman_info;
+
+int fftperm_brinkman_solve(uint64_t nx, uint64_t ny, uint64_t nz, const double* alpha, const double* viscosity,
+                           double relaxation, double tol, int max_iterations, const double force[3], int precision,
+                           double U[3], double* u, fftperm_brinkman_info* info);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

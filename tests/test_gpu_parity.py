@@ -1,0 +1,177 @@
+"""GPU parity tests: the HIP path through the C ABI against the FP64 oracle.
+
+Integer/index work (support maps) must be bit-exact; floating-point results are
+compared in relative Frobenius norm with the tolerances written in each test
+(1e-10 for permeability tensors, the contract of the FP64 path).
+"""
+import numpy as np
+import pytest
+
+from conftest import random_mask, rel
+
+pytestmark = pytest.mark.gpu
+
+SHAPES_SMALL = [(4, 4, 4), (2, 2, 1), (3, 3, 3), (6, 5, 4), (8, 8, 8), (5, 7, 3), (16, 12, 10)]
+
+
+def make_pair(fp, oracle_mod, shape, mask, support=1, mu=1.0, p=2, precision=0, h=1.0):
+    from fft_method_test_amd import permeability as P
+
+    g = P.Grid3D(*shape, h, h, h)
+    green = P.GreenOperator(g, mu, p, precision)
+    sup = P.ForceSupport(P.BinaryMedium(g, mask), support, green)
+    o = oracle_mod.Oracle(shape, h=h, mu=mu, p_radius=p)
+    o.set_medium(mask, support)
+    return green, sup, o
+
+
+@pytest.mark.parametrize("shape", SHAPES_SMALL + [(32, 32, 32), (64, 48, 40)])
+@pytest.mark.parametrize("support", [0, 1])
+def test_support_bit_exact(fp, oracle_mod, shape, support):
+    mask = random_mask(shape, 0.7, seed=sum(shape) + support)
+    green, sup, o = make_pair(fp, oracle_mod, shape, mask, support)
+    act_o, v2l_o = o.support()
+    assert sup.num_active_voxels() == o.nb
+    assert np.array_equal(sup.active_voxels().astype(np.uint64), act_o)
+    assert np.array_equal(sup.voxel_to_local(), v2l_o)
+
+
+def test_interface_block_reference_case(fp):
+    """tests/test_interface_voxels.cpp: 2x2x2 solid block in 4^3 -> 8 sorted interface voxels."""
+    from fft_method_test_amd import permeability as P
+
+    g = P.Grid3D(4, 4, 4)
+    mask = np.ones(64, dtype=np.uint8)
+    for k in (1, 2):
+        for j in (1, 2):
+            for i in (1, 2):
+                mask[g.flat_index(i, j, k)] = 0
+    green = P.GreenOperator(g)
+    sup = P.ForceSupport(P.BinaryMedium(g, mask), P.SUPPORT_INTERFACE, green)
+    act = sup.active_voxels()
+    assert sup.num_active_voxels() == 8
+    assert np.all(np.diff(act.astype(np.int64)) > 0)
+
+
+@pytest.mark.parametrize("shape", [(4, 4, 4), (6, 5, 4), (16, 16, 16), (9, 10, 7)])
+@pytest.mark.parametrize("p", [0, 2])
+def test_green_tensor_matches_oracle(fp, oracle_mod, shape, p):
+    """The GPU Green tensor uses the reference's expressions: equal to the oracle within 2 ulp."""
+    from fft_method_test_amd import permeability as P
+
+    green = P.GreenOperator(P.Grid3D(*shape), 1.3, p)
+    o = oracle_mod.Oracle(shape, mu=1.3, p_radius=p)
+    tg, to = green.tensor(), o.green_tensor()
+    scale = np.abs(to).max()
+    assert np.abs(tg - to).max() <= 4e-16 * scale
+    assert np.all(tg[:, 0] == 0.0)
+
+
+@pytest.mark.parametrize("shape", [(4, 4, 4), (2, 2, 1), (3, 3, 3), (8, 6, 5), (32, 32, 32), (16, 20, 12)])
+def test_fft_hooks_match_numpy(fp, shape):
+    from fft_method_test_amd import permeability as P
+
+    nx, ny, nz = shape
+    N = nx * ny * nz
+    green = P.GreenOperator(P.Grid3D(*shape))
+    rng = np.random.default_rng(7)
+    f = rng.standard_normal(3 * N)
+    spec = green.fft_forward(f).reshape(3, nz, ny, nx)
+    ref = np.stack([np.fft.fftn(f[c * N:(c + 1) * N].reshape(nz, ny, nx)) for c in range(3)])
+    assert rel(spec, ref) < 1e-14
+    back = green.fft_inverse(spec.reshape(-1))
+    assert rel(back, f) < 1e-14
+    # inverse of a non-Hermitian spectrum = real part of the C2C inverse (src/fft/FFTWBackend.cpp:292)
+    X = rng.standard_normal((3, nz, ny, nx)) + 1j * rng.standard_normal((3, nz, ny, nx))
+    got = green.fft_inverse(X.reshape(-1))
+    want = np.stack([np.fft.ifftn(X[c]).real for c in range(3)]).reshape(-1)
+    assert rel(got, want) < 1e-14
+
+
+@pytest.mark.parametrize("shape", SHAPES_SMALL + [(32, 32, 32), (24, 20, 18)])
+def test_green_apply_matches_oracle(fp, oracle_mod, shape):
+    mask = random_mask(shape, 0.6, seed=3)
+    green, sup, o = make_pair(fp, oracle_mod, shape, mask)
+    N = int(np.prod(shape))
+    f = np.random.default_rng(1).standard_normal(3 * N)
+    assert rel(green.apply(f), o.green_apply(f)) < 1e-13
+
+
+@pytest.mark.parametrize("shape", SHAPES_SMALL + [(32, 32, 32)])
+def test_rhs_operator_recover_match_oracle(fp, oracle_mod, shape):
+    from fft_method_test_amd import permeability as P
+
+    mask = random_mask(shape, 0.65, seed=11)
+    green, sup, o = make_pair(fp, oracle_mod, shape, mask)
+    if sup.num_active_voxels() == 0:
+        pytest.skip("empty support")
+    grad = np.array([1.0, -0.4, 0.25])
+    bg = P.BackgroundForce(P.BinaryMedium(green.grid(), mask), sup)
+    assert np.array_equal(bg.build(grad), o.background_force(grad))
+    b = P.VariationalRHS(sup, green, bg).build(grad)
+    assert rel(b, o.rhs(grad)) < 1e-12
+    x = np.random.default_rng(5).standard_normal(sup.size())
+    op = P.VariationalOperator(sup, green)
+    assert rel(op.apply(x), o.apply(x)) < 1e-12
+    rec = P.VelocityRecovery(sup, green, bg)
+    u = rec.recover(grad, x)
+    avg_o, u_o = o.recover(grad, x, want_field=True)
+    assert rel(u, u_o) < 1e-12
+    assert np.allclose(rec.average_over_domain(u), avg_o, rtol=1e-10, atol=1e-14 * np.abs(u_o).max())
+
+
+def test_minres_matches_oracle_sphere32(fp, oracle_mod):
+    """C1 cell (32^3 sphere, r = 0.3 L): same iteration count and history as the oracle."""
+    from fft_method_test_amd import geometry as G, permeability as P
+
+    mask = G.sphere_cell(32).reshape(-1)
+    green, sup, o = make_pair(fp, oracle_mod, (32, 32, 32), mask)
+    b = o.rhs([1.0, 0.0, 0.0])
+    res = P.MinresSolver(1e-10, 2000).solve(P.VariationalOperator(sup, green), b)
+    xo, info, ho = o.minres(b, 1e-10, 2000)
+    assert res.converged and info.converged
+    assert abs(res.iterations - info.iterations) <= 1
+    n = min(len(ho), len(res.residual_history))
+    assert np.allclose(res.residual_history[: n - 2], ho[: n - 2], rtol=1e-6, atol=1e-14)
+    assert rel(res.solution, xo) < 1e-8
+
+
+def test_k_tensor_sphere32_fp64(fp, oracle_mod):
+    """Full 3x3 K on the C1 cell: GPU FP64 within 1e-10 of the oracle."""
+    from fft_method_test_amd import geometry as G, permeability as P
+
+    mask = G.sphere_cell(32).reshape(-1)
+    green, sup, o = make_pair(fp, oracle_mod, (32, 32, 32), mask)
+    K, infos = P.solve_permeability_tensor(green, 1e-10, 10000)
+    Ko, io = o.solve_tensor(1e-10, 10000)
+    assert all(i.converged for i in infos)
+    assert rel(K, Ko) < 1e-10
+    assert K[0, 0] > 0
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_k_tensor_random_porous_fp64(fp, oracle_mod, seed):
+    from fft_method_test_amd import geometry as G, permeability as P
+
+    mask = G.boolean_spheres(24, 3.0, 0.6, seed).reshape(-1)
+    green, sup, o = make_pair(fp, oracle_mod, (24, 24, 24), mask)
+    K, infos = P.solve_permeability_tensor(green, 1e-10, 10000)
+    Ko, io = o.solve_tensor(1e-10, 10000)
+    assert rel(K, Ko) < 1e-10
+    for a, b in zip(infos, io):
+        assert abs(int(a.iterations) - int(b.iterations)) <= 2
+
+
+def test_symmetry_gap_fp64(fp, oracle_mod):
+    """tests/test_variational_operator_symmetry.cpp at FP64: <x,Ay> = <Ax,y>."""
+    from fft_method_test_amd import permeability as P
+
+    mask = random_mask((16, 16, 16), 0.6, seed=4)
+    green, sup, o = make_pair(fp, oracle_mod, (16, 16, 16), mask)
+    rng = np.random.default_rng(42)
+    x = rng.uniform(-1, 1, sup.size())
+    y = rng.uniform(-1, 1, sup.size())
+    op = P.VariationalOperator(sup, green)
+    gap = P.Diagnostics.relative_symmetry_gap(op, x, y)
+    assert gap < 1e-13
+    assert abs(gap - o.symmetry_gap(x, y)) < 1e-13

@@ -1,0 +1,232 @@
+"""Pins of the oracle that do not depend on the reference build (CPU).
+
+FFT: numpy.fft and a brute-force DFT.  Green tensor: an independent vectorised
+evaluation of the truncated Eq. (14) sum.  Green apply: numpy full C2C with the
+real part taken (src/fft/FFTWBackend.cpp:292), a direct real-space convolution, and
+the symmetrised-tensor identity the GPU path relies on.  MINRES: scipy and pinv on
+dense symmetric matrices.  Each check is chosen so that a dropped term, a sign, a
+transposed index or a missing 1/N fails it.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import random_mask, rel
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "reference_vectors.npz")
+
+
+@pytest.mark.parametrize("shape", [(4, 4, 4), (2, 3, 5), (7, 11, 13), (16, 8, 12), (17, 3, 2), (1, 1, 6), (32, 32, 32)])
+def test_fft3d_matches_numpy(oracle_mod, shape):
+    nx, ny, nz = shape
+    rng = np.random.default_rng(nx * 100 + ny)
+    a = rng.standard_normal((nz, ny, nx)) + 1j * rng.standard_normal((nz, ny, nx))
+    assert rel(oracle_mod.fft3d(a, -1), np.fft.fftn(a)) < 1e-14
+    assert rel(oracle_mod.fft3d(a, +1), np.fft.ifftn(a) * a.size) < 1e-14
+
+
+@pytest.mark.parametrize("shape", [(3, 2, 2), (4, 4, 1), (5, 3, 2)])
+def test_fft3d_brute_force_dft(oracle_mod, shape):
+    nx, ny, nz = shape
+    rng = np.random.default_rng(5)
+    a = rng.standard_normal((nz, ny, nx)) + 1j * rng.standard_normal((nz, ny, nx))
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    out = np.zeros_like(a)
+    for kz in range(nz):
+        for ky in range(ny):
+            for kx in range(nx):
+                ph = np.exp(-2j * np.pi * (kx * x / nx + ky * y / ny + kz * z / nz))
+                out[kz, ky, kx] = np.sum(a * ph)
+    assert rel(oracle_mod.fft3d(a, -1), out) < 1e-14
+
+
+def green_numpy(shape, h, mu, P):
+    """Eq. (14) truncated to |p_i| <= P over the unshifted index k in [0, n), vectorised."""
+    nx, ny, nz = shape
+    L = np.array([h * nx, h * ny, h * nz])
+    kz, ky, kx = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    G = np.zeros((6,) + kx.shape)
+    ps = np.arange(-P, P + 1)
+    for px in ps:
+        for py in ps:
+            for pz in ps:
+                m = [kx + px * nx, ky + py * ny, kz + pz * nz]
+                w = np.ones(kx.shape)
+                q = []
+                for a, n, Li in zip(m, (nx, ny, nz), L):
+                    arg = np.pi * a / n
+                    s = np.where(np.abs(arg) < 1e-12, 1.0, np.sin(arg) / np.where(arg == 0, 1, arg))
+                    w = w * s * s
+                    q.append(2 * np.pi * a / Li)
+                q2 = q[0] ** 2 + q[1] ** 2 + q[2] ** 2
+                ok = q2 > np.finfo(float).eps
+                q2s = np.where(ok, q2, 1.0)
+                fac = np.where(ok, w / (mu * q2s), 0.0)
+                comps = [(0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2)]
+                for g, (i, j) in enumerate(comps):
+                    G[g] += fac * ((1.0 if i == j else 0.0) - q[i] * q[j] / q2s)
+    G[:, 0, 0, 0] = 0.0
+    return G.reshape(6, -1)
+
+
+@pytest.mark.parametrize("shape,h,mu,P", [((4, 4, 4), 1.0, 1.0, 0), ((6, 5, 4), 0.5, 1.3, 1), ((8, 8, 8), 1.0, 1.0, 2),
+                                          ((5, 7, 3), 2.0, 0.7, 2)])
+def test_green_tensor_independent_sum(oracle_mod, shape, h, mu, P):
+    o = oracle_mod.Oracle(shape, h=h, mu=mu, p_radius=P)
+    G = o.green_tensor()
+    Gn = green_numpy(shape, h, mu, P)
+    assert np.abs(G - Gn).max() <= 1e-13 * np.abs(Gn).max()
+
+
+def test_green_tensor_invariants(oracle_mod):
+    """tests/test_projector_green.cpp: finite, zero at k = 0; plus trace = 2 sum w/(mu q^2) (projector rank 2)."""
+    o = oracle_mod.Oracle((8, 6, 4), p_radius=2)
+    G = o.green_tensor()
+    assert np.all(np.isfinite(G))
+    assert np.all(np.abs(G[:, 0]) < 1e-15)
+    # G q = 0 for the single-term (P = 0) operator: the projector annihilates q
+    o0 = oracle_mod.Oracle((8, 6, 4), p_radius=0)
+    G0 = o0.green_tensor().reshape(6, 4, 6, 8)
+    kz, ky, kx = np.meshgrid(np.arange(4), np.arange(6), np.arange(8), indexing="ij")
+    q = [2 * np.pi * kx / 8, 2 * np.pi * ky / 6, 2 * np.pi * kz / 4]
+    xx, xy, xz, yy, yz, zz = G0
+    Gq = [xx * q[0] + xy * q[1] + xz * q[2], xy * q[0] + yy * q[1] + yz * q[2], xz * q[0] + yz * q[1] + zz * q[2]]
+    for c in Gq:
+        assert np.abs(c).max() < 1e-14
+
+
+def _green_numpy_apply(G, shape, f):
+    nx, ny, nz = shape
+    N = nx * ny * nz
+    F = [np.fft.fftn(f[c * N:(c + 1) * N].reshape(nz, ny, nx)) for c in range(3)]
+    xx, xy, xz, yy, yz, zz = [g.reshape(nz, ny, nx) for g in G]
+    V = [xx * F[0] + xy * F[1] + xz * F[2], xy * F[0] + yy * F[1] + yz * F[2], xz * F[0] + yz * F[1] + zz * F[2]]
+    return np.concatenate([np.fft.ifftn(v).real.reshape(-1) for v in V])
+
+
+@pytest.mark.parametrize("shape", [(4, 4, 4), (6, 5, 4), (16, 16, 16), (9, 7, 5)])
+def test_green_apply_matches_numpy(oracle_mod, shape):
+    o = oracle_mod.Oracle(shape, mu=1.1)
+    N = int(np.prod(shape))
+    f = np.random.default_rng(2).standard_normal(3 * N)
+    assert rel(o.green_apply(f), _green_numpy_apply(o.green_tensor(), shape, f)) < 1e-14
+
+
+def test_green_apply_golden_tensor(oracle_mod):
+    """The reference's own tensor (fixture b3) through numpy equals the oracle's FFT-mode apply."""
+    g = np.load(GOLDEN, allow_pickle=False)
+    shape = tuple(int(v) for v in g["b3_shape"])
+    _, _, _, mu, P, h = g["b3_params"]
+    o = oracle_mod.Oracle(shape, h=float(h), mu=float(mu), p_radius=int(P))
+    f = g["b3_f"]
+    assert rel(o.green_apply(f), _green_numpy_apply(g["b3_green"], shape, f)) < 1e-14
+
+
+def test_symmetrised_tensor_identity(oracle_mod):
+    """Re IFFT(G(k) F) == IFFT(G_sym(k) F) with G_sym(k) = (G(k) + G(-k))/2 for real input: the
+    truncated G is not even in k, and the half-spectrum GPU path must use G_sym."""
+    shape = (8, 6, 5)
+    nx, ny, nz = shape
+    N = nx * ny * nz
+    o = oracle_mod.Oracle(shape, p_radius=2)
+    G = o.green_tensor().reshape(6, nz, ny, nx)
+    Gm = G[:, (-np.arange(nz)) % nz][:, :, (-np.arange(ny)) % ny][:, :, :, (-np.arange(nx)) % nx]
+    assert np.abs(G - Gm).max() > 1e-6 * np.abs(G).max()  # truncation breaks the parity
+    Gs = 0.5 * (G + Gm)
+    f = np.random.default_rng(3).standard_normal(3 * N)
+    want = o.green_apply(f)
+    got = _green_numpy_apply(Gs.reshape(6, -1), shape, f)
+    assert rel(got, want) < 1e-14
+    # and the symmetrised product is Hermitian, so the inverse is real without taking .real
+    F = np.fft.fftn(f[:N].reshape(nz, ny, nx))
+    V = Gs[0] * F
+    assert np.abs(np.fft.ifftn(V).imag).max() < 1e-14 * np.abs(np.fft.ifftn(V).real).max()
+
+
+def test_green_apply_direct_convolution(oracle_mod):
+    """u = g * f with g = IFFT(G_sym) summed directly in real space (O(N^2)) on 4x3x5."""
+    shape = (4, 3, 5)
+    nx, ny, nz = shape
+    N = nx * ny * nz
+    o = oracle_mod.Oracle(shape)
+    G = o.green_tensor().reshape(6, nz, ny, nx)
+    Gm = G[:, (-np.arange(nz)) % nz][:, :, (-np.arange(ny)) % ny][:, :, :, (-np.arange(nx)) % nx]
+    g = np.fft.ifftn(0.5 * (G + Gm), axes=(1, 2, 3)).real
+    comps = {(0, 0): 0, (0, 1): 1, (0, 2): 2, (1, 1): 3, (1, 2): 4, (2, 2): 5}
+    f = np.random.default_rng(4).standard_normal(3 * N)
+    fr = f.reshape(3, nz, ny, nx)
+    u = np.zeros((3, nz, ny, nx))
+    for i in range(3):
+        for j in range(3):
+            gij = g[comps[(min(i, j), max(i, j))]]
+            for z in range(nz):
+                for y in range(ny):
+                    for x in range(nx):
+                        sh = gij[(z - np.arange(nz))[:, None, None] % nz, (y - np.arange(ny))[None, :, None] % ny,
+                                 (x - np.arange(nx))[None, None, :] % nx]
+                        u[i, z, y, x] += np.sum(sh * fr[j])
+    assert rel(o.green_apply(f), u.reshape(-1)) < 1e-13
+
+
+def test_operator_dense_symmetric_psd(oracle_mod):
+    """Dense A (4^3): symmetric, positive semidefinite, constants per component in the null space."""
+    shape = (4, 4, 4)
+    mask = random_mask(shape, 0.6, seed=8)
+    o = oracle_mod.Oracle(shape)
+    nb = o.set_medium(mask)
+    L = 3 * nb
+    A = np.zeros((L, L))
+    for j in range(L):
+        e = np.zeros(L)
+        e[j] = 1.0
+        A[:, j] = o.apply(e)
+    assert np.abs(A - A.T).max() < 1e-13 * np.abs(A).max()
+    w = np.linalg.eigvalsh(0.5 * (A + A.T))
+    assert w.min() > -1e-12 * w.max()
+    for c in range(3):
+        e = np.zeros(L)
+        e[c::3] = 1.0
+        assert np.abs(A @ e).max() < 1e-13 * np.abs(A).max()
+
+
+def test_minres_dense_vs_scipy_and_pinv(oracle_mod):
+    import scipy.sparse.linalg as sla
+
+    rng = np.random.default_rng(11)
+    n = 40
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    d = np.concatenate([rng.uniform(0.1, 10, n - 3), np.zeros(3)])  # semidefinite, like A
+    A = (Q * d) @ Q.T
+    b = A @ rng.standard_normal(n)  # consistent
+    x, info, hist = oracle_mod.minres_dense(A, b, 1e-12, 500)
+    assert info.converged
+    xs, _ = sla.minres(A, b, rtol=1e-13, maxiter=500)
+    xp = np.linalg.pinv(A) @ b
+    assert rel(x, xp) < 1e-9
+    assert rel(x, xs) < 1e-9
+    assert np.linalg.norm(A @ x - b) <= 1e-10 * np.linalg.norm(b)
+    # residual estimates are non-increasing (MINRES minimises ||r|| over growing Krylov spaces)
+    assert np.all(np.diff(hist) <= 1e-12)
+
+
+def test_minres_indefinite_dense(oracle_mod):
+    rng = np.random.default_rng(12)
+    n = 30
+    M = rng.standard_normal((n, n))
+    A = M + M.T
+    b = rng.standard_normal(n)
+    x, info, _ = oracle_mod.minres_dense(A, b, 1e-12, 400)
+    assert rel(x, np.linalg.solve(A, b)) < 1e-8
+
+
+def test_operator_solution_solves_dense_system(oracle_mod):
+    """The MINRES solution of the variational problem satisfies A x = b (4^3, FFT mode)."""
+    shape = (6, 6, 6)
+    mask = random_mask(shape, 0.7, seed=13)
+    o = oracle_mod.Oracle(shape)
+    o.set_medium(mask)
+    b = o.rhs([1.0, 0.0, 0.0])
+    x, info, _ = o.minres(b, 1e-12, 1000)
+    assert info.converged
+    assert o.residual_norm(x, b) <= 1e-10 * np.linalg.norm(b)
