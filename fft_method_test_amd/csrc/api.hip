@@ -171,9 +171,9 @@ struct fftperm_ctx {
     unsigned long long* counts = nullptr;
 
     // reductions
-    double* part_x = nullptr;  // 6 x xinv blocks
-    double* part = nullptr;    // 6 x kRedBlocks
-    double* small = nullptr;   // 16 scalars
+    double* part_x = nullptr;  // kXinvParts x xinv blocks
+    double* part = nullptr;    // 8 x kRedBlocks
+    double* small = nullptr;   // 32 scalars: [0,3) input mean, [4,12) x-inverse sums, [12,15) average, 16 norm
     long xblocks = 0;
 
     // Krylov vectors (L each)
@@ -389,7 +389,7 @@ void device_apply(fftperm_ctx* c, const double* xin, double* yout) {
     sk.compact = yout;
     sk.partials = c->part_x;
     c->chain<R>(sr, sk, nullptr);
-    launch_finish_sum(c->part_x, (int)c->xblocks, 6, inv_nb, c->small + 4, nullptr, c->s);
+    launch_finish_sum(c->part_x, (int)c->xblocks, kXinvParts, inv_nb, c->small + 4, nullptr, c->s);
     launch_sub_mean(yout, c->small + 4, yout, nb, 1.0, nullptr, c->s);
 }
 
@@ -409,7 +409,7 @@ void device_rhs(fftperm_ctx* c, const double grad[3], double* bout) {
     sk.compact = c->tmp;
     sk.partials = c->part_x;
     c->chain<double>(sr, sk, nullptr);
-    launch_finish_sum(c->part_x, (int)c->xblocks, 6, 1.0 / (double)nb, c->small + 4, nullptr, c->s);
+    launch_finish_sum(c->part_x, (int)c->xblocks, kXinvParts, 1.0 / (double)nb, c->small + 4, nullptr, c->s);
     launch_sub_mean(c->tmp, c->small + 4, bout, nb, -1.0, nullptr, c->s);
 }
 
@@ -425,14 +425,16 @@ void minres_iteration(fftperm_ctx* c) {
     XSink sk = sink_none(c);
     sk.compact = c->z;
     sk.partials = c->part_x;
+    sk.dotv = c->v;
     c->chain<R>(sr, sk, &st->done);
     launch_mr_finish_mean_out(c->part_x, (int)c->xblocks, st, inv_nb, c->s);
-    launch_mr_sub_dot(c->z, c->v, st, nb, c->part, c->s);
-    launch_mr_finish_alpha(c->part, st, c->s);
-    launch_mr_axpy_norm(c->z, c->v, c->v_old, st, 3 * nb, c->part, c->s);
+    launch_mr_axpy_norm(c->z, c->v, c->v_old, st, nb, c->part, c->s);
     launch_mr_givens(c->part, st, c->hist, c->s);
-    launch_mr_update(c->x, c->v_old, c->v, c->z, c->w_prev, c->w, st, nb, c->part, c->s);
+    launch_mr_update(c->x, c->v, c->z, c->w_prev, c->w, c->v_old, st, nb, c->part, c->s);
     launch_mr_finish_mean_in(c->part, st, inv_nb, c->s);
+    // v_old <- v, v <- v_new (written over v_old); w_prev <- w, w <- w_next (written over w_prev)
+    std::swap(c->v_old, c->v);
+    std::swap(c->w_prev, c->w);
 }
 
 struct MinresOut {
@@ -504,6 +506,7 @@ MinresOut device_minres(fftperm_ctx* c, const double* bdev, const double* x0dev,
     HIPCHK(hipMemcpyAsync(c->st, &h, sizeof(h), hipMemcpyHostToDevice, c->s));
     launch_compact_sums(c->v, nb, c->part, nullptr, c->s);
     launch_finish_sum(c->part, kRedBlocks, 3, 1.0 / (double)nb, c->st->mean_in, nullptr, c->s);
+    launch_finish_sum(c->part, kRedBlocks, 3, 1.0, c->st->vsum, nullptr, c->s);
     const uint64_t chunk = (c->g.N >= (1L << 21)) ? 4 : 32;
     uint64_t launched = 0;
     int done = h.done;
@@ -610,7 +613,7 @@ void device_recover(fftperm_ctx* c, const double grad[3], const double* xdev, do
     sk.field = field;
     sk.partials = c->part_x;
     c->chain<double>(sr, sk, nullptr);
-    launch_finish_sum(c->part_x, (int)c->xblocks, 6, inv_nb, c->small + 4, nullptr, c->s);
+    launch_finish_sum(c->part_x, (int)c->xblocks, kXinvParts, inv_nb, c->small + 4, nullptr, c->s);
     double mb[3];
     HIPCHK(hipMemcpyAsync(mb, c->small + 4, 3 * sizeof(double), hipMemcpyDeviceToHost, c->s));
     c->sync();
@@ -781,8 +784,8 @@ int fftperm_create(const fftperm_grid* grid, double mu, int p_radius, int precis
         launch_green_sym(c->gt, g.nx, g.ny, g.nz, c->G64, c->G32, c->s);
         HIPCHK(hipGetLastError());
         c->xblocks = xinv_blocks(g, c->t64.px);
-        c->part_x = dalloc<double>(6 * (size_t)c->xblocks);
-        c->part = dalloc<double>(6 * (size_t)kRedBlocks);
+        c->part_x = dalloc<double>((size_t)kXinvParts * c->xblocks);
+        c->part = dalloc<double>(8 * (size_t)kRedBlocks);
         c->small = dalloc<double>(32);
         c->st = dalloc<MinresState>(1);
         c->sync();

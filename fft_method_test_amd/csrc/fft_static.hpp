@@ -1,0 +1,121 @@
+// fft_static.hpp -- compile-time specialised line transforms for the hot grid sizes.
+//
+// Same Stockham algorithm as fft_engine.hpp, with the line length N, the radix sequence,
+// the number of lines per workgroup (LINES) and the workgroup size (THREADS) known at
+// compile time, so index arithmetic folds to shifts and every register array has a
+// static size.  Butterflies are mapped line-fastest (thread t -> line t % LINES), which
+// with an LDS pitch of N + 1 complex values keeps the eight lanes that share an LDS
+// cycle on distinct banks for both the strided reads and the Stockham writes.  The
+// twiddle index is uniform across a group of LINES lanes and is served as an LDS
+// broadcast.
+#pragma once
+
+#include "fft_engine.hpp"
+
+namespace fftperm {
+
+template <int... Rs>
+struct Radices {};
+
+template <int N>
+struct StaticPlan;
+template <> struct StaticPlan<16> { using type = Radices<16>; };
+template <> struct StaticPlan<32> { using type = Radices<8, 4>; };
+template <> struct StaticPlan<64> { using type = Radices<8, 8>; };
+template <> struct StaticPlan<128> { using type = Radices<16, 8>; };
+template <> struct StaticPlan<256> { using type = Radices<16, 16>; };
+template <> struct StaticPlan<512> { using type = Radices<8, 8, 8>; };
+
+template <typename P>
+struct PlanInfo;
+template <int... Rs>
+struct PlanInfo<Radices<Rs...>> {
+    static constexpr int min_radix() {
+        int m = 1 << 30;
+        for (int r : {Rs...}) m = r < m ? r : m;
+        return m;
+    }
+    static constexpr int max_radix() {
+        int m = 0;
+        for (int r : {Rs...}) m = r > m ? r : m;
+        return m;
+    }
+};
+
+// One stage: each thread owns K = (N/RAD)/T butterflies of one line.
+template <typename R, int SIGN, int N, int RAD, int NS, int LINES, int T, int LD>
+__device__ __forceinline__ void sstage(cx<R>* __restrict__ buf, const cx<R>* __restrict__ tw, int nlines) {
+    constexpr int M = N / RAD;
+    static_assert(M % T == 0, "butterflies per line must split evenly over the line's threads");
+    constexpr int K = M / T;
+    constexpr int TS = N / (NS * RAD);
+    const int ci = threadIdx.x % LINES;
+    const int jt = threadIdx.x / LINES;
+    const bool act = ci < nlines;
+    cx<R> v[K][RAD];
+    cx<R>* L = buf + ci * LD;
+    if (act) {
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+            const int j = jt + q * T;
+            const int k = j % NS;
+#pragma unroll
+            for (int r = 0; r < RAD; ++r) v[q][r] = L[j + r * M];
+            if (NS > 1) {
+#pragma unroll
+                for (int r = 1; r < RAD; ++r) {
+                    cx<R> w = tw[r * k * TS];
+                    if (SIGN > 0) w.y = -w.y;
+                    v[q][r] = cmul<R>(v[q][r], w);
+                }
+            }
+            dft_small<R, SIGN, RAD>(v[q]);
+        }
+    }
+    __syncthreads();
+    if (act) {
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+            const int j = jt + q * T;
+            const int k = j % NS;
+            const int base = (j / NS) * NS * RAD + k;
+#pragma unroll
+            for (int r = 0; r < RAD; ++r) L[base + r * NS] = v[q][r];
+        }
+    }
+    __syncthreads();
+}
+
+template <typename R, int SIGN, int N, int LINES, int T, int LD, int NS>
+__device__ __forceinline__ void sstages(cx<R>*, const cx<R>*, int, Radices<>) {}
+
+template <typename R, int SIGN, int N, int LINES, int T, int LD, int NS, int R0, int... Rs>
+__device__ __forceinline__ void sstages(cx<R>* buf, const cx<R>* tw, int nlines, Radices<R0, Rs...>) {
+    sstage<R, SIGN, N, R0, NS, LINES, T, LD>(buf, tw, nlines);
+    sstages<R, SIGN, N, LINES, T, LD, NS * R0>(buf, tw, nlines, Radices<Rs...>{});
+}
+
+// Transform `nlines` (<= LINES) lines of length N held in LDS at pitch LD.
+// T threads per line: THREADS = LINES * T.  Caller synchronised after filling buf / tw.
+template <typename R, int SIGN, int N, int LINES, int T, int LD>
+__device__ __forceinline__ void sfft_lines(cx<R>* buf, const cx<R>* tw, int nlines) {
+    sstages<R, SIGN, N, LINES, T, LD, 1>(buf, tw, nlines, typename StaticPlan<N>::type{});
+}
+
+// Threads per line: the butterfly count of the largest-radix stage, so that every stage's
+// butterflies split evenly over them (radices are powers of two here).
+template <int N>
+constexpr int threads_per_line() {
+    return N / PlanInfo<typename StaticPlan<N>::type>::max_radix();
+}
+
+// Block index remap so that consecutive tiles land on the same XCD (workgroups are dealt to
+// the 8 XCDs round-robin; each XCD has its own L2).
+__device__ __forceinline__ long xcd_tile(long b, long nblocks) {
+    constexpr int kXcd = 8;
+    const long per = nblocks / kXcd;
+    if (per == 0 || b >= per * kXcd) return b;
+    return (b % kXcd) * per + b / kXcd;
+}
+
+}  // namespace fftperm

@@ -24,13 +24,16 @@ struct XSource {
 // Sink of the x-inverse pass.  value = Re(line) * inv_n.
 //   compact  != null : compact[3*l + c] = value on support voxels
 //   field    != null : field[c*N + v] = value
-//   partials != null : per-block sums {support sum x,y,z, all-voxel sum x,y,z}
+//   partials != null : per-block sums (kXinvParts wide): {support sums x,y,z, all-voxel sums x,y,z,
+//                      sum over the support of dotv[3l+c] * value (when dotv != null), 0}
 struct XSink {
     const int32_t* code;
     double* compact;
     double* field;
     double* partials;
+    const double* dotv;
 };
+constexpr int kXinvParts = 8;
 
 struct PassCfg {
     int lines;

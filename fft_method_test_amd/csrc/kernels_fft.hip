@@ -13,6 +13,7 @@
 //               (sink: fused gather to the compact vector and/or field write,
 //               per-block partial sums for the support/domain means)
 #include "fft_engine.hpp"
+#include "fft_static.hpp"
 #include "kernels.hpp"
 
 #include <algorithm>
@@ -63,8 +64,365 @@ PassCfg cfg_zgreen(const LinePlan& p, long ncols) { return make_cfg(p, ncols, 3,
 void set_lds_limit(size_t bytes) { g_lds_max = bytes; }
 size_t lds_limit() { return g_lds_max; }
 
+
+// =====================================================================================
+// Compile-time specialised passes for N in {32, 64, 128, 256, 512}.
+// =====================================================================================
+namespace {
+
+template <typename R, int N>
+struct Shape {
+    static constexpr int T = threads_per_line<N>();                  // threads per line
+    static constexpr int LD = N + 1;                                  // LDS pitch (complex values)
+    static constexpr int LINES = (256 / T) > 0 ? (256 / T) : 1;       // x / column passes
+    static constexpr int THREADS = LINES * T;
+    static constexpr size_t LDS1 = (size_t)(N + LINES * LD) * sizeof(cx<R>);
+    // fused z pass: three components of ZL columns in LDS, at most ~100 KB and 1024 threads
+    static constexpr int zl_raw() {
+        int z = 16;
+        while (z > 1 && ((size_t)(3 * z * LD + N) * sizeof(cx<R>) > 102400u || 3 * z * T > 1024)) z /= 2;
+        return z;
+    }
+    static constexpr int ZL = zl_raw();
+    static constexpr int ZTHREADS = 3 * ZL * T;
+    static constexpr size_t LDSZ = (size_t)(N + 3 * ZL * LD) * sizeof(cx<R>);
+};
+
+template <typename R, int N>
+__global__ __launch_bounds__((Shape<R, N>::THREADS)) void k_xfwd_s(XSource src, cx<R>* __restrict__ S, long nlines,
+                                                                     long N3, long Nh, const cx<R>* __restrict__ twg,
+                                                                     const int* skip) {
+    if (skip != nullptr && *skip) return;
+    using SH = Shape<R, N>;
+    constexpr int LINES = SH::LINES, LD = SH::LD, THREADS = SH::THREADS, NXH = N / 2 + 1;
+    constexpr int KE = LINES * N / THREADS;  // elements per thread per component
+    static_assert((LINES * N) % THREADS == 0, "element split");
+    extern __shared__ unsigned char smem_raw[];
+    cx<R>* tw = reinterpret_cast<cx<R>*>(smem_raw);
+    cx<R>* buf = tw + N;
+    for (int i = threadIdx.x; i < N; i += THREADS) tw[i] = twg[i];
+    const long line0 = xcd_tile(blockIdx.x, gridDim.x) * LINES;
+    const int nl = (int)min((long)LINES, nlines - line0);
+    // every global load of the block is issued up front: the support code once per voxel, then
+    // the three compact components (or the three field components) of each owned voxel
+    double v[3][KE];
+    if (src.field != nullptr) {
+#pragma unroll
+        for (int k = 0; k < KE; ++k) {
+            const int idx = threadIdx.x + k * THREADS;
+            const int line = idx / N;
+            const long vox = (line0 + line) * N + idx % N;
+            const bool ok = line < nl;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) v[c][k] = ok ? src.field[c * N3 + vox] : 0.0;
+        }
+    } else {
+        int code[KE];
+#pragma unroll
+        for (int k = 0; k < KE; ++k) {
+            const int idx = threadIdx.x + k * THREADS;
+            const int line = idx / N;
+            code[k] = (line < nl) ? src.code[(line0 + line) * N + idx % N] : -2;
+        }
+        double mc[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) mc[c] = (src.mean != nullptr) ? src.mean[c] : 0.0;
+#pragma unroll
+        for (int k = 0; k < KE; ++k) {
+            const int cd = code[k];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const double f0 = (cd == -1) ? src.a_fluid[c] : ((cd >= 0) ? src.a_support[c] : 0.0);
+                double fx = 0.0;
+                if (src.compact != nullptr && cd >= 0) fx = src.compact[3L * cd + c] - mc[c];
+                v[c][k] = f0 + fx;
+            }
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+#pragma unroll
+        for (int k = 0; k < KE; ++k) {
+            const int idx = threadIdx.x + k * THREADS;
+            buf[(idx / N) * LD + idx % N] = mkc<R>((R)v[c][k], (R)0);
+        }
+        __syncthreads();
+        sfft_lines<R, -1, N, LINES, SH::T, LD>(buf, tw, nl);
+        cx<R>* Sc = S + c * Nh;
+        for (int idx = threadIdx.x; idx < LINES * NXH; idx += THREADS) {
+            const int line = idx / NXH;
+            const int kx = idx % NXH;
+            if (line < nl) Sc[(line0 + line) * NXH + kx] = buf[line * LD + kx];
+        }
+        __syncthreads();
+    }
+}
+
+template <typename R, int N>
+__global__ __launch_bounds__((Shape<R, N>::THREADS)) void k_xinv_s(const cx<R>* __restrict__ S, XSink sink, long nlines,
+                                                                     long N3, long Nh, const cx<R>* __restrict__ twg,
+                                                                     double inv_n, const int* skip) {
+    if (skip != nullptr && *skip) return;
+    using SH = Shape<R, N>;
+    constexpr int LINES = SH::LINES, LD = SH::LD, THREADS = SH::THREADS, NXH = N / 2 + 1, NM = N - NXH;
+    constexpr int KE = LINES * N / THREADS;
+    constexpr int KH = (LINES * NXH + THREADS - 1) / THREADS;  // half-spectrum loads per thread
+    extern __shared__ unsigned char smem_raw[];
+    cx<R>* tw = reinterpret_cast<cx<R>*>(smem_raw);
+    cx<R>* buf = tw + N;
+    for (int i = threadIdx.x; i < N; i += THREADS) tw[i] = twg[i];
+    const long line0 = xcd_tile(blockIdx.x, gridDim.x) * LINES;
+    const int nl = (int)min((long)LINES, nlines - line0);
+    // support codes of the owned voxels and the first component's half spectrum up front; the next
+    // component's half spectrum is loaded while the current one is transformed
+    int code[KE];
+#pragma unroll
+    for (int k = 0; k < KE; ++k) {
+        const int idx = threadIdx.x + k * THREADS;
+        const int line = idx / N;
+        code[k] = (line < nl) ? sink.code[(line0 + line) * N + idx % N] : -2;
+    }
+    cx<R> h[KH];
+    auto load_half = [&](int c) {
+        const cx<R>* Sc = S + c * Nh;
+#pragma unroll
+        for (int k = 0; k < KH; ++k) {
+            const int idx = threadIdx.x + k * THREADS;
+            const int line = idx / NXH;
+            h[k] = (idx < LINES * NXH && line < nl) ? Sc[(line0 + line) * NXH + idx % NXH] : mkc<R>((R)0, (R)0);
+        }
+    };
+    load_half(0);
+    double out[3][KE];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+#pragma unroll
+        for (int k = 0; k < KH; ++k) {
+            const int idx = threadIdx.x + k * THREADS;
+            if (idx < LINES * NXH) buf[(idx / NXH) * LD + idx % NXH] = h[k];
+        }
+        if (c < 2) load_half(c + 1);
+        __syncthreads();
+        for (int idx = threadIdx.x; idx < LINES * NM; idx += THREADS) {
+            const int line = idx / NM;
+            const int kx = NXH + idx % NM;
+            buf[line * LD + kx] = conjc<R>(buf[line * LD + (N - kx)]);
+        }
+        __syncthreads();
+        sfft_lines<R, +1, N, LINES, SH::T, LD>(buf, tw, nl);
+#pragma unroll
+        for (int k = 0; k < KE; ++k) {
+            const int idx = threadIdx.x + k * THREADS;
+            out[c][k] = (double)buf[(idx / N) * LD + idx % N].x * inv_n;
+        }
+        __syncthreads();
+    }
+    double acc[kXinvParts] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < KE; ++k) {
+        const int idx = threadIdx.x + k * THREADS;
+        const int line = idx / N;
+        if (line >= nl) continue;
+        const long vox = (line0 + line) * N + idx % N;
+        if (sink.field != nullptr) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) sink.field[c * N3 + vox] = out[c][k];
+        }
+        const int cd = code[k];
+        if (cd >= 0) {
+            if (sink.compact != nullptr) {
+#pragma unroll
+                for (int c = 0; c < 3; ++c) sink.compact[3L * cd + c] = out[c][k];
+            }
+            if (sink.dotv != nullptr) {
+#pragma unroll
+                for (int c = 0; c < 3; ++c) acc[6] += sink.dotv[3L * cd + c] * out[c][k];
+            }
+#pragma unroll
+            for (int c = 0; c < 3; ++c) acc[c] += out[c][k];
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) acc[3 + c] += out[c][k];
+    }
+    if (sink.partials != nullptr) block_sum<kXinvParts>(acc, sink.partials + (long)kXinvParts * blockIdx.x);
+}
+
+// Column pass: column col has base (col / inner) * outer_stride + col % inner, element e at
+// base + e * estride.  Thread t always serves column t % LINES in the load/store loops.
+template <typename R, int N, int SIGN>
+__global__ __launch_bounds__((Shape<R, N>::THREADS)) void k_col_s(cx<R>* __restrict__ S, long ncols, long inner,
+                                                                  long outer_stride, long estride, long comp_stride,
+                                                                  const cx<R>* __restrict__ twg, const int* skip) {
+    if (skip != nullptr && *skip) return;
+    using SH = Shape<R, N>;
+    constexpr int LINES = SH::LINES, LD = SH::LD, THREADS = SH::THREADS;
+    constexpr int EST = THREADS / LINES;
+    extern __shared__ unsigned char smem_raw[];
+    cx<R>* tw = reinterpret_cast<cx<R>*>(smem_raw);
+    cx<R>* buf = tw + N;
+    for (int i = threadIdx.x; i < N; i += THREADS) tw[i] = twg[i];
+    const long col0 = xcd_tile(blockIdx.x, gridDim.x) * LINES;
+    const int ncl = (int)min((long)LINES, ncols - col0);
+    cx<R>* Sc = S + blockIdx.y * comp_stride;
+    const int ci = threadIdx.x % LINES;
+    const long col = col0 + ci;
+    const unsigned cu = (unsigned)col, iu = (unsigned)inner;
+    const long base = (long)(cu / iu) * outer_stride + (long)(cu % iu);
+    const bool own = ci < ncl;
+#pragma unroll 4
+    for (int e = threadIdx.x / LINES; e < N; e += EST)
+        if (own) buf[ci * LD + e] = Sc[base + e * estride];
+    __syncthreads();
+    sfft_lines<R, SIGN, N, LINES, SH::T, LD>(buf, tw, ncl);
+#pragma unroll 4
+    for (int e = threadIdx.x / LINES; e < N; e += EST)
+        if (own) Sc[base + e * estride] = buf[ci * LD + e];
+}
+
+// Fused z pass: lines 0..3*ZL-1 of the LDS buffer hold components x | y | z of ZL columns.
+template <typename R, int N>
+__global__ __launch_bounds__((Shape<R, N>::ZTHREADS)) void k_zgreen_s(cx<R>* __restrict__ S, const R* __restrict__ G6,
+                                                                      long ncols, long Nh,
+                                                                      const cx<R>* __restrict__ twg, const int* skip) {
+    if (skip != nullptr && *skip) return;
+    using SH = Shape<R, N>;
+    constexpr int ZL = SH::ZL, LD = SH::LD, THREADS = SH::ZTHREADS, L3 = 3 * ZL;
+    constexpr int EST = THREADS / L3;
+    extern __shared__ unsigned char smem_raw[];
+    cx<R>* tw = reinterpret_cast<cx<R>*>(smem_raw);
+    cx<R>* buf = tw + N;
+    for (int i = threadIdx.x; i < N; i += THREADS) tw[i] = twg[i];
+    const long col0 = xcd_tile(blockIdx.x, gridDim.x) * ZL;
+    const int ncl = (int)min((long)ZL, ncols - col0);
+    // loads: thread t serves line t % L3 = comp * ZL + ci
+    const int lt = threadIdx.x % L3;
+    const int comp = lt / ZL;
+    const int ci = lt % ZL;
+    const bool own = ci < ncl;
+    const cx<R>* Sc = S + comp * Nh + col0 + ci;
+#pragma unroll 4
+    for (int e = threadIdx.x / L3; e < N; e += EST)
+        buf[lt * LD + e] = own ? Sc[e * ncols] : mkc<R>((R)0, (R)0);
+    __syncthreads();
+    sfft_lines<R, -1, N, L3, SH::T, LD>(buf, tw, L3);
+    // symmetric Green multiply (src/bignonnet/GreenOperator.cpp:84-92 with G_sym)
+    {
+        constexpr int MT = THREADS / ZL;
+        const int mci = threadIdx.x % ZL;
+        const bool mown = mci < ncl;
+        const R* Gc = G6 + col0 + mci;
+#pragma unroll 2
+        for (int e = threadIdx.x / ZL; e < N; e += MT) {
+            if (!mown) continue;
+            const long gi = (long)e * ncols;
+            const R txx = Gc[gi], txy = Gc[Nh + gi], txz = Gc[2 * Nh + gi];
+            const R tyy = Gc[3 * Nh + gi], tyz = Gc[4 * Nh + gi], tzz = Gc[5 * Nh + gi];
+            cx<R>* px = buf + (0 * ZL + mci) * LD + e;
+            cx<R>* py = buf + (1 * ZL + mci) * LD + e;
+            cx<R>* pz = buf + (2 * ZL + mci) * LD + e;
+            const cx<R> fx = *px, fy = *py, fz = *pz;
+            *px = mkc<R>(txx * fx.x + txy * fy.x + txz * fz.x, txx * fx.y + txy * fy.y + txz * fz.y);
+            *py = mkc<R>(txy * fx.x + tyy * fy.x + tyz * fz.x, txy * fx.y + tyy * fy.y + tyz * fz.y);
+            *pz = mkc<R>(txz * fx.x + tyz * fy.x + tzz * fz.x, txz * fx.y + tyz * fy.y + tzz * fz.y);
+        }
+    }
+    __syncthreads();
+    sfft_lines<R, +1, N, L3, SH::T, LD>(buf, tw, L3);
+    cx<R>* Sw = S + comp * Nh + col0 + ci;
+#pragma unroll 4
+    for (int e = threadIdx.x / L3; e < N; e += EST)
+        if (own) Sw[e * ncols] = buf[lt * LD + e];
+}
+
+static_assert(Shape<double, 256>::LINES == Shape<float, 256>::LINES, "x-pass shape independent of precision");
+
+template <typename R>
+bool static_xfwd(int n, const XSource& src, cx<R>* S, const Geometry3& g, const FFTTables<R>& t, const int* skip,
+                 hipStream_t st) {
+    const long nlines = (long)g.ny * g.nz;
+#define FFTPERM_XF(NN)                                                                                          \
+    case NN: {                                                                                                  \
+        using SH = Shape<R, NN>;                                                                                \
+        const long blocks = (nlines + SH::LINES - 1) / SH::LINES;                                              \
+        hipLaunchKernelGGL((k_xfwd_s<R, NN>), dim3((unsigned)blocks), dim3(SH::THREADS), SH::LDS1, st, src, S,  \
+                           nlines, g.N, g.Nh, t.twx, skip);                                                     \
+        return true;                                                                                            \
+    }
+    switch (n) { FFTPERM_XF(32) FFTPERM_XF(64) FFTPERM_XF(128) FFTPERM_XF(256) FFTPERM_XF(512) default: return false; }
+#undef FFTPERM_XF
+}
+
+template <typename R>
+bool static_xinv(int n, const cx<R>* S, const XSink& sink, const Geometry3& g, const FFTTables<R>& t,
+                 const int* skip, hipStream_t st) {
+    const long nlines = (long)g.ny * g.nz;
+    const double inv_n = 1.0 / (double)g.N;
+#define FFTPERM_XI(NN)                                                                                          \
+    case NN: {                                                                                                  \
+        using SH = Shape<R, NN>;                                                                                \
+        const long blocks = (nlines + SH::LINES - 1) / SH::LINES;                                              \
+        hipLaunchKernelGGL((k_xinv_s<R, NN>), dim3((unsigned)blocks), dim3(SH::THREADS), SH::LDS1, st, S, sink, \
+                           nlines, g.N, g.Nh, t.twx, inv_n, skip);                                              \
+        return true;                                                                                            \
+    }
+    switch (n) { FFTPERM_XI(32) FFTPERM_XI(64) FFTPERM_XI(128) FFTPERM_XI(256) FFTPERM_XI(512) default: return false; }
+#undef FFTPERM_XI
+}
+
+template <typename R>
+bool static_col(int n, int sign, cx<R>* S, long ncols, long inner, long outer, long estride, long comp_stride,
+                const cx<R>* tw, const int* skip, hipStream_t st) {
+#define FFTPERM_CO(NN)                                                                                          \
+    case NN: {                                                                                                  \
+        using SH = Shape<R, NN>;                                                                                \
+        const dim3 grid((unsigned)((ncols + SH::LINES - 1) / SH::LINES), 3);                                    \
+        if (sign < 0)                                                                                           \
+            hipLaunchKernelGGL((k_col_s<R, NN, -1>), grid, dim3(SH::THREADS), SH::LDS1, st, S, ncols, inner,    \
+                               outer, estride, comp_stride, tw, skip);                                          \
+        else                                                                                                    \
+            hipLaunchKernelGGL((k_col_s<R, NN, +1>), grid, dim3(SH::THREADS), SH::LDS1, st, S, ncols, inner,    \
+                               outer, estride, comp_stride, tw, skip);                                          \
+        return true;                                                                                            \
+    }
+    switch (n) { FFTPERM_CO(32) FFTPERM_CO(64) FFTPERM_CO(128) FFTPERM_CO(256) FFTPERM_CO(512) default: return false; }
+#undef FFTPERM_CO
+}
+
+template <typename R>
+bool static_zgreen(int n, cx<R>* S, const R* G6, long ncols, long Nh, const cx<R>* tw, const int* skip,
+                   hipStream_t st) {
+#define FFTPERM_ZG(NN)                                                                                          \
+    case NN: {                                                                                                  \
+        using SH = Shape<R, NN>;                                                                                \
+        const long blocks = (ncols + SH::ZL - 1) / SH::ZL;                                                      \
+        hipLaunchKernelGGL((k_zgreen_s<R, NN>), dim3((unsigned)blocks), dim3(SH::ZTHREADS), SH::LDSZ, st, S, G6, \
+                           ncols, Nh, tw, skip);                                                                \
+        return true;                                                                                            \
+    }
+    switch (n) { FFTPERM_ZG(32) FFTPERM_ZG(64) FFTPERM_ZG(128) FFTPERM_ZG(256) FFTPERM_ZG(512) default: return false; }
+#undef FFTPERM_ZG
+}
+
+// The specialised kernels need their static LDS budget; fall back to the runtime kernels on
+// devices with less LDS per workgroup.
+bool static_ok() { return g_lds_max >= 102400u + 8192u; }
+
+}  // namespace
+
 long xinv_blocks(const Geometry3& g, const LinePlan& px) {
-    return cfg_xpass(px, (long)g.ny * g.nz).blocks;
+    // exactly the number of blocks (= partial sums) the x-inverse launcher will use
+    const long nlines = (long)g.ny * g.nz;
+    if (static_ok()) {
+        switch (g.nx) {
+            case 32: return (nlines + Shape<double, 32>::LINES - 1) / Shape<double, 32>::LINES;
+            case 64: return (nlines + Shape<double, 64>::LINES - 1) / Shape<double, 64>::LINES;
+            case 128: return (nlines + Shape<double, 128>::LINES - 1) / Shape<double, 128>::LINES;
+            case 256: return (nlines + Shape<double, 256>::LINES - 1) / Shape<double, 256>::LINES;
+            case 512: return (nlines + Shape<double, 512>::LINES - 1) / Shape<double, 512>::LINES;
+            default: break;
+        }
+    }
+    return cfg_xpass(px, nlines).blocks;
 }
 
 // ------------------------------------------------------------------ x forward
@@ -116,6 +474,7 @@ __global__ void k_xfwd(XSource src, cx<R>* __restrict__ S, int nx, long nlines, 
 template <typename R>
 void launch_xfwd(const XSource& src, cx<R>* S, const Geometry3& g, const FFTTables<R>& t, const int* skip,
                  hipStream_t st) {
+    if (static_ok() && static_xfwd<R>(g.nx, src, S, g, t, skip, st)) return;
     const long nlines = (long)g.ny * g.nz;
     PassCfg c = cfg_xpass(t.px, nlines);
     hipLaunchKernelGGL(k_xfwd<R>, dim3((unsigned)c.blocks), dim3(c.threads), c.lds, st, src, S, g.nx, nlines,
@@ -134,7 +493,7 @@ __global__ void k_xinv(const cx<R>* __restrict__ S, XSink sink, int nx, long nli
     for (int i = threadIdx.x; i < nx; i += blockDim.x) tw[i] = twg[i];
     const long line0 = (long)blockIdx.x * lpw;
     const int nl = (int)min((long)lpw, nlines - line0);
-    double acc[6] = {0, 0, 0, 0, 0, 0};
+    double acc[kXinvParts] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (int c = 0; c < 3; ++c) {
         const cx<R>* Sc = S + c * Nh;
         const int cnth = nl * nxh;
@@ -162,20 +521,26 @@ __global__ void k_xinv(const cx<R>* __restrict__ S, XSink sink, int nx, long nli
             const int code = sink.code[vox];
             if (code >= 0) {
                 if (sink.compact != nullptr) sink.compact[3L * code + c] = val;
-                acc[c] += val;
+                if (sink.dotv != nullptr) acc[6] += sink.dotv[3L * code + c] * val;
+                if (c == 0) acc[0] += val;
+                if (c == 1) acc[1] += val;
+                if (c == 2) acc[2] += val;
             }
-            acc[3 + c] += val;
+            if (c == 0) acc[3] += val;
+            if (c == 1) acc[4] += val;
+            if (c == 2) acc[5] += val;
         }
         __syncthreads();
     }
     if (sink.partials != nullptr) {
-        block_sum<6>(acc, sink.partials + 6L * blockIdx.x);
+        block_sum<kXinvParts>(acc, sink.partials + (long)kXinvParts * blockIdx.x);
     }
 }
 
 template <typename R>
 void launch_xinv(const cx<R>* S, const XSink& sink, const Geometry3& g, const FFTTables<R>& t, const int* skip,
                  hipStream_t st) {
+    if (static_ok() && static_xinv<R>(g.nx, S, sink, g, t, skip, st)) return;
     const long nlines = (long)g.ny * g.nz;
     PassCfg c = cfg_xpass(t.px, nlines);
     const double inv_n = 1.0 / (double)g.N;
@@ -222,6 +587,9 @@ template <typename R>
 void launch_ypass(cx<R>* S, int sign, const Geometry3& g, const FFTTables<R>& t, const int* skip,
                   hipStream_t st) {
     const long ncols = (long)g.nz * g.nxh;
+    if (static_ok() && static_col<R>(g.ny, sign, S, ncols, (long)g.nxh, (long)g.ny * g.nxh, (long)g.nxh, g.Nh, t.twy,
+                                     skip, st))
+        return;
     PassCfg c = cfg_colpass(t.py, ncols);
     dim3 grid((unsigned)c.blocks, 3);
     if (sign < 0)
@@ -236,6 +604,7 @@ template <typename R>
 void launch_zpass(cx<R>* S, int sign, const Geometry3& g, const FFTTables<R>& t, const int* skip,
                   hipStream_t st) {
     const long ncols = (long)g.ny * g.nxh;
+    if (static_ok() && static_col<R>(g.nz, sign, S, ncols, ncols, 0L, ncols, g.Nh, t.twz, skip, st)) return;
     PassCfg c = cfg_colpass(t.pz, ncols);
     dim3 grid((unsigned)c.blocks, 3);
     if (sign < 0)
@@ -303,6 +672,7 @@ template <typename R>
 void launch_zgreen(cx<R>* S, const R* G6, const Geometry3& g, const FFTTables<R>& t, const int* skip,
                    hipStream_t st) {
     const long ncols = (long)g.ny * g.nxh;
+    if (static_ok() && static_zgreen<R>(g.nz, S, G6, ncols, g.Nh, t.twz, skip, st)) return;
     PassCfg c = cfg_zgreen(t.pz, ncols);
     hipLaunchKernelGGL(k_zgreen<R>, dim3((unsigned)c.blocks), dim3(c.threads), c.lds, st, S, G6, g.nz, ncols,
                        g.Nh, t.pz, t.twz, c.lines, skip);

@@ -23,12 +23,14 @@ inline unsigned red_grid(long n) {
 __global__ void k_finish_sum(const double* __restrict__ partials, int nparts, int nv, double scale,
                              double* __restrict__ out, const int* skip) {
     if (skip != nullptr && *skip) return;
-    double acc[6] = {0, 0, 0, 0, 0, 0};
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (int p = threadIdx.x; p < nparts; p += blockDim.x) {
-        for (int j = 0; j < nv; ++j) acc[j] += partials[(long)p * nv + j];
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            if (j < nv) acc[j] += partials[(long)p * nv + j];
     }
-    __shared__ double res[6];
-    block_sum<6>(acc, res);
+    __shared__ double res[8];
+    block_sum<8>(acc, res);
     if (threadIdx.x < (unsigned)nv) out[threadIdx.x] = res[threadIdx.x] * scale;
 }
 
@@ -162,15 +164,24 @@ void launch_scale_copy(const double* x, double scale_div, double* out, long n, h
 __global__ void k_mr_finish_mean_out(const double* __restrict__ partials, int nparts, MinresState* st,
                                      double inv_nb) {
     if (st->done) return;
-    double acc[3] = {0, 0, 0};
+    double acc[4] = {0, 0, 0, 0};
     for (int p = threadIdx.x; p < nparts; p += blockDim.x) {
-        acc[0] += partials[6L * p];
-        acc[1] += partials[6L * p + 1];
-        acc[2] += partials[6L * p + 2];
+        const double* q = partials + (long)kXinvParts * p;
+        acc[0] += q[0];
+        acc[1] += q[1];
+        acc[2] += q[2];
+        acc[3] += q[6];
     }
-    __shared__ double res[3];
-    block_sum<3>(acc, res);
-    if (threadIdx.x < 3) st->mean_out[threadIdx.x] = res[threadIdx.x] * inv_nb;
+    __shared__ double res[4];
+    block_sum<4>(acc, res);
+    if (threadIdx.x == 0) {
+        const double m0 = res[0] * inv_nb, m1 = res[1] * inv_nb, m2 = res[2] * inv_nb;
+        st->mean_out[0] = m0;
+        st->mean_out[1] = m1;
+        st->mean_out[2] = m2;
+        // alpha = <v, z_raw - m> = <v, z_raw> - sum_c m_c (sum of component c of v)
+        st->alpha = res[3] - (m0 * st->vsum[0] + m1 * st->vsum[1] + m2 * st->vsum[2]);
+    }
 }
 
 void launch_mr_finish_mean_out(const double* partials, int nparts, MinresState* st, double inv_nb,
@@ -213,22 +224,29 @@ void launch_mr_finish_alpha(const double* partials, MinresState* st, hipStream_t
 }
 
 __global__ void k_mr_axpy_norm(double* __restrict__ z, const double* __restrict__ v,
-                               const double* __restrict__ v_old, MinresState* st, long n,
+                               const double* __restrict__ v_old, MinresState* st, long nb,
                                double* __restrict__ partials) {
     if (st->done) return;
     const double a = st->alpha, b = st->beta;
+    const double m0 = st->mean_out[0], m1 = st->mean_out[1], m2 = st->mean_out[2];
     double acc[1] = {0.0};
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-        const double zi = z[i] - (a * v[i] + b * v_old[i]);
-        z[i] = zi;
-        acc[0] += zi * zi;
+    for (long l = (long)blockIdx.x * blockDim.x + threadIdx.x; l < nb; l += (long)gridDim.x * blockDim.x) {
+        const long i = 3 * l;
+        // z = P_B(z_raw) (Algorithm 1 step 17), then z -= alpha v + beta v_old
+        const double z0 = (z[i] - m0) - (a * v[i] + b * v_old[i]);
+        const double z1 = (z[i + 1] - m1) - (a * v[i + 1] + b * v_old[i + 1]);
+        const double z2 = (z[i + 2] - m2) - (a * v[i + 2] + b * v_old[i + 2]);
+        z[i] = z0;
+        z[i + 1] = z1;
+        z[i + 2] = z2;
+        acc[0] += z0 * z0 + z1 * z1 + z2 * z2;
     }
     block_sum<1>(acc, partials + blockIdx.x);
 }
 
-void launch_mr_axpy_norm(double* z, const double* v, const double* v_old, MinresState* st, long n,
+void launch_mr_axpy_norm(double* z, const double* v, const double* v_old, MinresState* st, long nb,
                          double* partials, hipStream_t s) {
-    hipLaunchKernelGGL(k_mr_axpy_norm, dim3(kRedBlocks), dim3(kRedThreads), 0, s, z, v, v_old, st, n, partials);
+    hipLaunchKernelGGL(k_mr_axpy_norm, dim3(kRedBlocks), dim3(kRedThreads), 0, s, z, v, v_old, st, nb, partials);
 }
 
 // Steps 4-12 of the reference iteration (src/solver/MinresSolver.cpp:135-202) on scalars.
@@ -283,8 +301,8 @@ void launch_mr_givens(const double* partials, MinresState* st, double* history, 
     hipLaunchKernelGGL(k_mr_givens, dim3(1), dim3(kRedThreads), 0, s, partials, st, history);
 }
 
-__global__ void k_mr_update(double* __restrict__ x, double* __restrict__ v_old, double* __restrict__ v,
-                            const double* __restrict__ z, double* __restrict__ w_prev, double* __restrict__ w,
+__global__ void k_mr_update(double* __restrict__ x, const double* __restrict__ v, const double* __restrict__ z,
+                            double* __restrict__ w_prev_io, const double* __restrict__ w, double* __restrict__ v_new,
                             MinresState* st, long nb, double* __restrict__ partials) {
     if (st->done) return;
     const double beta_new = st->beta_new;
@@ -295,25 +313,23 @@ __global__ void k_mr_update(double* __restrict__ x, double* __restrict__ v_old, 
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const long i = 3 * l + c;
-            const double vo = v[i];
+            // v_old <- v (pointer swap on the host); v <- z / beta_new (into the old v_old buffer)
             const double vn = vz ? 0.0 : z[i] / beta_new;
-            const double wp = w_prev[i];
-            const double wc = w[i];
-            const double wn = wz ? 0.0 : (vo - eps * wp - delta * wc) / rho;
+            // w_next = (v_old - eps w_prev - delta w) / rho with v_old = the current v;
+            // written over w_prev (pointer swap of w_prev and w on the host)
+            const double wn = wz ? 0.0 : (v[i] - eps * w_prev_io[i] - delta * w[i]) / rho;
             x[i] += xc * wn;
-            v_old[i] = vo;
-            v[i] = vn;
-            w_prev[i] = wc;
-            w[i] = wn;
+            v_new[i] = vn;
+            w_prev_io[i] = wn;
             acc[c] += vn;
         }
     }
     block_sum<3>(acc, partials + 3L * blockIdx.x);
 }
 
-void launch_mr_update(double* x, double* v_old, double* v, const double* z, double* w_prev, double* w,
-                      MinresState* st, long nb, double* partials, hipStream_t s) {
-    hipLaunchKernelGGL(k_mr_update, dim3(kRedBlocks), dim3(kRedThreads), 0, s, x, v_old, v, z, w_prev, w, st, nb,
+void launch_mr_update(double* x, const double* v, const double* z, double* w_prev_io, const double* w,
+                      double* v_new, MinresState* st, long nb, double* partials, hipStream_t s) {
+    hipLaunchKernelGGL(k_mr_update, dim3(kRedBlocks), dim3(kRedThreads), 0, s, x, v, z, w_prev_io, w, v_new, st, nb,
                        partials);
 }
 
@@ -328,6 +344,9 @@ __global__ void k_mr_finish_mean_in(const double* __restrict__ partials, MinresS
     __shared__ double res[3];
     block_sum<3>(acc, res);
     if (threadIdx.x == 0) {
+        st->vsum[0] = res[0];
+        st->vsum[1] = res[1];
+        st->vsum[2] = res[2];
         st->mean_in[0] = res[0] * inv_nb;
         st->mean_in[1] = res[1] * inv_nb;
         st->mean_in[2] = res[2] * inv_nb;

@@ -21,6 +21,7 @@ struct MinresState {
     int v_zero, w_zero;
     double rel_res;
     double mean_in[3];   // support mean of v (input projection of the next apply)
+    double vsum[3];      // per-component sums of v
     double mean_out[3];  // support mean of the gathered apply output
     long long iterations;
     long long maxit;
@@ -33,18 +34,19 @@ struct MinresState {
 // z = z - mean_out ; partials: sum v.z                     (Algorithm 1 step 17 + alpha)
 void launch_mr_sub_dot(double* z, const double* v, MinresState* st, long nb, double* partials, hipStream_t s);
 void launch_mr_finish_alpha(const double* partials, MinresState* st, hipStream_t s);
-// z -= alpha v + beta v_old ; partials: sum z^2
-void launch_mr_axpy_norm(double* z, const double* v, const double* v_old, MinresState* st, long n,
+// z = (z_raw - mean_out) - (alpha v + beta v_old) ; partials: sum z^2
+void launch_mr_axpy_norm(double* z, const double* v, const double* v_old, MinresState* st, long nb,
                          double* partials, hipStream_t s);
 // beta_new, Givens rotation, phi_bar, relative residual estimate, stop flags, history
 void launch_mr_givens(const double* partials, MinresState* st, double* history, hipStream_t s);
-// v_old <- v, v <- z / beta_new, w_next, x += (cs_new phi_bar) w_next, w_prev <- w, w <- w_next ;
-// partials: per-component sums of the new v
-void launch_mr_update(double* x, double* v_old, double* v, const double* z, double* w_prev, double* w,
-                      MinresState* st, long nb, double* partials, hipStream_t s);
-// mean_in = sums / nb ; done = pending_done
+// v_new = z / beta_new (into the buffer that held v_old), w_next = (v - eps w_prev - delta w) / rho
+// (over w_prev), x += (cs_new phi_bar) w_next; the host then swaps (v_old, v) and (w_prev, w).
+// partials: per-component sums of v_new
+void launch_mr_update(double* x, const double* v, const double* z, double* w_prev_io, const double* w,
+                      double* v_new, MinresState* st, long nb, double* partials, hipStream_t s);
+// vsum = sums, mean_in = sums / nb ; done = pending_done
 void launch_mr_finish_mean_in(const double* partials, MinresState* st, double inv_nb, hipStream_t s);
-// mean_out = support sums of the x-inverse partials / nb
+// mean_out = support sums of the x-inverse partials / nb ; alpha = <v, z_raw> - sum_c mean_out_c vsum_c
 void launch_mr_finish_mean_out(const double* partials, int nparts, MinresState* st, double inv_nb,
                                hipStream_t s);
 

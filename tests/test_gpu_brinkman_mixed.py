@@ -106,3 +106,18 @@ def test_mixed_precision_true_residual(fp):
     assert res.converged
     # the FP64 true residual of the refined solution meets the tolerance
     assert P.Diagnostics.residual_norm(op, res.solution, b) <= 1e-10 * np.linalg.norm(b) * 1.01
+
+
+def test_mixed_precision_k_tensor_64(fp, oracle_mod):
+    """Mixed path on a 64^3 Boolean medium (specialised FP32 kernels): K within 1e-6 of the FP64 oracle."""
+    from fft_method_test_amd import geometry as G, permeability as P
+
+    mask = G.boolean_spheres(64, 4.0, 0.6, 17).reshape(-1)
+    g = P.Grid3D(64, 64, 64)
+    green = P.GreenOperator(g, 1.0, 2, P.PRECISION_MIXED)
+    P.ForceSupport(P.BinaryMedium(g, mask), P.SUPPORT_INTERFACE, green)
+    K, infos = P.solve_permeability_tensor(green, 1e-10, 20000)
+    o = oracle_mod.Oracle((64, 64, 64))
+    o.set_medium(mask)
+    Ko, _ = o.solve_tensor(1e-10, 20000)
+    assert rel(K, Ko) < 1e-6

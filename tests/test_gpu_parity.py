@@ -177,3 +177,48 @@ def test_symmetry_gap_fp64(fp, oracle_mod):
     gap = P.Diagnostics.relative_symmetry_gap(op, x, y)
     assert gap < 1e-13
     assert abs(gap - o.symmetry_gap(x, y)) < 1e-13
+
+
+# every compile-time specialised line length on every axis (x R2C/C2R, y columns, fused z)
+STATIC_SHAPES = [(512, 4, 6), (6, 512, 4), (4, 6, 512), (256, 10, 6), (10, 256, 6), (6, 10, 256),
+                 (128, 32, 64), (64, 128, 32), (32, 64, 128), (64, 64, 64)]
+
+
+@pytest.mark.parametrize("shape", STATIC_SHAPES)
+def test_green_apply_static_sizes(fp, oracle_mod, shape):
+    mask = random_mask(shape, 0.6, seed=21)
+    green, sup, o = make_pair(fp, oracle_mod, shape, mask)
+    N = int(np.prod(shape))
+    f = np.random.default_rng(2).standard_normal(3 * N)
+    assert rel(green.apply(f), o.green_apply(f)) < 1e-13
+    x = np.random.default_rng(3).standard_normal(sup.size())
+    from fft_method_test_amd import permeability as P
+
+    assert rel(P.VariationalOperator(sup, green).apply(x), o.apply(x)) < 1e-12
+
+
+@pytest.mark.parametrize("shape", [(128, 128, 128), (256, 256, 256)])
+def test_operator_symmetry_large(fp, shape):
+    """Size-independent property at full size: <x, A y> = <A x, y> for the FP64 device operator."""
+    from fft_method_test_amd import geometry as G, permeability as P
+
+    mask = G.boolean_spheres(shape[0], 8.0 * shape[0] / 256, 0.6, 2001).reshape(-1)
+    g = P.Grid3D(*shape)
+    green = P.GreenOperator(g)
+    sup = P.ForceSupport(P.BinaryMedium(g, mask), P.SUPPORT_INTERFACE, green)
+    rng = np.random.default_rng(42)
+    x = rng.uniform(-1, 1, sup.size())
+    y = rng.uniform(-1, 1, sup.size())
+    assert P.Diagnostics.relative_symmetry_gap(P.VariationalOperator(sup, green), x, y) < 1e-12
+
+
+def test_green_apply_128_cube_vs_oracle(fp, oracle_mod):
+    shape = (128, 128, 128)
+    mask = random_mask(shape, 0.6, seed=5)
+    green, sup, o = make_pair(fp, oracle_mod, shape, mask)
+    f = np.random.default_rng(4).standard_normal(3 * 128 ** 3)
+    assert rel(green.apply(f), o.green_apply(f)) < 1e-13
+    b = o.rhs([0.0, 0.0, 1.0])
+    from fft_method_test_amd import permeability as P
+
+    assert rel(P.VariationalRHS(sup, green, None).build([0.0, 0.0, 1.0]), b) < 1e-12
