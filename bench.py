@@ -88,7 +88,7 @@ def run_precision(args, P, _lib, mask_dev, precision):
         sweeps=[int(i.refinement_sweeps) for i in infos], converged=[bool(i.converged) for i in infos],
         final_rel=[float(i.final_rel_residual) for i in infos], setup_s=setup_s, nb=int(nb.value),
         prof=dict(z_ms=list(prof.zgreen_ms), z_n=list(prof.zgreen_launches), a_ms=list(prof.apply_ms),
-                  a_n=list(prof.apply_launches), N=int(prof.N), Nh=int(prof.Nh)),
+                  a_n=list(prof.apply_launches), N=int(prof.N), Nh=int(prof.Nh), tile=list(prof.zgreen_tile)),
     )
     green.close()
     return res
@@ -217,6 +217,7 @@ def main():
             "algorithmic_bytes_per_launch": z_bytes,
             "avg_launch_ms": z_ms,
             "launches": pr["z_n"][pi],
+            "autotuned_variant_10xcols_plus_oversub": pr["tile"][pi],
         },
         "apply": {
             "avg_ms": a_ms,

@@ -45,7 +45,7 @@ class SolveInfo(C.Structure):
 class ProfileStats(C.Structure):
     _fields_ = [("zgreen_ms", C.c_double * 2), ("zgreen_launches", C.c_uint64 * 2),
                 ("apply_ms", C.c_double * 2), ("apply_launches", C.c_uint64 * 2),
-                ("N", C.c_uint64), ("Nh", C.c_uint64), ("nb", C.c_uint64)]
+                ("N", C.c_uint64), ("Nh", C.c_uint64), ("nb", C.c_uint64), ("zgreen_tile", C.c_int * 2)]
 
 
 class BrinkmanInfo(C.Structure):

@@ -159,6 +159,7 @@ struct fftperm_ctx {
     double* G64 = nullptr;
     float* G32 = nullptr;
     double* gtab = nullptr;  // per-axis Green tables (6 arrays)
+    std::vector<std::pair<int, float>> tune64, tune32;  // z-pass autotune timings (width, ms)
     GreenAxisTables gt{};
 
     // medium
@@ -783,6 +784,10 @@ int fftperm_create(const fftperm_grid* grid, double mu, int p_radius, int precis
         HIPCHK(hipMemcpy(c->gtab, host.data(), tot * sizeof(double), hipMemcpyHostToDevice));
         launch_green_sym(c->gt, g.nx, g.ny, g.nz, c->G64, c->G32, c->s);
         HIPCHK(hipGetLastError());
+        // autotune the fused z pass tile width on this grid (cached per process)
+        c->tune64 = autotune_zgreen<double>(c->S64, c->G64, g, c->t64, c->s);
+        if (precision == FFTPERM_PRECISION_MIXED) c->tune32 = autotune_zgreen<float>(c->S32, c->G32, g, c->t32, c->s);
+        HIPCHK(hipGetLastError());
         c->xblocks = xinv_blocks(g, c->t64.px);
         c->part_x = dalloc<double>((size_t)kXinvParts * c->xblocks);
         c->part = dalloc<double>(8 * (size_t)kRedBlocks);
@@ -1122,6 +1127,7 @@ int fftperm_profile_read(fftperm_ctx* ctx, fftperm_profile_stats* out) {
         out->N = (uint64_t)ctx->g.N;
         out->Nh = (uint64_t)ctx->g.Nh;
         out->nb = ctx->nb;
+        for (int i = 0; i < 2; ++i) out->zgreen_tile[i] = zgreen_tile(i, ctx->g.nz, (long)ctx->g.ny * ctx->g.nxh);
     });
 }
 

@@ -51,7 +51,7 @@ __device__ __forceinline__ void sstage(cx<R>* __restrict__ buf, const cx<R>* __r
     constexpr int TS = N / (NS * RAD);
     const int ci = threadIdx.x % LINES;
     const int jt = threadIdx.x / LINES;
-    const bool act = ci < nlines;
+    const bool act = ci < nlines && jt < T;  // workgroups may carry extra threads for the memory phases
     cx<R> v[K][RAD];
     cx<R>* L = buf + ci * LD;
     if (act) {
@@ -96,7 +96,8 @@ __device__ __forceinline__ void sstages(cx<R>* buf, const cx<R>* tw, int nlines,
 }
 
 // Transform `nlines` (<= LINES) lines of length N held in LDS at pitch LD.
-// T threads per line: THREADS = LINES * T.  Caller synchronised after filling buf / tw.
+// T threads per line do the butterflies (threads LINES*T and above idle through the stages).
+// Caller synchronised after filling buf / tw.
 template <typename R, int SIGN, int N, int LINES, int T, int LD>
 __device__ __forceinline__ void sfft_lines(cx<R>* buf, const cx<R>* tw, int nlines) {
     sstages<R, SIGN, N, LINES, T, LD, 1>(buf, tw, nlines, typename StaticPlan<N>::type{});

@@ -3,6 +3,9 @@
 
 #include "common.hpp"
 
+#include <utility>
+#include <vector>
+
 namespace fftperm {
 
 // ---------------------------------------------------------------- FFT passes
@@ -76,11 +79,20 @@ template <typename R>
 void launch_zpass(cx<R>* S, int sign, const Geometry3& g, const FFTTables<R>& t, const int* skip,
                   hipStream_t st);
 // Fused z forward transform, symmetric Green multiply, z backward transform.
-// G6 = 6 SoA arrays of Nh (xx,xy,xz,yy,yz,zz) on the half spectrum.
+// G6 = the symmetrised tensor on the half spectrum, AoS: G6[6*i + g], g = xx,xy,xz,yy,yz,zz.
 template <typename R>
 void launch_zgreen(cx<R>* S, const R* G6, const Geometry3& g, const FFTTables<R>& t, const int* skip,
                    hipStream_t st);
 long xinv_blocks(const Geometry3& g, const LinePlan& px);
+// Times every tile width of the fused z pass that fits the device on the given buffers and
+// caches the fastest for (precision, nz, columns); returns (width, ms) pairs (empty when the
+// size has no specialised kernel or was tuned before).  Overwrites S.
+// Autotuned z-pass variant (10 * columns per workgroup + threads per butterfly) for (precision
+// 0 = FP64 / 1 = FP32, nz, columns), 0 if untuned.
+int zgreen_tile(int prec, int n, long ncols);
+template <typename R>
+std::vector<std::pair<int, float>> autotune_zgreen(cx<R>* S, const R* G6, const Geometry3& g,
+                                                   const FFTTables<R>& t, hipStream_t st);
 
 // ------------------------------------------------------------ setup kernels
 // Support classification + ascending compaction.  Fills code (N) and active (NB);
@@ -101,7 +113,8 @@ struct GreenAxisTables {
     int P;
     double mu;
 };
-// Half-spectrum symmetrised tensor 0.5 (G(k) + G(-k)) in double (and float if G6f != null).
+// Half-spectrum symmetrised tensor 0.5 (G(k) + G(-k)), AoS G6[6*i + g], in double (and float if
+// G6f != null).
 void launch_green_sym(const GreenAxisTables& t, int nx, int ny, int nz, double* G6, float* G6f,
                       hipStream_t st);
 // Reference-layout tensor over the full unshifted spectrum (test hook).
@@ -117,7 +130,7 @@ void launch_hermitian_half(const double2* X, cx<R>* S, const Geometry3& g, hipSt
 void launch_spectral_green_full(const double* T6, const double2* F, double2* V, long N, hipStream_t st);
 
 // ------------------------------------------------------------ reductions
-constexpr int kRedBlocks = 1024;
+constexpr int kRedBlocks = 4096;
 constexpr int kRedThreads = 256;
 // Sum of nparts consecutive NV-vectors in partials -> out[NV] (single block, fixed order).
 void launch_finish_sum(const double* partials, int nparts, int nv, double scale, double* out,

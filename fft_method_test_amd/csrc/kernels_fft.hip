@@ -17,6 +17,10 @@
 #include "kernels.hpp"
 
 #include <algorithm>
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
 
 namespace fftperm {
 
@@ -75,7 +79,9 @@ struct Shape {
     static constexpr int T = threads_per_line<N>();                  // threads per line
     static constexpr int LD = N + 1;                                  // LDS pitch (complex values)
     static constexpr int LINES = (256 / T) > 0 ? (256 / T) : 1;       // x / column passes
-    static constexpr int THREADS = LINES * T;
+    // twice the butterfly threads: the extra waves keep loads and stores in flight while other
+    // workgroups of the CU are in their LDS stages
+    static constexpr int THREADS = 2 * LINES * T <= 1024 ? 2 * LINES * T : LINES * T;
     static constexpr size_t LDS1 = (size_t)(N + LINES * LD) * sizeof(cx<R>);
     // fused z pass: three components of ZL columns in LDS, at most ~100 KB and 1024 threads
     static constexpr int zl_raw() {
@@ -88,6 +94,10 @@ struct Shape {
     static constexpr size_t LDSZ = (size_t)(N + 3 * ZL * LD) * sizeof(cx<R>);
 };
 
+// Real x-lines are transformed two at a time as one complex line z = a + i b
+// ("two for one"): round A packs components x and y of each line, round B packs component z of
+// lines 2m and 2m+1.  The half spectra follow from A[k] = (Z[k] + conj Z[N-k]) / 2 and
+// B[k] = (Z[k] - conj Z[N-k]) / 2i, which halves the FFT and LDS work of the R2C pass.
 template <typename R, int N>
 __global__ __launch_bounds__((Shape<R, N>::THREADS)) void k_xfwd_s(XSource src, cx<R>* __restrict__ S, long nlines,
                                                                      long N3, long Nh, const cx<R>* __restrict__ twg,
@@ -96,7 +106,7 @@ __global__ __launch_bounds__((Shape<R, N>::THREADS)) void k_xfwd_s(XSource src, 
     using SH = Shape<R, N>;
     constexpr int LINES = SH::LINES, LD = SH::LD, THREADS = SH::THREADS, NXH = N / 2 + 1;
     constexpr int KE = LINES * N / THREADS;  // elements per thread per component
-    static_assert((LINES * N) % THREADS == 0, "element split");
+    static_assert((LINES * N) % THREADS == 0 && LINES % 2 == 0, "element split");
     extern __shared__ unsigned char smem_raw[];
     cx<R>* tw = reinterpret_cast<cx<R>*>(smem_raw);
     cx<R>* buf = tw + N;
@@ -139,42 +149,85 @@ __global__ __launch_bounds__((Shape<R, N>::THREADS)) void k_xfwd_s(XSource src, 
             }
         }
     }
+    // round A: z = x + i y per line
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
+    for (int k = 0; k < KE; ++k) {
+        const int idx = threadIdx.x + k * THREADS;
+        buf[(idx / N) * LD + idx % N] = mkc<R>((R)v[0][k], (R)v[1][k]);
+    }
+    __syncthreads();
+    sfft_lines<R, -1, N, LINES, SH::T, LD>(buf, tw, nl);
+    for (int idx = threadIdx.x; idx < LINES * NXH; idx += THREADS) {
+        const int line = idx / NXH;
+        const int kx = idx % NXH;
+        if (line < nl) {
+            const cx<R> zk = buf[line * LD + kx];
+            const cx<R> zm = buf[line * LD + (N - kx) % N];
+            const long o = (line0 + line) * NXH + kx;
+            S[o] = mkc<R>((R)0.5 * (zk.x + zm.x), (R)0.5 * (zk.y - zm.y));
+            S[Nh + o] = mkc<R>((R)0.5 * (zk.y + zm.y), (R)0.5 * (zm.x - zk.x));
+        }
+    }
+    __syncthreads();
+    // round B: z = z_{2m} + i z_{2m+1} (component z of two adjacent lines)
+    R* bufr = reinterpret_cast<R*>(buf);
 #pragma unroll
-        for (int k = 0; k < KE; ++k) {
-            const int idx = threadIdx.x + k * THREADS;
-            buf[(idx / N) * LD + idx % N] = mkc<R>((R)v[c][k], (R)0);
-        }
-        __syncthreads();
-        sfft_lines<R, -1, N, LINES, SH::T, LD>(buf, tw, nl);
-        cx<R>* Sc = S + c * Nh;
-        for (int idx = threadIdx.x; idx < LINES * NXH; idx += THREADS) {
-            const int line = idx / NXH;
-            const int kx = idx % NXH;
-            if (line < nl) Sc[(line0 + line) * NXH + kx] = buf[line * LD + kx];
-        }
-        __syncthreads();
+    for (int k = 0; k < KE; ++k) {
+        const int idx = threadIdx.x + k * THREADS;
+        const int line = idx / N;
+        bufr[2 * ((line >> 1) * LD + idx % N) + (line & 1)] = (R)v[2][k];
+    }
+    __syncthreads();
+    const int nl2 = (nl + 1) / 2;
+    sfft_lines<R, -1, N, LINES, SH::T, LD>(buf, tw, nl2);
+    cx<R>* Sz = S + 2 * Nh;
+    for (int idx = threadIdx.x; idx < (LINES / 2) * NXH; idx += THREADS) {
+        const int m = idx / NXH;
+        const int kx = idx % NXH;
+        const cx<R> zk = buf[m * LD + kx];
+        const cx<R> zm = buf[m * LD + (N - kx) % N];
+        const int la = 2 * m, lb = 2 * m + 1;
+        if (la < nl) Sz[(line0 + la) * NXH + kx] = mkc<R>((R)0.5 * (zk.x + zm.x), (R)0.5 * (zk.y - zm.y));
+        if (lb < nl) Sz[(line0 + lb) * NXH + kx] = mkc<R>((R)0.5 * (zk.y + zm.y), (R)0.5 * (zm.x - zk.x));
     }
 }
 
+// Z[k] = A + iB and Z[N-k] = conj(A) + i conj(B) for 0 < k < N - k; the self-conjugate bins keep
+// their real parts only.
+template <typename R, int N, int LD>
+__device__ __forceinline__ void xinv_put(cx<R>* buf, int line, int kx, cx<R> a, cx<R> b) {
+    const bool self = (kx == 0 || 2 * kx == N);
+    if (self) {
+        a.y = (R)0;
+        b.y = (R)0;
+    }
+    buf[line * LD + kx] = mkc<R>(a.x - b.y, a.y + b.x);
+    if (!self) buf[line * LD + (N - kx)] = mkc<R>(a.x + b.y, b.x - a.y);
+}
+
+// C2R of two half spectra at once: Z[k] = A[k] + i B[k] over the Hermitian completions of A and
+// B, then a = Re ifft(Z), b = Im ifft(Z).  The imaginary parts of the self-conjugate bins (0 and
+// N/2) are dropped, which is what taking the real part of a full C2C inverse does
+// (src/fft/FFTWBackend.cpp:292).
 template <typename R, int N>
 __global__ __launch_bounds__((Shape<R, N>::THREADS)) void k_xinv_s(const cx<R>* __restrict__ S, XSink sink, long nlines,
                                                                      long N3, long Nh, const cx<R>* __restrict__ twg,
                                                                      double inv_n, const int* skip) {
     if (skip != nullptr && *skip) return;
     using SH = Shape<R, N>;
-    constexpr int LINES = SH::LINES, LD = SH::LD, THREADS = SH::THREADS, NXH = N / 2 + 1, NM = N - NXH;
+    constexpr int LINES = SH::LINES, LD = SH::LD, THREADS = SH::THREADS, NXH = N / 2 + 1;
     constexpr int KE = LINES * N / THREADS;
-    constexpr int KH = (LINES * NXH + THREADS - 1) / THREADS;  // half-spectrum loads per thread
+    constexpr int KA = (LINES * NXH + THREADS - 1) / THREADS;        // round A loads per thread
+    constexpr int KB = ((LINES / 2) * NXH + THREADS - 1) / THREADS;  // round B loads per thread
     extern __shared__ unsigned char smem_raw[];
     cx<R>* tw = reinterpret_cast<cx<R>*>(smem_raw);
     cx<R>* buf = tw + N;
     for (int i = threadIdx.x; i < N; i += THREADS) tw[i] = twg[i];
     const long line0 = xcd_tile(blockIdx.x, gridDim.x) * LINES;
     const int nl = (int)min((long)LINES, nlines - line0);
-    // support codes of the owned voxels and the first component's half spectrum up front; the next
-    // component's half spectrum is loaded while the current one is transformed
+    const cx<R> zero = mkc<R>((R)0, (R)0);
+    // all global loads up front: support codes, the x and y half spectra, component z, then the v
+    // values of the alpha dot product (these depend on the codes, so they go last)
     int code[KE];
 #pragma unroll
     for (int k = 0; k < KE; ++k) {
@@ -182,40 +235,64 @@ __global__ __launch_bounds__((Shape<R, N>::THREADS)) void k_xinv_s(const cx<R>* 
         const int line = idx / N;
         code[k] = (line < nl) ? sink.code[(line0 + line) * N + idx % N] : -2;
     }
-    cx<R> h[KH];
-    auto load_half = [&](int c) {
-        const cx<R>* Sc = S + c * Nh;
+    cx<R> ha[KA], hb[KA];
 #pragma unroll
-        for (int k = 0; k < KH; ++k) {
-            const int idx = threadIdx.x + k * THREADS;
-            const int line = idx / NXH;
-            h[k] = (idx < LINES * NXH && line < nl) ? Sc[(line0 + line) * NXH + idx % NXH] : mkc<R>((R)0, (R)0);
-        }
-    };
-    load_half(0);
+    for (int k = 0; k < KA; ++k) {
+        const int idx = threadIdx.x + k * THREADS;
+        const int line = idx / NXH;
+        const bool ok = idx < LINES * NXH && line < nl;
+        const long o = (line0 + line) * NXH + idx % NXH;
+        ha[k] = ok ? S[o] : zero;
+        hb[k] = ok ? S[Nh + o] : zero;
+    }
+    cx<R> ga[KB], gb[KB];
+#pragma unroll
+    for (int k = 0; k < KB; ++k) {
+        const int idx = threadIdx.x + k * THREADS;
+        const int m = idx / NXH;
+        const int kx = idx % NXH;
+        const bool in = idx < (LINES / 2) * NXH;
+        ga[k] = (in && 2 * m < nl) ? S[2 * Nh + (line0 + 2 * m) * NXH + kx] : zero;
+        gb[k] = (in && 2 * m + 1 < nl) ? S[2 * Nh + (line0 + 2 * m + 1) * NXH + kx] : zero;
+    }
+    double dv[3][KE];
+#pragma unroll
+    for (int k = 0; k < KE; ++k) {
+        const int cd = code[k];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) dv[c][k] = (sink.dotv != nullptr && cd >= 0) ? sink.dotv[3L * cd + c] : 0.0;
+    }
     double out[3][KE];
+    // round A: components x and y
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
+    for (int k = 0; k < KA; ++k) {
+        const int idx = threadIdx.x + k * THREADS;
+        if (idx < LINES * NXH) xinv_put<R, N, LD>(buf, idx / NXH, idx % NXH, ha[k], hb[k]);
+    }
+    __syncthreads();
+    sfft_lines<R, +1, N, LINES, SH::T, LD>(buf, tw, nl);
 #pragma unroll
-        for (int k = 0; k < KH; ++k) {
-            const int idx = threadIdx.x + k * THREADS;
-            if (idx < LINES * NXH) buf[(idx / NXH) * LD + idx % NXH] = h[k];
-        }
-        if (c < 2) load_half(c + 1);
-        __syncthreads();
-        for (int idx = threadIdx.x; idx < LINES * NM; idx += THREADS) {
-            const int line = idx / NM;
-            const int kx = NXH + idx % NM;
-            buf[line * LD + kx] = conjc<R>(buf[line * LD + (N - kx)]);
-        }
-        __syncthreads();
-        sfft_lines<R, +1, N, LINES, SH::T, LD>(buf, tw, nl);
+    for (int k = 0; k < KE; ++k) {
+        const int idx = threadIdx.x + k * THREADS;
+        const cx<R> z = buf[(idx / N) * LD + idx % N];
+        out[0][k] = (double)z.x * inv_n;
+        out[1][k] = (double)z.y * inv_n;
+    }
+    __syncthreads();
+    // round B: component z of lines 2m (real part) and 2m+1 (imaginary part)
 #pragma unroll
-        for (int k = 0; k < KE; ++k) {
-            const int idx = threadIdx.x + k * THREADS;
-            out[c][k] = (double)buf[(idx / N) * LD + idx % N].x * inv_n;
-        }
-        __syncthreads();
+    for (int k = 0; k < KB; ++k) {
+        const int idx = threadIdx.x + k * THREADS;
+        if (idx < (LINES / 2) * NXH) xinv_put<R, N, LD>(buf, idx / NXH, idx % NXH, ga[k], gb[k]);
+    }
+    __syncthreads();
+    sfft_lines<R, +1, N, LINES, SH::T, LD>(buf, tw, (nl + 1) / 2);
+    const R* bufr = reinterpret_cast<const R*>(buf);
+#pragma unroll
+    for (int k = 0; k < KE; ++k) {
+        const int idx = threadIdx.x + k * THREADS;
+        const int line = idx / N;
+        out[2][k] = (double)bufr[2 * ((line >> 1) * LD + idx % N) + (line & 1)] * inv_n;
     }
     double acc[kXinvParts] = {0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
@@ -234,10 +311,8 @@ __global__ __launch_bounds__((Shape<R, N>::THREADS)) void k_xinv_s(const cx<R>* 
 #pragma unroll
                 for (int c = 0; c < 3; ++c) sink.compact[3L * cd + c] = out[c][k];
             }
-            if (sink.dotv != nullptr) {
 #pragma unroll
-                for (int c = 0; c < 3; ++c) acc[6] += sink.dotv[3L * cd + c] * out[c][k];
-            }
+            for (int c = 0; c < 3; ++c) acc[6] += dv[c][k] * out[c][k];
 #pragma unroll
             for (int c = 0; c < 3; ++c) acc[c] += out[c][k];
         }
@@ -280,13 +355,21 @@ __global__ __launch_bounds__((Shape<R, N>::THREADS)) void k_col_s(cx<R>* __restr
 }
 
 // Fused z pass: lines 0..3*ZL-1 of the LDS buffer hold components x | y | z of ZL columns.
-template <typename R, int N>
-__global__ __launch_bounds__((Shape<R, N>::ZTHREADS)) void k_zgreen_s(cx<R>* __restrict__ S, const R* __restrict__ G6,
-                                                                      long ncols, long Nh,
-                                                                      const cx<R>* __restrict__ twg, const int* skip) {
+// OS = threads per butterfly thread (1 or 2): extra threads only help the memory phases
+template <typename R, int N, int ZL, int OS>
+constexpr int zgreen_threads() {
+    return OS * 3 * ZL * Shape<R, N>::T;
+}
+
+template <typename R, int N, int ZL, int OS>
+__global__ __launch_bounds__((zgreen_threads<R, N, ZL, OS>())) void k_zgreen_s(cx<R>* __restrict__ S,
+                                                                                 const R* __restrict__ G6, long ncols,
+                                                                                 long Nh,
+                                                                                 const cx<R>* __restrict__ twg,
+                                                                                 const int* skip) {
     if (skip != nullptr && *skip) return;
     using SH = Shape<R, N>;
-    constexpr int ZL = SH::ZL, LD = SH::LD, THREADS = SH::ZTHREADS, L3 = 3 * ZL;
+    constexpr int LD = SH::LD, THREADS = zgreen_threads<R, N, ZL, OS>(), L3 = 3 * ZL;
     constexpr int EST = THREADS / L3;
     extern __shared__ unsigned char smem_raw[];
     cx<R>* tw = reinterpret_cast<cx<R>*>(smem_raw);
@@ -310,13 +393,13 @@ __global__ __launch_bounds__((Shape<R, N>::ZTHREADS)) void k_zgreen_s(cx<R>* __r
         constexpr int MT = THREADS / ZL;
         const int mci = threadIdx.x % ZL;
         const bool mown = mci < ncl;
-        const R* Gc = G6 + col0 + mci;
+        const R* Gc = G6 + 6L * (col0 + mci);
 #pragma unroll 2
         for (int e = threadIdx.x / ZL; e < N; e += MT) {
             if (!mown) continue;
-            const long gi = (long)e * ncols;
-            const R txx = Gc[gi], txy = Gc[Nh + gi], txz = Gc[2 * Nh + gi];
-            const R tyy = Gc[3 * Nh + gi], tyz = Gc[4 * Nh + gi], tzz = Gc[5 * Nh + gi];
+            const R* gp = Gc + 6L * e * ncols;
+            const R txx = gp[0], txy = gp[1], txz = gp[2];
+            const R tyy = gp[3], tyz = gp[4], tzz = gp[5];
             cx<R>* px = buf + (0 * ZL + mci) * LD + e;
             cx<R>* py = buf + (1 * ZL + mci) * LD + e;
             cx<R>* pz = buf + (2 * ZL + mci) * LD + e;
@@ -388,19 +471,90 @@ bool static_col(int n, int sign, cx<R>* S, long ncols, long inner, long outer, l
 #undef FFTPERM_CO
 }
 
+template <typename R, int N, int ZL>
+constexpr size_t zgreen_lds() {
+    return (size_t)(N + 3 * ZL * Shape<R, N>::LD) * sizeof(cx<R>);
+}
+
+template <typename R, int N, int ZL>
+bool zgreen_fits() {
+    return zgreen_lds<R, N, ZL>() <= g_lds_max;
+}
+
+template <typename R, int N, int ZL, int OS>
+bool launch_zgreen_zl(cx<R>* S, const R* G6, long ncols, long Nh, const cx<R>* tw, const int* skip, hipStream_t st) {
+    if constexpr (zgreen_threads<R, N, ZL, OS>() > 1024) {
+        return false;
+    } else {
+        if (!zgreen_fits<R, N, ZL>()) return false;
+        const long blocks = (ncols + ZL - 1) / ZL;
+        constexpr int threads = zgreen_threads<R, N, ZL, OS>();
+        constexpr size_t lds = zgreen_lds<R, N, ZL>();
+        hipLaunchKernelGGL((k_zgreen_s<R, N, ZL, OS>), dim3((unsigned)blocks), dim3(threads), lds, st, S, G6, ncols,
+                           Nh, tw, skip);
+        return true;
+    }
+}
+
+// Autotuned choice of the fused z-pass tile width ZL (columns per workgroup), keyed by
+// (precision, N, columns); filled by autotune_zgreen, read by every launch.
+std::mutex g_tune_mu;
+std::map<std::tuple<int, int, long>, int> g_zl_choice;
+
+int zl_choice(int prec, int n, long ncols) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_zl_choice.find(std::make_tuple(prec, n, ncols));
+    return it == g_zl_choice.end() ? 0 : it->second;
+}
+
+// variant code = 10 * ZL + OS
+template <typename R, int N>
+bool launch_zgreen_n(int code, cx<R>* S, const R* G6, long ncols, long Nh, const cx<R>* tw, const int* skip,
+                     hipStream_t st) {
+    switch (code) {
+        case 11: return launch_zgreen_zl<R, N, 1, 1>(S, G6, ncols, Nh, tw, skip, st);
+        case 12: return launch_zgreen_zl<R, N, 1, 2>(S, G6, ncols, Nh, tw, skip, st);
+        case 21: return launch_zgreen_zl<R, N, 2, 1>(S, G6, ncols, Nh, tw, skip, st);
+        case 22: return launch_zgreen_zl<R, N, 2, 2>(S, G6, ncols, Nh, tw, skip, st);
+        case 41: return launch_zgreen_zl<R, N, 4, 1>(S, G6, ncols, Nh, tw, skip, st);
+        case 42: return launch_zgreen_zl<R, N, 4, 2>(S, G6, ncols, Nh, tw, skip, st);
+        case 81: return launch_zgreen_zl<R, N, 8, 1>(S, G6, ncols, Nh, tw, skip, st);
+        case 82: return launch_zgreen_zl<R, N, 8, 2>(S, G6, ncols, Nh, tw, skip, st);
+        case 161: return launch_zgreen_zl<R, N, 16, 1>(S, G6, ncols, Nh, tw, skip, st);
+        case 162: return launch_zgreen_zl<R, N, 16, 2>(S, G6, ncols, Nh, tw, skip, st);
+        default: return false;
+    }
+}
+
+template <typename R>
+bool static_zgreen_zl(int n, int zl, cx<R>* S, const R* G6, long ncols, long Nh, const cx<R>* tw, const int* skip,
+                      hipStream_t st) {
+    switch (n) {
+        case 32: return launch_zgreen_n<R, 32>(zl, S, G6, ncols, Nh, tw, skip, st);
+        case 64: return launch_zgreen_n<R, 64>(zl, S, G6, ncols, Nh, tw, skip, st);
+        case 128: return launch_zgreen_n<R, 128>(zl, S, G6, ncols, Nh, tw, skip, st);
+        case 256: return launch_zgreen_n<R, 256>(zl, S, G6, ncols, Nh, tw, skip, st);
+        case 512: return launch_zgreen_n<R, 512>(zl, S, G6, ncols, Nh, tw, skip, st);
+        default: return false;
+    }
+}
+
 template <typename R>
 bool static_zgreen(int n, cx<R>* S, const R* G6, long ncols, long Nh, const cx<R>* tw, const int* skip,
                    hipStream_t st) {
-#define FFTPERM_ZG(NN)                                                                                          \
-    case NN: {                                                                                                  \
-        using SH = Shape<R, NN>;                                                                                \
-        const long blocks = (ncols + SH::ZL - 1) / SH::ZL;                                                      \
-        hipLaunchKernelGGL((k_zgreen_s<R, NN>), dim3((unsigned)blocks), dim3(SH::ZTHREADS), SH::LDSZ, st, S, G6, \
-                           ncols, Nh, tw, skip);                                                                \
-        return true;                                                                                            \
+    int code = zl_choice(sizeof(R) == 8 ? 0 : 1, n, ncols);
+    if (code == 0) {
+        // untuned: the widest tile that fits ~100 KB of LDS, one thread per butterfly
+        switch (n) {
+            case 32: code = 10 * Shape<R, 32>::ZL + 1; break;
+            case 64: code = 10 * Shape<R, 64>::ZL + 1; break;
+            case 128: code = 10 * Shape<R, 128>::ZL + 1; break;
+            case 256: code = 10 * Shape<R, 256>::ZL + 1; break;
+            case 512: code = 10 * Shape<R, 512>::ZL + 1; break;
+            default: return false;
+        }
     }
-    switch (n) { FFTPERM_ZG(32) FFTPERM_ZG(64) FFTPERM_ZG(128) FFTPERM_ZG(256) FFTPERM_ZG(512) default: return false; }
-#undef FFTPERM_ZG
+    return static_zgreen_zl<R>(n, code, S, G6, ncols, Nh, tw, skip, st);
 }
 
 // The specialised kernels need their static LDS budget; fall back to the runtime kernels on
@@ -408,6 +562,53 @@ bool static_zgreen(int n, cx<R>* S, const R* G6, long ncols, long Nh, const cx<R
 bool static_ok() { return g_lds_max >= 102400u + 8192u; }
 
 }  // namespace
+
+template <typename R>
+std::vector<std::pair<int, float>> autotune_zgreen(cx<R>* S, const R* G6, const Geometry3& g, const FFTTables<R>& t,
+                                                   hipStream_t st) {
+    std::vector<std::pair<int, float>> timings;
+    const long ncols = (long)g.ny * g.nxh;
+    const int prec = sizeof(R) == 8 ? 0 : 1;
+    if (!static_ok() || zl_choice(prec, g.nz, ncols) != 0) return timings;
+    hipEvent_t a, b;
+    if (hipEventCreate(&a) != hipSuccess) return timings;
+    if (hipEventCreate(&b) != hipSuccess) {
+        (void)hipEventDestroy(a);
+        return timings;
+    }
+    (void)hipMemsetAsync(S, 0, 3 * (size_t)g.Nh * sizeof(cx<R>), st);
+    int best = 0;
+    float best_ms = 1e30f;
+    for (int zl : {11, 12, 21, 22, 41, 42, 81, 82, 161, 162}) {
+        if (!static_zgreen_zl<R>(g.nz, zl, S, G6, ncols, g.Nh, t.twz, nullptr, st)) continue;  // warm-up
+        (void)hipEventRecord(a, st);
+        for (int rep = 0; rep < 3; ++rep) static_zgreen_zl<R>(g.nz, zl, S, G6, ncols, g.Nh, t.twz, nullptr, st);
+        (void)hipEventRecord(b, st);
+        if (hipEventSynchronize(b) != hipSuccess) break;
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, a, b);
+        ms /= 3.f;
+        timings.emplace_back(zl, ms);
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = zl;
+        }
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (best != 0) {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_zl_choice[std::make_tuple(prec, g.nz, ncols)] = best;
+    }
+    return timings;
+}
+
+int zgreen_tile(int prec, int n, long ncols) { return zl_choice(prec, n, ncols); }
+
+template std::vector<std::pair<int, float>> autotune_zgreen<double>(cx<double>*, const double*, const Geometry3&,
+                                                                    const FFTTables<double>&, hipStream_t);
+template std::vector<std::pair<int, float>> autotune_zgreen<float>(cx<float>*, const float*, const Geometry3&,
+                                                                   const FFTTables<float>&, hipStream_t);
 
 long xinv_blocks(const Geometry3& g, const LinePlan& px) {
     // exactly the number of blocks (= partial sums) the x-inverse launcher will use
@@ -644,9 +845,9 @@ __global__ void k_zgreen(cx<R>* __restrict__ S, const R* __restrict__ G6, int nz
     for (int idx = threadIdx.x; idx < cnt; idx += blockDim.x) {
         const int e = idx / ncl;
         const int ci = idx - e * ncl;
-        const long gi = col0 + ci + e * ncols;
-        const R txx = G6[gi], txy = G6[Nh + gi], txz = G6[2 * Nh + gi];
-        const R tyy = G6[3 * Nh + gi], tyz = G6[4 * Nh + gi], tzz = G6[5 * Nh + gi];
+        const R* gp = G6 + 6L * (col0 + ci + e * ncols);
+        const R txx = gp[0], txy = gp[1], txz = gp[2];
+        const R tyy = gp[3], tyz = gp[4], tzz = gp[5];
         cx<R>* px = buf + (0 * ncl + ci) * ld + e;
         cx<R>* py = buf + (1 * ncl + ci) * ld + e;
         cx<R>* pz = buf + (2 * ncl + ci) * ld + e;

@@ -223,30 +223,30 @@ void launch_mr_finish_alpha(const double* partials, MinresState* st, hipStream_t
     hipLaunchKernelGGL(k_mr_finish_alpha, dim3(1), dim3(kRedThreads), 0, s, partials, st);
 }
 
-__global__ void k_mr_axpy_norm(double* __restrict__ z, const double* __restrict__ v,
-                               const double* __restrict__ v_old, MinresState* st, long nb,
-                               double* __restrict__ partials) {
+// Flat element loops over the AoS vectors: blocks of kFlatThreads (a multiple of 3) and a grid
+// stride that is a multiple of 3 give every thread one fixed component c = i % 3.
+constexpr int kFlatThreads = 384;
+
+__global__ __launch_bounds__(kFlatThreads) void k_mr_axpy_norm(double* __restrict__ z, const double* __restrict__ v,
+                                                               const double* __restrict__ v_old, MinresState* st,
+                                                               long n, double* __restrict__ partials) {
     if (st->done) return;
     const double a = st->alpha, b = st->beta;
-    const double m0 = st->mean_out[0], m1 = st->mean_out[1], m2 = st->mean_out[2];
+    const long i0 = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const double m = st->mean_out[i0 % 3];
     double acc[1] = {0.0};
-    for (long l = (long)blockIdx.x * blockDim.x + threadIdx.x; l < nb; l += (long)gridDim.x * blockDim.x) {
-        const long i = 3 * l;
+    for (long i = i0; i < n; i += (long)gridDim.x * blockDim.x) {
         // z = P_B(z_raw) (Algorithm 1 step 17), then z -= alpha v + beta v_old
-        const double z0 = (z[i] - m0) - (a * v[i] + b * v_old[i]);
-        const double z1 = (z[i + 1] - m1) - (a * v[i + 1] + b * v_old[i + 1]);
-        const double z2 = (z[i + 2] - m2) - (a * v[i + 2] + b * v_old[i + 2]);
-        z[i] = z0;
-        z[i + 1] = z1;
-        z[i + 2] = z2;
-        acc[0] += z0 * z0 + z1 * z1 + z2 * z2;
+        const double zi = (z[i] - m) - (a * v[i] + b * v_old[i]);
+        z[i] = zi;
+        acc[0] += zi * zi;
     }
     block_sum<1>(acc, partials + blockIdx.x);
 }
 
 void launch_mr_axpy_norm(double* z, const double* v, const double* v_old, MinresState* st, long nb,
                          double* partials, hipStream_t s) {
-    hipLaunchKernelGGL(k_mr_axpy_norm, dim3(kRedBlocks), dim3(kRedThreads), 0, s, z, v, v_old, st, nb, partials);
+    hipLaunchKernelGGL(k_mr_axpy_norm, dim3(kRedBlocks), dim3(kFlatThreads), 0, s, z, v, v_old, st, 3 * nb, partials);
 }
 
 // Steps 4-12 of the reference iteration (src/solver/MinresSolver.cpp:135-202) on scalars.
@@ -279,6 +279,9 @@ __global__ void k_mr_givens(const double* __restrict__ partials, MinresState* st
     st->c_eps = eps;
     st->c_delta = delta;
     st->c_rho = rho;
+    // the vector updates multiply by these reciprocals (within one rounding of the reference's divisions)
+    st->c_inv_beta = (beta_new > kBreakdownEps) ? 1.0 / beta_new : 0.0;
+    st->c_inv_rho = (rho > kBreakdownEps) ? 1.0 / rho : 0.0;
     st->c_xcoef = cs_new * st->phi_bar;
     const double phi_bar = st->phi_bar * sn_new;
     st->phi_bar = phi_bar;
@@ -301,36 +304,37 @@ void launch_mr_givens(const double* partials, MinresState* st, double* history, 
     hipLaunchKernelGGL(k_mr_givens, dim3(1), dim3(kRedThreads), 0, s, partials, st, history);
 }
 
-__global__ void k_mr_update(double* __restrict__ x, const double* __restrict__ v, const double* __restrict__ z,
-                            double* __restrict__ w_prev_io, const double* __restrict__ w, double* __restrict__ v_new,
-                            MinresState* st, long nb, double* __restrict__ partials) {
+__global__ __launch_bounds__(kFlatThreads) void k_mr_update(double* __restrict__ x, const double* __restrict__ v,
+                                                            const double* __restrict__ z,
+                                                            double* __restrict__ w_prev_io,
+                                                            const double* __restrict__ w, double* __restrict__ v_new,
+                                                            MinresState* st, long n, double* __restrict__ partials) {
     if (st->done) return;
-    const double beta_new = st->beta_new;
-    const double eps = st->c_eps, delta = st->c_delta, rho = st->c_rho, xc = st->c_xcoef;
+    const double inv_beta = st->c_inv_beta, inv_rho = st->c_inv_rho;
+    const double eps = st->c_eps, delta = st->c_delta, xc = st->c_xcoef;
     const int vz = st->v_zero, wz = st->w_zero;
-    double acc[3] = {0, 0, 0};
-    for (long l = (long)blockIdx.x * blockDim.x + threadIdx.x; l < nb; l += (long)gridDim.x * blockDim.x) {
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            const long i = 3 * l + c;
-            // v_old <- v (pointer swap on the host); v <- z / beta_new (into the old v_old buffer)
-            const double vn = vz ? 0.0 : z[i] / beta_new;
-            // w_next = (v_old - eps w_prev - delta w) / rho with v_old = the current v;
-            // written over w_prev (pointer swap of w_prev and w on the host)
-            const double wn = wz ? 0.0 : (v[i] - eps * w_prev_io[i] - delta * w[i]) / rho;
-            x[i] += xc * wn;
-            v_new[i] = vn;
-            w_prev_io[i] = wn;
-            acc[c] += vn;
-        }
+    const long i0 = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double s = 0.0;
+    for (long i = i0; i < n; i += (long)gridDim.x * blockDim.x) {
+        // v_old <- v (pointer swap on the host); v <- z / beta_new (into the old v_old buffer)
+        const double vn = vz ? 0.0 : z[i] * inv_beta;
+        // w_next = (v_old - eps w_prev - delta w) / rho with v_old = the current v; written over
+        // w_prev (pointer swap of w_prev and w on the host)
+        const double wn = wz ? 0.0 : (v[i] - eps * w_prev_io[i] - delta * w[i]) * inv_rho;
+        x[i] += xc * wn;
+        v_new[i] = vn;
+        w_prev_io[i] = wn;
+        s += vn;
     }
+    const int c = (int)(i0 % 3);
+    double acc[3] = {c == 0 ? s : 0.0, c == 1 ? s : 0.0, c == 2 ? s : 0.0};
     block_sum<3>(acc, partials + 3L * blockIdx.x);
 }
 
 void launch_mr_update(double* x, const double* v, const double* z, double* w_prev_io, const double* w,
                       double* v_new, MinresState* st, long nb, double* partials, hipStream_t s) {
-    hipLaunchKernelGGL(k_mr_update, dim3(kRedBlocks), dim3(kRedThreads), 0, s, x, v, z, w_prev_io, w, v_new, st, nb,
-                       partials);
+    hipLaunchKernelGGL(k_mr_update, dim3(kRedBlocks), dim3(kFlatThreads), 0, s, x, v, z, w_prev_io, w, v_new, st,
+                       3 * nb, partials);
 }
 
 __global__ void k_mr_finish_mean_in(const double* __restrict__ partials, MinresState* st, double inv_nb) {

@@ -208,10 +208,11 @@ __global__ void k_green_sym(GreenAxisTables t, int nx, int ny, int nz, int nxh, 
     double a[6], b[6];
     green_at(t, kx, ky, kz, a);
     green_at(t, (nx - kx) % nx, (ny - ky) % ny, (nz - kz) % nz, b);
+    // AoS: the six components of one frequency are contiguous (one 48-byte run per frequency)
     for (int g = 0; g < 6; ++g) {
         const double s = 0.5 * (a[g] + b[g]);
-        G6[g * Nh + i] = s;
-        if (G6f != nullptr) G6f[g * Nh + i] = (float)s;
+        G6[6 * i + g] = s;
+        if (G6f != nullptr) G6f[6 * i + g] = (float)s;
     }
 }
 

@@ -17,7 +17,7 @@ struct MinresState {
     double rhs_norm, tol;
     double alpha;
     // coefficients of the current iteration's vector update
-    double beta_new, c_eps, c_delta, c_rho, c_xcoef;
+    double beta_new, c_eps, c_delta, c_rho, c_xcoef, c_inv_beta, c_inv_rho;
     int v_zero, w_zero;
     double rel_res;
     double mean_in[3];   // support mean of v (input projection of the next apply)

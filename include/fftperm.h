@@ -153,6 +153,7 @@ typedef struct {
     double apply_ms[2];
     uint64_t apply_launches[2];
     uint64_t N, Nh, nb;          /* grid size, half-spectrum size, support voxels */
+    int zgreen_tile[2];          /* autotuned z-pass variant 10*columns + threads per butterfly [FP64, FP32]; 0 = not tuned */
 } fftperm_profile_stats;
 
 int fftperm_profile(fftperm_ctx* ctx, int enable);
