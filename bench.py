@@ -3,11 +3,17 @@
 
 Workload (the config the metric is quoted on): a periodic 256^3 Boolean model of
 overlapping radius-8 solid spheres at porosity 0.6 (seed 2001), full 3x3
-permeability tensor from the three unit load directions, Bignonnet MINRES to a
-relative recurrence residual of 1e-10.  One "step" = support detection from the
-device-resident mask + the three direction solves (RHS, MINRES, velocity
-recovery, domain average).  The Green tensor is precomputed once per grid and
-viscosity (reported separately as setup).
+permeability tensor from the three unit load directions.  One "step" = support
+detection from the device-resident mask + the three direction solves (RHS,
+MINRES, velocity recovery, domain average).  The Green tensor is precomputed
+once per grid and viscosity (reported separately as setup).
+
+The metric is quoted at <= 1e-6 relative K error, so both the plain FP64 path and
+the mixed-precision path are timed at the same MINRES tolerance (--tol, default
+1e-6) and each one's K error is measured against an untimed FP64 solve of the
+same medium at --ref-tol (1e-10): ||K - K_ref||_F / ||K_ref||_F.  The headline
+value is the mixed path's solves/s when its K error meets 1e-6, else the FP64
+path's.
 
 Prints ONE JSON line.  --gpus N with N > 1 exits: this build runs on one GPU.
 """
@@ -26,6 +32,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "256³ k-tensor solves/s, mixed+learned vs plain FP64 GPU, at ≤1e-6 rel k error"
+K_ERR_BAR = 1e-6
 HBM_PEAK_GBS = 8000.0
 
 
@@ -48,12 +55,15 @@ def bytes_apply(N, Nh, L, s_c=16, s_r=8):
     return xf + 2 * yp + zg + xi
 
 
-def run_precision(args, P, _lib, mask_dev, precision):
+def run_precision(args, P, _lib, mask_dev, precision, tol=None, steps=None, warmup=None, label=""):
     import ctypes as C
 
     import torch
 
     n = args.n
+    tol = args.tol if tol is None else tol
+    steps = args.steps if steps is None else steps
+    warmup = args.warmup if warmup is None else warmup
     grid = P.Grid3D(n, n, n)
     t0 = time.perf_counter()
     green = P.GreenOperator(grid, 1.0, 2, precision)
@@ -65,38 +75,39 @@ def run_precision(args, P, _lib, mask_dev, precision):
     def step():
         _lib.check(_lib.load().fftperm_set_medium_dev(green._h, C.c_void_p(mask_dev.data_ptr()),
                                                        _lib.SUPPORT_INTERFACE, C.byref(nb)))
-        _lib.check(_lib.load().fftperm_solve_tensor(green._h, args.tol, args.maxit, K, C.cast(infos, C.c_void_p)))
+        _lib.check(_lib.load().fftperm_solve_tensor(green._h, tol, args.maxit, K, C.cast(infos, C.c_void_p)))
 
-    for w in range(args.warmup):
+    for w in range(warmup):
         step()
-        log(f"[bench] warmup {w + 1}/{args.warmup} precision={precision} iterations="
-            f"{[int(i.iterations) for i in infos]}")
+        log(f"[bench] {label} warmup {w + 1}/{warmup} iterations={[int(i.iterations) for i in infos]}")
     _lib.check(_lib.load().fftperm_profile(green._h, 1))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for s in range(args.steps):
+    for s in range(steps):
         step()
-        log(f"[bench] step {s + 1}/{args.steps} precision={precision} iterations="
-            f"{[int(i.iterations) for i in infos]}")
+        log(f"[bench] {label} step {s + 1}/{steps} iterations={[int(i.iterations) for i in infos]}")
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     prof = _lib.ProfileStats()
     _lib.check(_lib.load().fftperm_profile_read(green._h, C.byref(prof)))
     res = dict(
-        precision=precision, elapsed=elapsed, solves_per_s=args.steps / elapsed, ms_per_step=1e3 * elapsed / args.steps,
+        precision=precision, tol=tol, elapsed=elapsed, solves_per_s=steps / elapsed, ms_per_step=1e3 * elapsed / steps,
         K=K.reshape(3, 3).copy(), iterations=[int(i.iterations) for i in infos],
         sweeps=[int(i.refinement_sweeps) for i in infos], converged=[bool(i.converged) for i in infos],
         final_rel=[float(i.final_rel_residual) for i in infos], setup_s=setup_s, nb=int(nb.value),
         prof=dict(z_ms=list(prof.zgreen_ms), z_n=list(prof.zgreen_launches), a_ms=list(prof.apply_ms),
-                  a_n=list(prof.apply_launches), N=int(prof.N), Nh=int(prof.Nh), tile=list(prof.zgreen_tile)),
+                  a_n=list(prof.apply_launches), N=int(prof.N), Nh=int(prof.Nh),
+                  tiles=dict(zgreen=list(prof.zgreen_tile), xfwd=list(prof.xfwd_tile), xinv=list(prof.xinv_tile),
+                             ypass=list(prof.ypass_tile))),
     )
     green.close()
     return res
 
 
-def cpu_baseline(args, mask, applies_per_solve):
-    """Oracle (FP64 restatement of the reference path) on the host cores: one operator apply at the
-    same size, scaled by the applies one tensor solve needs."""
+def cpu_baseline(args, mask, applies_per_solve, min_seconds=10.0):
+    """Oracle (FP64 restatement of the reference path) on the host cores: repeated operator applies
+    at the same size (a bounded sample of about min_seconds), scaled by the applies one tensor solve
+    needs."""
     import oracle
 
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
@@ -104,18 +115,24 @@ def cpu_baseline(args, mask, applies_per_solve):
     o = oracle.Oracle((n, n, n))
     o.set_medium(mask.reshape(-1))
     x = np.random.default_rng(0).standard_normal(3 * o.nb)
+    reps = 0
     t0 = time.perf_counter()
-    o.apply(x)
-    t_apply = time.perf_counter() - t0
+    while True:
+        o.apply(x)
+        reps += 1
+        if time.perf_counter() - t0 >= min_seconds and reps >= 2:
+            break
+    t_apply = (time.perf_counter() - t0) / reps
     o.close()
     return {
         "value": 1.0 / (t_apply * applies_per_solve),
         "unit": "k-tensor solves/s",
         "cores": threads,
         "kind": "port",
-        "sample": (f"one FP64 VariationalOperator apply of the oracle at {n}^3 ({t_apply:.2f} s, 6 C2C 3D FFTs + "
-                   f"Green multiply, {threads} OpenMP threads), scaled by the {applies_per_solve} applies of one "
-                   f"tensor solve (RHS + MINRES iterations + recovery, 3 directions); Green-tensor setup excluded"),
+        "sample": (f"{reps} FP64 VariationalOperator applies of the oracle at {n}^3 ({t_apply:.3f} s each: 6 C2C 3D "
+                   f"FFTs + Green multiply, {threads} OpenMP threads), scaled by the {applies_per_solve} applies of "
+                   f"one FP64 tensor solve at tol {args.tol:g} (RHS + MINRES iterations + recovery, 3 directions); "
+                   f"Green-tensor setup excluded"),
     }
 
 
@@ -128,8 +145,11 @@ def main():
     ap.add_argument("--radius", type=float, default=8.0)
     ap.add_argument("--porosity", type=float, default=0.6)
     ap.add_argument("--seed", type=int, default=2001)
-    ap.add_argument("--tol", type=float, default=1e-10)
-    ap.add_argument("--maxit", type=int, default=10000)
+    ap.add_argument("--tol", type=float, default=1e-6, help="MINRES tolerance of the timed solves")
+    ap.add_argument("--ref-tol", type=float, default=1e-10,
+                    help="tolerance of the untimed FP64 reference solve the K errors are measured against "
+                         "(<= 0: no reference solve, K errors not reported)")
+    ap.add_argument("--maxit", type=int, default=20000)
     ap.add_argument("--precision", choices=["fp64", "mixed", "both"], default="both")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -151,18 +171,25 @@ def main():
     mask_dev = torch.from_numpy(mask.reshape(-1).copy()).to("cuda:0")
     torch.cuda.synchronize()
 
+    ref = None
+    if args.ref_tol > 0:
+        ref = run_precision(args, P, _lib, mask_dev, _lib.PRECISION_FP64, tol=args.ref_tol, steps=1, warmup=0,
+                            label=f"reference fp64 tol={args.ref_tol:g}")
     runs = {}
     if args.precision in ("fp64", "both"):
-        runs["fp64"] = run_precision(args, P, _lib, mask_dev, _lib.PRECISION_FP64)
+        runs["fp64"] = run_precision(args, P, _lib, mask_dev, _lib.PRECISION_FP64, label=f"fp64 tol={args.tol:g}")
     if args.precision in ("mixed", "both"):
-        runs["mixed"] = run_precision(args, P, _lib, mask_dev, _lib.PRECISION_MIXED)
+        runs["mixed"] = run_precision(args, P, _lib, mask_dev, _lib.PRECISION_MIXED, label=f"mixed tol={args.tol:g}")
 
-    k_err = None
-    if "fp64" in runs and "mixed" in runs:
-        Kf, Km = runs["fp64"]["K"], runs["mixed"]["K"]
-        k_err = float(np.linalg.norm(Km - Kf) / np.linalg.norm(Kf))
-    # the headline value: mixed if it meets the 1e-6 relative K error bar against FP64, else FP64
-    if "mixed" in runs and k_err is not None and k_err <= 1e-6:
+    def k_err(r):
+        if ref is None:
+            return None
+        return float(np.linalg.norm(r["K"] - ref["K"]) / np.linalg.norm(ref["K"]))
+
+    errs = {k: k_err(r) for k, r in runs.items()}
+    meets = {k: (e is not None and e <= K_ERR_BAR) for k, e in errs.items()}
+    # headline: the mixed path when it meets the K-error bar, else the FP64 path
+    if "mixed" in runs and meets.get("mixed"):
         head = runs["mixed"]
         dtype = "f32 FFT/Green + f64 Krylov"
     elif "fp64" in runs:
@@ -200,15 +227,17 @@ def main():
         "config": {
             "workload": (f"{args.n}^3 periodic Boolean model of overlapping radius-{args.radius:g} spheres, porosity "
                          f"{args.porosity:g} (seed {args.seed}); full 3x3 K from 3 load directions; Bignonnet MINRES "
-                         f"tol {args.tol:g}, p_radius 2, mu 1; step = support detection + 3 direction solves"),
+                         f"tol {args.tol:g}, p_radius 2, mu 1; step = support detection + 3 direction solves; K error "
+                         f"against an untimed FP64 solve at tol {args.ref_tol:g}"),
             "n": args.n,
+            "tol": args.tol,
             "porosity_measured": float(mask.mean()),
             "support_voxels": head["nb"],
             "parallelism": "single GPU",
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_zgreen (fused z FFT, symmetric Green multiply, z inverse FFT)",
+            "kernel": "k_zgreen_s (fused z FFT, symmetric Green multiply, z inverse FFT)",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
@@ -217,7 +246,7 @@ def main():
             "algorithmic_bytes_per_launch": z_bytes,
             "avg_launch_ms": z_ms,
             "launches": pr["z_n"][pi],
-            "autotuned_variant_10xcols_plus_oversub": pr["tile"][pi],
+            "precision": "fp32" if pi == 1 else "fp64",
         },
         "apply": {
             "avg_ms": a_ms,
@@ -225,14 +254,19 @@ def main():
             "achieved_GBs": a_bytes / (a_ms * 1e-3) / 1e9,
             "frac": a_bytes / (a_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
         },
+        "k_error_bar": K_ERR_BAR,
         "runs": {
             k: {
-                "solves_per_s": r["solves_per_s"], "ms_per_step": r["ms_per_step"], "iterations": r["iterations"],
+                "solves_per_s": r["solves_per_s"], "ms_per_step": r["ms_per_step"], "tol": r["tol"],
+                "k_rel_error": errs[k], "meets_k_error_bar": meets[k], "iterations": r["iterations"],
                 "refinement_sweeps": r["sweeps"], "converged": r["converged"], "final_rel_residual": r["final_rel"],
-                "setup_s": r["setup_s"], "K": r["K"].tolist(),
+                "setup_s": r["setup_s"], "variants": r["prof"]["tiles"], "K": r["K"].tolist(),
             } for k, r in runs.items()
         },
-        "k_rel_error_mixed_vs_fp64": k_err,
+        "reference": None if ref is None else {
+            "tol": ref["tol"], "solves_per_s": ref["solves_per_s"], "iterations": ref["iterations"],
+            "K": ref["K"].tolist(),
+        },
         "speedup_mixed_vs_fp64": (runs["mixed"]["solves_per_s"] / runs["fp64"]["solves_per_s"]
                                   if "fp64" in runs and "mixed" in runs else None),
         "cpu_baseline": None,

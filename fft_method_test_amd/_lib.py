@@ -19,13 +19,16 @@ LIB_PATH = os.path.join(HERE, "libfftperm.so")
 OK, EINVAL, ERANGE, ERUNTIME, EDEVICE, EUNSUPPORTED = range(6)
 SUPPORT_FULLSOLID, SUPPORT_INTERFACE = 0, 1
 PRECISION_FP64, PRECISION_MIXED = 0, 1
+PASS_XFWD, PASS_XINV, PASS_YPASS, PASS_ZGREEN = 0, 1, 2, 3
+# tile variants the specialised passes offer at a length (codes as in fftperm_profile_stats)
+ZGREEN_CODES = (11, 12, 21, 22, 41, 42, 81, 82, 161, 162, 44, 84, 164, 324)
 
 EXPORTED_SYMBOLS = (
     "fftperm_last_error", "fftperm_device_info", "fftperm_create", "fftperm_destroy",
     "fftperm_set_medium", "fftperm_set_medium_dev", "fftperm_porosity", "fftperm_get_support",
     "fftperm_green_tensor", "fftperm_fft_forward", "fftperm_fft_inverse", "fftperm_spectral_green",
     "fftperm_green_apply", "fftperm_background_force", "fftperm_rhs", "fftperm_operator_apply",
-    "fftperm_minres", "fftperm_recover", "fftperm_solve_direction", "fftperm_solve_tensor",
+    "fftperm_minres", "fftperm_set_refinement", "fftperm_pin_variant", "fftperm_recover", "fftperm_solve_direction", "fftperm_solve_tensor",
     "fftperm_residual_norm", "fftperm_symmetry_gap", "fftperm_profile", "fftperm_profile_read",
     "fftperm_brinkman_solve",
 )
@@ -45,7 +48,8 @@ class SolveInfo(C.Structure):
 class ProfileStats(C.Structure):
     _fields_ = [("zgreen_ms", C.c_double * 2), ("zgreen_launches", C.c_uint64 * 2),
                 ("apply_ms", C.c_double * 2), ("apply_launches", C.c_uint64 * 2),
-                ("N", C.c_uint64), ("Nh", C.c_uint64), ("nb", C.c_uint64), ("zgreen_tile", C.c_int * 2)]
+                ("N", C.c_uint64), ("Nh", C.c_uint64), ("nb", C.c_uint64), ("zgreen_tile", C.c_int * 2),
+                ("xfwd_tile", C.c_int * 2), ("xinv_tile", C.c_int * 2), ("ypass_tile", C.c_int * 2)]
 
 
 class BrinkmanInfo(C.Structure):
@@ -84,6 +88,8 @@ def load():
     L.fftperm_rhs.argtypes = [_vp, _dp, _dp]
     L.fftperm_operator_apply.argtypes = [_vp, _dp, _dp]
     L.fftperm_minres.argtypes = [_vp, _dp, _vp, C.c_double, C.c_uint64, _dp, _vp, C.POINTER(SolveInfo)]
+    L.fftperm_set_refinement.argtypes = [_vp, C.c_double, C.c_double, C.c_uint32]
+    L.fftperm_pin_variant.argtypes = [_vp, C.c_int, C.c_int, C.c_int]
     L.fftperm_recover.argtypes = [_vp, _dp, _dp, _vp, _dp]
     L.fftperm_solve_direction.argtypes = [_vp, _dp, C.c_double, C.c_uint64, _vp, _dp, _vp, C.POINTER(SolveInfo)]
     L.fftperm_solve_tensor.argtypes = [_vp, C.c_double, C.c_uint64, _dp, _vp]

@@ -13,7 +13,8 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libfftperm.so")
 ARCH = os.environ.get("FFTPERM_ARCH", "gfx950")
 
-SOURCES = ["api.hip", "kernels_fft.hip", "kernels_setup.hip", "kernels_krylov.hip", "brinkman.hip"]
+SOURCES = ["api.hip", "kernels_fft.hip", "passes_x.hip", "passes_col.hip", "kernels_setup.hip", "kernels_krylov.hip",
+           "brinkman.hip"]
 
 HIPCC_FLAGS = [
     "-O3",
@@ -35,7 +36,7 @@ def hipcc() -> str:
     return "hipcc"
 
 
-def build(verbose: bool = False, jobs: int = 5) -> str:
+def build(verbose: bool = False, jobs: int = 7) -> str:
     """Compile each source to an object (in parallel), then link the shared library."""
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)

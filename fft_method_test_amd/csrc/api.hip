@@ -149,6 +149,10 @@ struct fftperm_ctx {
     double mu = 1.0;
     int P = 2;
     int precision = FFTPERM_PRECISION_FP64;
+    double mixed_switch = 2.0;      // recurrence residual below which sweep 1 applies the FP32 operator
+    double mixed_floor = 3e-6;      // inner recurrence-residual floor of a refinement sweep
+    uint32_t mixed_sweeps = 64;     // sweeps per solve at most
+    uint64_t fp32_iters = 0;        // MINRES iterations with the FP32 operator (whole context)
     int device = 0;
     hipStream_t s = nullptr;
 
@@ -159,7 +163,7 @@ struct fftperm_ctx {
     double* G64 = nullptr;
     float* G32 = nullptr;
     double* gtab = nullptr;  // per-axis Green tables (6 arrays)
-    std::vector<std::pair<int, float>> tune64, tune32;  // z-pass autotune timings (width, ms)
+    std::vector<std::pair<int, float>> tune64, tune32;  // autotune timings (variant code, ms) of this context
     GreenAxisTables gt{};
 
     // medium
@@ -172,10 +176,12 @@ struct fftperm_ctx {
     unsigned long long* counts = nullptr;
 
     // reductions
-    double* part_x = nullptr;  // kXinvParts x xinv blocks
+    double* part_x = nullptr;  // kXinvParts x x-inverse workgroups
+    double* part_z = nullptr;  // one quadratic-form partial per fused z-pass workgroup
     double* part = nullptr;    // 8 x kRedBlocks
-    double* small = nullptr;   // 32 scalars: [0,3) input mean, [4,12) x-inverse sums, [12,15) average, 16 norm
-    long xblocks = 0;
+    double* small = nullptr;   // 32 scalars: [0,3) input mean, [4,10) x-inverse sums, [12,15) average, 16 norm
+    long xparts = 0;           // partial rows written by the last x-inverse launch
+    long zparts = 0;           // partials written by the last fused z-pass launch
 
     // Krylov vectors (L each)
     size_t Lcap = 0;
@@ -243,7 +249,7 @@ struct fftperm_ctx {
         dfree(t32.twx); dfree(t32.twy); dfree(t32.twz);
         dfree(S64); dfree(S32); dfree(G64); dfree(G32); dfree(gtab);
         dfree(mask); dfree(code); dfree(active); dfree(scan_scratch); dfree(counts);
-        dfree(part_x); dfree(part); dfree(small);
+        dfree(part_x); dfree(part_z); dfree(part); dfree(small);
         dfree(x); dfree(v_old); dfree(v); dfree(z); dfree(w_prev); dfree(w);
         dfree(b); dfree(tmp); dfree(r); dfree(dx);
         dfree(st); dfree(hist); dfree(field); dfree(full_a); dfree(t6);
@@ -299,7 +305,7 @@ struct fftperm_ctx {
     }
 
     template <typename R>
-    void chain(const XSource& src, const XSink& sink, const int* skip);
+    void chain(const XSource& src, const XSink& sink, const int* skip, double* qpart = nullptr);
 
     double host_scalar(const double* d) {
         double h;
@@ -311,7 +317,7 @@ struct fftperm_ctx {
 
 template <typename R, typename S_, typename G_, typename T_>
 static void run_chain(fftperm_ctx* c, S_* S, const G_* G, const T_& t, const XSource& src, const XSink& sink,
-                      const int* skip, int fp32) {
+                      const int* skip, double* qpart, int fp32) {
     fftperm_ctx::ProfRec rec{};
     if (c->prof) {
         rec.a = c->take_event();
@@ -325,10 +331,10 @@ static void run_chain(fftperm_ctx* c, S_* S, const G_* G, const T_& t, const XSo
     launch_xfwd<R>(src, S, c->g, t, skip, c->s);
     launch_ypass<R>(S, -1, c->g, t, skip, c->s);
     if (c->prof) HIPCHK(hipEventRecord(rec.b, c->s));
-    launch_zgreen<R>(S, G, c->g, t, skip, c->s);
+    c->zparts = launch_zgreen<R>(S, G, c->g, t, qpart, skip, c->s);
     if (c->prof) HIPCHK(hipEventRecord(rec.c, c->s));
     launch_ypass<R>(S, +1, c->g, t, skip, c->s);
-    launch_xinv<R>(S, sink, c->g, t, skip, c->s);
+    c->xparts = launch_xinv<R>(S, sink, c->g, t, skip, c->s);
     if (c->prof) {
         HIPCHK(hipEventRecord(rec.d, c->s));
         c->prof_pending.push_back(rec);
@@ -336,13 +342,13 @@ static void run_chain(fftperm_ctx* c, S_* S, const G_* G, const T_& t, const XSo
 }
 
 template <>
-void fftperm_ctx::chain<double>(const XSource& src, const XSink& sink, const int* skip) {
-    run_chain<double>(this, S64, G64, t64, src, sink, skip, 0);
+void fftperm_ctx::chain<double>(const XSource& src, const XSink& sink, const int* skip, double* qpart) {
+    run_chain<double>(this, S64, G64, t64, src, sink, skip, qpart, 0);
 }
 
 template <>
-void fftperm_ctx::chain<float>(const XSource& src, const XSink& sink, const int* skip) {
-    run_chain<float>(this, S32, G32, t32, src, sink, skip, 1);
+void fftperm_ctx::chain<float>(const XSource& src, const XSink& sink, const int* skip, double* qpart) {
+    run_chain<float>(this, S32, G32, t32, src, sink, skip, qpart, 1);
 }
 
 namespace {
@@ -390,7 +396,7 @@ void device_apply(fftperm_ctx* c, const double* xin, double* yout) {
     sk.compact = yout;
     sk.partials = c->part_x;
     c->chain<R>(sr, sk, nullptr);
-    launch_finish_sum(c->part_x, (int)c->xblocks, kXinvParts, inv_nb, c->small + 4, nullptr, c->s);
+    launch_finish_sum(c->part_x, (int)c->xparts, kXinvParts, inv_nb, c->small + 4, nullptr, c->s);
     launch_sub_mean(yout, c->small + 4, yout, nb, 1.0, nullptr, c->s);
 }
 
@@ -410,7 +416,7 @@ void device_rhs(fftperm_ctx* c, const double grad[3], double* bout) {
     sk.compact = c->tmp;
     sk.partials = c->part_x;
     c->chain<double>(sr, sk, nullptr);
-    launch_finish_sum(c->part_x, (int)c->xblocks, kXinvParts, 1.0 / (double)nb, c->small + 4, nullptr, c->s);
+    launch_finish_sum(c->part_x, (int)c->xparts, kXinvParts, 1.0 / (double)nb, c->small + 4, nullptr, c->s);
     launch_sub_mean(c->tmp, c->small + 4, bout, nb, -1.0, nullptr, c->s);
 }
 
@@ -426,9 +432,10 @@ void minres_iteration(fftperm_ctx* c) {
     XSink sk = sink_none(c);
     sk.compact = c->z;
     sk.partials = c->part_x;
-    sk.dotv = c->v;
-    c->chain<R>(sr, sk, &st->done);
-    launch_mr_finish_mean_out(c->part_x, (int)c->xblocks, st, inv_nb, c->s);
+    // alpha = <P_B v, G scatter(P_B v)> comes from the fused z pass (Parseval)
+    c->chain<R>(sr, sk, &st->done, c->part_z);
+    launch_mr_finish_mean_out(c->part_x, (int)c->xparts, c->part_z, (int)c->zparts, st, inv_nb,
+                              1.0 / (double)c->g.N, c->s);
     launch_mr_axpy_norm(c->z, c->v, c->v_old, st, nb, c->part, c->s);
     launch_mr_givens(c->part, st, c->hist, c->s);
     launch_mr_update(c->x, c->v, c->z, c->w_prev, c->w, c->v_old, st, nb, c->part, c->s);
@@ -447,8 +454,12 @@ struct MinresOut {
 
 // MinresSolver::solve (src/solver/MinresSolver.cpp:46-218) with the device operator.
 // bdev: rhs; x0dev: initial guess or null; result in c->x; history in c->hist[0..hist_len).
-template <typename R>
-MinresOut device_minres(fftperm_ctx* c, const double* bdev, const double* x0dev, double tol, uint64_t maxit) {
+// Precision schedule of the operator applies: FP64 while the recurrence relative residual is
+// above fp32_below, FP32 from the first chunk boundary at or below it (fp32_below <= 0: FP64
+// throughout; >= 1: FP32 throughout).  The initial residual and every Lanczos scalar stay FP64.
+// *fp32_iters (may be null) receives the number of iterations that used the FP32 operator.
+MinresOut device_minres(fftperm_ctx* c, const double* bdev, const double* x0dev, double tol, uint64_t maxit,
+                        double fp32_below, uint64_t* fp32_iters) {
     if (!(tol > 0.0)) throw Err(FFTPERM_EINVAL, "MinresSolver tolerance must be positive");
     const long L = c->L3();
     const long nb = (long)c->nb;
@@ -470,8 +481,12 @@ MinresOut device_minres(fftperm_ctx* c, const double* bdev, const double* x0dev,
     }
     const double rhs_norm = std::max(rhs_raw_norm, DBL_MIN);
     // r0 = b - A x0 (A 0 = 0 exactly, so the empty guess gives r0 = b)
+    bool use32 = fp32_below >= 1.0;
     if (x0dev) {
-        device_apply<R>(c, c->x, c->z);
+        if (use32)
+            device_apply<float>(c, c->x, c->z);
+        else
+            device_apply<double>(c, c->x, c->z);
         launch_axpby(1.0, bdev, -1.0, c->z, c->z, L, c->s);
     } else {
         HIPCHK(hipMemcpyAsync(c->z, bdev, L * sizeof(double), hipMemcpyDeviceToDevice, c->s));
@@ -509,18 +524,35 @@ MinresOut device_minres(fftperm_ctx* c, const double* bdev, const double* x0dev,
     launch_finish_sum(c->part, kRedBlocks, 3, 1.0 / (double)nb, c->st->mean_in, nullptr, c->s);
     launch_finish_sum(c->part, kRedBlocks, 3, 1.0, c->st->vsum, nullptr, c->s);
     const uint64_t chunk = (c->g.N >= (1L << 21)) ? 4 : 32;
-    uint64_t launched = 0;
+    uint64_t launched = 0, switched_at = use32 ? 0 : UINT64_MAX;
+    if (!use32 && fp32_below > 0.0 && rel0 <= fp32_below) {
+        use32 = true;
+        switched_at = 0;
+    }
     int done = h.done;
     while (!done && launched < maxit) {
         const uint64_t k = std::min<uint64_t>(chunk, maxit - launched);
         for (uint64_t i = 0; i < k; ++i) {
             c->cur_tag = (long long)(launched + i);
-            minres_iteration<R>(c);
+            if (use32)
+                minres_iteration<float>(c);
+            else
+                minres_iteration<double>(c);
         }
         c->cur_tag = -1;
         launched += k;
-        HIPCHK(hipMemcpyAsync(&done, &c->st->done, sizeof(int), hipMemcpyDeviceToHost, c->s));
-        c->sync();
+        if (!use32 && fp32_below > 0.0) {
+            HIPCHK(hipMemcpyAsync(&h, c->st, sizeof(h), hipMemcpyDeviceToHost, c->s));
+            c->sync();
+            done = h.done;
+            if (!done && h.rel_res <= fp32_below) {
+                use32 = true;
+                switched_at = launched;
+            }
+        } else {
+            HIPCHK(hipMemcpyAsync(&done, &c->st->done, sizeof(int), hipMemcpyDeviceToHost, c->s));
+            c->sync();
+        }
     }
     HIPCHK(hipMemcpyAsync(&h, c->st, sizeof(h), hipMemcpyDeviceToHost, c->s));
     c->sync();
@@ -529,10 +561,19 @@ MinresOut device_minres(fftperm_ctx* c, const double* bdev, const double* x0dev,
     out.final_rel = (h.iterations > 0) ? h.rel_res : rel0;
     out.converged = h.converged || (out.final_rel <= tol);
     out.hist_len = (uint64_t)h.iterations + 1;
+    if (fp32_iters)
+        *fp32_iters = (switched_at == UINT64_MAX || (uint64_t)h.iterations <= switched_at)
+                          ? 0
+                          : (uint64_t)h.iterations - switched_at;
     return out;
 }
 
-// Mixed precision: FP64 iterative refinement around FP32-operator MINRES sweeps.
+// Mixed precision: FP64 iterative refinement around MINRES sweeps on r = b - A x, each to the
+// recurrence residual max(c->mixed_floor, 0.5 tol / r) with r the FP64 true relative residual.
+// Sweep 1 may start with FP64 operator applies and switch to FP32 once its recurrence residual
+// is below c->mixed_switch (default: FP32 throughout; measured at 256^3, an FP64 head start does
+// not remove the second sweep, because the FP32-storage operator error sets the attainable
+// residual whichever iterations it enters); later sweeps apply the FP32 operator throughout.
 MinresOut device_minres_mixed(fftperm_ctx* c, const double* bdev, const double* x0dev, double tol, uint64_t maxit,
                               uint64_t* sweeps_out, std::vector<double>* hist_out) {
     if (!(tol > 0.0)) throw Err(FFTPERM_EINVAL, "MinresSolver tolerance must be positive");
@@ -570,10 +611,13 @@ MinresOut device_minres_mixed(fftperm_ctx* c, const double* bdev, const double* 
     double rel = true_residual();
     if (hist_out) hist_out->push_back(rel);
     uint64_t sweeps = 0;
-    const double inner_tol = 3e-6;
-    while (rel > tol && out.iterations < maxit && sweeps < 64) {
-        const double itol = std::max(inner_tol, 0.5 * tol / rel);
-        MinresOut in = device_minres<float>(c, c->r, nullptr, itol, maxit - out.iterations);
+    while (rel > tol && out.iterations < maxit && sweeps < c->mixed_sweeps) {
+        const bool first = (sweeps == 0);
+        const double itol = std::max(c->mixed_floor, 0.5 * tol / rel);
+        uint64_t it32 = 0;
+        MinresOut in = device_minres(c, c->r, nullptr, itol, maxit - out.iterations, first ? c->mixed_switch : 2.0,
+                                     &it32);
+        c->fp32_iters += it32;
         out.iterations += in.iterations;
         ++sweeps;
         // x += dx (MINRES result in c->x)
@@ -596,7 +640,7 @@ MinresOut run_minres(fftperm_ctx* c, const double* bdev, const double* x0dev, do
                      uint64_t* sweeps, std::vector<double>* mixed_hist) {
     *sweeps = 0;
     if (c->precision == FFTPERM_PRECISION_MIXED) return device_minres_mixed(c, bdev, x0dev, tol, maxit, sweeps, mixed_hist);
-    return device_minres<double>(c, bdev, x0dev, tol, maxit);
+    return device_minres(c, bdev, x0dev, tol, maxit, 0.0, nullptr);
 }
 
 // VelocityRecovery::recover + average_over_domain (src/bignonnet/VelocityRecovery.cpp:12-47), FP64.
@@ -614,7 +658,7 @@ void device_recover(fftperm_ctx* c, const double grad[3], const double* xdev, do
     sk.field = field;
     sk.partials = c->part_x;
     c->chain<double>(sr, sk, nullptr);
-    launch_finish_sum(c->part_x, (int)c->xblocks, kXinvParts, inv_nb, c->small + 4, nullptr, c->s);
+    launch_finish_sum(c->part_x, (int)c->xparts, kXinvParts, inv_nb, c->small + 4, nullptr, c->s);
     double mb[3];
     HIPCHK(hipMemcpyAsync(mb, c->small + 4, 3 * sizeof(double), hipMemcpyDeviceToHost, c->s));
     c->sync();
@@ -658,6 +702,40 @@ void solve_direction_dev(fftperm_ctx* c, const double grad[3], double tol, uint6
     fill_info(info, o, sweeps);
 }
 
+// First medium of a grid shape in this process: time the x-pass variants on its support map
+// (scatter from and gather to compact vectors, as in the solver) and keep the fastest.
+void tune_x_passes(fftperm_ctx* c) {
+    if (!static_passes_ok()) return;
+    const Geometry3& g = c->g;
+    const long nlines = (long)g.ny * g.nz;
+    auto untuned = [&](int prec) {
+        return tuned_variant(kKindXfwd, prec, g.nx, nlines) < 0 || tuned_variant(kKindXinv, prec, g.nx, nlines) < 0;
+    };
+    const bool need64 = untuned(0);
+    const bool need32 = c->precision == FFTPERM_PRECISION_MIXED && untuned(1);
+    if (!need64 && !need32) return;
+    c->ensure_vectors();
+    HIPCHK(hipMemsetAsync(c->x, 0, c->L3() * sizeof(double), c->s));
+    XSource sr = src_none(c);
+    sr.compact = c->x;
+    for (int k = 0; k < 3; ++k) {
+        sr.a_fluid[k] = 1.0;
+        sr.a_support[k] = 0.5;
+    }
+    XSink sk = sink_none(c);
+    sk.compact = c->tmp;
+    sk.partials = c->part_x;
+    if (need64) {
+        auto t = autotune_x<double>(sr, sk, c->S64, g, c->t64, c->s);
+        c->tune64.insert(c->tune64.end(), t.begin(), t.end());
+    }
+    if (need32) {
+        auto t = autotune_x<float>(sr, sk, c->S32, g, c->t32, c->s);
+        c->tune32.insert(c->tune32.end(), t.begin(), t.end());
+    }
+    c->sync();
+}
+
 void set_medium_impl(fftperm_ctx* c, const uint8_t* mask, bool host, int mode, uint64_t* nb_out) {
     if (mode != FFTPERM_SUPPORT_INTERFACE && mode != FFTPERM_SUPPORT_FULLSOLID)
         throw Err(FFTPERM_EINVAL, "unknown support mode");
@@ -679,6 +757,7 @@ void set_medium_impl(fftperm_ctx* c, const uint8_t* mask, bool host, int mode, u
     c->nf = cnt[1];
     c->has_medium = true;
     if (nb_out) *nb_out = c->nb;
+    if (c->nb > 0) tune_x_passes(c);
 }
 
 }  // namespace
@@ -788,8 +867,17 @@ int fftperm_create(const fftperm_grid* grid, double mu, int p_radius, int precis
         c->tune64 = autotune_zgreen<double>(c->S64, c->G64, g, c->t64, c->s);
         if (precision == FFTPERM_PRECISION_MIXED) c->tune32 = autotune_zgreen<float>(c->S32, c->G32, g, c->t32, c->s);
         HIPCHK(hipGetLastError());
-        c->xblocks = xinv_blocks(g, c->t64.px);
-        c->part_x = dalloc<double>((size_t)kXinvParts * c->xblocks);
+        {
+            auto t = autotune_col<double>(c->S64, g, c->t64, c->s);
+            c->tune64.insert(c->tune64.end(), t.begin(), t.end());
+        }
+        if (precision == FFTPERM_PRECISION_MIXED) {
+            auto t = autotune_col<float>(c->S32, g, c->t32, c->s);
+            c->tune32.insert(c->tune32.end(), t.begin(), t.end());
+        }
+        HIPCHK(hipGetLastError());
+        c->part_x = dalloc<double>((size_t)kXinvParts * xinv_max_blocks(g));
+        c->part_z = dalloc<double>((size_t)zgreen_max_blocks(g));
         c->part = dalloc<double>(8 * (size_t)kRedBlocks);
         c->small = dalloc<double>(32);
         c->st = dalloc<MinresState>(1);
@@ -1016,6 +1104,18 @@ int fftperm_minres(fftperm_ctx* ctx, const double* b, const double* x0, double t
     });
 }
 
+int fftperm_set_refinement(fftperm_ctx* ctx, double fp32_below, double inner_floor, uint32_t max_sweeps) {
+    return guarded([&] {
+        check_ctx(ctx);
+        if (!(fp32_below >= 0.0)) throw Err(FFTPERM_EINVAL, "fp32_below must be >= 0");
+        if (!(inner_floor > 0.0 && inner_floor < 1.0)) throw Err(FFTPERM_EINVAL, "inner_floor must lie in (0, 1)");
+        if (max_sweeps == 0) throw Err(FFTPERM_EINVAL, "max_sweeps must be positive");
+        ctx->mixed_switch = fp32_below;
+        ctx->mixed_floor = inner_floor;
+        ctx->mixed_sweeps = max_sweeps;
+    });
+}
+
 int fftperm_recover(fftperm_ctx* ctx, const double grad[3], const double* x, double* u, double avg[3]) {
     return guarded([&] {
         check_ctx(ctx);
@@ -1101,6 +1201,32 @@ int fftperm_symmetry_gap(fftperm_ctx* ctx, const double* x, const double* y, dou
     });
 }
 
+int fftperm_pin_variant(fftperm_ctx* ctx, int kind, int precision, int code) {
+    return guarded([&] {
+        check_ctx(ctx);
+        if (kind < kKindXfwd || kind > kKindZgreen) throw Err(FFTPERM_EINVAL, "unknown pass kind");
+        if (precision != FFTPERM_PRECISION_FP64 && precision != FFTPERM_PRECISION_MIXED)
+            throw Err(FFTPERM_EINVAL, "unknown precision mode");
+        const Geometry3& g = ctx->g;
+        const int prec = precision == FFTPERM_PRECISION_MIXED ? 1 : 0;
+        int n = 0;
+        long count = 0;
+        switch (kind) {
+            case kKindXfwd:
+            case kKindXinv: n = g.nx; count = (long)g.ny * g.nz; break;
+            case kKindCol: n = g.ny; count = (long)g.nz * g.nxh; break;
+            default: n = g.nz; count = (long)g.ny * g.nxh; break;
+        }
+        if (code == 0) {
+            clear_tuned_variant(kind, prec, n, count);
+            return;
+        }
+        const int v = variant_of_code(kind, n, code);
+        if (v < 0) throw Err(FFTPERM_EINVAL, "unknown variant code " + std::to_string(code));
+        set_tuned_variant(kind, prec, n, count, v);
+    });
+}
+
 int fftperm_profile(fftperm_ctx* ctx, int enable) {
     return guarded([&] {
         check_ctx(ctx);
@@ -1127,7 +1253,13 @@ int fftperm_profile_read(fftperm_ctx* ctx, fftperm_profile_stats* out) {
         out->N = (uint64_t)ctx->g.N;
         out->Nh = (uint64_t)ctx->g.Nh;
         out->nb = ctx->nb;
-        for (int i = 0; i < 2; ++i) out->zgreen_tile[i] = zgreen_tile(i, ctx->g.nz, (long)ctx->g.ny * ctx->g.nxh);
+        const Geometry3& g = ctx->g;
+        for (int i = 0; i < 2; ++i) {
+            out->zgreen_tile[i] = variant_code(kKindZgreen, i, g.nz, (long)g.ny * g.nxh);
+            out->xfwd_tile[i] = variant_code(kKindXfwd, i, g.nx, (long)g.ny * g.nz);
+            out->xinv_tile[i] = variant_code(kKindXinv, i, g.nx, (long)g.ny * g.nz);
+            out->ypass_tile[i] = variant_code(kKindCol, i, g.ny, (long)g.nz * g.nxh);
+        }
     });
 }
 

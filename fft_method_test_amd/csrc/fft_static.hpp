@@ -110,6 +110,35 @@ constexpr int threads_per_line() {
     return N / PlanInfo<typename StaticPlan<N>::type>::max_radix();
 }
 
+// Tile variants of the x and column passes.  Variant v in [0, kPassVariants) puts LINES lines in
+// a workgroup with OS threads per butterfly thread (the extra threads only widen the memory
+// phases); code = 10 * LINES + OS names it in timings.  LINES starts from the count that gives
+// 256 butterfly threads and is halved twice.
+constexpr int kPassVariants = 5;
+
+template <int N>
+constexpr int pass_base_lines() {
+    return (256 / threads_per_line<N>()) > 2 ? 256 / threads_per_line<N>() : 2;
+}
+
+template <int N>
+constexpr int pass_lines(int v) {
+    return v < 2 ? pass_base_lines<N>() : (v < 4 ? pass_base_lines<N>() / 2 : pass_base_lines<N>() / 4);
+}
+
+constexpr int pass_os(int v) { return (v == 0 || v == 2) ? 2 : 4; }
+
+template <typename R, int N, int LINES, int OS>
+struct PassShape {
+    static constexpr int T = threads_per_line<N>();
+    static constexpr int LD = N + 1;
+    static constexpr int THREADS = OS * LINES * T;
+    static constexpr size_t LDS = (size_t)(N + LINES * LD) * sizeof(cx<R>);
+    // every thread owns KE values of each line element set
+    static constexpr bool valid = LINES >= 2 && LINES % 2 == 0 && THREADS >= 64 && THREADS <= 1024 &&
+                                  (LINES * N) % THREADS == 0 && LDS <= 100u * 1024u;
+};
+
 // Block index remap so that consecutive tiles land on the same XCD (workgroups are dealt to
 // the 8 XCDs round-robin; each XCD has its own L2).
 __device__ __forceinline__ long xcd_tile(long b, long nblocks) {

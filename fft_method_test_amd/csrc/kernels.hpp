@@ -27,16 +27,14 @@ struct XSource {
 // Sink of the x-inverse pass.  value = Re(line) * inv_n.
 //   compact  != null : compact[3*l + c] = value on support voxels
 //   field    != null : field[c*N + v] = value
-//   partials != null : per-block sums (kXinvParts wide): {support sums x,y,z, all-voxel sums x,y,z,
-//                      sum over the support of dotv[3l+c] * value (when dotv != null), 0}
+//   partials != null : per-block sums (kXinvParts wide): {support sums x,y,z, all-voxel sums x,y,z}
 struct XSink {
     const int32_t* code;
     double* compact;
     double* field;
     double* partials;
-    const double* dotv;
 };
-constexpr int kXinvParts = 8;
+constexpr int kXinvParts = 6;
 
 struct PassCfg {
     int lines;
@@ -67,8 +65,9 @@ size_t lds_limit();
 template <typename R>
 void launch_xfwd(const XSource& src, cx<R>* S, const Geometry3& g, const FFTTables<R>& t,
                  const int* skip, hipStream_t st);
+// Returns the number of workgroups (= partial-sum rows written when sink.partials != null).
 template <typename R>
-void launch_xinv(const cx<R>* S, const XSink& sink, const Geometry3& g, const FFTTables<R>& t,
+long launch_xinv(const cx<R>* S, const XSink& sink, const Geometry3& g, const FFTTables<R>& t,
                  const int* skip, hipStream_t st);
 // y-axis transform of all three components (sign -1 forward, +1 backward).
 template <typename R>
@@ -80,19 +79,57 @@ void launch_zpass(cx<R>* S, int sign, const Geometry3& g, const FFTTables<R>& t,
                   hipStream_t st);
 // Fused z forward transform, symmetric Green multiply, z backward transform.
 // G6 = the symmetrised tensor on the half spectrum, AoS: G6[6*i + g], g = xx,xy,xz,yy,yz,zz.
+// qpart != null: one partial per workgroup of the Hermitian-weighted quadratic form
+// sum_k w(kx) Re(conj(f(k)) . G(k) f(k)) over the forward spectrum f (w = 1 on the
+// self-conjugate planes kx = 0 and 2 kx = nx, else 2); divided by N it is <u, G u> over the
+// grid for the real field u that was transformed (Parseval).  Returns the workgroup count.
 template <typename R>
-void launch_zgreen(cx<R>* S, const R* G6, const Geometry3& g, const FFTTables<R>& t, const int* skip,
-                   hipStream_t st);
-long xinv_blocks(const Geometry3& g, const LinePlan& px);
-// Times every tile width of the fused z pass that fits the device on the given buffers and
-// caches the fastest for (precision, nz, columns); returns (width, ms) pairs (empty when the
-// size has no specialised kernel or was tuned before).  Overwrites S.
-// Autotuned z-pass variant (10 * columns per workgroup + threads per butterfly) for (precision
-// 0 = FP64 / 1 = FP32, nz, columns), 0 if untuned.
-int zgreen_tile(int prec, int n, long ncols);
+long launch_zgreen(cx<R>* S, const R* G6, const Geometry3& g, const FFTTables<R>& t, double* qpart,
+                   const int* skip, hipStream_t st);
+// Upper bounds of the workgroup counts of launch_xinv / launch_zgreen over every variant.
+long xinv_max_blocks(const Geometry3& g);
+long zgreen_max_blocks(const Geometry3& g);
+
+// Specialised passes (x-line / column lengths 32 ... 512) in kPassVariants tile shapes; each
+// returns the workgroup count, or -1 when the size or variant has no kernel on this device.
+bool static_passes_ok();
+template <typename R>
+long static_xfwd(int v, const XSource& src, cx<R>* S, const Geometry3& g, const cx<R>* tw, const int* skip,
+                 hipStream_t st);
+template <typename R>
+long static_xinv(int v, const cx<R>* S, const XSink& sink, const Geometry3& g, const cx<R>* tw, const int* skip,
+                 hipStream_t st);
+template <typename R>
+long static_col(int v, int n, int sign, cx<R>* S, long ncols, long inner, long outer, long estride, long comp_stride,
+                const cx<R>* tw, const int* skip, hipStream_t st);
+// 10 * lines per workgroup + threads per butterfly thread of variant v at length n (0 if none).
+int pass_variant_code(int n, int v);
+
+// Autotuner registry, keyed by (pass kind, precision 0 = FP64 / 1 = FP32, transform length,
+// line or column count).  Values: pass variant index for the x and column passes, variant code
+// (10 * columns per workgroup + threads per butterfly thread) for the fused z pass; -1 untuned.
+enum PassKind { kKindXfwd = 0, kKindXinv = 1, kKindCol = 2, kKindZgreen = 3 };
+int tuned_variant(int kind, int prec, int n, long count);
+void set_tuned_variant(int kind, int prec, int n, long count, int v);
+void clear_tuned_variant(int kind, int prec, int n, long count);
+// Registry value of a report code (pass variant index or z-pass code), -1 when unknown.
+int variant_of_code(int kind, int n, int code);
+// Report code of the variant a launch of this kind and size uses now (0 = runtime kernel).
+int variant_code(int kind, int prec, int n, long count);
+
+// Each autotuner times every variant that fits the device on the given buffers (3 launches
+// after a warm-up), records the fastest and returns (code, ms) pairs; empty when the size has
+// no specialised kernel or was tuned before in this process.  They overwrite S (and the sink
+// outputs of the x-inverse).
 template <typename R>
 std::vector<std::pair<int, float>> autotune_zgreen(cx<R>* S, const R* G6, const Geometry3& g,
                                                    const FFTTables<R>& t, hipStream_t st);
+template <typename R>
+std::vector<std::pair<int, float>> autotune_col(cx<R>* S, const Geometry3& g, const FFTTables<R>& t,
+                                                hipStream_t st);
+template <typename R>
+std::vector<std::pair<int, float>> autotune_x(const XSource& src, const XSink& sink, cx<R>* S, const Geometry3& g,
+                                              const FFTTables<R>& t, hipStream_t st);
 
 // ------------------------------------------------------------ setup kernels
 // Support classification + ascending compaction.  Fills code (N) and active (NB);

@@ -15,6 +15,7 @@
 #include "fft_engine.hpp"
 #include "fft_static.hpp"
 #include "kernels.hpp"
+#include "zgreen_reg.hpp"
 
 #include <algorithm>
 #include <map>
@@ -70,305 +71,43 @@ size_t lds_limit() { return g_lds_max; }
 
 
 // =====================================================================================
-// Compile-time specialised passes for N in {32, 64, 128, 256, 512}.
+// Compile-time specialised fused z pass for N in {32, 64, 128, 256, 512}; the specialised x and
+// column passes live in passes_x.hip and passes_col.hip.
 // =====================================================================================
 namespace {
 
 template <typename R, int N>
-struct Shape {
-    static constexpr int T = threads_per_line<N>();                  // threads per line
-    static constexpr int LD = N + 1;                                  // LDS pitch (complex values)
-    static constexpr int LINES = (256 / T) > 0 ? (256 / T) : 1;       // x / column passes
-    // twice the butterfly threads: the extra waves keep loads and stores in flight while other
-    // workgroups of the CU are in their LDS stages
-    static constexpr int THREADS = 2 * LINES * T <= 1024 ? 2 * LINES * T : LINES * T;
-    static constexpr size_t LDS1 = (size_t)(N + LINES * LD) * sizeof(cx<R>);
-    // fused z pass: three components of ZL columns in LDS, at most ~100 KB and 1024 threads
+struct ZShape {
+    static constexpr int T = threads_per_line<N>();  // threads per line
+    static constexpr int LD = N + 1;                  // LDS pitch (complex values)
+    // three components of ZL columns in LDS, at most ~100 KB and 1024 threads
     static constexpr int zl_raw() {
         int z = 16;
         while (z > 1 && ((size_t)(3 * z * LD + N) * sizeof(cx<R>) > 102400u || 3 * z * T > 1024)) z /= 2;
         return z;
     }
     static constexpr int ZL = zl_raw();
-    static constexpr int ZTHREADS = 3 * ZL * T;
-    static constexpr size_t LDSZ = (size_t)(N + 3 * ZL * LD) * sizeof(cx<R>);
 };
 
-// Real x-lines are transformed two at a time as one complex line z = a + i b
-// ("two for one"): round A packs components x and y of each line, round B packs component z of
-// lines 2m and 2m+1.  The half spectra follow from A[k] = (Z[k] + conj Z[N-k]) / 2 and
-// B[k] = (Z[k] - conj Z[N-k]) / 2i, which halves the FFT and LDS work of the R2C pass.
-template <typename R, int N>
-__global__ __launch_bounds__((Shape<R, N>::THREADS)) void k_xfwd_s(XSource src, cx<R>* __restrict__ S, long nlines,
-                                                                     long N3, long Nh, const cx<R>* __restrict__ twg,
-                                                                     const int* skip) {
-    if (skip != nullptr && *skip) return;
-    using SH = Shape<R, N>;
-    constexpr int LINES = SH::LINES, LD = SH::LD, THREADS = SH::THREADS, NXH = N / 2 + 1;
-    constexpr int KE = LINES * N / THREADS;  // elements per thread per component
-    static_assert((LINES * N) % THREADS == 0 && LINES % 2 == 0, "element split");
-    extern __shared__ unsigned char smem_raw[];
-    cx<R>* tw = reinterpret_cast<cx<R>*>(smem_raw);
-    cx<R>* buf = tw + N;
-    for (int i = threadIdx.x; i < N; i += THREADS) tw[i] = twg[i];
-    const long line0 = xcd_tile(blockIdx.x, gridDim.x) * LINES;
-    const int nl = (int)min((long)LINES, nlines - line0);
-    // every global load of the block is issued up front: the support code once per voxel, then
-    // the three compact components (or the three field components) of each owned voxel
-    double v[3][KE];
-    if (src.field != nullptr) {
-#pragma unroll
-        for (int k = 0; k < KE; ++k) {
-            const int idx = threadIdx.x + k * THREADS;
-            const int line = idx / N;
-            const long vox = (line0 + line) * N + idx % N;
-            const bool ok = line < nl;
-#pragma unroll
-            for (int c = 0; c < 3; ++c) v[c][k] = ok ? src.field[c * N3 + vox] : 0.0;
-        }
-    } else {
-        int code[KE];
-#pragma unroll
-        for (int k = 0; k < KE; ++k) {
-            const int idx = threadIdx.x + k * THREADS;
-            const int line = idx / N;
-            code[k] = (line < nl) ? src.code[(line0 + line) * N + idx % N] : -2;
-        }
-        double mc[3];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) mc[c] = (src.mean != nullptr) ? src.mean[c] : 0.0;
-#pragma unroll
-        for (int k = 0; k < KE; ++k) {
-            const int cd = code[k];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                const double f0 = (cd == -1) ? src.a_fluid[c] : ((cd >= 0) ? src.a_support[c] : 0.0);
-                double fx = 0.0;
-                if (src.compact != nullptr && cd >= 0) fx = src.compact[3L * cd + c] - mc[c];
-                v[c][k] = f0 + fx;
-            }
-        }
-    }
-    // round A: z = x + i y per line
-#pragma unroll
-    for (int k = 0; k < KE; ++k) {
-        const int idx = threadIdx.x + k * THREADS;
-        buf[(idx / N) * LD + idx % N] = mkc<R>((R)v[0][k], (R)v[1][k]);
-    }
-    __syncthreads();
-    sfft_lines<R, -1, N, LINES, SH::T, LD>(buf, tw, nl);
-    for (int idx = threadIdx.x; idx < LINES * NXH; idx += THREADS) {
-        const int line = idx / NXH;
-        const int kx = idx % NXH;
-        if (line < nl) {
-            const cx<R> zk = buf[line * LD + kx];
-            const cx<R> zm = buf[line * LD + (N - kx) % N];
-            const long o = (line0 + line) * NXH + kx;
-            S[o] = mkc<R>((R)0.5 * (zk.x + zm.x), (R)0.5 * (zk.y - zm.y));
-            S[Nh + o] = mkc<R>((R)0.5 * (zk.y + zm.y), (R)0.5 * (zm.x - zk.x));
-        }
-    }
-    __syncthreads();
-    // round B: z = z_{2m} + i z_{2m+1} (component z of two adjacent lines)
-    R* bufr = reinterpret_cast<R*>(buf);
-#pragma unroll
-    for (int k = 0; k < KE; ++k) {
-        const int idx = threadIdx.x + k * THREADS;
-        const int line = idx / N;
-        bufr[2 * ((line >> 1) * LD + idx % N) + (line & 1)] = (R)v[2][k];
-    }
-    __syncthreads();
-    const int nl2 = (nl + 1) / 2;
-    sfft_lines<R, -1, N, LINES, SH::T, LD>(buf, tw, nl2);
-    cx<R>* Sz = S + 2 * Nh;
-    for (int idx = threadIdx.x; idx < (LINES / 2) * NXH; idx += THREADS) {
-        const int m = idx / NXH;
-        const int kx = idx % NXH;
-        const cx<R> zk = buf[m * LD + kx];
-        const cx<R> zm = buf[m * LD + (N - kx) % N];
-        const int la = 2 * m, lb = 2 * m + 1;
-        if (la < nl) Sz[(line0 + la) * NXH + kx] = mkc<R>((R)0.5 * (zk.x + zm.x), (R)0.5 * (zk.y - zm.y));
-        if (lb < nl) Sz[(line0 + lb) * NXH + kx] = mkc<R>((R)0.5 * (zk.y + zm.y), (R)0.5 * (zm.x - zk.x));
-    }
-}
-
-// Z[k] = A + iB and Z[N-k] = conj(A) + i conj(B) for 0 < k < N - k; the self-conjugate bins keep
-// their real parts only.
-template <typename R, int N, int LD>
-__device__ __forceinline__ void xinv_put(cx<R>* buf, int line, int kx, cx<R> a, cx<R> b) {
-    const bool self = (kx == 0 || 2 * kx == N);
-    if (self) {
-        a.y = (R)0;
-        b.y = (R)0;
-    }
-    buf[line * LD + kx] = mkc<R>(a.x - b.y, a.y + b.x);
-    if (!self) buf[line * LD + (N - kx)] = mkc<R>(a.x + b.y, b.x - a.y);
-}
-
-// C2R of two half spectra at once: Z[k] = A[k] + i B[k] over the Hermitian completions of A and
-// B, then a = Re ifft(Z), b = Im ifft(Z).  The imaginary parts of the self-conjugate bins (0 and
-// N/2) are dropped, which is what taking the real part of a full C2C inverse does
-// (src/fft/FFTWBackend.cpp:292).
-template <typename R, int N>
-__global__ __launch_bounds__((Shape<R, N>::THREADS)) void k_xinv_s(const cx<R>* __restrict__ S, XSink sink, long nlines,
-                                                                     long N3, long Nh, const cx<R>* __restrict__ twg,
-                                                                     double inv_n, const int* skip) {
-    if (skip != nullptr && *skip) return;
-    using SH = Shape<R, N>;
-    constexpr int LINES = SH::LINES, LD = SH::LD, THREADS = SH::THREADS, NXH = N / 2 + 1;
-    constexpr int KE = LINES * N / THREADS;
-    constexpr int KA = (LINES * NXH + THREADS - 1) / THREADS;        // round A loads per thread
-    constexpr int KB = ((LINES / 2) * NXH + THREADS - 1) / THREADS;  // round B loads per thread
-    extern __shared__ unsigned char smem_raw[];
-    cx<R>* tw = reinterpret_cast<cx<R>*>(smem_raw);
-    cx<R>* buf = tw + N;
-    for (int i = threadIdx.x; i < N; i += THREADS) tw[i] = twg[i];
-    const long line0 = xcd_tile(blockIdx.x, gridDim.x) * LINES;
-    const int nl = (int)min((long)LINES, nlines - line0);
-    const cx<R> zero = mkc<R>((R)0, (R)0);
-    // all global loads up front: support codes, the x and y half spectra, component z, then the v
-    // values of the alpha dot product (these depend on the codes, so they go last)
-    int code[KE];
-#pragma unroll
-    for (int k = 0; k < KE; ++k) {
-        const int idx = threadIdx.x + k * THREADS;
-        const int line = idx / N;
-        code[k] = (line < nl) ? sink.code[(line0 + line) * N + idx % N] : -2;
-    }
-    cx<R> ha[KA], hb[KA];
-#pragma unroll
-    for (int k = 0; k < KA; ++k) {
-        const int idx = threadIdx.x + k * THREADS;
-        const int line = idx / NXH;
-        const bool ok = idx < LINES * NXH && line < nl;
-        const long o = (line0 + line) * NXH + idx % NXH;
-        ha[k] = ok ? S[o] : zero;
-        hb[k] = ok ? S[Nh + o] : zero;
-    }
-    cx<R> ga[KB], gb[KB];
-#pragma unroll
-    for (int k = 0; k < KB; ++k) {
-        const int idx = threadIdx.x + k * THREADS;
-        const int m = idx / NXH;
-        const int kx = idx % NXH;
-        const bool in = idx < (LINES / 2) * NXH;
-        ga[k] = (in && 2 * m < nl) ? S[2 * Nh + (line0 + 2 * m) * NXH + kx] : zero;
-        gb[k] = (in && 2 * m + 1 < nl) ? S[2 * Nh + (line0 + 2 * m + 1) * NXH + kx] : zero;
-    }
-    double dv[3][KE];
-#pragma unroll
-    for (int k = 0; k < KE; ++k) {
-        const int cd = code[k];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) dv[c][k] = (sink.dotv != nullptr && cd >= 0) ? sink.dotv[3L * cd + c] : 0.0;
-    }
-    double out[3][KE];
-    // round A: components x and y
-#pragma unroll
-    for (int k = 0; k < KA; ++k) {
-        const int idx = threadIdx.x + k * THREADS;
-        if (idx < LINES * NXH) xinv_put<R, N, LD>(buf, idx / NXH, idx % NXH, ha[k], hb[k]);
-    }
-    __syncthreads();
-    sfft_lines<R, +1, N, LINES, SH::T, LD>(buf, tw, nl);
-#pragma unroll
-    for (int k = 0; k < KE; ++k) {
-        const int idx = threadIdx.x + k * THREADS;
-        const cx<R> z = buf[(idx / N) * LD + idx % N];
-        out[0][k] = (double)z.x * inv_n;
-        out[1][k] = (double)z.y * inv_n;
-    }
-    __syncthreads();
-    // round B: component z of lines 2m (real part) and 2m+1 (imaginary part)
-#pragma unroll
-    for (int k = 0; k < KB; ++k) {
-        const int idx = threadIdx.x + k * THREADS;
-        if (idx < (LINES / 2) * NXH) xinv_put<R, N, LD>(buf, idx / NXH, idx % NXH, ga[k], gb[k]);
-    }
-    __syncthreads();
-    sfft_lines<R, +1, N, LINES, SH::T, LD>(buf, tw, (nl + 1) / 2);
-    const R* bufr = reinterpret_cast<const R*>(buf);
-#pragma unroll
-    for (int k = 0; k < KE; ++k) {
-        const int idx = threadIdx.x + k * THREADS;
-        const int line = idx / N;
-        out[2][k] = (double)bufr[2 * ((line >> 1) * LD + idx % N) + (line & 1)] * inv_n;
-    }
-    double acc[kXinvParts] = {0, 0, 0, 0, 0, 0, 0, 0};
-#pragma unroll
-    for (int k = 0; k < KE; ++k) {
-        const int idx = threadIdx.x + k * THREADS;
-        const int line = idx / N;
-        if (line >= nl) continue;
-        const long vox = (line0 + line) * N + idx % N;
-        if (sink.field != nullptr) {
-#pragma unroll
-            for (int c = 0; c < 3; ++c) sink.field[c * N3 + vox] = out[c][k];
-        }
-        const int cd = code[k];
-        if (cd >= 0) {
-            if (sink.compact != nullptr) {
-#pragma unroll
-                for (int c = 0; c < 3; ++c) sink.compact[3L * cd + c] = out[c][k];
-            }
-#pragma unroll
-            for (int c = 0; c < 3; ++c) acc[6] += dv[c][k] * out[c][k];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) acc[c] += out[c][k];
-        }
-#pragma unroll
-        for (int c = 0; c < 3; ++c) acc[3 + c] += out[c][k];
-    }
-    if (sink.partials != nullptr) block_sum<kXinvParts>(acc, sink.partials + (long)kXinvParts * blockIdx.x);
-}
-
-// Column pass: column col has base (col / inner) * outer_stride + col % inner, element e at
-// base + e * estride.  Thread t always serves column t % LINES in the load/store loops.
-template <typename R, int N, int SIGN>
-__global__ __launch_bounds__((Shape<R, N>::THREADS)) void k_col_s(cx<R>* __restrict__ S, long ncols, long inner,
-                                                                  long outer_stride, long estride, long comp_stride,
-                                                                  const cx<R>* __restrict__ twg, const int* skip) {
-    if (skip != nullptr && *skip) return;
-    using SH = Shape<R, N>;
-    constexpr int LINES = SH::LINES, LD = SH::LD, THREADS = SH::THREADS;
-    constexpr int EST = THREADS / LINES;
-    extern __shared__ unsigned char smem_raw[];
-    cx<R>* tw = reinterpret_cast<cx<R>*>(smem_raw);
-    cx<R>* buf = tw + N;
-    for (int i = threadIdx.x; i < N; i += THREADS) tw[i] = twg[i];
-    const long col0 = xcd_tile(blockIdx.x, gridDim.x) * LINES;
-    const int ncl = (int)min((long)LINES, ncols - col0);
-    cx<R>* Sc = S + blockIdx.y * comp_stride;
-    const int ci = threadIdx.x % LINES;
-    const long col = col0 + ci;
-    const unsigned cu = (unsigned)col, iu = (unsigned)inner;
-    const long base = (long)(cu / iu) * outer_stride + (long)(cu % iu);
-    const bool own = ci < ncl;
-#pragma unroll 4
-    for (int e = threadIdx.x / LINES; e < N; e += EST)
-        if (own) buf[ci * LD + e] = Sc[base + e * estride];
-    __syncthreads();
-    sfft_lines<R, SIGN, N, LINES, SH::T, LD>(buf, tw, ncl);
-#pragma unroll 4
-    for (int e = threadIdx.x / LINES; e < N; e += EST)
-        if (own) Sc[base + e * estride] = buf[ci * LD + e];
-}
+// Hermitian weight of column kx of the half spectrum in a sum over the full spectrum.
+__device__ __forceinline__ double herm_weight(int kx, int nx) { return (kx == 0 || 2 * kx == nx) ? 1.0 : 2.0; }
 
 // Fused z pass: lines 0..3*ZL-1 of the LDS buffer hold components x | y | z of ZL columns.
 // OS = threads per butterfly thread (1 or 2): extra threads only help the memory phases
 template <typename R, int N, int ZL, int OS>
 constexpr int zgreen_threads() {
-    return OS * 3 * ZL * Shape<R, N>::T;
+    return OS * 3 * ZL * ZShape<R, N>::T;
 }
 
 template <typename R, int N, int ZL, int OS>
 __global__ __launch_bounds__((zgreen_threads<R, N, ZL, OS>())) void k_zgreen_s(cx<R>* __restrict__ S,
                                                                                  const R* __restrict__ G6, long ncols,
-                                                                                 long Nh,
+                                                                                 long Nh, int nx, int nxh,
                                                                                  const cx<R>* __restrict__ twg,
+                                                                                 double* __restrict__ qpart,
                                                                                  const int* skip) {
     if (skip != nullptr && *skip) return;
-    using SH = Shape<R, N>;
+    using SH = ZShape<R, N>;
     constexpr int LD = SH::LD, THREADS = zgreen_threads<R, N, ZL, OS>(), L3 = 3 * ZL;
     constexpr int EST = THREADS / L3;
     extern __shared__ unsigned char smem_raw[];
@@ -377,254 +116,410 @@ __global__ __launch_bounds__((zgreen_threads<R, N, ZL, OS>())) void k_zgreen_s(c
     for (int i = threadIdx.x; i < N; i += THREADS) tw[i] = twg[i];
     const long col0 = xcd_tile(blockIdx.x, gridDim.x) * ZL;
     const int ncl = (int)min((long)ZL, ncols - col0);
-    // loads: thread t serves line t % L3 = comp * ZL + ci
+    // loads: thread t serves line t % L3 = comp * ZL + ci, elements t / L3 + k * EST
     const int lt = threadIdx.x % L3;
     const int comp = lt / ZL;
     const int ci = lt % ZL;
     const bool own = ci < ncl;
+    constexpr int KL = N / EST;
+    static_assert(N % EST == 0, "column elements split evenly over the load threads");
     const cx<R>* Sc = S + comp * Nh + col0 + ci;
-#pragma unroll 4
-    for (int e = threadIdx.x / L3; e < N; e += EST)
-        buf[lt * LD + e] = own ? Sc[e * ncols] : mkc<R>((R)0, (R)0);
+    cx<R> ld[KL];
+#pragma unroll
+    for (int k = 0; k < KL; ++k) {
+        ld[k] = mkc<R>((R)0, (R)0);
+        if (own) ld[k] = Sc[(long)(threadIdx.x / L3 + k * EST) * ncols];
+    }
+    // the Green tensor of the frequencies this thread multiplies, fetched now so that its latency
+    // overlaps the column loads and the forward transform
+    constexpr int MT = THREADS / ZL;
+    constexpr int KM = (N + MT - 1) / MT;
+    const int mci = threadIdx.x % ZL;
+    const bool mown = mci < ncl;
+    R gt[KM][6];
+    {
+        const R* Gc = G6 + 6L * (col0 + mci);
+#pragma unroll
+        for (int m = 0; m < KM; ++m) {
+            const int e = threadIdx.x / ZL + m * MT;
+#pragma unroll
+            for (int q = 0; q < 6; ++q) gt[m][q] = (R)0;
+            if (mown && e < N) {
+                const R* gp = Gc + 6L * e * ncols;
+#pragma unroll
+                for (int q = 0; q < 6; ++q) gt[m][q] = gp[q];
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < KL; ++k) buf[lt * LD + threadIdx.x / L3 + k * EST] = ld[k];
     __syncthreads();
     sfft_lines<R, -1, N, L3, SH::T, LD>(buf, tw, L3);
-    // symmetric Green multiply (src/bignonnet/GreenOperator.cpp:84-92 with G_sym)
+    // symmetric Green multiply (src/bignonnet/GreenOperator.cpp:84-92 with G_sym), and the
+    // quadratic form conj(f) . G f of the forward spectrum when asked for
     {
-        constexpr int MT = THREADS / ZL;
-        const int mci = threadIdx.x % ZL;
-        const bool mown = mci < ncl;
-        const R* Gc = G6 + 6L * (col0 + mci);
-#pragma unroll 2
-        for (int e = threadIdx.x / ZL; e < N; e += MT) {
-            if (!mown) continue;
-            const R* gp = Gc + 6L * e * ncols;
-            const R txx = gp[0], txy = gp[1], txz = gp[2];
-            const R tyy = gp[3], tyz = gp[4], tzz = gp[5];
+        double q[1] = {0.0};
+#pragma unroll
+        for (int m = 0; m < KM; ++m) {
+            const int e = threadIdx.x / ZL + m * MT;
+            if (!mown || e >= N) continue;
+            const R txx = gt[m][0], txy = gt[m][1], txz = gt[m][2];
+            const R tyy = gt[m][3], tyz = gt[m][4], tzz = gt[m][5];
             cx<R>* px = buf + (0 * ZL + mci) * LD + e;
             cx<R>* py = buf + (1 * ZL + mci) * LD + e;
             cx<R>* pz = buf + (2 * ZL + mci) * LD + e;
             const cx<R> fx = *px, fy = *py, fz = *pz;
-            *px = mkc<R>(txx * fx.x + txy * fy.x + txz * fz.x, txx * fx.y + txy * fy.y + txz * fz.y);
-            *py = mkc<R>(txy * fx.x + tyy * fy.x + tyz * fz.x, txy * fx.y + tyy * fy.y + tyz * fz.y);
-            *pz = mkc<R>(txz * fx.x + tyz * fy.x + tzz * fz.x, txz * fx.y + tyz * fy.y + tzz * fz.y);
+            const cx<R> gx = mkc<R>(txx * fx.x + txy * fy.x + txz * fz.x, txx * fx.y + txy * fy.y + txz * fz.y);
+            const cx<R> gy = mkc<R>(txy * fx.x + tyy * fy.x + tyz * fz.x, txy * fx.y + tyy * fy.y + tyz * fz.y);
+            const cx<R> gz = mkc<R>(txz * fx.x + tyz * fy.x + tzz * fz.x, txz * fx.y + tyz * fy.y + tzz * fz.y);
+            *px = gx;
+            *py = gy;
+            *pz = gz;
+            if (qpart != nullptr)
+                q[0] += (double)fx.x * gx.x + (double)fx.y * gx.y + (double)fy.x * gy.x + (double)fy.y * gy.y +
+                        (double)fz.x * gz.x + (double)fz.y * gz.y;
+        }
+        if (qpart != nullptr) {
+            q[0] *= herm_weight((int)((col0 + mci) % nxh), nx);
+            block_sum<1>(q, qpart + blockIdx.x);
         }
     }
     __syncthreads();
     sfft_lines<R, +1, N, L3, SH::T, LD>(buf, tw, L3);
     cx<R>* Sw = S + comp * Nh + col0 + ci;
-#pragma unroll 4
-    for (int e = threadIdx.x / L3; e < N; e += EST)
-        if (own) Sw[e * ncols] = buf[lt * LD + e];
+#pragma unroll
+    for (int k = 0; k < KL; ++k) {
+        const int e = threadIdx.x / L3 + k * EST;
+        if (own) Sw[(long)e * ncols] = buf[lt * LD + e];
+    }
 }
 
-static_assert(Shape<double, 256>::LINES == Shape<float, 256>::LINES, "x-pass shape independent of precision");
-
-template <typename R>
-bool static_xfwd(int n, const XSource& src, cx<R>* S, const Geometry3& g, const FFTTables<R>& t, const int* skip,
-                 hipStream_t st) {
-    const long nlines = (long)g.ny * g.nz;
-#define FFTPERM_XF(NN)                                                                                          \
-    case NN: {                                                                                                  \
-        using SH = Shape<R, NN>;                                                                                \
-        const long blocks = (nlines + SH::LINES - 1) / SH::LINES;                                              \
-        hipLaunchKernelGGL((k_xfwd_s<R, NN>), dim3((unsigned)blocks), dim3(SH::THREADS), SH::LDS1, st, src, S,  \
-                           nlines, g.N, g.Nh, t.twx, skip);                                                     \
-        return true;                                                                                            \
+// Register-resident variant (zgreen_reg.hpp): ZL columns, N/4 threads per column.
+template <typename R, int N, int ZL>
+__global__ __launch_bounds__(ZL * N / 4) void k_zgreen_r(cx<R>* __restrict__ S, const R* __restrict__ G6, long ncols,
+                                                         long Nh, int nx, int nxh, const cx<R>* __restrict__ twg,
+                                                         double* __restrict__ qpart, const int* skip) {
+    if (skip != nullptr && *skip) return;
+    constexpr int THREADS = ZL * N / 4, LD = N + 1, Q = N / 4;
+    extern __shared__ unsigned char smem_raw[];
+    cx<R>* tw = reinterpret_cast<cx<R>*>(smem_raw);
+    cx<R>* buf = tw + N;
+    for (int i = threadIdx.x; i < N; i += THREADS) tw[i] = twg[i];
+    const long col0 = xcd_tile(blockIdx.x, gridDim.x) * ZL;
+    const int ncl = (int)min((long)ZL, ncols - col0);
+    const int ci = threadIdx.x % ZL;
+    const int j = threadIdx.x / ZL;
+    const bool own = ci < ncl;
+    const long colg = col0 + ci;
+    // element offsets inside one component (Nh < 2^31 for every grid this kernel serves) off a
+    // wave-uniform component base: one 32-bit offset per lane instead of a 64-bit address per access
+    const int nc = (int)ncols;
+    const int off0 = (int)colg + j * nc;
+    // element j + r N/4 of the three components
+    cx<R> f[3][4];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const cx<R>* Sc = S + c * Nh;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            f[c][r] = mkc<R>((R)0, (R)0);
+            if (own) f[c][r] = Sc[off0 + r * Q * nc];
+        }
     }
-    switch (n) { FFTPERM_XF(32) FFTPERM_XF(64) FFTPERM_XF(128) FFTPERM_XF(256) FFTPERM_XF(512) default: return false; }
-#undef FFTPERM_XF
+    __syncthreads();
+    cx<R>* col = buf + ci * LD;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) rfft_fwd<R, N>(f[c], col, tw, j);
+    // symmetric Green multiply (src/bignonnet/GreenOperator.cpp:84-92 with G_sym) on registers; the
+    // tensor is read here rather than up front, which keeps the workgroup's register footprint
+    // small enough for several workgroups per CU (they hide each other's load latency)
+    double qf[1] = {0.0};
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const cx<R> fx = f[0][r], fy = f[1][r], fz = f[2][r];
+        R t6[6];
+        const R* gp = G6 + 6L * (off0 + r * Q * nc);
+#pragma unroll
+        for (int q = 0; q < 6; ++q) t6[q] = own ? gp[q] : (R)0;
+        const R txx = t6[0], txy = t6[1], txz = t6[2], tyy = t6[3], tyz = t6[4], tzz = t6[5];
+        const cx<R> gx = mkc<R>(txx * fx.x + txy * fy.x + txz * fz.x, txx * fx.y + txy * fy.y + txz * fz.y);
+        const cx<R> gy = mkc<R>(txy * fx.x + tyy * fy.x + tyz * fz.x, txy * fx.y + tyy * fy.y + tyz * fz.y);
+        const cx<R> gz = mkc<R>(txz * fx.x + tyz * fy.x + tzz * fz.x, txz * fx.y + tyz * fy.y + tzz * fz.y);
+        if (qpart != nullptr)
+            qf[0] += (double)fx.x * gx.x + (double)fx.y * gx.y + (double)fy.x * gy.x + (double)fy.y * gy.y +
+                     (double)fz.x * gz.x + (double)fz.y * gz.y;
+        f[0][r] = gx;
+        f[1][r] = gy;
+        f[2][r] = gz;
+        // two frequencies per scheduling group: the tensor loads of the next pair are not hoisted
+        // above this point, which caps the live registers (FP64: 4 x 6 tensor values would not fit
+        // next to the 12 spectrum values at two workgroups per CU)
+        if (r % 2 == 1) __builtin_amdgcn_sched_barrier(0);
+    }
+    if (qpart != nullptr) {
+        qf[0] = own ? qf[0] * herm_weight((int)(colg % nxh), nx) : 0.0;
+        block_sum<1>(qf, qpart + blockIdx.x);
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        cx<R>* Sc = S + c * Nh;
+        rfft_bwd<R, N>(f[c], col, tw, j, [&](int e, cx<R> v) {
+            if (own) Sc[(int)colg + e * nc] = v;
+        });
+    }
 }
 
-template <typename R>
-bool static_xinv(int n, const cx<R>* S, const XSink& sink, const Geometry3& g, const FFTTables<R>& t,
-                 const int* skip, hipStream_t st) {
-    const long nlines = (long)g.ny * g.nz;
-    const double inv_n = 1.0 / (double)g.N;
-#define FFTPERM_XI(NN)                                                                                          \
-    case NN: {                                                                                                  \
-        using SH = Shape<R, NN>;                                                                                \
-        const long blocks = (nlines + SH::LINES - 1) / SH::LINES;                                              \
-        hipLaunchKernelGGL((k_xinv_s<R, NN>), dim3((unsigned)blocks), dim3(SH::THREADS), SH::LDS1, st, S, sink, \
-                           nlines, g.N, g.Nh, t.twx, inv_n, skip);                                              \
-        return true;                                                                                            \
-    }
-    switch (n) { FFTPERM_XI(32) FFTPERM_XI(64) FFTPERM_XI(128) FFTPERM_XI(256) FFTPERM_XI(512) default: return false; }
-#undef FFTPERM_XI
+template <typename R, int N, int ZL>
+constexpr size_t zgreen_r_lds() {
+    return (size_t)(N + ZL * (N + 1)) * sizeof(cx<R>);
 }
 
-template <typename R>
-bool static_col(int n, int sign, cx<R>* S, long ncols, long inner, long outer, long estride, long comp_stride,
-                const cx<R>* tw, const int* skip, hipStream_t st) {
-#define FFTPERM_CO(NN)                                                                                          \
-    case NN: {                                                                                                  \
-        using SH = Shape<R, NN>;                                                                                \
-        const dim3 grid((unsigned)((ncols + SH::LINES - 1) / SH::LINES), 3);                                    \
-        if (sign < 0)                                                                                           \
-            hipLaunchKernelGGL((k_col_s<R, NN, -1>), grid, dim3(SH::THREADS), SH::LDS1, st, S, ncols, inner,    \
-                               outer, estride, comp_stride, tw, skip);                                          \
-        else                                                                                                    \
-            hipLaunchKernelGGL((k_col_s<R, NN, +1>), grid, dim3(SH::THREADS), SH::LDS1, st, S, ncols, inner,    \
-                               outer, estride, comp_stride, tw, skip);                                          \
-        return true;                                                                                            \
+template <typename R, int N, int ZL>
+long launch_zgreen_r(cx<R>* S, const R* G6, long ncols, const Geometry3& g, const cx<R>* tw, double* qpart,
+                     const int* skip, hipStream_t st) {
+    constexpr int threads = ZL * N / 4;
+    if constexpr (threads > 1024 || threads < 64) {
+        return -1;
+    } else {
+        constexpr size_t lds = zgreen_r_lds<R, N, ZL>();
+        if (lds > g_lds_max) return -1;
+        const long blocks = (ncols + ZL - 1) / ZL;
+        hipLaunchKernelGGL((k_zgreen_r<R, N, ZL>), dim3((unsigned)blocks), dim3(threads), lds, st, S, G6, ncols, g.Nh,
+                           g.nx, g.nxh, tw, qpart, skip);
+        return blocks;
     }
-    switch (n) { FFTPERM_CO(32) FFTPERM_CO(64) FFTPERM_CO(128) FFTPERM_CO(256) FFTPERM_CO(512) default: return false; }
-#undef FFTPERM_CO
 }
 
 template <typename R, int N, int ZL>
 constexpr size_t zgreen_lds() {
-    return (size_t)(N + 3 * ZL * Shape<R, N>::LD) * sizeof(cx<R>);
-}
-
-template <typename R, int N, int ZL>
-bool zgreen_fits() {
-    return zgreen_lds<R, N, ZL>() <= g_lds_max;
+    return (size_t)(N + 3 * ZL * ZShape<R, N>::LD) * sizeof(cx<R>);
 }
 
 template <typename R, int N, int ZL, int OS>
-bool launch_zgreen_zl(cx<R>* S, const R* G6, long ncols, long Nh, const cx<R>* tw, const int* skip, hipStream_t st) {
+long launch_zgreen_zl(cx<R>* S, const R* G6, long ncols, const Geometry3& g, const cx<R>* tw, double* qpart,
+                      const int* skip, hipStream_t st) {
     if constexpr (zgreen_threads<R, N, ZL, OS>() > 1024) {
-        return false;
+        return -1;
     } else {
-        if (!zgreen_fits<R, N, ZL>()) return false;
+        constexpr size_t lds = zgreen_lds<R, N, ZL>();
+        if (lds > g_lds_max) return -1;
         const long blocks = (ncols + ZL - 1) / ZL;
         constexpr int threads = zgreen_threads<R, N, ZL, OS>();
-        constexpr size_t lds = zgreen_lds<R, N, ZL>();
         hipLaunchKernelGGL((k_zgreen_s<R, N, ZL, OS>), dim3((unsigned)blocks), dim3(threads), lds, st, S, G6, ncols,
-                           Nh, tw, skip);
-        return true;
+                           g.Nh, g.nx, g.nxh, tw, qpart, skip);
+        return blocks;
     }
 }
 
-// Autotuned choice of the fused z-pass tile width ZL (columns per workgroup), keyed by
-// (precision, N, columns); filled by autotune_zgreen, read by every launch.
-std::mutex g_tune_mu;
-std::map<std::tuple<int, int, long>, int> g_zl_choice;
+// variant code = 10 * ZL + OS for the all-in-LDS kernel (OS = 1, 2 threads per butterfly thread),
+// 10 * ZL + 4 for the register-resident radix-4 kernel
+constexpr int kZgreenCodes[] = {11, 12, 21, 22, 41, 42, 81, 82, 161, 162, 44, 84, 164, 324};
 
-int zl_choice(int prec, int n, long ncols) {
-    std::lock_guard<std::mutex> lk(g_tune_mu);
-    auto it = g_zl_choice.find(std::make_tuple(prec, n, ncols));
-    return it == g_zl_choice.end() ? 0 : it->second;
-}
-
-// variant code = 10 * ZL + OS
 template <typename R, int N>
-bool launch_zgreen_n(int code, cx<R>* S, const R* G6, long ncols, long Nh, const cx<R>* tw, const int* skip,
-                     hipStream_t st) {
+long launch_zgreen_n(int code, cx<R>* S, const R* G6, long ncols, const Geometry3& g, const cx<R>* tw,
+                     double* qpart, const int* skip, hipStream_t st) {
     switch (code) {
-        case 11: return launch_zgreen_zl<R, N, 1, 1>(S, G6, ncols, Nh, tw, skip, st);
-        case 12: return launch_zgreen_zl<R, N, 1, 2>(S, G6, ncols, Nh, tw, skip, st);
-        case 21: return launch_zgreen_zl<R, N, 2, 1>(S, G6, ncols, Nh, tw, skip, st);
-        case 22: return launch_zgreen_zl<R, N, 2, 2>(S, G6, ncols, Nh, tw, skip, st);
-        case 41: return launch_zgreen_zl<R, N, 4, 1>(S, G6, ncols, Nh, tw, skip, st);
-        case 42: return launch_zgreen_zl<R, N, 4, 2>(S, G6, ncols, Nh, tw, skip, st);
-        case 81: return launch_zgreen_zl<R, N, 8, 1>(S, G6, ncols, Nh, tw, skip, st);
-        case 82: return launch_zgreen_zl<R, N, 8, 2>(S, G6, ncols, Nh, tw, skip, st);
-        case 161: return launch_zgreen_zl<R, N, 16, 1>(S, G6, ncols, Nh, tw, skip, st);
-        case 162: return launch_zgreen_zl<R, N, 16, 2>(S, G6, ncols, Nh, tw, skip, st);
-        default: return false;
+        case 11: return launch_zgreen_zl<R, N, 1, 1>(S, G6, ncols, g, tw, qpart, skip, st);
+        case 12: return launch_zgreen_zl<R, N, 1, 2>(S, G6, ncols, g, tw, qpart, skip, st);
+        case 21: return launch_zgreen_zl<R, N, 2, 1>(S, G6, ncols, g, tw, qpart, skip, st);
+        case 22: return launch_zgreen_zl<R, N, 2, 2>(S, G6, ncols, g, tw, qpart, skip, st);
+        case 41: return launch_zgreen_zl<R, N, 4, 1>(S, G6, ncols, g, tw, qpart, skip, st);
+        case 42: return launch_zgreen_zl<R, N, 4, 2>(S, G6, ncols, g, tw, qpart, skip, st);
+        case 81: return launch_zgreen_zl<R, N, 8, 1>(S, G6, ncols, g, tw, qpart, skip, st);
+        case 82: return launch_zgreen_zl<R, N, 8, 2>(S, G6, ncols, g, tw, qpart, skip, st);
+        case 161: return launch_zgreen_zl<R, N, 16, 1>(S, G6, ncols, g, tw, qpart, skip, st);
+        case 162: return launch_zgreen_zl<R, N, 16, 2>(S, G6, ncols, g, tw, qpart, skip, st);
+        case 44: return launch_zgreen_r<R, N, 4>(S, G6, ncols, g, tw, qpart, skip, st);
+        case 84: return launch_zgreen_r<R, N, 8>(S, G6, ncols, g, tw, qpart, skip, st);
+        case 164: return launch_zgreen_r<R, N, 16>(S, G6, ncols, g, tw, qpart, skip, st);
+        case 324: return launch_zgreen_r<R, N, 32>(S, G6, ncols, g, tw, qpart, skip, st);
+        default: return -1;
     }
 }
 
 template <typename R>
-bool static_zgreen_zl(int n, int zl, cx<R>* S, const R* G6, long ncols, long Nh, const cx<R>* tw, const int* skip,
-                      hipStream_t st) {
+long static_zgreen_code(int code, cx<R>* S, const R* G6, long ncols, const Geometry3& g, const cx<R>* tw,
+                        double* qpart, const int* skip, hipStream_t st) {
+    switch (g.nz) {
+        case 32: return launch_zgreen_n<R, 32>(code, S, G6, ncols, g, tw, qpart, skip, st);
+        case 64: return launch_zgreen_n<R, 64>(code, S, G6, ncols, g, tw, qpart, skip, st);
+        case 128: return launch_zgreen_n<R, 128>(code, S, G6, ncols, g, tw, qpart, skip, st);
+        case 256: return launch_zgreen_n<R, 256>(code, S, G6, ncols, g, tw, qpart, skip, st);
+        case 512: return launch_zgreen_n<R, 512>(code, S, G6, ncols, g, tw, qpart, skip, st);
+        default: return -1;
+    }
+}
+
+// untuned: the widest tile that fits ~100 KB of LDS, one thread per butterfly
+int zgreen_default_code(int n, int prec) {
     switch (n) {
-        case 32: return launch_zgreen_n<R, 32>(zl, S, G6, ncols, Nh, tw, skip, st);
-        case 64: return launch_zgreen_n<R, 64>(zl, S, G6, ncols, Nh, tw, skip, st);
-        case 128: return launch_zgreen_n<R, 128>(zl, S, G6, ncols, Nh, tw, skip, st);
-        case 256: return launch_zgreen_n<R, 256>(zl, S, G6, ncols, Nh, tw, skip, st);
-        case 512: return launch_zgreen_n<R, 512>(zl, S, G6, ncols, Nh, tw, skip, st);
-        default: return false;
+        case 32: return 10 * (prec ? ZShape<float, 32>::ZL : ZShape<double, 32>::ZL) + 1;
+        case 64: return 10 * (prec ? ZShape<float, 64>::ZL : ZShape<double, 64>::ZL) + 1;
+        case 128: return 10 * (prec ? ZShape<float, 128>::ZL : ZShape<double, 128>::ZL) + 1;
+        case 256: return 10 * (prec ? ZShape<float, 256>::ZL : ZShape<double, 256>::ZL) + 1;
+        case 512: return 10 * (prec ? ZShape<float, 512>::ZL : ZShape<double, 512>::ZL) + 1;
+        default: return 0;
     }
 }
 
+std::mutex g_tune_mu;
+std::map<std::tuple<int, int, int, long>, int> g_tuned;
+
 template <typename R>
-bool static_zgreen(int n, cx<R>* S, const R* G6, long ncols, long Nh, const cx<R>* tw, const int* skip,
-                   hipStream_t st) {
-    int code = zl_choice(sizeof(R) == 8 ? 0 : 1, n, ncols);
-    if (code == 0) {
-        // untuned: the widest tile that fits ~100 KB of LDS, one thread per butterfly
-        switch (n) {
-            case 32: code = 10 * Shape<R, 32>::ZL + 1; break;
-            case 64: code = 10 * Shape<R, 64>::ZL + 1; break;
-            case 128: code = 10 * Shape<R, 128>::ZL + 1; break;
-            case 256: code = 10 * Shape<R, 256>::ZL + 1; break;
-            case 512: code = 10 * Shape<R, 512>::ZL + 1; break;
-            default: return false;
-        }
-    }
-    return static_zgreen_zl<R>(n, code, S, G6, ncols, Nh, tw, skip, st);
+constexpr int prec_of() {
+    return sizeof(R) == 8 ? 0 : 1;
 }
 
-// The specialised kernels need their static LDS budget; fall back to the runtime kernels on
-// devices with less LDS per workgroup.
-bool static_ok() { return g_lds_max >= 102400u + 8192u; }
-
-}  // namespace
-
-template <typename R>
-std::vector<std::pair<int, float>> autotune_zgreen(cx<R>* S, const R* G6, const Geometry3& g, const FFTTables<R>& t,
-                                                   hipStream_t st) {
+// Times `launch` (which returns < 0 when the variant does not exist) for every candidate and
+// records the fastest under (kind, prec, n, count).
+template <typename F>
+std::vector<std::pair<int, float>> tune_candidates(int kind, int prec, int n, long count, const std::vector<int>& cands,
+                                                   F&& launch, int (*code_of)(int, int), hipStream_t st) {
     std::vector<std::pair<int, float>> timings;
-    const long ncols = (long)g.ny * g.nxh;
-    const int prec = sizeof(R) == 8 ? 0 : 1;
-    if (!static_ok() || zl_choice(prec, g.nz, ncols) != 0) return timings;
+    if (!static_passes_ok() || tuned_variant(kind, prec, n, count) >= 0) return timings;
     hipEvent_t a, b;
     if (hipEventCreate(&a) != hipSuccess) return timings;
     if (hipEventCreate(&b) != hipSuccess) {
         (void)hipEventDestroy(a);
         return timings;
     }
-    (void)hipMemsetAsync(S, 0, 3 * (size_t)g.Nh * sizeof(cx<R>), st);
-    int best = 0;
+    int best = -1;
     float best_ms = 1e30f;
-    for (int zl : {11, 12, 21, 22, 41, 42, 81, 82, 161, 162}) {
-        if (!static_zgreen_zl<R>(g.nz, zl, S, G6, ncols, g.Nh, t.twz, nullptr, st)) continue;  // warm-up
+    for (int v : cands) {
+        if (launch(v) < 0) continue;  // warm-up (and existence check)
         (void)hipEventRecord(a, st);
-        for (int rep = 0; rep < 3; ++rep) static_zgreen_zl<R>(g.nz, zl, S, G6, ncols, g.Nh, t.twz, nullptr, st);
+        for (int rep = 0; rep < 3; ++rep) launch(v);
         (void)hipEventRecord(b, st);
         if (hipEventSynchronize(b) != hipSuccess) break;
         float ms = 0.f;
         (void)hipEventElapsedTime(&ms, a, b);
         ms /= 3.f;
-        timings.emplace_back(zl, ms);
+        timings.emplace_back(code_of(n, v), ms);
         if (ms < best_ms) {
             best_ms = ms;
-            best = zl;
+            best = v;
         }
     }
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
-    if (best != 0) {
-        std::lock_guard<std::mutex> lk(g_tune_mu);
-        g_zl_choice[std::make_tuple(prec, g.nz, ncols)] = best;
-    }
+    if (best >= 0) set_tuned_variant(kind, prec, n, count, best);
     return timings;
 }
 
-int zgreen_tile(int prec, int n, long ncols) { return zl_choice(prec, n, ncols); }
+int zgreen_code_identity(int, int v) { return v; }
+
+std::vector<int> pass_candidates() {
+    std::vector<int> v;
+    for (int i = 0; i < kPassVariants; ++i) v.push_back(i);
+    return v;
+}
+
+int pass_variant_or_default(int kind, int prec, int n, long count) {
+    const int v = tuned_variant(kind, prec, n, count);
+    return v < 0 ? 0 : v;
+}
+
+}  // namespace
+
+// The specialised kernels need their static LDS budget; the runtime kernels cover devices with
+// less LDS per workgroup and every other length.
+bool static_passes_ok() { return g_lds_max >= 102400u + 8192u; }
+
+int tuned_variant(int kind, int prec, int n, long count) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tuned.find(std::make_tuple(kind, prec, n, count));
+    return it == g_tuned.end() ? -1 : it->second;
+}
+
+void set_tuned_variant(int kind, int prec, int n, long count, int v) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tuned[std::make_tuple(kind, prec, n, count)] = v;
+}
+
+void clear_tuned_variant(int kind, int prec, int n, long count) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tuned.erase(std::make_tuple(kind, prec, n, count));
+}
+
+int variant_of_code(int kind, int n, int code) {
+    if (kind == kKindZgreen) {
+        for (int c : kZgreenCodes)
+            if (c == code) return c;
+        return -1;
+    }
+    for (int v = 0; v < kPassVariants; ++v)
+        if (pass_variant_code(n, v) == code && code != 0) return v;
+    return -1;
+}
+
+int variant_code(int kind, int prec, int n, long count) {
+    if (!static_passes_ok()) return 0;
+    if (kind == kKindZgreen) {
+        const int c = tuned_variant(kind, prec, n, count);
+        return c < 0 ? zgreen_default_code(n, prec) : c;
+    }
+    return pass_variant_code(n, pass_variant_or_default(kind, prec, n, count));
+}
+
+template <typename R>
+std::vector<std::pair<int, float>> autotune_zgreen(cx<R>* S, const R* G6, const Geometry3& g, const FFTTables<R>& t,
+                                                   hipStream_t st) {
+    const long ncols = (long)g.ny * g.nxh;
+    (void)hipMemsetAsync(S, 0, 3 * (size_t)g.Nh * sizeof(cx<R>), st);
+    std::vector<int> cands(std::begin(kZgreenCodes), std::end(kZgreenCodes));
+    return tune_candidates(
+        kKindZgreen, prec_of<R>(), g.nz, ncols, cands,
+        [&](int code) { return static_zgreen_code<R>(code, S, G6, ncols, g, t.twz, nullptr, nullptr, st); },
+        zgreen_code_identity, st);
+}
+
+template <typename R>
+std::vector<std::pair<int, float>> autotune_col(cx<R>* S, const Geometry3& g, const FFTTables<R>& t, hipStream_t st) {
+    const long ncols = (long)g.nz * g.nxh;
+    (void)hipMemsetAsync(S, 0, 3 * (size_t)g.Nh * sizeof(cx<R>), st);
+    return tune_candidates(
+        kKindCol, prec_of<R>(), g.ny, ncols, pass_candidates(),
+        [&](int v) {
+            return static_col<R>(v, g.ny, -1, S, ncols, (long)g.nxh, (long)g.ny * g.nxh, (long)g.nxh, g.Nh, t.twy,
+                                 nullptr, st);
+        },
+        pass_variant_code, st);
+}
+
+template <typename R>
+std::vector<std::pair<int, float>> autotune_x(const XSource& src, const XSink& sink, cx<R>* S, const Geometry3& g,
+                                              const FFTTables<R>& t, hipStream_t st) {
+    const long nlines = (long)g.ny * g.nz;
+    auto tf = tune_candidates(
+        kKindXfwd, prec_of<R>(), g.nx, nlines, pass_candidates(),
+        [&](int v) { return static_xfwd<R>(v, src, S, g, t.twx, nullptr, st); }, pass_variant_code, st);
+    auto ti = tune_candidates(
+        kKindXinv, prec_of<R>(), g.nx, nlines, pass_candidates(),
+        [&](int v) { return static_xinv<R>(v, S, sink, g, t.twx, nullptr, st); }, pass_variant_code, st);
+    tf.insert(tf.end(), ti.begin(), ti.end());
+    return tf;
+}
 
 template std::vector<std::pair<int, float>> autotune_zgreen<double>(cx<double>*, const double*, const Geometry3&,
                                                                     const FFTTables<double>&, hipStream_t);
 template std::vector<std::pair<int, float>> autotune_zgreen<float>(cx<float>*, const float*, const Geometry3&,
                                                                    const FFTTables<float>&, hipStream_t);
+template std::vector<std::pair<int, float>> autotune_col<double>(cx<double>*, const Geometry3&,
+                                                                 const FFTTables<double>&, hipStream_t);
+template std::vector<std::pair<int, float>> autotune_col<float>(cx<float>*, const Geometry3&, const FFTTables<float>&,
+                                                                hipStream_t);
+template std::vector<std::pair<int, float>> autotune_x<double>(const XSource&, const XSink&, cx<double>*,
+                                                               const Geometry3&, const FFTTables<double>&,
+                                                               hipStream_t);
+template std::vector<std::pair<int, float>> autotune_x<float>(const XSource&, const XSink&, cx<float>*,
+                                                              const Geometry3&, const FFTTables<float>&, hipStream_t);
 
-long xinv_blocks(const Geometry3& g, const LinePlan& px) {
-    // exactly the number of blocks (= partial sums) the x-inverse launcher will use
-    const long nlines = (long)g.ny * g.nz;
-    if (static_ok()) {
-        switch (g.nx) {
-            case 32: return (nlines + Shape<double, 32>::LINES - 1) / Shape<double, 32>::LINES;
-            case 64: return (nlines + Shape<double, 64>::LINES - 1) / Shape<double, 64>::LINES;
-            case 128: return (nlines + Shape<double, 128>::LINES - 1) / Shape<double, 128>::LINES;
-            case 256: return (nlines + Shape<double, 256>::LINES - 1) / Shape<double, 256>::LINES;
-            case 512: return (nlines + Shape<double, 512>::LINES - 1) / Shape<double, 512>::LINES;
-            default: break;
-        }
-    }
-    return cfg_xpass(px, nlines).blocks;
+long xinv_max_blocks(const Geometry3& g) {
+    // every variant and the runtime kernel put at least one line in a workgroup
+    return (long)g.ny * g.nz;
 }
+
+long zgreen_max_blocks(const Geometry3& g) { return (long)g.ny * g.nxh; }
 
 // ------------------------------------------------------------------ x forward
 template <typename R>
@@ -675,8 +570,10 @@ __global__ void k_xfwd(XSource src, cx<R>* __restrict__ S, int nx, long nlines, 
 template <typename R>
 void launch_xfwd(const XSource& src, cx<R>* S, const Geometry3& g, const FFTTables<R>& t, const int* skip,
                  hipStream_t st) {
-    if (static_ok() && static_xfwd<R>(g.nx, src, S, g, t, skip, st)) return;
     const long nlines = (long)g.ny * g.nz;
+    if (static_passes_ok() &&
+        static_xfwd<R>(pass_variant_or_default(kKindXfwd, prec_of<R>(), g.nx, nlines), src, S, g, t.twx, skip, st) >= 0)
+        return;
     PassCfg c = cfg_xpass(t.px, nlines);
     hipLaunchKernelGGL(k_xfwd<R>, dim3((unsigned)c.blocks), dim3(c.threads), c.lds, st, src, S, g.nx, nlines,
                        g.nxh, g.N, g.Nh, t.px, t.twx, c.lines, skip);
@@ -694,7 +591,7 @@ __global__ void k_xinv(const cx<R>* __restrict__ S, XSink sink, int nx, long nli
     for (int i = threadIdx.x; i < nx; i += blockDim.x) tw[i] = twg[i];
     const long line0 = (long)blockIdx.x * lpw;
     const int nl = (int)min((long)lpw, nlines - line0);
-    double acc[kXinvParts] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double acc[kXinvParts] = {0, 0, 0, 0, 0, 0};
     for (int c = 0; c < 3; ++c) {
         const cx<R>* Sc = S + c * Nh;
         const int cnth = nl * nxh;
@@ -722,7 +619,6 @@ __global__ void k_xinv(const cx<R>* __restrict__ S, XSink sink, int nx, long nli
             const int code = sink.code[vox];
             if (code >= 0) {
                 if (sink.compact != nullptr) sink.compact[3L * code + c] = val;
-                if (sink.dotv != nullptr) acc[6] += sink.dotv[3L * code + c] * val;
                 if (c == 0) acc[0] += val;
                 if (c == 1) acc[1] += val;
                 if (c == 2) acc[2] += val;
@@ -739,14 +635,19 @@ __global__ void k_xinv(const cx<R>* __restrict__ S, XSink sink, int nx, long nli
 }
 
 template <typename R>
-void launch_xinv(const cx<R>* S, const XSink& sink, const Geometry3& g, const FFTTables<R>& t, const int* skip,
+long launch_xinv(const cx<R>* S, const XSink& sink, const Geometry3& g, const FFTTables<R>& t, const int* skip,
                  hipStream_t st) {
-    if (static_ok() && static_xinv<R>(g.nx, S, sink, g, t, skip, st)) return;
     const long nlines = (long)g.ny * g.nz;
+    if (static_passes_ok()) {
+        const long b =
+            static_xinv<R>(pass_variant_or_default(kKindXinv, prec_of<R>(), g.nx, nlines), S, sink, g, t.twx, skip, st);
+        if (b >= 0) return b;
+    }
     PassCfg c = cfg_xpass(t.px, nlines);
     const double inv_n = 1.0 / (double)g.N;
     hipLaunchKernelGGL(k_xinv<R>, dim3((unsigned)c.blocks), dim3(c.threads), c.lds, st, S, sink, g.nx, nlines,
                        g.nxh, g.N, g.Nh, t.px, t.twx, c.lines, inv_n, skip);
+    return c.blocks;
 }
 
 // ------------------------------------------------------------- column passes
@@ -788,8 +689,9 @@ template <typename R>
 void launch_ypass(cx<R>* S, int sign, const Geometry3& g, const FFTTables<R>& t, const int* skip,
                   hipStream_t st) {
     const long ncols = (long)g.nz * g.nxh;
-    if (static_ok() && static_col<R>(g.ny, sign, S, ncols, (long)g.nxh, (long)g.ny * g.nxh, (long)g.nxh, g.Nh, t.twy,
-                                     skip, st))
+    if (static_passes_ok() &&
+        static_col<R>(pass_variant_or_default(kKindCol, prec_of<R>(), g.ny, ncols), g.ny, sign, S, ncols, (long)g.nxh,
+                      (long)g.ny * g.nxh, (long)g.nxh, g.Nh, t.twy, skip, st) >= 0)
         return;
     PassCfg c = cfg_colpass(t.py, ncols);
     dim3 grid((unsigned)c.blocks, 3);
@@ -805,7 +707,8 @@ template <typename R>
 void launch_zpass(cx<R>* S, int sign, const Geometry3& g, const FFTTables<R>& t, const int* skip,
                   hipStream_t st) {
     const long ncols = (long)g.ny * g.nxh;
-    if (static_ok() && static_col<R>(g.nz, sign, S, ncols, ncols, 0L, ncols, g.Nh, t.twz, skip, st)) return;
+    if (static_passes_ok() && static_col<R>(0, g.nz, sign, S, ncols, ncols, 0L, ncols, g.Nh, t.twz, skip, st) >= 0)
+        return;
     PassCfg c = cfg_colpass(t.pz, ncols);
     dim3 grid((unsigned)c.blocks, 3);
     if (sign < 0)
@@ -818,8 +721,9 @@ void launch_zpass(cx<R>* S, int sign, const Geometry3& g, const FFTTables<R>& t,
 
 // ------------------------------------------------------------- fused z + Green
 template <typename R>
-__global__ void k_zgreen(cx<R>* __restrict__ S, const R* __restrict__ G6, int nz, long ncols, long Nh,
-                         LinePlan pl, const cx<R>* __restrict__ twg, int tl, const int* skip) {
+__global__ void k_zgreen(cx<R>* __restrict__ S, const R* __restrict__ G6, int nz, long ncols, long Nh, int nx,
+                         int nxh, LinePlan pl, const cx<R>* __restrict__ twg, int tl, double* __restrict__ qpart,
+                         const int* skip) {
     if (skip != nullptr && *skip) return;
     extern __shared__ unsigned char smem_raw[];
     cx<R>* tw = reinterpret_cast<cx<R>*>(smem_raw);
@@ -841,7 +745,9 @@ __global__ void k_zgreen(cx<R>* __restrict__ S, const R* __restrict__ G6, int nz
     __syncthreads();
     fft_lines<R, -1>(buf, ld, 3 * ncl, pl, tw);
     // Symmetric Green multiply, as GreenOperator::apply_spectral_green_tensor
-    // (src/bignonnet/GreenOperator.cpp:84-92) with the symmetrised tensor.
+    // (src/bignonnet/GreenOperator.cpp:84-92) with the symmetrised tensor, plus the weighted
+    // quadratic form conj(f) . G f when asked for.
+    double q[1] = {0.0};
     for (int idx = threadIdx.x; idx < cnt; idx += blockDim.x) {
         const int e = idx / ncl;
         const int ci = idx - e * ncl;
@@ -852,10 +758,18 @@ __global__ void k_zgreen(cx<R>* __restrict__ S, const R* __restrict__ G6, int nz
         cx<R>* py = buf + (1 * ncl + ci) * ld + e;
         cx<R>* pz = buf + (2 * ncl + ci) * ld + e;
         const cx<R> fx = *px, fy = *py, fz = *pz;
-        *px = mkc<R>(txx * fx.x + txy * fy.x + txz * fz.x, txx * fx.y + txy * fy.y + txz * fz.y);
-        *py = mkc<R>(txy * fx.x + tyy * fy.x + tyz * fz.x, txy * fx.y + tyy * fy.y + tyz * fz.y);
-        *pz = mkc<R>(txz * fx.x + tyz * fy.x + tzz * fz.x, txz * fx.y + tyz * fy.y + tzz * fz.y);
+        const cx<R> gx = mkc<R>(txx * fx.x + txy * fy.x + txz * fz.x, txx * fx.y + txy * fy.y + txz * fz.y);
+        const cx<R> gy = mkc<R>(txy * fx.x + tyy * fy.x + tyz * fz.x, txy * fx.y + tyy * fy.y + tyz * fz.y);
+        const cx<R> gz = mkc<R>(txz * fx.x + tyz * fy.x + tzz * fz.x, txz * fx.y + tyz * fy.y + tzz * fz.y);
+        *px = gx;
+        *py = gy;
+        *pz = gz;
+        if (qpart != nullptr)
+            q[0] += herm_weight((int)((col0 + ci) % nxh), nx) *
+                    ((double)fx.x * gx.x + (double)fx.y * gx.y + (double)fy.x * gy.x + (double)fy.y * gy.y +
+                     (double)fz.x * gz.x + (double)fz.y * gz.y);
     }
+    if (qpart != nullptr) block_sum<1>(q, qpart + blockIdx.x);
     __syncthreads();
     fft_lines<R, +1>(buf, ld, 3 * ncl, pl, tw);
     for (int c = 0; c < 3; ++c) {
@@ -870,13 +784,19 @@ __global__ void k_zgreen(cx<R>* __restrict__ S, const R* __restrict__ G6, int nz
 }
 
 template <typename R>
-void launch_zgreen(cx<R>* S, const R* G6, const Geometry3& g, const FFTTables<R>& t, const int* skip,
+long launch_zgreen(cx<R>* S, const R* G6, const Geometry3& g, const FFTTables<R>& t, double* qpart, const int* skip,
                    hipStream_t st) {
     const long ncols = (long)g.ny * g.nxh;
-    if (static_ok() && static_zgreen<R>(g.nz, S, G6, ncols, g.Nh, t.twz, skip, st)) return;
+    if (static_passes_ok()) {
+        int code = tuned_variant(kKindZgreen, prec_of<R>(), g.nz, ncols);
+        if (code < 0) code = zgreen_default_code(g.nz, prec_of<R>());
+        const long b = static_zgreen_code<R>(code, S, G6, ncols, g, t.twz, qpart, skip, st);
+        if (b >= 0) return b;
+    }
     PassCfg c = cfg_zgreen(t.pz, ncols);
     hipLaunchKernelGGL(k_zgreen<R>, dim3((unsigned)c.blocks), dim3(c.threads), c.lds, st, S, G6, g.nz, ncols,
-                       g.Nh, t.pz, t.twz, c.lines, skip);
+                       g.Nh, g.nx, g.nxh, t.pz, t.twz, c.lines, qpart, skip);
+    return c.blocks;
 }
 
 // ------------------------------------------------------- spectrum utilities
@@ -958,14 +878,14 @@ void launch_spectral_green_full(const double* T6, const double2* F, double2* V, 
 #define FFTPERM_INST(R)                                                                                       \
     template void launch_xfwd<R>(const XSource&, cx<R>*, const Geometry3&, const FFTTables<R>&, const int*,   \
                                  hipStream_t);                                                                \
-    template void launch_xinv<R>(const cx<R>*, const XSink&, const Geometry3&, const FFTTables<R>&,           \
+    template long launch_xinv<R>(const cx<R>*, const XSink&, const Geometry3&, const FFTTables<R>&,           \
                                  const int*, hipStream_t);                                                    \
     template void launch_ypass<R>(cx<R>*, int, const Geometry3&, const FFTTables<R>&, const int*,             \
                                   hipStream_t);                                                               \
     template void launch_zpass<R>(cx<R>*, int, const Geometry3&, const FFTTables<R>&, const int*,             \
                                   hipStream_t);                                                               \
-    template void launch_zgreen<R>(cx<R>*, const R*, const Geometry3&, const FFTTables<R>&, const int*,       \
-                                   hipStream_t);                                                              \
+    template long launch_zgreen<R>(cx<R>*, const R*, const Geometry3&, const FFTTables<R>&, double*,          \
+                                   const int*, hipStream_t);                                                  \
     template void launch_expand_full<R>(const cx<R>*, double2*, const Geometry3&, hipStream_t);               \
     template void launch_hermitian_half<R>(const double2*, cx<R>*, const Geometry3&, hipStream_t);
 

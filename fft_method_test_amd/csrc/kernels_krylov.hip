@@ -161,32 +161,34 @@ void launch_scale_copy(const double* x, double scale_div, double* out, long n, h
 }
 
 // ------------------------------------------------------------ MINRES steps
-__global__ void k_mr_finish_mean_out(const double* __restrict__ partials, int nparts, MinresState* st,
-                                     double inv_nb) {
+__global__ void k_mr_finish_mean_out(const double* __restrict__ xpart, int nxparts, const double* __restrict__ zpart,
+                                     int nzparts, MinresState* st, double inv_nb, double inv_n) {
     if (st->done) return;
     double acc[4] = {0, 0, 0, 0};
-    for (int p = threadIdx.x; p < nparts; p += blockDim.x) {
-        const double* q = partials + (long)kXinvParts * p;
+    for (int p = threadIdx.x; p < nxparts; p += blockDim.x) {
+        const double* q = xpart + (long)kXinvParts * p;
         acc[0] += q[0];
         acc[1] += q[1];
         acc[2] += q[2];
-        acc[3] += q[6];
     }
+    for (int p = threadIdx.x; p < nzparts; p += blockDim.x) acc[3] += zpart[p];
     __shared__ double res[4];
     block_sum<4>(acc, res);
     if (threadIdx.x == 0) {
-        const double m0 = res[0] * inv_nb, m1 = res[1] * inv_nb, m2 = res[2] * inv_nb;
-        st->mean_out[0] = m0;
-        st->mean_out[1] = m1;
-        st->mean_out[2] = m2;
-        // alpha = <v, z_raw - m> = <v, z_raw> - sum_c m_c (sum of component c of v)
-        st->alpha = res[3] - (m0 * st->vsum[0] + m1 * st->vsum[1] + m2 * st->vsum[2]);
+        st->mean_out[0] = res[0] * inv_nb;
+        st->mean_out[1] = res[1] * inv_nb;
+        st->mean_out[2] = res[2] * inv_nb;
+        // alpha = <v, P_B gather(G scatter(P_B v))> = <scatter(P_B v), G scatter(P_B v)> over the grid
+        //       = (1/N) sum_k conj(f(k)) . G(k) f(k)                                  (Parseval)
+        st->alpha = res[3] * inv_n;
     }
 }
 
-void launch_mr_finish_mean_out(const double* partials, int nparts, MinresState* st, double inv_nb,
-                               hipStream_t s) {
-    hipLaunchKernelGGL(k_mr_finish_mean_out, dim3(1), dim3(kRedThreads), 0, s, partials, nparts, st, inv_nb);
+void launch_mr_finish_mean_out(const double* xpart, int nxparts, const double* zpart, int nzparts, MinresState* st,
+                               double inv_nb, double inv_n, hipStream_t s) {
+    // thousands of partial rows: a full 1024-thread block keeps this single-block finish short
+    hipLaunchKernelGGL(k_mr_finish_mean_out, dim3(1), dim3(1024), 0, s, xpart, nxparts, zpart, nzparts, st, inv_nb,
+                       inv_n);
 }
 
 __global__ void k_mr_sub_dot(double* __restrict__ z, const double* __restrict__ v, MinresState* st, long nb,

@@ -46,9 +46,10 @@ void launch_mr_update(double* x, const double* v, const double* z, double* w_pre
                       double* v_new, MinresState* st, long nb, double* partials, hipStream_t s);
 // vsum = sums, mean_in = sums / nb ; done = pending_done
 void launch_mr_finish_mean_in(const double* partials, MinresState* st, double inv_nb, hipStream_t s);
-// mean_out = support sums of the x-inverse partials / nb ; alpha = <v, z_raw> - sum_c mean_out_c vsum_c
-void launch_mr_finish_mean_out(const double* partials, int nparts, MinresState* st, double inv_nb,
-                               hipStream_t s);
+// mean_out = support sums of the x-inverse partials / nb ; alpha = (sum of the fused z-pass
+// quadratic-form partials) / N
+void launch_mr_finish_mean_out(const double* xpart, int nxparts, const double* zpart, int nzparts, MinresState* st,
+                               double inv_nb, double inv_n, hipStream_t s);
 
 // Generic helpers
 void launch_dot_partials(const double* a, const double* b, long n, double* partials, hipStream_t s);

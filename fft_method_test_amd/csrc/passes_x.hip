@@ -1,0 +1,349 @@
+// passes_x.hip -- compile-time specialised x passes (R2C with fused scatter, C2R with fused
+// gather and partial sums) for x-line lengths 32, 64, 128, 256 and 512, in kPassVariants tile
+// shapes each (lines per workgroup x threads per butterfly thread).
+//
+// Real x-lines are transformed two at a time as one complex line z = a + i b ("two for one"):
+// round A packs components x and y of each line, round B packs component z of lines 2m and
+// 2m+1.  Forward: A[k] = (Z[k] + conj Z[N-k]) / 2, B[k] = (Z[k] - conj Z[N-k]) / 2i.  Inverse:
+// Z = A + iB over the Hermitian completions of A and B, a = Re ifft(Z), b = Im ifft(Z).  This
+// halves the FFT and LDS work of both x passes against one complex transform per real line.
+#include "fft_static.hpp"
+#include "kernels.hpp"
+
+namespace fftperm {
+
+namespace {
+
+template <typename R, int N, int LINES, int OS>
+__global__ __launch_bounds__((PassShape<R, N, LINES, OS>::THREADS)) void k_xfwd_s(XSource src, cx<R>* __restrict__ S,
+                                                                                long nlines, long N3, long Nh,
+                                                                                const cx<R>* __restrict__ twg,
+                                                                                const int* skip) {
+    if (skip != nullptr && *skip) return;
+    using SH = PassShape<R, N, LINES, OS>;
+    constexpr int LD = SH::LD, THREADS = SH::THREADS, NXH = N / 2 + 1;
+    constexpr int KE = LINES * N / THREADS;  // elements per thread per component
+    extern __shared__ unsigned char smem_raw[];
+    cx<R>* tw = reinterpret_cast<cx<R>*>(smem_raw);
+    cx<R>* buf = tw + N;
+    for (int i = threadIdx.x; i < N; i += THREADS) tw[i] = twg[i];
+    const long line0 = xcd_tile(blockIdx.x, gridDim.x) * LINES;
+    const int nl = (int)min((long)LINES, nlines - line0);
+    // every global load of the block is issued up front: the support code once per voxel, then
+    // the three compact components (or the three field components) of each owned voxel
+    double v[3][KE];
+    if (src.field != nullptr) {
+#pragma unroll
+        for (int k = 0; k < KE; ++k) {
+            const int idx = threadIdx.x + k * THREADS;
+            const int line = idx / N;
+            const long vox = (line0 + line) * N + idx % N;
+            const bool ok = line < nl;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) v[c][k] = ok ? src.field[c * N3 + vox] : 0.0;
+        }
+    } else {
+        int code[KE];
+#pragma unroll
+        for (int k = 0; k < KE; ++k) {
+            const int idx = threadIdx.x + k * THREADS;
+            const int line = idx / N;
+            code[k] = (line < nl) ? src.code[(line0 + line) * N + idx % N] : -2;
+        }
+        double mc[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) mc[c] = (src.mean != nullptr) ? src.mean[c] : 0.0;
+#pragma unroll
+        for (int k = 0; k < KE; ++k) {
+            const int cd = code[k];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const double f0 = (cd == -1) ? src.a_fluid[c] : ((cd >= 0) ? src.a_support[c] : 0.0);
+                double fx = 0.0;
+                if (src.compact != nullptr && cd >= 0) fx = src.compact[3L * cd + c] - mc[c];
+                v[c][k] = f0 + fx;
+            }
+        }
+    }
+    // round A: z = x + i y per line
+#pragma unroll
+    for (int k = 0; k < KE; ++k) {
+        const int idx = threadIdx.x + k * THREADS;
+        buf[(idx / N) * LD + idx % N] = mkc<R>((R)v[0][k], (R)v[1][k]);
+    }
+    __syncthreads();
+    sfft_lines<R, -1, N, LINES, SH::T, LD>(buf, tw, nl);
+    for (int idx = threadIdx.x; idx < LINES * NXH; idx += THREADS) {
+        const int line = idx / NXH;
+        const int kx = idx % NXH;
+        if (line < nl) {
+            const cx<R> zk = buf[line * LD + kx];
+            const cx<R> zm = buf[line * LD + (N - kx) % N];
+            const long o = (line0 + line) * NXH + kx;
+            S[o] = mkc<R>((R)0.5 * (zk.x + zm.x), (R)0.5 * (zk.y - zm.y));
+            S[Nh + o] = mkc<R>((R)0.5 * (zk.y + zm.y), (R)0.5 * (zm.x - zk.x));
+        }
+    }
+    __syncthreads();
+    // round B: z = z_{2m} + i z_{2m+1} (component z of two adjacent lines)
+    R* bufr = reinterpret_cast<R*>(buf);
+#pragma unroll
+    for (int k = 0; k < KE; ++k) {
+        const int idx = threadIdx.x + k * THREADS;
+        const int line = idx / N;
+        bufr[2 * ((line >> 1) * LD + idx % N) + (line & 1)] = (R)v[2][k];
+    }
+    __syncthreads();
+    const int nl2 = (nl + 1) / 2;
+    sfft_lines<R, -1, N, LINES, SH::T, LD>(buf, tw, nl2);
+    cx<R>* Sz = S + 2 * Nh;
+    for (int idx = threadIdx.x; idx < (LINES / 2) * NXH; idx += THREADS) {
+        const int m = idx / NXH;
+        const int kx = idx % NXH;
+        const cx<R> zk = buf[m * LD + kx];
+        const cx<R> zm = buf[m * LD + (N - kx) % N];
+        const int la = 2 * m, lb = 2 * m + 1;
+        if (la < nl) Sz[(line0 + la) * NXH + kx] = mkc<R>((R)0.5 * (zk.x + zm.x), (R)0.5 * (zk.y - zm.y));
+        if (lb < nl) Sz[(line0 + lb) * NXH + kx] = mkc<R>((R)0.5 * (zk.y + zm.y), (R)0.5 * (zm.x - zk.x));
+    }
+}
+
+// Z[k] = A + iB and Z[N-k] = conj(A) + i conj(B) for 0 < k < N - k; the self-conjugate bins keep
+// their real parts only.
+template <typename R, int N, int LD>
+__device__ __forceinline__ void xinv_put(cx<R>* buf, int line, int kx, cx<R> a, cx<R> b) {
+    const bool self = (kx == 0 || 2 * kx == N);
+    if (self) {
+        a.y = (R)0;
+        b.y = (R)0;
+    }
+    buf[line * LD + kx] = mkc<R>(a.x - b.y, a.y + b.x);
+    if (!self) buf[line * LD + (N - kx)] = mkc<R>(a.x + b.y, b.x - a.y);
+}
+
+// Writes one real output value of component c at voxel vox / support index cd and accumulates
+// the partial sums {support sum of c, all-voxel sum of c}.
+__device__ __forceinline__ void xinv_emit(const XSink& sink, long N3, int c, long vox, int cd, double val,
+                                          double (&acc)[kXinvParts]) {
+    if (sink.field != nullptr) sink.field[c * N3 + vox] = val;
+    if (cd >= 0) {
+        if (sink.compact != nullptr) sink.compact[3L * cd + c] = val;
+        acc[c] += val;
+    }
+    acc[3 + c] += val;
+}
+
+// C2R of two half spectra at once.  The imaginary parts of the self-conjugate bins (0 and N/2)
+// are dropped, which is what taking the real part of a full C2C inverse does
+// (src/fft/FFTWBackend.cpp:292).  Components x and y are written out as soon as round A has
+// produced them, so only the codes and the round-B inputs stay in registers across a transform.
+template <typename R, int N, int LINES, int OS>
+__global__ __launch_bounds__((PassShape<R, N, LINES, OS>::THREADS)) void k_xinv_s(const cx<R>* __restrict__ S,
+                                                                                XSink sink, long nlines, long N3,
+                                                                                long Nh,
+                                                                                const cx<R>* __restrict__ twg,
+                                                                                double inv_n, const int* skip) {
+    if (skip != nullptr && *skip) return;
+    using SH = PassShape<R, N, LINES, OS>;
+    constexpr int LD = SH::LD, THREADS = SH::THREADS, NXH = N / 2 + 1;
+    constexpr int KE = LINES * N / THREADS;
+    constexpr int KA = (LINES * NXH + THREADS - 1) / THREADS;        // round A loads per thread
+    constexpr int KB = ((LINES / 2) * NXH + THREADS - 1) / THREADS;  // round B loads per thread
+    extern __shared__ unsigned char smem_raw[];
+    cx<R>* tw = reinterpret_cast<cx<R>*>(smem_raw);
+    cx<R>* buf = tw + N;
+    for (int i = threadIdx.x; i < N; i += THREADS) tw[i] = twg[i];
+    const long line0 = xcd_tile(blockIdx.x, gridDim.x) * LINES;
+    const int nl = (int)min((long)LINES, nlines - line0);
+    // all global loads up front: support codes, the x and y half spectra, then component z
+    int code[KE];
+#pragma unroll
+    for (int k = 0; k < KE; ++k) {
+        const int idx = threadIdx.x + k * THREADS;
+        const int line = idx / N;
+        code[k] = (line < nl) ? sink.code[(line0 + line) * N + idx % N] : -2;
+    }
+    cx<R> ha[KA], hb[KA];
+#pragma unroll
+    for (int k = 0; k < KA; ++k) {
+        const int idx = threadIdx.x + k * THREADS;
+        const int line = idx / NXH;
+        const bool ok = idx < LINES * NXH && line < nl;
+        const long o = (line0 + line) * NXH + idx % NXH;
+        // predicated loads into zeroed registers (a select between the spectrum and a zero
+        // constant in memory would become a flat load through a private stack slot)
+        ha[k] = mkc<R>((R)0, (R)0);
+        hb[k] = mkc<R>((R)0, (R)0);
+        if (ok) {
+            ha[k] = S[o];
+            hb[k] = S[Nh + o];
+        }
+    }
+    cx<R> ga[KB], gb[KB];
+#pragma unroll
+    for (int k = 0; k < KB; ++k) {
+        const int idx = threadIdx.x + k * THREADS;
+        const int m = idx / NXH;
+        const int kx = idx % NXH;
+        const bool in = idx < (LINES / 2) * NXH;
+        ga[k] = mkc<R>((R)0, (R)0);
+        gb[k] = mkc<R>((R)0, (R)0);
+        if (in && 2 * m < nl) ga[k] = S[2 * Nh + (line0 + 2 * m) * NXH + kx];
+        if (in && 2 * m + 1 < nl) gb[k] = S[2 * Nh + (line0 + 2 * m + 1) * NXH + kx];
+    }
+    double acc[kXinvParts] = {0, 0, 0, 0, 0, 0};
+    // round A: components x and y
+#pragma unroll
+    for (int k = 0; k < KA; ++k) {
+        const int idx = threadIdx.x + k * THREADS;
+        if (idx < LINES * NXH) xinv_put<R, N, LD>(buf, idx / NXH, idx % NXH, ha[k], hb[k]);
+    }
+    __syncthreads();
+    sfft_lines<R, +1, N, LINES, SH::T, LD>(buf, tw, nl);
+#pragma unroll
+    for (int k = 0; k < KE; ++k) {
+        const int idx = threadIdx.x + k * THREADS;
+        const int line = idx / N;
+        const cx<R> z = buf[line * LD + idx % N];
+        if (line < nl) {
+            const long vox = (line0 + line) * N + idx % N;
+            xinv_emit(sink, N3, 0, vox, code[k], (double)z.x * inv_n, acc);
+            xinv_emit(sink, N3, 1, vox, code[k], (double)z.y * inv_n, acc);
+        }
+    }
+    __syncthreads();
+    // round B: component z of lines 2m (real part) and 2m+1 (imaginary part)
+#pragma unroll
+    for (int k = 0; k < KB; ++k) {
+        const int idx = threadIdx.x + k * THREADS;
+        if (idx < (LINES / 2) * NXH) xinv_put<R, N, LD>(buf, idx / NXH, idx % NXH, ga[k], gb[k]);
+    }
+    __syncthreads();
+    sfft_lines<R, +1, N, LINES, SH::T, LD>(buf, tw, (nl + 1) / 2);
+    const R* bufr = reinterpret_cast<const R*>(buf);
+#pragma unroll
+    for (int k = 0; k < KE; ++k) {
+        const int idx = threadIdx.x + k * THREADS;
+        const int line = idx / N;
+        const double val = (double)bufr[2 * ((line >> 1) * LD + idx % N) + (line & 1)] * inv_n;
+        if (line < nl) xinv_emit(sink, N3, 2, (line0 + line) * N + idx % N, code[k], val, acc);
+    }
+    if (sink.partials != nullptr) block_sum<kXinvParts>(acc, sink.partials + (long)kXinvParts * blockIdx.x);
+}
+
+template <typename R, int N, int V>
+long xfwd_variant(const XSource& src, cx<R>* S, const Geometry3& g, const cx<R>* tw, const int* skip,
+                  hipStream_t st) {
+    constexpr int LINES = pass_lines<N>(V), OS = pass_os(V);
+    using SH = PassShape<R, N, LINES, OS>;
+    if constexpr (!SH::valid) {
+        return -1;
+    } else {
+        if (SH::LDS > lds_limit()) return -1;
+        const long nlines = (long)g.ny * g.nz;
+        const long blocks = (nlines + LINES - 1) / LINES;
+        hipLaunchKernelGGL((k_xfwd_s<R, N, LINES, OS>), dim3((unsigned)blocks), dim3(SH::THREADS), SH::LDS, st, src,
+                           S, nlines, g.N, g.Nh, tw, skip);
+        return blocks;
+    }
+}
+
+template <typename R, int N, int V>
+long xinv_variant(const cx<R>* S, const XSink& sink, const Geometry3& g, const cx<R>* tw, const int* skip,
+                  hipStream_t st) {
+    constexpr int LINES = pass_lines<N>(V), OS = pass_os(V);
+    using SH = PassShape<R, N, LINES, OS>;
+    if constexpr (!SH::valid) {
+        return -1;
+    } else {
+        if (SH::LDS > lds_limit()) return -1;
+        const long nlines = (long)g.ny * g.nz;
+        const long blocks = (nlines + LINES - 1) / LINES;
+        const double inv_n = 1.0 / (double)g.N;
+        hipLaunchKernelGGL((k_xinv_s<R, N, LINES, OS>), dim3((unsigned)blocks), dim3(SH::THREADS), SH::LDS, st, S,
+                           sink, nlines, g.N, g.Nh, tw, inv_n, skip);
+        return blocks;
+    }
+}
+
+template <typename R, int N>
+long xfwd_n(int v, const XSource& src, cx<R>* S, const Geometry3& g, const cx<R>* tw, const int* skip,
+            hipStream_t st) {
+    switch (v) {
+        case 0: return xfwd_variant<R, N, 0>(src, S, g, tw, skip, st);
+        case 1: return xfwd_variant<R, N, 1>(src, S, g, tw, skip, st);
+        case 2: return xfwd_variant<R, N, 2>(src, S, g, tw, skip, st);
+        case 3: return xfwd_variant<R, N, 3>(src, S, g, tw, skip, st);
+        case 4: return xfwd_variant<R, N, 4>(src, S, g, tw, skip, st);
+        default: return -1;
+    }
+}
+
+template <typename R, int N>
+long xinv_n(int v, const cx<R>* S, const XSink& sink, const Geometry3& g, const cx<R>* tw, const int* skip,
+            hipStream_t st) {
+    switch (v) {
+        case 0: return xinv_variant<R, N, 0>(S, sink, g, tw, skip, st);
+        case 1: return xinv_variant<R, N, 1>(S, sink, g, tw, skip, st);
+        case 2: return xinv_variant<R, N, 2>(S, sink, g, tw, skip, st);
+        case 3: return xinv_variant<R, N, 3>(S, sink, g, tw, skip, st);
+        case 4: return xinv_variant<R, N, 4>(S, sink, g, tw, skip, st);
+        default: return -1;
+    }
+}
+
+template <int N>
+int code_of(int v) {
+    return 10 * pass_lines<N>(v) + pass_os(v);
+}
+
+}  // namespace
+
+template <typename R>
+long static_xfwd(int v, const XSource& src, cx<R>* S, const Geometry3& g, const cx<R>* tw, const int* skip,
+                 hipStream_t st) {
+    switch (g.nx) {
+        case 32: return xfwd_n<R, 32>(v, src, S, g, tw, skip, st);
+        case 64: return xfwd_n<R, 64>(v, src, S, g, tw, skip, st);
+        case 128: return xfwd_n<R, 128>(v, src, S, g, tw, skip, st);
+        case 256: return xfwd_n<R, 256>(v, src, S, g, tw, skip, st);
+        case 512: return xfwd_n<R, 512>(v, src, S, g, tw, skip, st);
+        default: return -1;
+    }
+}
+
+template <typename R>
+long static_xinv(int v, const cx<R>* S, const XSink& sink, const Geometry3& g, const cx<R>* tw, const int* skip,
+                 hipStream_t st) {
+    switch (g.nx) {
+        case 32: return xinv_n<R, 32>(v, S, sink, g, tw, skip, st);
+        case 64: return xinv_n<R, 64>(v, S, sink, g, tw, skip, st);
+        case 128: return xinv_n<R, 128>(v, S, sink, g, tw, skip, st);
+        case 256: return xinv_n<R, 256>(v, S, sink, g, tw, skip, st);
+        case 512: return xinv_n<R, 512>(v, S, sink, g, tw, skip, st);
+        default: return -1;
+    }
+}
+
+int pass_variant_code(int n, int v) {
+    if (v < 0 || v >= kPassVariants) return 0;
+    switch (n) {
+        case 32: return code_of<32>(v);
+        case 64: return code_of<64>(v);
+        case 128: return code_of<128>(v);
+        case 256: return code_of<256>(v);
+        case 512: return code_of<512>(v);
+        default: return 0;
+    }
+}
+
+template long static_xfwd<double>(int, const XSource&, cx<double>*, const Geometry3&, const cx<double>*, const int*,
+                                  hipStream_t);
+template long static_xfwd<float>(int, const XSource&, cx<float>*, const Geometry3&, const cx<float>*, const int*,
+                                 hipStream_t);
+template long static_xinv<double>(int, const cx<double>*, const XSink&, const Geometry3&, const cx<double>*,
+                                  const int*, hipStream_t);
+template long static_xinv<float>(int, const cx<float>*, const XSink&, const Geometry3&, const cx<float>*, const int*,
+                                 hipStream_t);
+
+}  // namespace fftperm

@@ -125,6 +125,11 @@ class GreenOperator:
         self._h = h
         self._tensor = None
 
+    def set_refinement(self, fp32_below=2.0, inner_floor=3e-6, max_sweeps=64):
+        """MIXED mode precision schedule: sweeps to max(inner_floor, 0.5 tol / r); the first one applies
+        the FP64 operator while its recurrence residual is above fp32_below (>= 1: FP32 throughout)."""
+        check(_lib.load().fftperm_set_refinement(self._h, float(fp32_below), float(inner_floor), int(max_sweeps)))
+
     def close(self):
         if getattr(self, "_h", None) and self._h.value:
             _lib.load().fftperm_destroy(self._h)

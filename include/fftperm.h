@@ -126,6 +126,15 @@ int fftperm_operator_apply(fftperm_ctx* ctx, const double* x, double* y);
 int fftperm_minres(fftperm_ctx* ctx, const double* b, const double* x0, double tol, uint64_t maxit,
                    double* x, double* history, fftperm_solve_info* info);
 
+/* MIXED-mode precision schedule (no counterpart in the reference, which builds one precision per
+ * binary).  Every sweep is a MINRES run on r = b - A x to the recurrence residual
+ * max(inner_floor, 0.5 tol / r), r = the current FP64 true relative residual; sweep 1 applies the
+ * FP64 operator while its recurrence residual is above fp32_below and the FP32 operator below it
+ * (0: FP64 throughout, >= 1: FP32 throughout), later sweeps FP32 throughout; max_sweeps sweeps at
+ * most.  Defaults 2 (FP32 throughout), 3e-6, 64.  fp32_below < 0, inner_floor outside (0, 1) or
+ * max_sweeps == 0 -> FFTPERM_EINVAL. */
+int fftperm_set_refinement(fftperm_ctx* ctx, double fp32_below, double inner_floor, uint32_t max_sweeps);
+
 /* VelocityRecovery::recover + average_over_domain (src/bignonnet/VelocityRecovery.cpp:12-47):
  * u (3N, may be NULL) and its domain average avg[3]. */
 int fftperm_recover(fftperm_ctx* ctx, const double grad[3], const double* x, double* u, double avg[3]);
@@ -153,8 +162,20 @@ typedef struct {
     double apply_ms[2];
     uint64_t apply_launches[2];
     uint64_t N, Nh, nb;          /* grid size, half-spectrum size, support voxels */
-    int zgreen_tile[2];          /* autotuned z-pass variant 10*columns + threads per butterfly [FP64, FP32]; 0 = not tuned */
+    int zgreen_tile[2];          /* fused z-pass variant in use: 10*columns + threads per butterfly [FP64, FP32]; 0 = runtime kernel */
+    int xfwd_tile[2];            /* x-forward variant in use: 10*lines + threads per butterfly [FP64, FP32]; 0 = runtime kernel */
+    int xinv_tile[2];            /* x-inverse variant in use, same code */
+    int ypass_tile[2];           /* y-pass variant in use, same code */
 } fftperm_profile_stats;
+
+/* Pin the tile variant a pass uses on this context's grid shape, process-wide (no counterpart in the
+ * reference; the autotuner chooses these otherwise).  kind: FFTPERM_PASS_XFWD, _XINV, _YPASS,
+ * _ZGREEN; precision: FFTPERM_PRECISION_FP64 or _MIXED (the FP32 transforms); code as reported in
+ * fftperm_profile_stats (10 * lines or columns per workgroup + threads per butterfly thread, or + 4
+ * for the register-resident z pass); code 0 returns the choice to the autotuner.  An unknown code
+ * -> FFTPERM_EINVAL; a known code that does not fit the device runs the generic kernel. */
+enum { FFTPERM_PASS_XFWD = 0, FFTPERM_PASS_XINV = 1, FFTPERM_PASS_YPASS = 2, FFTPERM_PASS_ZGREEN = 3 };
+int fftperm_pin_variant(fftperm_ctx* ctx, int kind, int precision, int code);
 
 int fftperm_profile(fftperm_ctx* ctx, int enable);
 int fftperm_profile_read(fftperm_ctx* ctx, fftperm_profile_stats* out);

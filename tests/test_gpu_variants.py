@@ -1,0 +1,142 @@
+"""Every tile variant of the specialised passes, pinned one at a time, against the FP64 oracle.
+
+The autotuner picks one variant per (pass, precision, length, count) at run time, so the parity
+tests elsewhere only see the variant that won on the test box.  Here each variant is forced
+through fftperm_pin_variant and the Green apply / variational operator are compared with the
+oracle (FP64: relative Frobenius error < 1e-13 / 1e-12).  FP32 variants (the transforms of the
+mixed-precision solves) are checked through the mixed K tensor of the 32^3 sphere cell: within
+1e-6 of the oracle's K, the contract of the mixed path.
+"""
+import contextlib
+
+import numpy as np
+import pytest
+
+from conftest import random_mask, rel
+
+pytestmark = pytest.mark.gpu
+
+# largest radix of the compile-time plan per length (fft_static.hpp StaticPlan)
+_MAX_RADIX = {32: 8, 64: 8, 128: 16, 256: 16, 512: 8}
+
+
+def pass_codes(n):
+    """Codes (10 * lines per workgroup + threads per butterfly thread) of the x / column pass
+    variants at length n, as fft_static.hpp enumerates them."""
+    t = n // _MAX_RADIX[n]
+    base = max(2, 256 // t)
+    out = []
+    for v in range(5):
+        lines = base if v < 2 else (base // 2 if v < 4 else base // 4)
+        os_ = 2 if v in (0, 2) else 4
+        if lines >= 2 and lines % 2 == 0 and 64 <= os_ * lines * t <= 1024:
+            out.append(10 * lines + os_)
+    return out
+
+
+@contextlib.contextmanager
+def pinned(green, kind, precision, code):
+    from fft_method_test_amd import _lib
+
+    _lib.check(_lib.load().fftperm_pin_variant(green._h, kind, precision, code))
+    try:
+        yield
+    finally:
+        _lib.check(_lib.load().fftperm_pin_variant(green._h, kind, precision, 0))
+
+
+def _pair(oracle_mod, shape, seed):
+    from fft_method_test_amd import permeability as P
+
+    mask = random_mask(shape, 0.6, seed=seed)
+    g = P.Grid3D(*shape)
+    green = P.GreenOperator(g)
+    sup = P.ForceSupport(P.BinaryMedium(g, mask), P.SUPPORT_INTERFACE, green)
+    o = oracle_mod.Oracle(shape)
+    o.set_medium(mask)
+    return green, sup, o
+
+
+def _check_apply(green, sup, o, shape, seed):
+    from fft_method_test_amd import permeability as P
+
+    N = int(np.prod(shape))
+    f = np.random.default_rng(seed).standard_normal(3 * N)
+    assert rel(green.apply(f), o.green_apply(f)) < 1e-13
+    x = np.random.default_rng(seed + 1).standard_normal(sup.size())
+    assert rel(P.VariationalOperator(sup, green).apply(x), o.apply(x)) < 1e-12
+
+
+# one shape per axis and length: the varied axis carries the specialised length
+X_SHAPES = {n: (n, 6, 10) for n in (32, 64, 128, 256, 512)}
+Y_SHAPES = {n: (6, n, 10) for n in (32, 64, 128, 256, 512)}
+Z_SHAPES = {n: (6, 10, n) for n in (32, 64, 128, 256, 512)}
+
+
+@pytest.mark.parametrize("n", [32, 64, 128, 256, 512])
+def test_x_pass_variants_fp64(fp, oracle_mod, n):
+    from fft_method_test_amd import _lib
+
+    shape = X_SHAPES[n]
+    green, sup, o = _pair(oracle_mod, shape, seed=n)
+    for code in pass_codes(n):
+        for kind in (_lib.PASS_XFWD, _lib.PASS_XINV):
+            with pinned(green, kind, _lib.PRECISION_FP64, code):
+                _check_apply(green, sup, o, shape, seed=code)
+
+
+@pytest.mark.parametrize("n", [32, 64, 128, 256, 512])
+def test_y_pass_variants_fp64(fp, oracle_mod, n):
+    from fft_method_test_amd import _lib
+
+    shape = Y_SHAPES[n]
+    green, sup, o = _pair(oracle_mod, shape, seed=n + 1)
+    for code in pass_codes(n):
+        with pinned(green, _lib.PASS_YPASS, _lib.PRECISION_FP64, code):
+            _check_apply(green, sup, o, shape, seed=code)
+
+
+@pytest.mark.parametrize("n", [32, 64, 128, 256, 512])
+def test_zgreen_variants_fp64(fp, oracle_mod, n):
+    from fft_method_test_amd import _lib
+
+    shape = Z_SHAPES[n]
+    green, sup, o = _pair(oracle_mod, shape, seed=n + 2)
+    for code in _lib.ZGREEN_CODES:
+        with pinned(green, _lib.PASS_ZGREEN, _lib.PRECISION_FP64, code):
+            _check_apply(green, sup, o, shape, seed=code)
+
+
+def test_zgreen_variants_minres_alpha(fp, oracle_mod):
+    """The fused z pass also returns the Lanczos alpha (Parseval): MINRES with each variant pinned
+    follows the oracle's recurrence (first 12 residual estimates within 1e-9 relative)."""
+    from fft_method_test_amd import _lib, permeability as P
+
+    shape = (16, 12, 32)
+    green, sup, o = _pair(oracle_mod, shape, seed=9)
+    b = o.rhs([1.0, 0.0, 0.0])
+    _, _, ref_hist = o.minres(b, tol=1e-10, maxit=12)
+    for code in _lib.ZGREEN_CODES:
+        with pinned(green, _lib.PASS_ZGREEN, _lib.PRECISION_FP64, code):
+            res = P.MinresSolver(1e-10, 12).solve(P.VariationalOperator(sup, green), b)
+            h = np.asarray(res.residual_history)
+            assert rel(h[:12], ref_hist[:12]) < 1e-9, code
+
+
+def test_fp32_variants_mixed_k_sphere32(fp, oracle_mod):
+    from fft_method_test_amd import _lib, geometry as G, permeability as P
+
+    n = 32
+    mask = G.sphere_cell(n, 0.3).reshape(-1)
+    o = oracle_mod.Oracle((n, n, n))
+    o.set_medium(mask)
+    k_ref, _ = o.solve_tensor(tol=1e-10, maxit=10000)
+    g = P.Grid3D(n, n, n)
+    green = P.GreenOperator(g, 1.0, 2, P.PRECISION_MIXED)
+    sup = P.ForceSupport(P.BinaryMedium(g, mask), P.SUPPORT_INTERFACE, green)
+    cases = [(_lib.PASS_ZGREEN, c) for c in _lib.ZGREEN_CODES]
+    cases += [(k, c) for k in (_lib.PASS_XFWD, _lib.PASS_XINV, _lib.PASS_YPASS) for c in pass_codes(n)]
+    for kind, code in cases:
+        with pinned(green, kind, _lib.PRECISION_MIXED, code):
+            K, _ = P.solve_permeability_tensor(green, 1e-9, 10000)
+            assert rel(K, k_ref) < 1e-6, (kind, code)
