@@ -28,9 +28,11 @@ EXPORTED_SYMBOLS = (
     "fftperm_set_medium", "fftperm_set_medium_dev", "fftperm_porosity", "fftperm_get_support",
     "fftperm_green_tensor", "fftperm_fft_forward", "fftperm_fft_inverse", "fftperm_spectral_green",
     "fftperm_green_apply", "fftperm_background_force", "fftperm_rhs", "fftperm_operator_apply",
-    "fftperm_minres", "fftperm_set_refinement", "fftperm_pin_variant", "fftperm_recover", "fftperm_solve_direction", "fftperm_solve_tensor",
-    "fftperm_residual_norm", "fftperm_symmetry_gap", "fftperm_profile", "fftperm_profile_read",
-    "fftperm_brinkman_solve",
+    "fftperm_minres", "fftperm_set_refinement", "fftperm_pin_variant", "fftperm_recover",
+    "fftperm_solve_direction", "fftperm_solve_tensor", "fftperm_residual_norm", "fftperm_symmetry_gap",
+    "fftperm_profile", "fftperm_profile_read", "fftperm_unet_create", "fftperm_unet_destroy",
+    "fftperm_unet_num_params", "fftperm_unet_get_params", "fftperm_unet_set_params", "fftperm_unet_loss_grad",
+    "fftperm_unet_predict", "fftperm_solve_tensor_warm", "fftperm_brinkman_solve",
 )
 
 
@@ -70,6 +72,10 @@ def load():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m fft_method_test_amd.build`")
+    # One HIP runtime per process: torch ships its own libamdhip64 and binds to whichever copy of
+    # that soname is loaded first.  Importing torch before this library makes both use torch's
+    # copy; loading this library first left torch with no usable device.
+    import torch  # noqa: F401
     L = C.CDLL(LIB_PATH)
     L.fftperm_last_error.restype = C.c_char_p
     L.fftperm_device_info.argtypes = [C.c_char_p, C.c_size_t]
@@ -99,6 +105,15 @@ def load():
     L.fftperm_profile_read.argtypes = [_vp, C.POINTER(ProfileStats)]
     L.fftperm_brinkman_solve.argtypes = [C.c_uint64] * 3 + [_dp, _dp, C.c_double, C.c_double, C.c_int, _dp, C.c_int,
                                                              _dp, _vp, C.POINTER(BrinkmanInfo)]
+    _fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+    L.fftperm_unet_create.argtypes = [C.c_uint64, C.POINTER(_vp)]
+    L.fftperm_unet_destroy.argtypes = [_vp]
+    L.fftperm_unet_num_params.argtypes = [_vp, C.POINTER(C.c_uint64)]
+    L.fftperm_unet_get_params.argtypes = [_vp, _fp]
+    L.fftperm_unet_set_params.argtypes = [_vp, _fp]
+    L.fftperm_unet_loss_grad.argtypes = [_vp, _vp, _vp, C.c_uint32, C.c_float, C.POINTER(C.c_double), _vp]
+    L.fftperm_unet_predict.argtypes = [_vp, _vp, C.c_uint32, _vp]
+    L.fftperm_solve_tensor_warm.argtypes = [_vp, _vp, C.c_double, C.c_uint64, _dp, _vp, _vp]
     for name in EXPORTED_SYMBOLS:
         getattr(L, name).restype = C.c_char_p if name == "fftperm_last_error" else C.c_int
     _lib = L

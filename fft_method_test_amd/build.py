@@ -14,6 +14,7 @@ LIB = os.path.join(HERE, "libfftperm.so")
 ARCH = os.environ.get("FFTPERM_ARCH", "gfx950")
 
 SOURCES = ["api.hip", "kernels_fft.hip", "passes_x.hip", "passes_col.hip", "kernels_setup.hip", "kernels_krylov.hip",
+           "unet.hip",
            "brinkman.hip"]
 
 HIPCC_FLAGS = [
@@ -54,7 +55,8 @@ def build(verbose: bool = False, jobs: int = 7) -> str:
     for p in procs:
         _wait(p)
     tmp = LIB + ".tmp"
-    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
+    # --no-undefined: a symbol missing from the objects fails here, not when the library is loaded
+    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-Wl,--no-undefined", "-o", tmp, *objs]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)

@@ -16,6 +16,7 @@
 #include "brinkman.hpp"
 #include "kernels.hpp"
 #include "minres.hpp"
+#include "unet.hpp"
 
 using namespace fftperm;
 
@@ -1259,6 +1260,105 @@ int fftperm_profile_read(fftperm_ctx* ctx, fftperm_profile_stats* out) {
             out->xfwd_tile[i] = variant_code(kKindXfwd, i, g.nx, (long)g.ny * g.nz);
             out->xinv_tile[i] = variant_code(kKindXinv, i, g.nx, (long)g.ny * g.nz);
             out->ypass_tile[i] = variant_code(kKindCol, i, g.ny, (long)g.nz * g.nxh);
+        }
+    });
+}
+
+int fftperm_unet_create(uint64_t seed, fftperm_unet** out) {
+    return guarded([&] {
+        if (!out) throw Err(FFTPERM_EINVAL, "null argument");
+        *out = reinterpret_cast<fftperm_unet*>(unet::create(seed));
+    });
+}
+
+int fftperm_unet_destroy(fftperm_unet* net) {
+    return guarded([&] { unet::destroy(reinterpret_cast<unet::Net*>(net)); });
+}
+
+int fftperm_unet_num_params(fftperm_unet* net, uint64_t* count) {
+    return guarded([&] {
+        if (!net || !count) throw Err(FFTPERM_EINVAL, "null argument");
+        *count = (uint64_t)unet::num_params(reinterpret_cast<unet::Net*>(net));
+    });
+}
+
+int fftperm_unet_get_params(fftperm_unet* net, float* params) {
+    return guarded([&] {
+        if (!net || !params) throw Err(FFTPERM_EINVAL, "null argument");
+        unet::get_params(reinterpret_cast<unet::Net*>(net), params);
+    });
+}
+
+int fftperm_unet_set_params(fftperm_unet* net, const float* params) {
+    return guarded([&] {
+        if (!net || !params) throw Err(FFTPERM_EINVAL, "null argument");
+        unet::set_params(reinterpret_cast<unet::Net*>(net), params);
+    });
+}
+
+namespace {
+void check_unet_edge(uint32_t n) {
+    if (n < 8 || n % 8 != 0) throw Err(FFTPERM_EUNSUPPORTED, "U-Net grid edge must be a positive multiple of 8");
+}
+}  // namespace
+
+int fftperm_unet_loss_grad(fftperm_unet* net, const uint8_t* fluid_mask_dev, const float* target_dev, uint32_t n,
+                           float lr, double* loss, float* grads) {
+    return guarded([&] {
+        if (!net || !fluid_mask_dev || !target_dev) throw Err(FFTPERM_EINVAL, "null argument");
+        check_unet_edge(n);
+        auto* u = reinterpret_cast<unet::Net*>(net);
+        const double l = unet::loss_grad(u, fluid_mask_dev, target_dev, (int)n, false, 0.f);
+        if (grads) unet::get_grads(u, grads);
+        if (lr > 0.f) unet::loss_grad(u, fluid_mask_dev, target_dev, (int)n, true, lr);
+        if (loss) *loss = l;
+    });
+}
+
+int fftperm_unet_predict(fftperm_unet* net, const uint8_t* fluid_mask_dev, uint32_t n, float* field_dev) {
+    return guarded([&] {
+        if (!net || !fluid_mask_dev || !field_dev) throw Err(FFTPERM_EINVAL, "null argument");
+        check_unet_edge(n);
+        auto* u = reinterpret_cast<unet::Net*>(net);
+        unet::predict(u, fluid_mask_dev, (int)n, 0);
+        unet::pred_field(u, field_dev);
+    });
+}
+
+int fftperm_solve_tensor_warm(fftperm_ctx* ctx, fftperm_unet* net, double tol, uint64_t maxit, double K[9],
+                              fftperm_solve_info* info, int* used_guess) {
+    return guarded([&] {
+        check_ctx(ctx);
+        if (!net || !K) throw Err(FFTPERM_EINVAL, "null argument");
+        const Geometry3& g = ctx->g;
+        if (g.nx != g.ny || g.nx != g.nz) throw Err(FFTPERM_EUNSUPPORTED, "the warm start needs a cubic grid");
+        check_unet_edge((uint32_t)g.nx);
+        ctx->require_support();
+        ctx->ensure_vectors();
+        auto* u = reinterpret_cast<unet::Net*>(net);
+        const long L = ctx->L3();
+        for (int col = 0; col < 3; ++col) {
+            double grad[3] = {0.0, 0.0, 0.0};
+            grad[col] = 1.0;
+            device_rhs(ctx, grad, ctx->b);
+            ctx->sync();
+            // the network predicts the x-direction density of the axis-swapped medium
+            double* x0 = (ctx->precision == FFTPERM_PRECISION_MIXED) ? ctx->dx : ctx->tmp;
+            unet::predict(u, ctx->mask, g.nx, col);
+            unet::gather_guess(u, col, ctx->active, (long)ctx->nb, x0);
+            // keep the guess only when it lowers the FP64 true residual below ||b||
+            const double bnorm = device_norm(ctx, ctx->b, L);
+            device_apply<double>(ctx, x0, ctx->r);
+            launch_axpby(1.0, ctx->b, -1.0, ctx->r, ctx->r, L, ctx->s);
+            const double rnorm = device_norm(ctx, ctx->r, L);
+            const bool use = std::isfinite(rnorm) && rnorm < bnorm;
+            if (used_guess) used_guess[col] = use ? 1 : 0;
+            uint64_t sweeps = 0;
+            MinresOut o = run_minres(ctx, ctx->b, use ? x0 : nullptr, tol, maxit, &sweeps, nullptr);
+            double avg[3];
+            device_recover(ctx, grad, ctx->x, avg);
+            fill_info(info ? &info[col] : nullptr, o, sweeps);
+            for (int r = 0; r < 3; ++r) K[r * 3 + col] = ctx->mu * avg[r];
         }
     });
 }

@@ -180,6 +180,34 @@ int fftperm_pin_variant(fftperm_ctx* ctx, int kind, int precision, int code);
 int fftperm_profile(fftperm_ctx* ctx, int enable);
 int fftperm_profile_read(fftperm_ctx* ctx, fftperm_profile_stats* out);
 
+/* ------------------------------------------------------------------ learned warm start
+ * No counterpart in the reference.  A 4-level 3D U-Net (16 base channels, BF16 weights and
+ * activations, FP32 accumulation on the matrix cores) maps a periodic fluid mask to the interface
+ * force density of the x-direction variational problem; the y and z directions use the axis-swapped
+ * medium (the Stokes problem is isotropic and the cell cubic) with the components exchanged back.
+ * Grids: cubic, edge a positive multiple of 8. */
+typedef struct fftperm_unet fftperm_unet;
+
+int fftperm_unet_create(uint64_t seed, fftperm_unet** out);
+int fftperm_unet_destroy(fftperm_unet* net);
+int fftperm_unet_num_params(fftperm_unet* net, uint64_t* count);
+/* flat FP32 parameters: per layer the weights [cout][27 or 1 taps][cin] then the biases [cout] */
+int fftperm_unet_get_params(fftperm_unet* net, float* params);
+int fftperm_unet_set_params(fftperm_unet* net, const float* params);
+/* Forward, mean squared error over the support voxels of the three predicted components, backward;
+ * with lr > 0 one Adam step (beta 0.9 / 0.999).  fluid_mask_dev (n^3 uint8, fluid = 1) and
+ * target_dev (n^3 x 3 FP32, the force density on support voxels) are device pointers.  loss and
+ * grads (num_params FP32, may be NULL) receive the loss and its gradient before the step. */
+int fftperm_unet_loss_grad(fftperm_unet* net, const uint8_t* fluid_mask_dev, const float* target_dev, uint32_t n,
+                           float lr, double* loss, float* grads);
+/* Prediction for the x direction: field_dev (device) n^3 x 3 FP32. */
+int fftperm_unet_predict(fftperm_unet* net, const uint8_t* fluid_mask_dev, uint32_t n, float* field_dev);
+/* fftperm_solve_tensor with the network's prediction (gathered on the support) as each direction's
+ * MINRES initial guess.  MINRES stays the judge: a guess whose FP64 true residual is not below ||b||
+ * is dropped and that direction starts cold.  used_guess[3] (may be NULL) reports the choice. */
+int fftperm_solve_tensor_warm(fftperm_ctx* ctx, fftperm_unet* net, double tol, uint64_t maxit, double K[9],
+                              fftperm_solve_info* info, int* used_guess);
+
 /* Brinkman penalisation: common::fluid::solveBrinkman (src/brinkman/FluidBrinkman.cpp:282-507) for
  * one forcing vector on an (nx,ny,nz) grid.  alpha / viscosity: per-voxel coefficient fields
  * (buildCoefficientFields, :152-191: alpha = penalty in solid and 0 in fluid; viscosity = the fluid's

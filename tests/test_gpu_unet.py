@@ -1,0 +1,137 @@
+"""GPU tests of the learned warm start: the BF16 MFMA U-Net against a plain PyTorch FP32 restatement
+(tests/unet_torch_ref.py), its training kernels, and the warm-started tensor solve.
+
+Tolerances: activations and weights are BF16 on the device (8 significant bits), so the forward
+output is compared with the reference that rounds its activations the same way (relative
+Frobenius error < 2e-2), and the gradients with the FP32 reference (relative error < 6e-2 over
+all parameters, cosine similarity > 0.99 per layer).
+"""
+import numpy as np
+import pytest
+import torch
+
+from conftest import random_mask, rel
+
+pytestmark = pytest.mark.gpu
+
+
+def _bf16_round(a):
+    return torch.as_tensor(np.asarray(a, dtype=np.float32)).to(torch.bfloat16).to(torch.float32).numpy()
+
+
+def _net(seed=3):
+    from fft_method_test_amd import warmstart as W
+
+    net = W.WarmStartNet(seed)
+    assert net.num_params == W.param_count()
+    p = _bf16_round(net.get_params())
+    # non-zero biases exercise the bias paths; scale keeps the activations O(1)
+    rng = np.random.default_rng(seed)
+    off = 0
+    for taps, c0, c1, co in W.LAYERS:
+        off += co * taps * (c0 + c1)
+        p[off:off + co] = _bf16_round(rng.uniform(-0.05, 0.05, co))
+        off += co
+    net.set_params(p)
+    return net, p
+
+
+def test_unet_params_roundtrip(fp):
+    net, p = _net()
+    assert np.array_equal(net.get_params(), p)
+
+
+@pytest.mark.parametrize("n", [16, 32])
+def test_unet_forward_matches_torch(fp, n):
+    import unet_torch_ref as R
+
+    net, p = _net()
+    mask = random_mask((n, n, n), 0.6, seed=n)
+    pred = net.predict(torch.from_numpy(mask).to("cuda:0"), n).cpu().numpy()
+    layers = R.unflatten(p)
+    with torch.no_grad():
+        ref = R.ncdhw_to_field(R.forward(layers, R.features(mask, n), bf16_acts=True))
+    assert rel(pred, ref) < 2e-2
+
+
+def test_unet_gradients_match_torch(fp):
+    import unet_torch_ref as R
+    from fft_method_test_amd import warmstart as W
+
+    n = 16
+    net, p = _net(seed=5)
+    mask = random_mask((n, n, n), 0.55, seed=11)
+    rng = np.random.default_rng(2)
+    target = rng.standard_normal((n ** 3, 3)).astype(np.float32) * 0.1
+    loss, g = net.loss_grad(torch.from_numpy(mask).to("cuda:0"), torch.from_numpy(target).to("cuda:0"), n,
+                            want_grad=True)
+    layers = R.unflatten(p)
+    for w, b in layers:
+        w.requires_grad_(True)
+        b.requires_grad_(True)
+    x = R.features(mask, n)
+    pred = R.forward(layers, x)
+    L = R.loss(pred, R.field_to_ncdhw(target, n), x[:, 1:2])
+    L.backward()
+    gr = R.flatten_grads(layers)
+    assert abs(loss - L.item()) <= 3e-2 * L.item()
+    assert rel(g, gr) < 6e-2
+    off = 0
+    for taps, c0, c1, co in W.LAYERS:
+        k = co * taps * (c0 + c1) + co
+        a, b = g[off:off + k], gr[off:off + k]
+        cos = float(a @ b / (np.linalg.norm(a) * np.linalg.norm(b) + 1e-30))
+        assert cos > 0.99, (taps, c0, c1, co, cos)
+        off += k
+
+
+def test_unet_training_reduces_loss(fp):
+    from fft_method_test_amd import warmstart as W
+
+    n = 16
+    net = W.WarmStartNet(7)
+    mask = random_mask((n, n, n), 0.6, seed=4)
+    # a local function of the medium (the network is translation-equivariant and sees only the
+    # mask, so the target must be one): fluid fraction of the 6 face neighbours, the centred
+    # x-difference of the fluid indicator, a constant
+    f = mask.reshape(n, n, n).astype(np.float32)
+    face = sum(np.roll(f, s, axis=a) for a in range(3) for s in (-1, 1)) / 6.0
+    dx = 0.5 * (np.roll(f, -1, axis=2) - np.roll(f, 1, axis=2))
+    t = np.stack([face, dx, np.full_like(f, 0.5)], -1)
+    target = torch.from_numpy(t.reshape(-1, 3).astype(np.float32)).to("cuda:0")
+    m = torch.from_numpy(mask).to("cuda:0")
+    first, _ = net.loss_grad(m, target, n, lr=2e-3)
+    for _ in range(60):
+        last, _ = net.loss_grad(m, target, n, lr=2e-3)
+    assert last < 0.5 * first
+
+
+def test_warm_start_solve_matches_cold(fp):
+    """Whatever the guess, the warm-started tensor equals the cold one to the solve tolerance, and
+    a guess is only used when it lowers the true residual."""
+    from fft_method_test_amd import geometry as G, permeability as P, warmstart as W
+
+    n = 32
+    mask = G.sphere_cell(n, 0.3).reshape(-1)
+    g = P.Grid3D(n, n, n)
+    green = P.GreenOperator(g)
+    P.ForceSupport(P.BinaryMedium(g, mask), P.SUPPORT_INTERFACE, green)
+    K_cold, infos_cold = P.solve_permeability_tensor(green, 1e-9, 10000)
+    net, _ = _net()
+    K_warm, infos_warm, used = W.solve_tensor_warm(green, net, 1e-9, 10000)
+    assert set(used) <= {0, 1}
+    assert rel(K_warm, K_cold) < 1e-7
+    for u, a, b in zip(used, infos_warm, infos_cold):
+        if not u:
+            assert a.iterations == b.iterations
+
+
+def test_warm_start_requires_cubic_multiple_of_8(fp):
+    from fft_method_test_amd import permeability as P, warmstart as W
+
+    g = P.Grid3D(12, 12, 12)
+    green = P.GreenOperator(g)
+    P.ForceSupport(P.BinaryMedium(g, random_mask((12, 12, 12), 0.6, seed=1)), P.SUPPORT_INTERFACE, green)
+    net = W.WarmStartNet(1)
+    with pytest.raises(Exception, match="multiple of 8"):
+        W.solve_tensor_warm(green, net, 1e-6, 100)
