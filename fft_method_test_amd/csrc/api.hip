@@ -173,6 +173,9 @@ struct fftperm_ctx {
     uint8_t* mask = nullptr;
     int32_t* code = nullptr;
     int64_t* active = nullptr;
+    uint32_t* lbits = nullptr;           // line bitmask of the support (specialised x passes)
+    int32_t* lstart = nullptr;           // support voxels before each x line
+    unsigned long long* lcount = nullptr;
     void* scan_scratch = nullptr;
     unsigned long long* counts = nullptr;
 
@@ -191,6 +194,26 @@ struct fftperm_ctx {
     MinresState* st = nullptr;
     double* hist = nullptr;
     size_t hist_cap = 0;
+
+    // hipGraphs of two MINRES iterations (two, so that the v / w pointer rotation ends where it
+    // started), keyed by everything a capture bakes in: precision, buffer addresses, sizes
+    struct GraphKey {
+        int fp32;
+        const void* p[9];
+        long nb, tuning;
+        bool operator==(const GraphKey& o) const {
+            if (fp32 != o.fp32 || nb != o.nb || tuning != o.tuning) return false;
+            for (int i = 0; i < 9; ++i)
+                if (p[i] != o.p[i]) return false;
+            return true;
+        }
+    };
+    struct GraphSlot {
+        GraphKey key;
+        hipGraphExec_t exec;
+    };
+    bool use_graphs = true;
+    std::vector<GraphSlot> graphs;
 
     // live kernel timing (fftperm_profile): HIP events on this context's stream around the
     // fused z-pass and around the whole five-sweep apply
@@ -246,10 +269,12 @@ struct fftperm_ctx {
 
     ~fftperm_ctx() {
         if (s) (void)hipStreamSynchronize(s);
+        for (GraphSlot& g : graphs) (void)hipGraphExecDestroy(g.exec);
         dfree(t64.twx); dfree(t64.twy); dfree(t64.twz);
         dfree(t32.twx); dfree(t32.twy); dfree(t32.twz);
         dfree(S64); dfree(S32); dfree(G64); dfree(G32); dfree(gtab);
         dfree(mask); dfree(code); dfree(active); dfree(scan_scratch); dfree(counts);
+        dfree(lbits); dfree(lstart); dfree(lcount);
         dfree(part_x); dfree(part_z); dfree(part); dfree(small);
         dfree(x); dfree(v_old); dfree(v); dfree(z); dfree(w_prev); dfree(w);
         dfree(b); dfree(tmp); dfree(r); dfree(dx);
@@ -354,10 +379,21 @@ void fftperm_ctx::chain<float>(const XSource& src, const XSink& sink, const int*
 
 namespace {
 
+LineBits line_bits(const fftperm_ctx* c) {
+    LineBits lb{};
+    if (c->has_medium && c->lbits != nullptr) {
+        lb.bits = c->lbits;
+        lb.lstart = c->lstart;
+        lb.nb = (long)c->nb;
+    }
+    return lb;
+}
+
 XSource src_none(const fftperm_ctx* c) {
     XSource sr;
     std::memset(&sr, 0, sizeof(sr));
     sr.code = c->code;
+    sr.lines = line_bits(c);
     return sr;
 }
 
@@ -365,6 +401,7 @@ XSink sink_none(const fftperm_ctx* c) {
     XSink sk;
     std::memset(&sk, 0, sizeof(sk));
     sk.code = c->code;
+    sk.lines = line_bits(c);
     return sk;
 }
 
@@ -444,6 +481,34 @@ void minres_iteration(fftperm_ctx* c) {
     // v_old <- v, v <- v_new (written over v_old); w_prev <- w, w <- w_next (written over w_prev)
     std::swap(c->v_old, c->v);
     std::swap(c->w_prev, c->w);
+}
+
+// Graph of two MINRES iterations for the current buffers (captured on first use).
+template <typename R>
+hipGraphExec_t minres_graph(fftperm_ctx* c) {
+    fftperm_ctx::GraphKey k{};
+    k.fp32 = sizeof(R) == 4 ? 1 : 0;
+    const void* ptrs[9] = {c->x, c->v_old, c->v, c->z, c->w_prev, c->w, c->st, c->hist, c->part_x};
+    for (int i = 0; i < 9; ++i) k.p[i] = ptrs[i];
+    k.nb = (long)c->nb;
+    k.tuning = tuning_generation();  // a pinned or re-tuned variant changes the captured launches
+    for (const auto& g : c->graphs)
+        if (g.key == k) return g.exec;
+    if (c->graphs.size() >= 8) {
+        HIPCHK(hipGraphExecDestroy(c->graphs.front().exec));
+        c->graphs.erase(c->graphs.begin());
+    }
+    hipGraph_t graph = nullptr;
+    HIPCHK(hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal));
+    minres_iteration<R>(c);
+    minres_iteration<R>(c);
+    HIPCHK(hipStreamEndCapture(c->s, &graph));
+    hipGraphExec_t exec = nullptr;
+    const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    HIPCHK(e);
+    c->graphs.push_back({k, exec});
+    return exec;
 }
 
 struct MinresOut {
@@ -533,7 +598,16 @@ MinresOut device_minres(fftperm_ctx* c, const double* bdev, const double* x0dev,
     int done = h.done;
     while (!done && launched < maxit) {
         const uint64_t k = std::min<uint64_t>(chunk, maxit - launched);
-        for (uint64_t i = 0; i < k; ++i) {
+        // pairs of iterations replay a captured graph (not while live timing records events)
+        const bool graph = c->use_graphs && !c->prof;
+        uint64_t i = 0;
+        if (graph) {
+            for (; i + 2 <= k; i += 2) {
+                hipGraphExec_t ex = use32 ? minres_graph<float>(c) : minres_graph<double>(c);
+                HIPCHK(hipGraphLaunch(ex, c->s));
+            }
+        }
+        for (; i < k; ++i) {
             c->cur_tag = (long long)(launched + i);
             if (use32)
                 minres_iteration<float>(c);
@@ -717,12 +791,8 @@ void tune_x_passes(fftperm_ctx* c) {
     if (!need64 && !need32) return;
     c->ensure_vectors();
     HIPCHK(hipMemsetAsync(c->x, 0, c->L3() * sizeof(double), c->s));
-    XSource sr = src_none(c);
+    XSource sr = src_none(c);  // f0 = 0: the source of every MINRES apply
     sr.compact = c->x;
-    for (int k = 0; k < 3; ++k) {
-        sr.a_fluid[k] = 1.0;
-        sr.a_support[k] = 0.5;
-    }
     XSink sk = sink_none(c);
     sk.compact = c->tmp;
     sk.partials = c->part_x;
@@ -751,6 +821,11 @@ void set_medium_impl(fftperm_ctx* c, const uint8_t* mask, bool host, int mode, u
     HIPCHK(hipMemsetAsync(c->counts, 0, 2 * sizeof(unsigned long long), c->s));
     launch_support(c->mask, c->g.nx, c->g.ny, c->g.nz, mode == FFTPERM_SUPPORT_INTERFACE ? 1 : 0, c->code, c->active,
                    c->counts, c->scan_scratch, c->s);
+    const long nlines = (long)c->g.ny * c->g.nz;
+    if (!c->lbits) c->lbits = dalloc<uint32_t>((size_t)nlines * ((c->g.nx + 31) / 32));
+    if (!c->lstart) c->lstart = dalloc<int32_t>((size_t)nlines);
+    if (!c->lcount) c->lcount = dalloc<unsigned long long>(1);
+    launch_line_bits(c->code, c->g.nx, nlines, c->lbits, c->lstart, c->lcount, c->s);
     unsigned long long cnt[2];
     HIPCHK(hipMemcpyAsync(cnt, c->counts, sizeof(cnt), hipMemcpyDeviceToHost, c->s));
     c->sync();
@@ -1228,6 +1303,13 @@ int fftperm_pin_variant(fftperm_ctx* ctx, int kind, int precision, int code) {
     });
 }
 
+int fftperm_set_graphs(fftperm_ctx* ctx, int enable) {
+    return guarded([&] {
+        check_ctx(ctx);
+        ctx->use_graphs = enable != 0;
+    });
+}
+
 int fftperm_profile(fftperm_ctx* ctx, int enable) {
     return guarded([&] {
         check_ctx(ctx);
@@ -1308,9 +1390,9 @@ int fftperm_unet_loss_grad(fftperm_unet* net, const uint8_t* fluid_mask_dev, con
         if (!net || !fluid_mask_dev || !target_dev) throw Err(FFTPERM_EINVAL, "null argument");
         check_unet_edge(n);
         auto* u = reinterpret_cast<unet::Net*>(net);
-        const double l = unet::loss_grad(u, fluid_mask_dev, target_dev, (int)n, false, 0.f);
+        // the Adam step reads the gradient buffers without changing them, so one pass serves both
+        const double l = unet::loss_grad(u, fluid_mask_dev, target_dev, (int)n, lr > 0.f, lr);
         if (grads) unet::get_grads(u, grads);
-        if (lr > 0.f) unet::loss_grad(u, fluid_mask_dev, target_dev, (int)n, true, lr);
         if (loss) *loss = l;
     });
 }
@@ -1346,12 +1428,23 @@ int fftperm_solve_tensor_warm(fftperm_ctx* ctx, fftperm_unet* net, double tol, u
             double* x0 = (ctx->precision == FFTPERM_PRECISION_MIXED) ? ctx->dx : ctx->tmp;
             unet::predict(u, ctx->mask, g.nx, col);
             unet::gather_guess(u, col, ctx->active, (long)ctx->nb, x0);
-            // keep the guess only when it lowers the FP64 true residual below ||b||
+            // the network fixes the shape of the density, the scale comes from the minimal-residual
+            // step along it: x0 = alpha p, alpha = <b, A p> / ||A p||^2, so ||b - A x0|| <= ||b||;
+            // the guess is kept only when it lowers the FP64 true residual below ||b||
             const double bnorm = device_norm(ctx, ctx->b, L);
             device_apply<double>(ctx, x0, ctx->r);
-            launch_axpby(1.0, ctx->b, -1.0, ctx->r, ctx->r, L, ctx->s);
+            auto dot = [&](const double* a, const double* bb) {
+                launch_dot_partials(a, bb, L, ctx->part, ctx->s);
+                launch_finish_sum(ctx->part, kRedBlocks, 1, 1.0, ctx->small + 16, nullptr, ctx->s);
+                return ctx->host_scalar(ctx->small + 16);
+            };
+            const double bap = dot(ctx->b, ctx->r);
+            const double apap = dot(ctx->r, ctx->r);
+            const double alpha = (apap > 0.0 && std::isfinite(apap)) ? bap / apap : 0.0;
+            launch_axpby(alpha, x0, 0.0, x0, x0, L, ctx->s);
+            launch_axpby(1.0, ctx->b, -alpha, ctx->r, ctx->r, L, ctx->s);
             const double rnorm = device_norm(ctx, ctx->r, L);
-            const bool use = std::isfinite(rnorm) && rnorm < bnorm;
+            const bool use = alpha != 0.0 && std::isfinite(rnorm) && rnorm < bnorm;
             if (used_guess) used_guess[col] = use ? 1 : 0;
             uint64_t sweeps = 0;
             MinresOut o = run_minres(ctx, ctx->b, use ? x0 : nullptr, tol, maxit, &sweeps, nullptr);

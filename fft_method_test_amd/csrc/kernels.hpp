@@ -15,6 +15,19 @@ namespace fftperm {
 //                   f0 = a_fluid[c] on fluid voxels, a_support[c] on support voxels, else 0
 //                   fx = compact[3*l + c] - mean[c] on support voxel l (when compact != null)
 // `code` holds the support map: l >= 0 support local index, -1 fluid, -2 other solid.
+// Line-compressed support map (specialised x passes): bit i % 32 of word i / 32 of line l is set
+// when voxel (l, i) is on the support (lines padded to ceil(nx / 32) words); lstart[l] = number of
+// support voxels before line l, nb the total.  The support index of a voxel is lstart[line] + the
+// set bits before it in its line, which is what `code` stores.
+struct LineBits {
+    const uint32_t* bits;
+    const int32_t* lstart;
+    long nb;
+};
+
+// With a_fluid = a_support = 0 and `lines.bits` set, the specialised forward pass reads the
+// support from the line bitmask instead of `code` and stages each tile's contiguous run of the
+// compact vector through LDS.
 struct XSource {
     const int32_t* code;
     const double* compact;
@@ -22,6 +35,7 @@ struct XSource {
     const double* field;
     double a_fluid[3];
     double a_support[3];
+    LineBits lines;
 };
 
 // Sink of the x-inverse pass.  value = Re(line) * inv_n.
@@ -33,6 +47,7 @@ struct XSink {
     double* compact;
     double* field;
     double* partials;
+    LineBits lines;  // when set, the specialised inverse pass reads the support from it
 };
 constexpr int kXinvParts = 6;
 
@@ -112,6 +127,8 @@ enum PassKind { kKindXfwd = 0, kKindXinv = 1, kKindCol = 2, kKindZgreen = 3 };
 int tuned_variant(int kind, int prec, int n, long count);
 void set_tuned_variant(int kind, int prec, int n, long count, int v);
 void clear_tuned_variant(int kind, int prec, int n, long count);
+// Incremented by every change of the registry (captured launch sequences depend on it).
+long tuning_generation();
 // Registry value of a report code (pass variant index or z-pass code), -1 when unknown.
 int variant_of_code(int kind, int n, int code);
 // Report code of the variant a launch of this kind and size uses now (0 = runtime kernel).
@@ -135,6 +152,10 @@ std::vector<std::pair<int, float>> autotune_x(const XSource& src, const XSink& s
 // Support classification + ascending compaction.  Fills code (N) and active (NB);
 // nb_out / nf_out are device counters (uint64).  scratch must hold scan_scratch_bytes(N).
 size_t scan_scratch_bytes(long N);
+// Line bitmask + line starts from the support map (bits: nlines * ceil(nx / 32) words; lstart:
+// nlines ints, scanned in place; total: device uint64 receiving the support count).
+void launch_line_bits(const int32_t* code, int nx, long nlines, uint32_t* bits, int32_t* lstart,
+                      unsigned long long* total, hipStream_t st);
 void launch_support(const uint8_t* mask, int nx, int ny, int nz, int interface_only, int32_t* code,
                     int64_t* active, unsigned long long* counts /*[2]: nb, nf*/, void* scratch,
                     hipStream_t st);

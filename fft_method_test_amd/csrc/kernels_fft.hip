@@ -364,6 +364,7 @@ int zgreen_default_code(int n, int prec) {
 
 std::mutex g_tune_mu;
 std::map<std::tuple<int, int, int, long>, int> g_tuned;
+long g_tune_gen = 0;
 
 template <typename R>
 constexpr int prec_of() {
@@ -434,11 +435,18 @@ int tuned_variant(int kind, int prec, int n, long count) {
 void set_tuned_variant(int kind, int prec, int n, long count, int v) {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     g_tuned[std::make_tuple(kind, prec, n, count)] = v;
+    ++g_tune_gen;
+}
+
+long tuning_generation() {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    return g_tune_gen;
 }
 
 void clear_tuned_variant(int kind, int prec, int n, long count) {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     g_tuned.erase(std::make_tuple(kind, prec, n, count));
+    ++g_tune_gen;
 }
 
 int variant_of_code(int kind, int n, int code) {

@@ -1,6 +1,7 @@
 // kernels_setup.hip -- support detection/compaction and the energy-consistent Green tensor.
 #include "kernels.hpp"
 
+#include <algorithm>
 #include <cfloat>
 
 namespace fftperm {
@@ -126,6 +127,32 @@ __global__ void k_support_write(const uint8_t* __restrict__ mask, int nx, int ny
             code[v] = (mask[v] != 0) ? -1 : -2;
         }
     }
+}
+
+__global__ void k_line_bits(const int32_t* __restrict__ code, int nx, long nlines, int W, uint32_t* __restrict__ bits,
+                            int* __restrict__ counts) {
+    // one thread per word; the W threads of a line add their popcounts into counts[line]
+    for (long wid = (long)blockIdx.x * blockDim.x + threadIdx.x; wid < nlines * W; wid += (long)gridDim.x * blockDim.x) {
+        const long line = wid / W;
+        const int w = (int)(wid % W);
+        uint32_t b = 0;
+        for (int q = 0; q < 32; ++q) {
+            const int i = w * 32 + q;
+            if (i < nx && code[line * nx + i] >= 0) b |= 1u << q;
+        }
+        bits[wid] = b;
+        if (b != 0) atomicAdd(&counts[line], __popc(b));
+    }
+}
+
+void launch_line_bits(const int32_t* code, int nx, long nlines, uint32_t* bits, int32_t* lstart,
+                      unsigned long long* total, hipStream_t st) {
+    const int W = (nx + 31) / 32;
+    (void)hipMemsetAsync(lstart, 0, nlines * sizeof(int32_t), st);
+    const long words = nlines * W;
+    hipLaunchKernelGGL(k_line_bits, dim3((unsigned)std::min<long>((words + 255) / 256, 65536)), dim3(256), 0, st,
+                       code, nx, nlines, W, bits, lstart);
+    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(kScanThreads), 0, st, lstart, nlines, total);
 }
 
 size_t scan_scratch_bytes(long N) {

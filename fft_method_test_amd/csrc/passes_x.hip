@@ -14,6 +14,46 @@ namespace fftperm {
 
 namespace {
 
+// LDS after the transform buffer: the tile's support words, their in-line prefix counts and the
+// line starts (line-bitmask path)
+template <int N, int LINES>
+constexpr size_t bits_lds() {
+    return (size_t)LINES * (2 * ((N + 31) / 32) + 1) * sizeof(int);
+}
+
+// Loads the tile's support words and line starts into LDS and computes the in-line word
+// prefixes; returns after a barrier.  sb: LINES*W words, sp: LINES*W prefixes, sl: LINES starts.
+template <int N, int LINES, int THREADS>
+__device__ __forceinline__ void load_tile_bits(const LineBits& lb, long line0, int nl, uint32_t* sb, int* sp,
+                                               int* sl) {
+    constexpr int W = (N + 31) / 32;
+    for (int t = threadIdx.x; t < LINES * W; t += THREADS) {
+        const int line = t / W;
+        sb[t] = (line < nl) ? lb.bits[(line0 + line) * W + t % W] : 0u;
+    }
+    for (int t = threadIdx.x; t < LINES; t += THREADS) sl[t] = (t < nl) ? lb.lstart[line0 + t] : 0;
+    __syncthreads();
+    for (int t = threadIdx.x; t < LINES; t += THREADS) {
+        int acc = 0;
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            sp[t * W + w] = acc;
+            acc += __popc(sb[t * W + w]);
+        }
+    }
+    __syncthreads();
+}
+
+// support index of voxel i of tile line `line` (-1 off the support)
+template <int N>
+__device__ __forceinline__ int tile_local(const uint32_t* sb, const int* sp, const int* sl, int line, int i) {
+    constexpr int W = (N + 31) / 32;
+    const int w = i >> 5, b = i & 31;
+    const uint32_t word = sb[line * W + w];
+    if (((word >> b) & 1u) == 0u) return -1;
+    return sl[line] + sp[line * W + w] + __popc(word & ((1u << b) - 1u));
+}
+
 template <typename R, int N, int LINES, int OS>
 __global__ __launch_bounds__((PassShape<R, N, LINES, OS>::THREADS)) void k_xfwd_s(XSource src, cx<R>* __restrict__ S,
                                                                                 long nlines, long N3, long Nh,
@@ -32,7 +72,42 @@ __global__ __launch_bounds__((PassShape<R, N, LINES, OS>::THREADS)) void k_xfwd_
     // every global load of the block is issued up front: the support code once per voxel, then
     // the three compact components (or the three field components) of each owned voxel
     double v[3][KE];
-    if (src.field != nullptr) {
+    const bool no_f0 = src.a_fluid[0] == 0.0 && src.a_fluid[1] == 0.0 && src.a_fluid[2] == 0.0 &&
+                       src.a_support[0] == 0.0 && src.a_support[1] == 0.0 && src.a_support[2] == 0.0;
+    if (src.field == nullptr && src.lines.bits != nullptr && no_f0) {
+        // the tile's support indices form one contiguous run [cs, ce) of the compact vector: it is
+        // staged through LDS (in the transform buffer) with coalesced loads, then each voxel reads
+        // its three values from there; runs larger than the buffer fall back to direct gathers
+        constexpr int W = (N + 31) / 32;
+        uint32_t* sb = reinterpret_cast<uint32_t*>(buf + LINES * LD);
+        int* sp = reinterpret_cast<int*>(sb + LINES * W);
+        int* sl = sp + LINES * W;
+        const long cs = src.lines.lstart[line0];
+        const long ce = (line0 + nl < nlines) ? (long)src.lines.lstart[line0 + nl] : src.lines.nb;
+        const long cnt3 = 3 * (ce - cs);
+        constexpr long CAP = (long)(LINES * LD * sizeof(cx<R>) / sizeof(double));
+        const bool staged = cnt3 <= CAP;
+        double* stage = reinterpret_cast<double*>(buf);
+        if (staged && src.compact != nullptr) {
+            const double* cc = src.compact + 3 * cs;
+#pragma unroll 4
+            for (long e = threadIdx.x; e < cnt3; e += THREADS) stage[e] = cc[e];
+        }
+        load_tile_bits<N, LINES, THREADS>(src.lines, line0, nl, sb, sp, sl);
+        double mc[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) mc[c] = (src.mean != nullptr) ? src.mean[c] : 0.0;
+#pragma unroll
+        for (int k = 0; k < KE; ++k) {
+            const int idx = threadIdx.x + k * THREADS;
+            const int line = idx / N;
+            const int l = (line < nl && src.compact != nullptr) ? tile_local<N>(sb, sp, sl, line, idx % N) : -1;
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                v[c][k] = (l < 0) ? 0.0 : (staged ? stage[3 * (l - cs) + c] : src.compact[3L * l + c]) - mc[c];
+        }
+        __syncthreads();  // the stage is the transform buffer
+    } else if (src.field != nullptr) {
 #pragma unroll
         for (int k = 0; k < KE; ++k) {
             const int idx = threadIdx.x + k * THREADS;
@@ -155,13 +230,17 @@ __global__ __launch_bounds__((PassShape<R, N, LINES, OS>::THREADS)) void k_xinv_
     for (int i = threadIdx.x; i < N; i += THREADS) tw[i] = twg[i];
     const long line0 = xcd_tile(blockIdx.x, gridDim.x) * LINES;
     const int nl = (int)min((long)LINES, nlines - line0);
-    // all global loads up front: support codes, the x and y half spectra, then component z
+    // all global loads up front: support codes (or the tile's support words), the x and y half
+    // spectra, then component z
+    const bool use_bits = sink.lines.bits != nullptr;
     int code[KE];
+    if (!use_bits) {
 #pragma unroll
-    for (int k = 0; k < KE; ++k) {
-        const int idx = threadIdx.x + k * THREADS;
-        const int line = idx / N;
-        code[k] = (line < nl) ? sink.code[(line0 + line) * N + idx % N] : -2;
+        for (int k = 0; k < KE; ++k) {
+            const int idx = threadIdx.x + k * THREADS;
+            const int line = idx / N;
+            code[k] = (line < nl) ? sink.code[(line0 + line) * N + idx % N] : -2;
+        }
     }
     cx<R> ha[KA], hb[KA];
 #pragma unroll
@@ -190,6 +269,19 @@ __global__ __launch_bounds__((PassShape<R, N, LINES, OS>::THREADS)) void k_xinv_
         gb[k] = mkc<R>((R)0, (R)0);
         if (in && 2 * m < nl) ga[k] = S[2 * Nh + (line0 + 2 * m) * NXH + kx];
         if (in && 2 * m + 1 < nl) gb[k] = S[2 * Nh + (line0 + 2 * m + 1) * NXH + kx];
+    }
+    if (use_bits) {
+        constexpr int W = (N + 31) / 32;
+        uint32_t* sb = reinterpret_cast<uint32_t*>(buf + LINES * LD);
+        int* sp = reinterpret_cast<int*>(sb + LINES * W);
+        int* sl = sp + LINES * W;
+        load_tile_bits<N, LINES, THREADS>(sink.lines, line0, nl, sb, sp, sl);
+#pragma unroll
+        for (int k = 0; k < KE; ++k) {
+            const int idx = threadIdx.x + k * THREADS;
+            const int line = idx / N;
+            code[k] = (line < nl) ? tile_local<N>(sb, sp, sl, line, idx % N) : -2;
+        }
     }
     double acc[kXinvParts] = {0, 0, 0, 0, 0, 0};
     // round A: components x and y
@@ -239,11 +331,12 @@ long xfwd_variant(const XSource& src, cx<R>* S, const Geometry3& g, const cx<R>*
     if constexpr (!SH::valid) {
         return -1;
     } else {
-        if (SH::LDS > lds_limit()) return -1;
+        constexpr size_t lds = SH::LDS + bits_lds<N, LINES>();
+        if (lds > lds_limit()) return -1;
         const long nlines = (long)g.ny * g.nz;
         const long blocks = (nlines + LINES - 1) / LINES;
-        hipLaunchKernelGGL((k_xfwd_s<R, N, LINES, OS>), dim3((unsigned)blocks), dim3(SH::THREADS), SH::LDS, st, src,
-                           S, nlines, g.N, g.Nh, tw, skip);
+        hipLaunchKernelGGL((k_xfwd_s<R, N, LINES, OS>), dim3((unsigned)blocks), dim3(SH::THREADS),
+                           lds, st, src, S, nlines, g.N, g.Nh, tw, skip);
         return blocks;
     }
 }
@@ -256,12 +349,13 @@ long xinv_variant(const cx<R>* S, const XSink& sink, const Geometry3& g, const c
     if constexpr (!SH::valid) {
         return -1;
     } else {
-        if (SH::LDS > lds_limit()) return -1;
+        constexpr size_t lds = SH::LDS + bits_lds<N, LINES>();
+        if (lds > lds_limit()) return -1;
         const long nlines = (long)g.ny * g.nz;
         const long blocks = (nlines + LINES - 1) / LINES;
         const double inv_n = 1.0 / (double)g.N;
-        hipLaunchKernelGGL((k_xinv_s<R, N, LINES, OS>), dim3((unsigned)blocks), dim3(SH::THREADS), SH::LDS, st, S,
-                           sink, nlines, g.N, g.Nh, tw, inv_n, skip);
+        hipLaunchKernelGGL((k_xinv_s<R, N, LINES, OS>), dim3((unsigned)blocks), dim3(SH::THREADS),
+                           lds, st, S, sink, nlines, g.N, g.Nh, tw, inv_n, skip);
         return blocks;
     }
 }

@@ -23,8 +23,9 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def evaluate(net, n, count, seed, tol, precision):
-    """Tensor solves cold vs warm on `count` held-out media: per medium iterations and seconds."""
+def evaluate(net, n, count, seed, tol, precision, media=None):
+    """Tensor solves cold vs warm on `count` held-out media (or the given masks): per medium
+    iterations and seconds."""
     import torch
 
     from . import permeability as P, warmstart as W
@@ -33,8 +34,10 @@ def evaluate(net, n, count, seed, tol, precision):
     g = P.Grid3D(n, n, n)
     green = P.GreenOperator(g, 1.0, 2, precision)
     rows = []
+    if media is not None:
+        count = len(media)
     for i in range(count):
-        mask = W.random_medium(n, rng).reshape(-1)
+        mask = (W.random_medium(n, rng) if media is None else media[i]).reshape(-1)
         sup = P.ForceSupport(P.BinaryMedium(g, mask), P.SUPPORT_INTERFACE, green)
         if sup.num_active_voxels() == 0:
             continue
@@ -66,12 +69,17 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--seed", type=int, default=10000)
     ap.add_argument("--init", default=None, help="start from saved parameters")
+    ap.add_argument("--objective", choices=["mse", "residual"], default="mse",
+                    help="mse: squared error against solver pairs; residual: relative residual of the "
+                         "minimal-residual-scaled guess (needs only media, no solves)")
     ap.add_argument("--out", default=None)
     ap.add_argument("--eval-n", type=int, default=0)
     ap.add_argument("--eval-count", type=int, default=4)
     ap.add_argument("--eval-seed", type=int, default=20000)
     ap.add_argument("--eval-tol", type=float, default=1e-6)
     ap.add_argument("--eval-precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--eval-bench", action="store_true",
+                    help="also evaluate on the benchmark medium (256^3 Boolean spheres r 8, porosity 0.6, seed 2001)")
     ap.add_argument("--report", default=None, help="write the training / evaluation summary (JSON)")
     args = ap.parse_args()
 
@@ -82,7 +90,19 @@ def main():
     if args.init:
         net.load(args.init)
     report = dict(n=args.n, pairs=args.pairs, epochs=args.epochs, lr=args.lr, tol=args.tol, seed=args.seed)
-    if args.pairs > 0 and args.epochs > 0:
+    if args.objective == "residual" and args.pairs > 0 and args.epochs > 0:
+        rng = np.random.default_rng(args.seed)
+        t0 = time.perf_counter()
+        masks = [W.random_medium(args.n, rng).reshape(-1) for _ in range(args.pairs)]
+        t1 = time.perf_counter()
+        hist = W.train_residual(net, masks, args.n, args.epochs, lr=args.lr, seed=args.seed, log=log)
+        t2 = time.perf_counter()
+        report.update(objective="residual", media_seconds=t1 - t0, train_seconds=t2 - t1, residual_history=hist)
+        if args.out:
+            os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
+            net.save(args.out)
+            log(f"[warmstart] saved {args.out}")
+    elif args.pairs > 0 and args.epochs > 0:
         t0 = time.perf_counter()
         pairs = W.generate_pairs(args.n, args.pairs, args.seed, tol=args.tol, log=log)
         t1 = time.perf_counter()
@@ -100,11 +120,20 @@ def main():
         e = report["eval"]
         log(f"[warmstart] eval n={e['n']}: iterations warm/cold = {e['warm_iterations']}/{e['cold_iterations']} "
             f"({e['iteration_ratio']:.3f}), seconds {e['warm_s']:.2f}/{e['cold_s']:.2f}")
+    if args.eval_bench:
+        from . import geometry
+
+        prec = _lib.PRECISION_MIXED if args.eval_precision == "mixed" else _lib.PRECISION_FP64
+        m = geometry.boolean_spheres(256, 8.0, 0.6, 2001)
+        report["eval_bench"] = evaluate(net, 256, 1, 0, args.eval_tol, prec, media=[m])
+        e = report["eval_bench"]
+        log(f"[warmstart] bench medium: iterations warm/cold = {e['warm_iterations']}/{e['cold_iterations']} "
+            f"({e['iteration_ratio']:.3f}), seconds {e['warm_s']:.2f}/{e['cold_s']:.2f}")
     if args.report:
         with open(args.report, "w") as f:
             json.dump(report, f)
-    print(json.dumps({k: v for k, v in report.items() if k != "eval"} |
-                     ({"eval": {k: v for k, v in report["eval"].items() if k != "rows"}} if "eval" in report else {})))
+    print(json.dumps({k: (v if not isinstance(v, dict) else {a: b for a, b in v.items() if a != "rows"})
+                      for k, v in report.items()}))
 
 
 if __name__ == "__main__":

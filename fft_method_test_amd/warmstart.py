@@ -103,15 +103,14 @@ def solve_tensor_warm(green, net: WarmStartNet, tolerance, max_iterations):
 
 
 def random_medium(n, rng):
-    """Training / evaluation media: overlapping radius-8-like spheres or prolate ellipsoids
-    (aspect 3, random orientation) at a porosity drawn from U[0.3, 0.8], radius scaled to n."""
+    """Training / evaluation media at the voxel scale of the benchmark problems: overlapping
+    radius-8 spheres or prolate ellipsoids (semi-axes 12, 4, 4, random orientation), half each, at
+    a porosity drawn from U[0.3, 0.8]."""
     phi = float(rng.uniform(0.3, 0.8))
     seed = int(rng.integers(0, 2 ** 31 - 1))
-    r = 8.0 * n / 128.0 * 2.0  # 16 voxels at 128^3 -> keeps several inclusions per cell at small n
     if rng.uniform() < 0.5:
-        return geometry.boolean_spheres(n, max(2.0, r / 2), phi, seed)
-    a = max(3.0, 1.5 * r)
-    return geometry.boolean_ellipsoids(n, (a, a / 3.0, a / 3.0), phi, seed)
+        return geometry.boolean_spheres(n, 8.0, phi, seed)
+    return geometry.boolean_ellipsoids(n, (12.0, 4.0, 4.0), phi, seed)
 
 
 def force_density_x(green, support, tol, maxit):
@@ -128,7 +127,8 @@ def force_density_x(green, support, tol, maxit):
 
 
 def generate_pairs(n, count, seed, tol=1e-6, maxit=20000, log=None):
-    """`count` (fluid mask, force density) pairs at n^3 from the FP64 GPU solver."""
+    """`count` (fluid mask, force density normalised to unit RMS on the support) pairs at n^3
+    from the FP64 GPU solver."""
     from . import permeability as P
 
     rng = np.random.default_rng(seed)
@@ -142,6 +142,11 @@ def generate_pairs(n, count, seed, tol=1e-6, maxit=20000, log=None):
         if sup.num_active_voxels() == 0:
             continue
         field, res = force_density_x(green, sup, tol, maxit)
+        # unit RMS over the support: the network learns the shape, the solve fixes the scale
+        # (minimal-residual step along the prediction)
+        rms = float(np.sqrt(np.mean(field[sup.active_voxels().astype(np.int64)] ** 2)))
+        if rms > 0:
+            field /= rms
         pairs.append((mask.astype(np.uint8), field))
         if log and (i + 1) % max(1, count // 10) == 0:
             log(f"[warmstart] pairs {i + 1}/{count} at {n}^3 ({time.perf_counter() - t0:.1f} s, last "
@@ -169,3 +174,61 @@ def train(net: WarmStartNet, pairs, n, epochs, lr=1e-3, seed=0, log=None):
         if log:
             log(f"[warmstart] epoch {ep + 1}/{epochs} mean loss {hist[-1]:.4e}")
     return hist
+
+
+def train_residual(net: WarmStartNet, masks, n, epochs, lr=2e-4, seed=0, log=None):
+    """Train on the quantity MINRES sees: the relative residual of the scaled guess,
+    loss = min_alpha ||b - alpha A p||^2 / ||b||^2 (p = the prediction gathered on the support,
+    b the direction-x RHS, A the FP64 variational operator).  At the optimal alpha its gradient
+    with respect to p is -2 alpha A r / ||b||^2, r = b - alpha A p (A is symmetric).  The network's
+    squared-error kernel delivers a prescribed output gradient g when trained against the target
+    pred - (3 count / 2) g, count = support voxels, which is how the step is taken.  Returns the
+    per-epoch mean relative residual (the square root of the loss)."""
+    import torch
+
+    from . import permeability as P
+
+    rng = np.random.default_rng(seed)
+    dev = torch.device("cuda:0")
+    g = P.Grid3D(n, n, n)
+    green = P.GreenOperator(g)
+    samples = []
+    for m in masks:
+        sup = P.ForceSupport(P.BinaryMedium(g, m), P.SUPPORT_INTERFACE, green)
+        if sup.num_active_voxels() == 0:
+            continue
+        b = P.VariationalRHS(sup, green, None).build([1.0, 0.0, 0.0])
+        samples.append((np.ascontiguousarray(m, dtype=np.uint8), sup.active_voxels().astype(np.int64), b))
+    hist = []
+    for ep in range(epochs):
+        tot = 0.0
+        for i in rng.permutation(len(samples)):
+            m, act, b = samples[i]
+            nb_ = np.zeros(1, dtype=np.uint64)
+            check(_lib.load().fftperm_set_medium(green._h, m, _lib.SUPPORT_INTERFACE,
+                                                 nb_.ctypes.data_as(C.POINTER(C.c_uint64))))
+            mdev = torch.from_numpy(m).to(dev)
+            pred = net.predict(mdev, n).cpu().numpy()
+            p = pred[act].astype(np.float64).reshape(-1)
+            Ap = _apply(green, p)
+            bb = float(b @ b)
+            apap = float(Ap @ Ap)
+            alpha = float(b @ Ap) / apap if apap > 0 else 0.0
+            r = b - alpha * Ap
+            tot += np.sqrt(float(r @ r) / bb)
+            grad_p = (-2.0 * alpha / bb) * _apply(green, r)
+            gfield = np.zeros_like(pred)
+            gfield[act] = grad_p.reshape(-1, 3)
+            target = (pred - 1.5 * act.size * gfield).astype(np.float32)
+            net.loss_grad(mdev, torch.from_numpy(target).to(dev), n, lr=lr)
+        hist.append(tot / max(1, len(samples)))
+        if log:
+            log(f"[warmstart] residual epoch {ep + 1}/{epochs} mean relative residual of the guess {hist[-1]:.4f}")
+    green.close()
+    return hist
+
+
+def _apply(green, x):
+    y = np.zeros_like(x)
+    check(_lib.load().fftperm_operator_apply(green._h, np.ascontiguousarray(x, dtype=np.float64), y))
+    return y

@@ -177,6 +177,11 @@ typedef struct {
 enum { FFTPERM_PASS_XFWD = 0, FFTPERM_PASS_XINV = 1, FFTPERM_PASS_YPASS = 2, FFTPERM_PASS_ZGREEN = 3 };
 int fftperm_pin_variant(fftperm_ctx* ctx, int kind, int precision, int code);
 
+/* MINRES iterations replay hipGraphs of two captured iterations (default on; no counterpart in the
+ * reference).  enable = 0 launches every kernel from the host instead; results are identical.
+ * While fftperm_profile is on, iterations launch eagerly (the live timing records events). */
+int fftperm_set_graphs(fftperm_ctx* ctx, int enable);
+
 int fftperm_profile(fftperm_ctx* ctx, int enable);
 int fftperm_profile_read(fftperm_ctx* ctx, fftperm_profile_stats* out);
 
@@ -202,9 +207,11 @@ int fftperm_unet_loss_grad(fftperm_unet* net, const uint8_t* fluid_mask_dev, con
                            float lr, double* loss, float* grads);
 /* Prediction for the x direction: field_dev (device) n^3 x 3 FP32. */
 int fftperm_unet_predict(fftperm_unet* net, const uint8_t* fluid_mask_dev, uint32_t n, float* field_dev);
-/* fftperm_solve_tensor with the network's prediction (gathered on the support) as each direction's
- * MINRES initial guess.  MINRES stays the judge: a guess whose FP64 true residual is not below ||b||
- * is dropped and that direction starts cold.  used_guess[3] (may be NULL) reports the choice. */
+/* fftperm_solve_tensor with the network's prediction p (gathered on the support) seeding each
+ * direction's MINRES: x0 = alpha p with alpha = <b, A p> / ||A p||^2 (the minimal-residual scale of
+ * the predicted shape, one FP64 apply).  MINRES stays the judge: a guess whose FP64 true residual is
+ * not below ||b|| is dropped and that direction starts cold.  used_guess[3] (may be NULL) reports
+ * the choice. */
 int fftperm_solve_tensor_warm(fftperm_ctx* ctx, fftperm_unet* net, double tol, uint64_t maxit, double K[9],
                               fftperm_solve_info* info, int* used_guess);
 

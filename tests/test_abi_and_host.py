@@ -104,3 +104,15 @@ def test_bench_refuses_multi_gpu():
 
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True, text=True)
     assert r.returncode == 2 and "one GPU" in r.stderr
+
+
+def test_c4_batch_generator_is_seeded():
+    """The uneven-size batch: sizes from {64, ..., 160}, porosity in [0.3, 0.8], seeded."""
+    from fft_method_test_amd import batch
+
+    a = batch.c4_batch(3, 4000)
+    b = batch.c4_batch(3, 4000)
+    assert [s[0] for s in a] == [s[0] for s in b]
+    assert all(np.array_equal(x[1], y[1]) for x, y in zip(a, b))
+    assert all(s[0] in (64, 80, 96, 112, 128, 144, 160) for s in a)
+    assert all(0.3 <= s[3] <= 0.8 for s in a)

@@ -222,3 +222,34 @@ def test_green_apply_128_cube_vs_oracle(fp, oracle_mod):
     from fft_method_test_amd import permeability as P
 
     assert rel(P.VariationalRHS(sup, green, None).build([0.0, 0.0, 1.0]), b) < 1e-12
+
+
+@pytest.mark.parametrize("shape,precision", [((32, 32, 32), 0), ((24, 20, 16), 0), ((32, 32, 32), 1)])
+def test_minres_graph_replay_matches_eager(fp, shape, precision):
+    """Graph-captured MINRES iteration pairs reproduce the eagerly launched iterations bit for bit,
+    over several solves in a row (replays with eager work and synchronisation in between)."""
+    from fft_method_test_amd import _lib, permeability as P
+
+    mask = random_mask(shape, 0.6, seed=17)
+    g = P.Grid3D(*shape)
+    green = P.GreenOperator(g, 1.0, 2, precision)
+    sup = P.ForceSupport(P.BinaryMedium(g, mask), P.SUPPORT_INTERFACE, green)
+    op = P.VariationalOperator(sup, green)
+    runs = {}
+    for graphs in (0, 1, 0, 1):
+        _lib.check(_lib.load().fftperm_set_graphs(green._h, graphs))
+        out = []
+        for d in range(3):
+            grad = [0.0, 0.0, 0.0]
+            grad[d] = 1.0
+            b = P.VariationalRHS(sup, green, None).build(grad)
+            res = P.MinresSolver(1e-9, 400).solve(op, b)
+            out.append((res.iterations, res.residual_history.copy(), res.solution.copy()))
+        runs.setdefault(graphs, []).append(out)
+    ref = runs[0][0]
+    for graphs in (0, 1):
+        for out in runs[graphs]:
+            for (it, h, x), (it0, h0, x0) in zip(out, ref):
+                assert it == it0
+                assert np.array_equal(h, h0)
+                assert np.array_equal(x, x0)
