@@ -12,8 +12,9 @@ The metric is quoted at <= 1e-6 relative K error, so both the plain FP64 path an
 the mixed-precision path are timed at the same MINRES tolerance (--tol, default
 1e-6) and each one's K error is measured against an untimed FP64 solve of the
 same medium at --ref-tol (1e-10): ||K - K_ref||_F / ||K_ref||_F.  The headline
-value is the mixed path's solves/s when its K error meets 1e-6, else the FP64
-path's.
+value is the mixed-precision path with the learned warm start (the metric's
+configuration; the network is fft_method_test_amd/data/warmstart_unet.npz) when
+its K error meets 1e-6, else mixed precision alone, else the FP64 path.
 
 Prints ONE JSON line.  --gpus N with N > 1 exits: this build runs on one GPU.
 """
@@ -55,7 +56,7 @@ def bytes_apply(N, Nh, L, s_c=16, s_r=8):
     return xf + 2 * yp + zg + xi
 
 
-def run_precision(args, P, _lib, mask_dev, precision, tol=None, steps=None, warmup=None, label=""):
+def run_precision(args, P, _lib, mask_dev, precision, tol=None, steps=None, warmup=None, label="", net=None):
     import ctypes as C
 
     import torch
@@ -71,11 +72,16 @@ def run_precision(args, P, _lib, mask_dev, precision, tol=None, steps=None, warm
     nb = C.c_uint64()
     K = np.zeros(9)
     infos = (_lib.SolveInfo * 3)()
+    used = (C.c_int * 3)()
 
     def step():
         _lib.check(_lib.load().fftperm_set_medium_dev(green._h, C.c_void_p(mask_dev.data_ptr()),
                                                        _lib.SUPPORT_INTERFACE, C.byref(nb)))
-        _lib.check(_lib.load().fftperm_solve_tensor(green._h, tol, args.maxit, K, C.cast(infos, C.c_void_p)))
+        if net is None:
+            _lib.check(_lib.load().fftperm_solve_tensor(green._h, tol, args.maxit, K, C.cast(infos, C.c_void_p)))
+        else:
+            _lib.check(_lib.load().fftperm_solve_tensor_warm(green._h, net._h, tol, args.maxit, K,
+                                                             C.cast(infos, C.c_void_p), C.cast(used, C.c_void_p)))
 
     for w in range(warmup):
         step()
@@ -95,6 +101,7 @@ def run_precision(args, P, _lib, mask_dev, precision, tol=None, steps=None, warm
         K=K.reshape(3, 3).copy(), iterations=[int(i.iterations) for i in infos],
         sweeps=[int(i.refinement_sweeps) for i in infos], converged=[bool(i.converged) for i in infos],
         final_rel=[float(i.final_rel_residual) for i in infos], setup_s=setup_s, nb=int(nb.value),
+        used_guess=[int(u) for u in used] if net is not None else None,
         prof=dict(z_ms=list(prof.zgreen_ms), z_n=list(prof.zgreen_launches), a_ms=list(prof.apply_ms),
                   a_n=list(prof.apply_launches), N=int(prof.N), Nh=int(prof.Nh),
                   tiles=dict(zgreen=list(prof.zgreen_tile), xfwd=list(prof.xfwd_tile), xinv=list(prof.xinv_tile),
@@ -151,6 +158,8 @@ def main():
                          "(<= 0: no reference solve, K errors not reported)")
     ap.add_argument("--maxit", type=int, default=20000)
     ap.add_argument("--precision", choices=["fp64", "mixed", "both"], default="both")
+    ap.add_argument("--warmstart", default=os.path.join(ROOT, "fft_method_test_amd", "data", "warmstart_unet.npz"),
+                    help="trained warm-start network for the mixed+learned run ('' to skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
 
@@ -180,6 +189,14 @@ def main():
         runs["fp64"] = run_precision(args, P, _lib, mask_dev, _lib.PRECISION_FP64, label=f"fp64 tol={args.tol:g}")
     if args.precision in ("mixed", "both"):
         runs["mixed"] = run_precision(args, P, _lib, mask_dev, _lib.PRECISION_MIXED, label=f"mixed tol={args.tol:g}")
+        if args.warmstart and os.path.exists(args.warmstart):
+            from fft_method_test_amd import warmstart as W
+
+            net = W.WarmStartNet(0)
+            net.load(args.warmstart)
+            runs["mixed_learned"] = run_precision(args, P, _lib, mask_dev, _lib.PRECISION_MIXED,
+                                                  label=f"mixed+learned tol={args.tol:g}", net=net)
+            net.close()
 
     def k_err(r):
         if ref is None:
@@ -188,8 +205,12 @@ def main():
 
     errs = {k: k_err(r) for k, r in runs.items()}
     meets = {k: (e is not None and e <= K_ERR_BAR) for k, e in errs.items()}
-    # headline: the mixed path when it meets the K-error bar, else the FP64 path
-    if "mixed" in runs and meets.get("mixed"):
+    # headline: mixed precision with the learned warm start (the metric's configuration) when it
+    # meets the K-error bar, else mixed precision alone, else the FP64 path
+    if "mixed_learned" in runs and meets.get("mixed_learned"):
+        head = runs["mixed_learned"]
+        dtype = "f32 FFT/Green + f64 Krylov, bf16 U-Net guess"
+    elif "mixed" in runs and meets.get("mixed"):
         head = runs["mixed"]
         dtype = "f32 FFT/Green + f64 Krylov"
     elif "fp64" in runs:
@@ -261,6 +282,7 @@ def main():
                 "k_rel_error": errs[k], "meets_k_error_bar": meets[k], "iterations": r["iterations"],
                 "refinement_sweeps": r["sweeps"], "converged": r["converged"], "final_rel_residual": r["final_rel"],
                 "setup_s": r["setup_s"], "variants": r["prof"]["tiles"], "K": r["K"].tolist(),
+                "used_guess": r["used_guess"],
             } for k, r in runs.items()
         },
         "reference": None if ref is None else {
@@ -269,6 +291,7 @@ def main():
         },
         "speedup_mixed_vs_fp64": (runs["mixed"]["solves_per_s"] / runs["fp64"]["solves_per_s"]
                                   if "fp64" in runs and "mixed" in runs else None),
+        "speedup_head_vs_fp64": (head["solves_per_s"] / runs["fp64"]["solves_per_s"] if "fp64" in runs else None),
         "cpu_baseline": None,
     }
     if not args.no_cpu_baseline:

@@ -872,6 +872,7 @@ int fftperm_create(const fftperm_grid* grid, double mu, int p_radius, int precis
             hipDeviceProp_t prop;
             HIPCHK(hipGetDeviceProperties(&prop, c->device));
             set_lds_limit(prop.sharedMemPerBlock);
+            set_device_shape(prop.multiProcessorCount, prop.maxSharedMemoryPerMultiProcessor);
         }
         HIPCHK(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
         Geometry3& g = c->g;

@@ -103,6 +103,22 @@ __device__ __forceinline__ void sfft_lines(cx<R>* buf, const cx<R>* tw, int nlin
     sstages<R, SIGN, N, LINES, T, LD, 1>(buf, tw, nlines, typename StaticPlan<N>::type{});
 }
 
+// The same with an explicit radix plan (PLAN = Radices<...>, product N).
+template <typename R, int SIGN, int N, int LINES, int T, int LD, typename PLAN>
+__device__ __forceinline__ void sfft_lines_plan(cx<R>* buf, const cx<R>* tw, int nlines) {
+    sstages<R, SIGN, N, LINES, T, LD, 1>(buf, tw, nlines, PLAN{});
+}
+
+// Radix <= 8 plans: smaller butterflies (32 live values in FP64 instead of 64) for kernels that
+// keep other data in registers across the transform.
+template <int N>
+struct SmallRadixPlan;
+template <> struct SmallRadixPlan<32> { using type = Radices<8, 4>; };
+template <> struct SmallRadixPlan<64> { using type = Radices<8, 8>; };
+template <> struct SmallRadixPlan<128> { using type = Radices<8, 4, 4>; };
+template <> struct SmallRadixPlan<256> { using type = Radices<8, 8, 4>; };
+template <> struct SmallRadixPlan<512> { using type = Radices<8, 8, 8>; };
+
 // Threads per line: the butterfly count of the largest-radix stage, so that every stage's
 // butterflies split evenly over them (radices are powers of two here).
 template <int N>

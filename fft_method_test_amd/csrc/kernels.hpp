@@ -75,6 +75,8 @@ PassCfg cfg_xpass(const LinePlan& p, long nlines);
 PassCfg cfg_colpass(const LinePlan& p, long ncols);
 PassCfg cfg_zgreen(const LinePlan& p, long ncols);
 void set_lds_limit(size_t bytes);
+// compute units and LDS per compute unit of the device (sizes the persistent z-pass grid)
+void set_device_shape(int cus, size_t lds_per_cu);
 size_t lds_limit();
 
 template <typename R>

@@ -28,6 +28,8 @@ namespace fftperm {
 namespace {
 
 size_t g_lds_max = 64 * 1024;  // per-workgroup LDS limit of the device (set at context creation)
+int g_cus = 256;               // compute units of the device (set at context creation)
+size_t g_lds_cu = 160 * 1024;  // LDS per compute unit
 
 size_t lds_single() { return std::min<size_t>(72 * 1024, g_lds_max); }
 size_t lds_triple() { return std::min<size_t>(104 * 1024, g_lds_max); }
@@ -67,6 +69,10 @@ PassCfg cfg_colpass(const LinePlan& p, long ncols) { return make_cfg(p, ncols, 1
 PassCfg cfg_zgreen(const LinePlan& p, long ncols) { return make_cfg(p, ncols, 3, lds_triple(), 16); }
 
 void set_lds_limit(size_t bytes) { g_lds_max = bytes; }
+void set_device_shape(int cus, size_t lds_per_cu) {
+    if (cus > 0) g_cus = cus;
+    if (lds_per_cu > 0) g_lds_cu = lds_per_cu;
+}
 size_t lds_limit() { return g_lds_max; }
 
 
@@ -194,6 +200,146 @@ __global__ __launch_bounds__((zgreen_threads<R, N, ZL, OS>())) void k_zgreen_s(c
     }
 }
 
+// Persistent variant: a grid of a few workgroups per CU walks the column tiles; before a tile's
+// transforms start, each thread issues the loads of its share of the NEXT tile into registers, so
+// one workgroup overlaps HBM latency with its own LDS / FFT phases (the all-in-LDS kernel needs
+// ~100 KB of LDS in FP64, which leaves one workgroup per CU and nothing to overlap with).  Tiles
+// are dealt per XCD: the workgroups of one XCD (blockIdx % 8) take consecutive tiles, whose rows
+// share L2 lines when a tile row is shorter than 128 bytes.
+template <int N>
+constexpr int zgreen_p_t() {
+    return N / PlanInfo<typename SmallRadixPlan<N>::type>::max_radix();
+}
+
+template <typename R, int N, int ZL>
+constexpr int zgreen_p_threads() {
+    return 3 * ZL * zgreen_p_t<N>();
+}
+
+template <typename R, int N, int ZL>
+__global__ __launch_bounds__((zgreen_p_threads<R, N, ZL>())) void k_zgreen_p(cx<R>* __restrict__ S,
+                                                                      const R* __restrict__ G6, long ncols, long Nh,
+                                                                      int nx, int nxh, const cx<R>* __restrict__ twg,
+                                                                      double* __restrict__ qpart, const int* skip) {
+    if (skip != nullptr && *skip) return;
+    using PLAN = typename SmallRadixPlan<N>::type;
+    constexpr int T = zgreen_p_t<N>();
+    constexpr int LD = N + 1, L3 = 3 * ZL, THREADS = L3 * T;
+    constexpr int EST = THREADS / L3;  // = T
+    constexpr int KL = N / EST;
+    constexpr int MT = THREADS / ZL;
+    constexpr int KM = (N + MT - 1) / MT;
+    extern __shared__ unsigned char smem_raw[];
+    cx<R>* tw = reinterpret_cast<cx<R>*>(smem_raw);
+    cx<R>* buf = tw + N;
+    for (int i = threadIdx.x; i < N; i += THREADS) tw[i] = twg[i];
+    const long ntiles = (ncols + ZL - 1) / ZL;
+    // tiles of this workgroup: XCD x = blockIdx % 8 owns tiles [x*per, min((x+1)*per, ntiles)),
+    // walked by the workgroups of that XCD in turn
+    const int nx8 = (int)(gridDim.x / 8);
+    const long per = (ntiles + 7) / 8;
+    const int xcd = (int)(blockIdx.x % 8), slot = (int)(blockIdx.x / 8);
+    const long tbeg = xcd * per, tend = min(ntiles, (long)(xcd + 1) * per);
+    const int lt = threadIdx.x % L3;
+    const int comp = lt / ZL;
+    const int ci = lt % ZL;
+    const int mci = threadIdx.x % ZL;
+    auto fetch = [&](long tile, cx<R> (&ld)[KL]) {
+        const long col0 = tile * ZL;
+        const bool own = col0 + ci < ncols;
+        const cx<R>* Sc = S + comp * Nh + col0 + ci;
+#pragma unroll
+        for (int k = 0; k < KL; ++k) {
+            ld[k] = mkc<R>((R)0, (R)0);
+            if (own) ld[k] = Sc[(long)(threadIdx.x / L3 + k * EST) * ncols];
+        }
+    };
+    double q[1] = {0.0};
+    cx<R> ld[KL];
+    long tile = tbeg + slot;
+    if (tile < tend) fetch(tile, ld);
+    __syncthreads();  // twiddles
+    for (; tile < tend; tile += nx8) {
+        const long col0 = tile * ZL;
+        const int ncl = (int)min((long)ZL, ncols - col0);
+#pragma unroll
+        for (int k = 0; k < KL; ++k) buf[lt * LD + threadIdx.x / L3 + k * EST] = ld[k];
+        // this tile's Green tensor, then the next tile's columns, in flight during the transforms
+        const bool mown = mci < ncl;
+        R gt[KM][6];
+        {
+            const R* Gc = G6 + 6L * (col0 + mci);
+#pragma unroll
+            for (int m = 0; m < KM; ++m) {
+                const int e = threadIdx.x / ZL + m * MT;
+#pragma unroll
+                for (int qq = 0; qq < 6; ++qq) gt[m][qq] = (R)0;
+                if (mown && e < N) {
+                    const R* gp = Gc + 6L * e * ncols;
+#pragma unroll
+                    for (int qq = 0; qq < 6; ++qq) gt[m][qq] = gp[qq];
+                }
+            }
+        }
+        const long next = tile + nx8;
+        if (next < tend) fetch(next, ld);
+        __syncthreads();
+        sfft_lines_plan<R, -1, N, L3, T, LD, PLAN>(buf, tw, L3);
+#pragma unroll
+        for (int m = 0; m < KM; ++m) {
+            const int e = threadIdx.x / ZL + m * MT;
+            if (!mown || e >= N) continue;
+            const R txx = gt[m][0], txy = gt[m][1], txz = gt[m][2];
+            const R tyy = gt[m][3], tyz = gt[m][4], tzz = gt[m][5];
+            cx<R>* px = buf + (0 * ZL + mci) * LD + e;
+            cx<R>* py = buf + (1 * ZL + mci) * LD + e;
+            cx<R>* pz = buf + (2 * ZL + mci) * LD + e;
+            const cx<R> fx = *px, fy = *py, fz = *pz;
+            const cx<R> gx = mkc<R>(txx * fx.x + txy * fy.x + txz * fz.x, txx * fx.y + txy * fy.y + txz * fz.y);
+            const cx<R> gy = mkc<R>(txy * fx.x + tyy * fy.x + tyz * fz.x, txy * fx.y + tyy * fy.y + tyz * fz.y);
+            const cx<R> gz = mkc<R>(txz * fx.x + tyz * fy.x + tzz * fz.x, txz * fx.y + tyz * fy.y + tzz * fz.y);
+            *px = gx;
+            *py = gy;
+            *pz = gz;
+            if (qpart != nullptr)
+                q[0] += herm_weight((int)((col0 + mci) % nxh), nx) *
+                        ((double)fx.x * gx.x + (double)fx.y * gx.y + (double)fy.x * gy.x + (double)fy.y * gy.y +
+                         (double)fz.x * gz.x + (double)fz.y * gz.y);
+        }
+        __syncthreads();
+        sfft_lines_plan<R, +1, N, L3, T, LD, PLAN>(buf, tw, L3);
+        cx<R>* Sw = S + comp * Nh + col0 + ci;
+        const bool own = ci < ncl;
+#pragma unroll
+        for (int k = 0; k < KL; ++k) {
+            const int e = threadIdx.x / L3 + k * EST;
+            if (own) Sw[(long)e * ncols] = buf[lt * LD + e];
+        }
+        __syncthreads();  // the next tile overwrites the buffer
+    }
+    if (qpart != nullptr) block_sum<1>(q, qpart + blockIdx.x);
+}
+
+template <typename R, int N, int ZL>
+long launch_zgreen_p(cx<R>* S, const R* G6, long ncols, const Geometry3& g, const cx<R>* tw, double* qpart,
+                     const int* skip, hipStream_t st) {
+    constexpr int threads = zgreen_p_threads<R, N, ZL>();
+    if constexpr (threads > 1024) {
+        return -1;
+    } else {
+        constexpr size_t lds = (size_t)(N + 3 * ZL * (N + 1)) * sizeof(cx<R>);
+        if (lds > g_lds_max) return -1;
+        const long ntiles = (ncols + ZL - 1) / ZL;
+        // as many workgroups as fit the device at once (LDS-limited), a multiple of 8 (XCDs)
+        const long per_cu = std::max<long>(1, (long)(g_lds_cu / lds));
+        long blocks = std::min<long>((long)g_cus * per_cu, ((ntiles + 7) / 8) * 8);
+        blocks = std::max<long>(8, blocks / 8 * 8);
+        hipLaunchKernelGGL((k_zgreen_p<R, N, ZL>), dim3((unsigned)blocks), dim3(threads), lds, st, S, G6, ncols, g.Nh,
+                           g.nx, g.nxh, tw, qpart, skip);
+        return blocks;
+    }
+}
+
 // Register-resident variant (zgreen_reg.hpp): ZL columns, N/4 threads per column.
 template <typename R, int N, int ZL>
 __global__ __launch_bounds__(ZL * N / 4) void k_zgreen_r(cx<R>* __restrict__ S, const R* __restrict__ G6, long ncols,
@@ -312,8 +458,8 @@ long launch_zgreen_zl(cx<R>* S, const R* G6, long ncols, const Geometry3& g, con
 }
 
 // variant code = 10 * ZL + OS for the all-in-LDS kernel (OS = 1, 2 threads per butterfly thread),
-// 10 * ZL + 4 for the register-resident radix-4 kernel
-constexpr int kZgreenCodes[] = {11, 12, 21, 22, 41, 42, 81, 82, 161, 162, 44, 84, 164, 324};
+// 10 * ZL + 4 for the register-resident radix-4 kernel, 10 * ZL + 5 for the persistent kernel
+constexpr int kZgreenCodes[] = {11, 12, 21, 22, 41, 42, 81, 82, 161, 162, 44, 84, 164, 324, 45, 85, 165};
 
 template <typename R, int N>
 long launch_zgreen_n(int code, cx<R>* S, const R* G6, long ncols, const Geometry3& g, const cx<R>* tw,
@@ -333,6 +479,9 @@ long launch_zgreen_n(int code, cx<R>* S, const R* G6, long ncols, const Geometry
         case 84: return launch_zgreen_r<R, N, 8>(S, G6, ncols, g, tw, qpart, skip, st);
         case 164: return launch_zgreen_r<R, N, 16>(S, G6, ncols, g, tw, qpart, skip, st);
         case 324: return launch_zgreen_r<R, N, 32>(S, G6, ncols, g, tw, qpart, skip, st);
+        case 45: return launch_zgreen_p<R, N, 4>(S, G6, ncols, g, tw, qpart, skip, st);
+        case 85: return launch_zgreen_p<R, N, 8>(S, G6, ncols, g, tw, qpart, skip, st);
+        case 165: return launch_zgreen_p<R, N, 16>(S, G6, ncols, g, tw, qpart, skip, st);
         default: return -1;
     }
 }

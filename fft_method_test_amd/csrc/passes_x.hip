@@ -44,6 +44,44 @@ __device__ __forceinline__ void load_tile_bits(const LineBits& lb, long line0, i
     __syncthreads();
 }
 
+// Two-phase variant for kernels that issue other loads in between: fetch the thread's share of the
+// tile words and line starts into registers first (so waiting for them does not wait for loads
+// issued later), then store them to LDS and compute the prefixes.
+template <int N, int LINES, int THREADS>
+struct TileBitsRegs {
+    static constexpr int W = (N + 31) / 32;
+    static constexpr int PW = (LINES * W + THREADS - 1) / THREADS;
+    uint32_t word[PW];
+    int start;
+    __device__ __forceinline__ void fetch(const LineBits& lb, long line0, int nl) {
+#pragma unroll
+        for (int q = 0; q < PW; ++q) {
+            const int t = threadIdx.x + q * THREADS;
+            const int line = t / W;
+            word[q] = (t < LINES * W && line < nl) ? lb.bits[(line0 + line) * W + t % W] : 0u;
+        }
+        start = ((int)threadIdx.x < nl) ? lb.lstart[line0 + threadIdx.x] : 0;
+    }
+    __device__ __forceinline__ void store(uint32_t* sb, int* sp, int* sl) const {
+#pragma unroll
+        for (int q = 0; q < PW; ++q) {
+            const int t = threadIdx.x + q * THREADS;
+            if (t < LINES * W) sb[t] = word[q];
+        }
+        if ((int)threadIdx.x < LINES) sl[threadIdx.x] = start;
+        __syncthreads();
+        for (int t = threadIdx.x; t < LINES; t += THREADS) {
+            int acc = 0;
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+                sp[t * W + w] = acc;
+                acc += __popc(sb[t * W + w]);
+            }
+        }
+        __syncthreads();
+    }
+};
+
 // support index of voxel i of tile line `line` (-1 off the support)
 template <int N>
 __device__ __forceinline__ int tile_local(const uint32_t* sb, const int* sp, const int* sl, int line, int i) {
@@ -233,6 +271,8 @@ __global__ __launch_bounds__((PassShape<R, N, LINES, OS>::THREADS)) void k_xinv_
     // all global loads up front: support codes (or the tile's support words), the x and y half
     // spectra, then component z
     const bool use_bits = sink.lines.bits != nullptr;
+    TileBitsRegs<N, LINES, THREADS> tb;
+    if (use_bits) tb.fetch(sink.lines, line0, nl);
     int code[KE];
     if (!use_bits) {
 #pragma unroll
@@ -275,7 +315,7 @@ __global__ __launch_bounds__((PassShape<R, N, LINES, OS>::THREADS)) void k_xinv_
         uint32_t* sb = reinterpret_cast<uint32_t*>(buf + LINES * LD);
         int* sp = reinterpret_cast<int*>(sb + LINES * W);
         int* sl = sp + LINES * W;
-        load_tile_bits<N, LINES, THREADS>(sink.lines, line0, nl, sb, sp, sl);
+        tb.store(sb, sp, sl);
 #pragma unroll
         for (int k = 0; k < KE; ++k) {
             const int idx = threadIdx.x + k * THREADS;

@@ -1,7 +1,10 @@
 """Train the warm-start U-Net on solver-generated pairs and evaluate it on held-out media.
 
-    python -m fft_method_test_amd.train_warmstart --n 128 --pairs 400 --epochs 4 \\
-        --out fft_method_test_amd/data/warmstart_unet.npz
+    python -m fft_method_test_amd.train_warmstart --n 64 --pairs 192 --epochs 10 --lr 5e-4 --out mse.npz
+    python -m fft_method_test_amd.train_warmstart --objective residual --init mse.npz --n 64 \\
+        --pairs 96 --epochs 6 --lr 2e-4 --bf16 --out fft_method_test_amd/data/warmstart_unet.npz
+
+(the two commands that produced the committed network)
 
 Pairs: random periodic media (overlapping spheres or prolate ellipsoids of aspect 3, porosity
 U[0.3, 0.8], training seeds from --seed) and their direction-x interface force density from the
@@ -73,6 +76,7 @@ def main():
                     help="mse: squared error against solver pairs; residual: relative residual of the "
                          "minimal-residual-scaled guess (needs only media, no solves)")
     ap.add_argument("--out", default=None)
+    ap.add_argument("--bf16", action="store_true", help="save the parameters as BF16 bit patterns")
     ap.add_argument("--eval-n", type=int, default=0)
     ap.add_argument("--eval-count", type=int, default=4)
     ap.add_argument("--eval-seed", type=int, default=20000)
@@ -100,7 +104,7 @@ def main():
         report.update(objective="residual", media_seconds=t1 - t0, train_seconds=t2 - t1, residual_history=hist)
         if args.out:
             os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
-            net.save(args.out)
+            net.save(args.out, bf16=args.bf16)
             log(f"[warmstart] saved {args.out}")
     elif args.pairs > 0 and args.epochs > 0:
         t0 = time.perf_counter()
@@ -112,7 +116,7 @@ def main():
         log(f"[warmstart] pairs {t1 - t0:.1f} s, training {t2 - t1:.1f} s")
         if args.out:
             os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
-            net.save(args.out)
+            net.save(args.out, bf16=args.bf16)
             log(f"[warmstart] saved {args.out}")
     if args.eval_n > 0:
         prec = _lib.PRECISION_MIXED if args.eval_precision == "mixed" else _lib.PRECISION_FP64

@@ -60,14 +60,25 @@ class WarmStartNet:
             raise ValueError("parameter vector size mismatch")
         check(_lib.load().fftperm_unet_set_params(self._h, p))
 
-    def save(self, path):
-        np.savez(path, params=self.get_params(), layers=np.asarray(LAYERS, dtype=np.int32))
+    def save(self, path, bf16=False):
+        """FP32 parameters, or (bf16=True) their BF16 bit patterns -- the precision the device
+        network computes with -- at half the size."""
+        p = self.get_params()
+        if bf16:
+            bits = (p.view(np.uint32) + 0x7FFF + ((p.view(np.uint32) >> 16) & 1)) >> 16
+            np.savez(path, params_bf16=bits.astype(np.uint16), layers=np.asarray(LAYERS, dtype=np.int32))
+        else:
+            np.savez(path, params=p, layers=np.asarray(LAYERS, dtype=np.int32))
 
     def load(self, path):
         with np.load(path, allow_pickle=False) as z:
             if not np.array_equal(z["layers"], np.asarray(LAYERS, dtype=np.int32)):
                 raise ValueError("saved network has a different architecture")
-            self.set_params(z["params"])
+            if "params_bf16" in z:
+                p = (z["params_bf16"].astype(np.uint32) << 16).view(np.float32)
+            else:
+                p = z["params"]
+            self.set_params(p)
 
     def loss_grad(self, mask_dev, target_dev, n, lr=0.0, want_grad=False):
         """mask_dev: torch uint8 (n^3, fluid = 1) on the GPU; target_dev: torch float32 (n^3, 3).
