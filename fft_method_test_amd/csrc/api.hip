@@ -153,7 +153,9 @@ struct fftperm_ctx {
     double mixed_switch = 2.0;      // recurrence residual below which sweep 1 applies the FP32 operator
     double mixed_floor = 3e-6;      // inner recurrence-residual floor of a refinement sweep
     uint32_t mixed_sweeps = 64;     // sweeps per solve at most
-    uint64_t fp32_iters = 0;        // MINRES iterations with the FP32 operator (whole context)
+    uint64_t fp32_iters = 0;        // iterations with the FP32 operator (whole context)
+    int mixed_solver = FFTPERM_MIXED_CG_RELIABLE;
+    double reliable_delta = 0.1;    // CG: reliable update when ||r|| falls below this x its last peak
     int device = 0;
     hipStream_t s = nullptr;
 
@@ -192,6 +194,7 @@ struct fftperm_ctx {
     double *x = nullptr, *v_old = nullptr, *v = nullptr, *z = nullptr, *w_prev = nullptr, *w = nullptr;
     double *b = nullptr, *tmp = nullptr, *r = nullptr, *dx = nullptr;
     MinresState* st = nullptr;
+    CgState* cst = nullptr;
     double* hist = nullptr;
     size_t hist_cap = 0;
 
@@ -199,10 +202,11 @@ struct fftperm_ctx {
     // started), keyed by everything a capture bakes in: precision, buffer addresses, sizes
     struct GraphKey {
         int fp32;
+        int solver;  // 0 MINRES, 1 CG
         const void* p[9];
         long nb, tuning;
         bool operator==(const GraphKey& o) const {
-            if (fp32 != o.fp32 || nb != o.nb || tuning != o.tuning) return false;
+            if (fp32 != o.fp32 || solver != o.solver || nb != o.nb || tuning != o.tuning) return false;
             for (int i = 0; i < 9; ++i)
                 if (p[i] != o.p[i]) return false;
             return true;
@@ -278,7 +282,7 @@ struct fftperm_ctx {
         dfree(part_x); dfree(part_z); dfree(part); dfree(small);
         dfree(x); dfree(v_old); dfree(v); dfree(z); dfree(w_prev); dfree(w);
         dfree(b); dfree(tmp); dfree(r); dfree(dx);
-        dfree(st); dfree(hist); dfree(field); dfree(full_a); dfree(t6);
+        dfree(st); dfree(cst); dfree(hist); dfree(field); dfree(full_a); dfree(t6);
         for (const ProfRec& r : prof_pending) {
             (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b);
             (void)hipEventDestroy(r.c); (void)hipEventDestroy(r.d);
@@ -711,9 +715,182 @@ MinresOut device_minres_mixed(fftperm_ctx* c, const double* bdev, const double* 
     return out;
 }
 
+// One CG iteration on the device (p = c->v, q = c->z, x = c->x group solution, r = c->r); every
+// kernel is a no-op once cst->done is set.
+template <typename R>
+void cg_iteration(fftperm_ctx* c) {
+    const long nb = (long)c->nb;
+    const double inv_nb = 1.0 / (double)nb;
+    CgState* st = c->cst;
+    XSource sr = src_none(c);
+    sr.compact = c->v;
+    sr.mean = st->mean_in;
+    XSink sk = sink_none(c);
+    sk.compact = c->z;
+    sk.partials = c->part_x;
+    // p.A p comes from the fused z pass (Parseval)
+    c->chain<R>(sr, sk, &st->done, c->part_z);
+    launch_cg_alpha(c->part_x, (int)c->xparts, c->part_z, (int)c->zparts, st, inv_nb, 1.0 / (double)c->g.N, c->s);
+    launch_cg_update(c->x, c->r, c->v, c->z, st, nb, c->part, c->s);
+    launch_cg_beta(c->part, st, c->hist, c->s);
+    launch_cg_direction(c->v, c->r, st, nb, c->part, c->s);
+    launch_cg_finish_mean_in(c->part, st, inv_nb, c->s);
+}
+
+template <typename R>
+hipGraphExec_t cg_graph(fftperm_ctx* c) {
+    fftperm_ctx::GraphKey k{};
+    k.fp32 = sizeof(R) == 4 ? 1 : 0;
+    k.solver = 1;
+    const void* ptrs[9] = {c->x, c->r, c->v, c->z, c->cst, c->hist, c->part_x, c->part_z, c->part};
+    for (int i = 0; i < 9; ++i) k.p[i] = ptrs[i];
+    k.nb = (long)c->nb;
+    k.tuning = tuning_generation();
+    for (const auto& g : c->graphs)
+        if (g.key == k) return g.exec;
+    if (c->graphs.size() >= 8) {
+        HIPCHK(hipGraphExecDestroy(c->graphs.front().exec));
+        c->graphs.erase(c->graphs.begin());
+    }
+    hipGraph_t graph = nullptr;
+    HIPCHK(hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal));
+    cg_iteration<R>(c);
+    cg_iteration<R>(c);
+    HIPCHK(hipStreamEndCapture(c->s, &graph));
+    hipGraphExec_t exec = nullptr;
+    const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    HIPCHK(e);
+    c->graphs.push_back({k, exec});
+    return exec;
+}
+
+// MIXED-mode solve by conjugate gradients with reliable updates: the iterations apply the operator
+// in precision R (FP32 in MIXED mode) with FP64 vectors; the recursive residual r drifts from the
+// true one by the accumulated FP32 apply errors, so whenever ||r|| falls below reliable_delta times
+// its peak since the last update (or below tol ||b||) the group solution is added to the FP64
+// accumulator and r is replaced by b - A64 x_acc.  The solve ends when that true residual meets
+// tol, or after maxit iterations.  Search directions survive the updates, so no Krylov
+// information is thrown away (unlike restarted refinement sweeps).  The variational operator is
+// symmetric positive semi-definite and b lies in its range, so CG applies.
+// Result in c->x; *updates_out = reliable updates done; hist_out: the true residuals.
+template <typename R>
+MinresOut device_cg_reliable(fftperm_ctx* c, const double* bdev, const double* x0dev, double tol, uint64_t maxit,
+                             uint64_t* updates_out, std::vector<double>* hist_out) {
+    if (!(tol > 0.0)) throw Err(FFTPERM_EINVAL, "MinresSolver tolerance must be positive");
+    const long L = c->L3();
+    const long nb = (long)c->nb;
+    const double inv_nb = 1.0 / (double)nb;
+    MinresOut out;
+    *updates_out = 0;
+    const double bnorm = device_norm(c, bdev, L);
+    double* xacc = c->dx;
+    if (x0dev) {
+        if (x0dev != xacc)
+            HIPCHK(hipMemcpyAsync(xacc, x0dev, L * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+    } else {
+        HIPCHK(hipMemsetAsync(xacc, 0, L * sizeof(double), c->s));
+    }
+    bool xacc_zero = (x0dev == nullptr);
+    auto finish = [&](double rel) {
+        HIPCHK(hipMemcpyAsync(c->x, xacc, L * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+        c->sync();
+        out.final_rel = rel;
+        out.converged = rel <= tol ? 1 : 0;
+        out.hist_len = hist_out ? hist_out->size() : 0;
+    };
+    if (bnorm == 0.0) {
+        if (hist_out) hist_out->push_back(0.0);
+        finish(0.0);
+        out.converged = 1;
+        return out;
+    }
+    // r = b - A64 x_acc, its norm relative to ||b||
+    auto true_residual = [&]() {
+        if (!xacc_zero) {
+            device_apply<double>(c, xacc, c->tmp);
+            launch_axpby(1.0, bdev, -1.0, c->tmp, c->r, L, c->s);
+        } else {
+            HIPCHK(hipMemcpyAsync(c->r, bdev, L * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+        }
+        return device_norm(c, c->r, L) / bnorm;
+    };
+    double rel = true_residual();
+    if (hist_out) hist_out->push_back(rel);
+    if (rel <= tol || maxit == 0) {
+        finish(rel);
+        return out;
+    }
+    c->ensure_hist((size_t)maxit + 2);
+    HIPCHK(hipMemsetAsync(c->x, 0, L * sizeof(double), c->s));
+    HIPCHK(hipMemcpyAsync(c->v, c->r, L * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+    CgState h;
+    std::memset(&h, 0, sizeof(h));
+    const double rn0 = rel * bnorm;
+    h.rr = rn0 * rn0;
+    h.rnorm_max = rn0;
+    h.rhs_norm = bnorm;
+    h.tol = tol;
+    h.delta = c->reliable_delta;
+    h.maxit = (long long)maxit;
+    HIPCHK(hipMemcpyAsync(c->cst, &h, sizeof(h), hipMemcpyHostToDevice, c->s));
+    launch_compact_sums(c->v, nb, c->part, nullptr, c->s);
+    launch_finish_sum(c->part, kRedBlocks, 3, inv_nb, c->cst->mean_in, nullptr, c->s);
+    const uint64_t chunk = (c->g.N >= (1L << 21)) ? 4 : 32;
+    long long seg_start = 0;  // iterations done before the current segment (between updates)
+    for (;;) {
+        // iterate until the device pauses (update owed) or stops (maxit, breakdown); live timing
+        // tags each eager launch with its index in the segment so that no-op launches are dropped
+        int done = 0;
+        long long seg_launch = 0;
+        while (!done) {
+            const bool graph = c->use_graphs && !c->prof;
+            uint64_t i = 0;
+            if (graph)
+                for (; i + 2 <= chunk; i += 2) HIPCHK(hipGraphLaunch(cg_graph<R>(c), c->s));
+            for (; i < chunk; ++i) {
+                c->cur_tag = seg_launch++;
+                cg_iteration<R>(c);
+            }
+            c->cur_tag = -1;
+            HIPCHK(hipMemcpyAsync(&done, &c->cst->done, sizeof(int), hipMemcpyDeviceToHost, c->s));
+            c->sync();
+        }
+        HIPCHK(hipMemcpyAsync(&h, c->cst, sizeof(h), hipMemcpyDeviceToHost, c->s));
+        c->sync();
+        c->prof_resolve(h.iterations - seg_start);
+        seg_start = h.iterations;
+        // fold the group solution into the accumulator and take the FP64 true residual
+        launch_axpby(1.0, xacc, 1.0, c->x, xacc, L, c->s);
+        xacc_zero = false;
+        HIPCHK(hipMemsetAsync(c->x, 0, L * sizeof(double), c->s));
+        rel = true_residual();
+        if (hist_out) hist_out->push_back(rel);
+        ++*updates_out;
+        if (rel <= tol || !h.pause || h.breakdown || h.iterations >= h.maxit) break;
+        // continue from the true residual with the same search direction
+        const double rn = rel * bnorm;
+        h.beta = (rn * rn) / h.rr;
+        h.rr = rn * rn;
+        h.rnorm_max = rn;
+        h.done = 0;
+        h.pause = 0;
+        HIPCHK(hipMemcpyAsync(c->cst, &h, sizeof(h), hipMemcpyHostToDevice, c->s));
+        launch_cg_direction(c->v, c->r, c->cst, nb, c->part, c->s);
+        launch_cg_finish_mean_in(c->part, c->cst, inv_nb, c->s);
+    }
+    c->prof_resolve(LLONG_MAX);
+    out.iterations = (uint64_t)h.iterations;
+    if (sizeof(R) == 4) c->fp32_iters += (uint64_t)h.iterations;
+    finish(rel);
+    return out;
+}
+
 MinresOut run_minres(fftperm_ctx* c, const double* bdev, const double* x0dev, double tol, uint64_t maxit,
                      uint64_t* sweeps, std::vector<double>* mixed_hist) {
     *sweeps = 0;
+    if (c->precision == FFTPERM_PRECISION_MIXED && c->mixed_solver == FFTPERM_MIXED_CG_RELIABLE)
+        return device_cg_reliable<float>(c, bdev, x0dev, tol, maxit, sweeps, mixed_hist);
     if (c->precision == FFTPERM_PRECISION_MIXED) return device_minres_mixed(c, bdev, x0dev, tol, maxit, sweeps, mixed_hist);
     return device_minres(c, bdev, x0dev, tol, maxit, 0.0, nullptr);
 }
@@ -958,6 +1135,7 @@ int fftperm_create(const fftperm_grid* grid, double mu, int p_radius, int precis
         c->part = dalloc<double>(8 * (size_t)kRedBlocks);
         c->small = dalloc<double>(32);
         c->st = dalloc<MinresState>(1);
+        c->cst = dalloc<CgState>(1);
         c->sync();
         *out = c.release();
     });
@@ -1153,6 +1331,22 @@ int fftperm_operator_apply(fftperm_ctx* ctx, const double* x, double* y) {
     });
 }
 
+int fftperm_operator_apply_fp32(fftperm_ctx* ctx, const double* x, double* y) {
+    return guarded([&] {
+        check_ctx(ctx);
+        ctx->require_support();
+        if (!x || !y) throw Err(FFTPERM_EINVAL, "null argument");
+        if (ctx->precision != FFTPERM_PRECISION_MIXED)
+            throw Err(FFTPERM_EINVAL, "the FP32 operator exists in MIXED contexts only");
+        ctx->ensure_vectors();
+        const long L = ctx->L3();
+        HIPCHK(hipMemcpyAsync(ctx->r, x, L * sizeof(double), hipMemcpyHostToDevice, ctx->s));
+        device_apply<float>(ctx, ctx->r, ctx->tmp);
+        HIPCHK(hipMemcpyAsync(y, ctx->tmp, L * sizeof(double), hipMemcpyDeviceToHost, ctx->s));
+        ctx->sync();
+    });
+}
+
 int fftperm_minres(fftperm_ctx* ctx, const double* b, const double* x0, double tol, uint64_t maxit, double* x,
                    double* history, fftperm_solve_info* info) {
     return guarded([&] {
@@ -1190,6 +1384,18 @@ int fftperm_set_refinement(fftperm_ctx* ctx, double fp32_below, double inner_flo
         ctx->mixed_switch = fp32_below;
         ctx->mixed_floor = inner_floor;
         ctx->mixed_sweeps = max_sweeps;
+    });
+}
+
+int fftperm_set_mixed_solver(fftperm_ctx* ctx, int solver, double reliable_delta) {
+    return guarded([&] {
+        check_ctx(ctx);
+        if (solver != FFTPERM_MIXED_CG_RELIABLE && solver != FFTPERM_MIXED_MINRES_REFINED)
+            throw Err(FFTPERM_EINVAL, "unknown mixed solver");
+        if (solver == FFTPERM_MIXED_CG_RELIABLE && !(reliable_delta > 0.0 && reliable_delta < 1.0))
+            throw Err(FFTPERM_EINVAL, "reliable_delta must lie in (0, 1)");
+        ctx->mixed_solver = solver;
+        if (solver == FFTPERM_MIXED_CG_RELIABLE) ctx->reliable_delta = reliable_delta;
     });
 }
 

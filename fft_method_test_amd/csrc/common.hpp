@@ -31,6 +31,9 @@ __device__ __forceinline__ cx<R> cmul(cx<R> a, cx<R> b) {
 }
 template <typename R>
 __device__ __forceinline__ cx<R> conjc(cx<R> a) { return mkc<R>(a.x, -a.y); }
+// complex value converted to the precision T (rounded to nearest when narrowing)
+template <typename T, typename C>
+__device__ __forceinline__ cx<T> cvtc(C a) { return mkc<T>((T)a.x, (T)a.y); }
 
 // One line-transform plan: n = prod(rad[0..nst)), Stockham stages in that order.
 struct LinePlan {

@@ -42,9 +42,11 @@ struct PlanInfo<Radices<Rs...>> {
     }
 };
 
-// One stage: each thread owns K = (N/RAD)/T butterflies of one line.
-template <typename R, int SIGN, int N, int RAD, int NS, int LINES, int T, int LD>
-__device__ __forceinline__ void sstage(cx<R>* __restrict__ buf, const cx<R>* __restrict__ tw, int nlines) {
+// One stage: each thread owns K = (N/RAD)/T butterflies of one line.  Values are stored in LDS
+// as cx<R> and computed in registers as cx<RC> with twiddles from a cx<RT> table; RC = double over
+// R = float keeps the FP32 storage but rounds once per stage instead of once per butterfly level.
+template <typename R, int SIGN, int N, int RAD, int NS, int LINES, int T, int LD, typename RC, typename RT>
+__device__ __forceinline__ void sstage(cx<R>* __restrict__ buf, const cx<RT>* __restrict__ tw, int nlines) {
     constexpr int M = N / RAD;
     static_assert(M % T == 0, "butterflies per line must split evenly over the line's threads");
     constexpr int K = M / T;
@@ -52,7 +54,7 @@ __device__ __forceinline__ void sstage(cx<R>* __restrict__ buf, const cx<R>* __r
     const int ci = threadIdx.x % LINES;
     const int jt = threadIdx.x / LINES;
     const bool act = ci < nlines && jt < T;  // workgroups may carry extra threads for the memory phases
-    cx<R> v[K][RAD];
+    cx<RC> v[K][RAD];
     cx<R>* L = buf + ci * LD;
     if (act) {
 #pragma unroll
@@ -60,16 +62,16 @@ __device__ __forceinline__ void sstage(cx<R>* __restrict__ buf, const cx<R>* __r
             const int j = jt + q * T;
             const int k = j % NS;
 #pragma unroll
-            for (int r = 0; r < RAD; ++r) v[q][r] = L[j + r * M];
+            for (int r = 0; r < RAD; ++r) v[q][r] = cvtc<RC>(L[j + r * M]);
             if (NS > 1) {
 #pragma unroll
                 for (int r = 1; r < RAD; ++r) {
-                    cx<R> w = tw[r * k * TS];
+                    cx<RC> w = cvtc<RC>(tw[r * k * TS]);
                     if (SIGN > 0) w.y = -w.y;
-                    v[q][r] = cmul<R>(v[q][r], w);
+                    v[q][r] = cmul<RC>(v[q][r], w);
                 }
             }
-            dft_small<R, SIGN, RAD>(v[q]);
+            dft_small<RC, SIGN, RAD>(v[q]);
         }
     }
     __syncthreads();
@@ -80,33 +82,33 @@ __device__ __forceinline__ void sstage(cx<R>* __restrict__ buf, const cx<R>* __r
             const int k = j % NS;
             const int base = (j / NS) * NS * RAD + k;
 #pragma unroll
-            for (int r = 0; r < RAD; ++r) L[base + r * NS] = v[q][r];
+            for (int r = 0; r < RAD; ++r) L[base + r * NS] = cvtc<R>(v[q][r]);
         }
     }
     __syncthreads();
 }
 
-template <typename R, int SIGN, int N, int LINES, int T, int LD, int NS>
-__device__ __forceinline__ void sstages(cx<R>*, const cx<R>*, int, Radices<>) {}
+template <typename R, typename RC, typename RT, int SIGN, int N, int LINES, int T, int LD, int NS>
+__device__ __forceinline__ void sstages(cx<R>*, const cx<RT>*, int, Radices<>) {}
 
-template <typename R, int SIGN, int N, int LINES, int T, int LD, int NS, int R0, int... Rs>
-__device__ __forceinline__ void sstages(cx<R>* buf, const cx<R>* tw, int nlines, Radices<R0, Rs...>) {
-    sstage<R, SIGN, N, R0, NS, LINES, T, LD>(buf, tw, nlines);
-    sstages<R, SIGN, N, LINES, T, LD, NS * R0>(buf, tw, nlines, Radices<Rs...>{});
+template <typename R, typename RC, typename RT, int SIGN, int N, int LINES, int T, int LD, int NS, int R0, int... Rs>
+__device__ __forceinline__ void sstages(cx<R>* buf, const cx<RT>* tw, int nlines, Radices<R0, Rs...>) {
+    sstage<R, SIGN, N, R0, NS, LINES, T, LD, RC, RT>(buf, tw, nlines);
+    sstages<R, RC, RT, SIGN, N, LINES, T, LD, NS * R0>(buf, tw, nlines, Radices<Rs...>{});
 }
 
 // Transform `nlines` (<= LINES) lines of length N held in LDS at pitch LD.
 // T threads per line do the butterflies (threads LINES*T and above idle through the stages).
-// Caller synchronised after filling buf / tw.
-template <typename R, int SIGN, int N, int LINES, int T, int LD>
-__device__ __forceinline__ void sfft_lines(cx<R>* buf, const cx<R>* tw, int nlines) {
-    sstages<R, SIGN, N, LINES, T, LD, 1>(buf, tw, nlines, typename StaticPlan<N>::type{});
+// Caller synchronised after filling buf / tw.  RC / RT: compute and twiddle precision (default R).
+template <typename R, int SIGN, int N, int LINES, int T, int LD, typename RC = R, typename RT = R>
+__device__ __forceinline__ void sfft_lines(cx<R>* buf, const cx<RT>* tw, int nlines) {
+    sstages<R, RC, RT, SIGN, N, LINES, T, LD, 1>(buf, tw, nlines, typename StaticPlan<N>::type{});
 }
 
 // The same with an explicit radix plan (PLAN = Radices<...>, product N).
-template <typename R, int SIGN, int N, int LINES, int T, int LD, typename PLAN>
-__device__ __forceinline__ void sfft_lines_plan(cx<R>* buf, const cx<R>* tw, int nlines) {
-    sstages<R, SIGN, N, LINES, T, LD, 1>(buf, tw, nlines, PLAN{});
+template <typename R, int SIGN, int N, int LINES, int T, int LD, typename PLAN, typename RC = R, typename RT = R>
+__device__ __forceinline__ void sfft_lines_plan(cx<R>* buf, const cx<RT>* tw, int nlines) {
+    sstages<R, RC, RT, SIGN, N, LINES, T, LD, 1>(buf, tw, nlines, PLAN{});
 }
 
 // Radix <= 8 plans: smaller butterflies (32 live values in FP64 instead of 64) for kernels that

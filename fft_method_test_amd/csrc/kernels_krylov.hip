@@ -364,4 +364,138 @@ void launch_mr_finish_mean_in(const double* partials, MinresState* st, double in
     hipLaunchKernelGGL(k_mr_finish_mean_in, dim3(1), dim3(kRedThreads), 0, s, partials, st, inv_nb);
 }
 
+// ------------------------------------------------------------ CG with reliable updates
+__global__ void k_cg_alpha(const double* __restrict__ xpart, int nxparts, const double* __restrict__ zpart,
+                           int nzparts, CgState* st, double inv_nb, double inv_n) {
+    if (st->done) return;
+    double acc[4] = {0, 0, 0, 0};
+    for (int p = threadIdx.x; p < nxparts; p += blockDim.x) {
+        const double* q = xpart + (long)kXinvParts * p;
+        acc[0] += q[0];
+        acc[1] += q[1];
+        acc[2] += q[2];
+    }
+    for (int p = threadIdx.x; p < nzparts; p += blockDim.x) acc[3] += zpart[p];
+    __shared__ double res[4];
+    block_sum<4>(acc, res);
+    if (threadIdx.x == 0) {
+        st->mean_out[0] = res[0] * inv_nb;
+        st->mean_out[1] = res[1] * inv_nb;
+        st->mean_out[2] = res[2] * inv_nb;
+        // p.A p = <scatter(P_B p), G scatter(P_B p)> over the grid (Parseval, from the z pass)
+        const double pap = res[3] * inv_n;
+        st->pap = pap;
+        if (pap > 0.0) {
+            st->alpha = st->rr / pap;
+        } else {
+            st->alpha = 0.0;
+            st->breakdown = 1;
+            st->done = 1;
+        }
+    }
+}
+
+void launch_cg_alpha(const double* xpart, int nxparts, const double* zpart, int nzparts, CgState* st, double inv_nb,
+                     double inv_n, hipStream_t s) {
+    hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(1024), 0, s, xpart, nxparts, zpart, nzparts, st, inv_nb, inv_n);
+}
+
+__global__ __launch_bounds__(kFlatThreads) void k_cg_update(double* __restrict__ x, double* __restrict__ r,
+                                                            const double* __restrict__ p,
+                                                            const double* __restrict__ q, CgState* st, long n,
+                                                            double* __restrict__ partials) {
+    if (st->done) return;
+    const double a = st->alpha;
+    const long i0 = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const double m = st->mean_out[i0 % 3];
+    double acc[1] = {0.0};
+    for (long i = i0; i < n; i += (long)gridDim.x * blockDim.x) {
+        x[i] += a * p[i];
+        const double ri = r[i] - a * (q[i] - m);
+        r[i] = ri;
+        acc[0] += ri * ri;
+    }
+    block_sum<1>(acc, partials + blockIdx.x);
+}
+
+void launch_cg_update(double* x, double* r, const double* p, const double* q, CgState* st, long nb, double* partials,
+                      hipStream_t s) {
+    hipLaunchKernelGGL(k_cg_update, dim3(kRedBlocks), dim3(kFlatThreads), 0, s, x, r, p, q, st, 3 * nb, partials);
+}
+
+__global__ void k_cg_beta(const double* __restrict__ partials, CgState* st, double* __restrict__ history) {
+    if (st->done) return;
+    double acc[1] = {0.0};
+    for (int p = threadIdx.x; p < kRedBlocks; p += blockDim.x) acc[0] += partials[p];
+    __shared__ double res[1];
+    block_sum<1>(acc, res);
+    if (threadIdx.x != 0) return;
+    const double rr_new = res[0] > 0.0 ? res[0] : 0.0;
+    const double rn = sqrt(rr_new);
+    const double rel = rn / st->rhs_norm;
+    const long long k = st->iterations + 1;
+    st->iterations = k;
+    st->rr_new = rr_new;
+    st->rel_res = rel;
+    if (history != nullptr) history[k] = rel;
+    if (rel <= st->tol || rn < st->delta * st->rnorm_max) {
+        // the host replaces r by the true residual (and stops if that meets tol)
+        st->pause = 1;
+        st->done = 1;
+        return;
+    }
+    if (k >= st->maxit) {
+        st->done = 1;
+        return;
+    }
+    if (rn > st->rnorm_max) st->rnorm_max = rn;
+    st->beta = rr_new / st->rr;
+    st->rr = rr_new;
+}
+
+void launch_cg_beta(const double* partials, CgState* st, double* history, hipStream_t s) {
+    hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(kRedThreads), 0, s, partials, st, history);
+}
+
+__global__ __launch_bounds__(kFlatThreads) void k_cg_direction(double* __restrict__ p, const double* __restrict__ r,
+                                                               CgState* st, long n, double* __restrict__ partials) {
+    if (st->done) return;
+    const double b = st->beta;
+    const long i0 = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double s = 0.0;
+    for (long i = i0; i < n; i += (long)gridDim.x * blockDim.x) {
+        const double pi = r[i] + b * p[i];
+        p[i] = pi;
+        s += pi;
+    }
+    const int c = (int)(i0 % 3);
+    double acc[3] = {c == 0 ? s : 0.0, c == 1 ? s : 0.0, c == 2 ? s : 0.0};
+    block_sum<3>(acc, partials + 3L * blockIdx.x);
+}
+
+void launch_cg_direction(double* p, const double* r, CgState* st, long nb, double* partials, hipStream_t s) {
+    hipLaunchKernelGGL(k_cg_direction, dim3(kRedBlocks), dim3(kFlatThreads), 0, s, p, r, st, 3 * nb, partials);
+}
+
+__global__ void k_cg_finish_mean_in(const double* __restrict__ partials, CgState* st, double inv_nb) {
+    if (st->done) return;
+    double acc[3] = {0, 0, 0};
+    for (int p = threadIdx.x; p < kRedBlocks; p += blockDim.x) {
+        acc[0] += partials[3 * p];
+        acc[1] += partials[3 * p + 1];
+        acc[2] += partials[3 * p + 2];
+    }
+    __shared__ double res[3];
+    block_sum<3>(acc, res);
+    if (threadIdx.x == 0) {
+        st->mean_in[0] = res[0] * inv_nb;
+        st->mean_in[1] = res[1] * inv_nb;
+        st->mean_in[2] = res[2] * inv_nb;
+    }
+}
+
+void launch_cg_finish_mean_in(const double* partials, CgState* st, double inv_nb, hipStream_t s) {
+    hipLaunchKernelGGL(k_cg_finish_mean_in, dim3(1), dim3(kRedThreads), 0, s, partials, st, inv_nb);
+}
+
 }  // namespace fftperm

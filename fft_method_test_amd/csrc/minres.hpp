@@ -51,6 +51,40 @@ void launch_mr_finish_mean_in(const double* partials, MinresState* st, double in
 void launch_mr_finish_mean_out(const double* xpart, int nxparts, const double* zpart, int nzparts, MinresState* st,
                                double inv_nb, double inv_n, hipStream_t s);
 
+// Conjugate gradients with reliable updates (the MIXED-mode solver): the iterations apply the FP32
+// operator and update x and the recursive residual r; whenever ||r|| drops below delta times its
+// largest value since the last update, or below tol ||b||, the iteration pauses (done = pause = 1)
+// and the host replaces r by the FP64 true residual of the accumulated solution.
+struct CgState {
+    double rr;           // r.r of the recursive residual the current beta / alpha use
+    double rr_new;       // r.r after the latest update
+    double alpha, beta, pap;
+    double mean_in[3];   // support mean of p (input projection of the next apply)
+    double mean_out[3];  // support mean of the raw apply output
+    double rnorm_max;    // largest ||r|| since the last reliable update
+    double rhs_norm, tol, delta;
+    double rel_res;      // ||r|| / ||b|| of the recursive residual
+    long long iterations;
+    long long maxit;
+    int done;            // every CG kernel returns at once while set
+    int pause;           // 1: the host owes a reliable update / convergence check
+    int breakdown;       // p.A p <= 0
+    int pad_;
+};
+
+// mean_out, p.A p (Parseval partials of the fused z pass / N), alpha = rr / pAp
+void launch_cg_alpha(const double* xpart, int nxparts, const double* zpart, int nzparts, CgState* st, double inv_nb,
+                     double inv_n, hipStream_t s);
+// x += alpha p ; r -= alpha (q - mean_out) ; partials: r.r
+void launch_cg_update(double* x, double* r, const double* p, const double* q, CgState* st, long nb, double* partials,
+                      hipStream_t s);
+// rr_new, relative residual, history, pause / stop decisions, beta = rr_new / rr
+void launch_cg_beta(const double* partials, CgState* st, double* history, hipStream_t s);
+// p = r + beta p ; partials: per-component sums of p
+void launch_cg_direction(double* p, const double* r, CgState* st, long nb, double* partials, hipStream_t s);
+// mean_in = sums / nb
+void launch_cg_finish_mean_in(const double* partials, CgState* st, double inv_nb, hipStream_t s);
+
 // Generic helpers
 void launch_dot_partials(const double* a, const double* b, long n, double* partials, hipStream_t s);
 void launch_axpby(double a, const double* x, double b, const double* y, double* out, long n, hipStream_t s);

@@ -21,32 +21,10 @@ constexpr size_t bits_lds() {
     return (size_t)LINES * (2 * ((N + 31) / 32) + 1) * sizeof(int);
 }
 
-// Loads the tile's support words and line starts into LDS and computes the in-line word
-// prefixes; returns after a barrier.  sb: LINES*W words, sp: LINES*W prefixes, sl: LINES starts.
-template <int N, int LINES, int THREADS>
-__device__ __forceinline__ void load_tile_bits(const LineBits& lb, long line0, int nl, uint32_t* sb, int* sp,
-                                               int* sl) {
-    constexpr int W = (N + 31) / 32;
-    for (int t = threadIdx.x; t < LINES * W; t += THREADS) {
-        const int line = t / W;
-        sb[t] = (line < nl) ? lb.bits[(line0 + line) * W + t % W] : 0u;
-    }
-    for (int t = threadIdx.x; t < LINES; t += THREADS) sl[t] = (t < nl) ? lb.lstart[line0 + t] : 0;
-    __syncthreads();
-    for (int t = threadIdx.x; t < LINES; t += THREADS) {
-        int acc = 0;
-#pragma unroll
-        for (int w = 0; w < W; ++w) {
-            sp[t * W + w] = acc;
-            acc += __popc(sb[t * W + w]);
-        }
-    }
-    __syncthreads();
-}
-
-// Two-phase variant for kernels that issue other loads in between: fetch the thread's share of the
-// tile words and line starts into registers first (so waiting for them does not wait for loads
-// issued later), then store them to LDS and compute the prefixes.
+// The tile's support words and line starts, in two phases so that kernels can issue other loads in
+// between: fetch the thread's share into registers first (waiting for them then does not wait for
+// loads issued later), then store them to LDS (sb: LINES*W words, sl: LINES starts) and compute
+// the in-line word prefixes (sp: LINES*W); store() returns after a barrier.
 template <int N, int LINES, int THREADS>
 struct TileBitsRegs {
     static constexpr int W = (N + 31) / 32;
@@ -120,6 +98,9 @@ __global__ __launch_bounds__((PassShape<R, N, LINES, OS>::THREADS)) void k_xfwd_
         uint32_t* sb = reinterpret_cast<uint32_t*>(buf + LINES * LD);
         int* sp = reinterpret_cast<int*>(sb + LINES * W);
         int* sl = sp + LINES * W;
+        // the support words are requested first, so their latency overlaps the compact-run loads
+        TileBitsRegs<N, LINES, THREADS> tb;
+        tb.fetch(src.lines, line0, nl);
         const long cs = src.lines.lstart[line0];
         const long ce = (line0 + nl < nlines) ? (long)src.lines.lstart[line0 + nl] : src.lines.nb;
         const long cnt3 = 3 * (ce - cs);
@@ -131,7 +112,7 @@ __global__ __launch_bounds__((PassShape<R, N, LINES, OS>::THREADS)) void k_xfwd_
 #pragma unroll 4
             for (long e = threadIdx.x; e < cnt3; e += THREADS) stage[e] = cc[e];
         }
-        load_tile_bits<N, LINES, THREADS>(src.lines, line0, nl, sb, sp, sl);
+        tb.store(sb, sp, sl);
         double mc[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) mc[c] = (src.mean != nullptr) ? src.mean[c] : 0.0;

@@ -130,6 +130,10 @@ class GreenOperator:
         the FP64 operator while its recurrence residual is above fp32_below (>= 1: FP32 throughout)."""
         check(_lib.load().fftperm_set_refinement(self._h, float(fp32_below), float(inner_floor), int(max_sweeps)))
 
+    def set_mixed_solver(self, solver=_lib.MIXED_CG_RELIABLE, reliable_delta=0.1):
+        """MIXED mode solver: CG with reliable updates (default) or refined MINRES sweeps."""
+        check(_lib.load().fftperm_set_mixed_solver(self._h, int(solver), float(reliable_delta)))
+
     def close(self):
         if getattr(self, "_h", None) and self._h.value:
             _lib.load().fftperm_destroy(self._h)
@@ -298,6 +302,15 @@ class VariationalOperator:
             raise ValueError("VariationalOperator::apply input size mismatch")
         y = np.zeros(self.size())
         check(_lib.load().fftperm_operator_apply(self._green._h, x, y))
+        return y
+
+    def apply_fp32(self, x):
+        """The FP32 apply the MIXED-mode sweeps use (MIXED contexts only; no reference counterpart)."""
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
+        if x.size != self.size():
+            raise ValueError("VariationalOperator::apply input size mismatch")
+        y = np.zeros(self.size())
+        check(_lib.load().fftperm_operator_apply_fp32(self._green._h, x, y))
         return y
 
 

@@ -45,10 +45,17 @@ enum {
 /* SupportMode (src/core/Types.hpp:26-29) */
 enum { FFTPERM_SUPPORT_FULLSOLID = 0, FFTPERM_SUPPORT_INTERFACE = 1 };
 /* FFTPERM_PRECISION_FP64: every step in FP64 (the reference with FFT_METHOD_PRECISION=DOUBLE).
- * FFTPERM_PRECISION_MIXED: FFTs and Green multiply in FP32 inside MINRES; Lanczos vectors,
- * inner products, Givens rotations, true residuals, the refinement update, RHS,
- * velocity recovery and averages in FP64. */
+ * FFTPERM_PRECISION_MIXED: FFTs and Green multiply in FP32 inside the iterative solve; Krylov
+ * vectors, inner products, recurrence scalars, true residuals, RHS, velocity recovery and
+ * averages in FP64.  The solve ends on the FP64 true residual ||b - A x|| <= tol ||b||. */
 enum { FFTPERM_PRECISION_FP64 = 0, FFTPERM_PRECISION_MIXED = 1 };
+/* MIXED-mode solvers (fftperm_set_mixed_solver):
+ * FFTPERM_MIXED_CG_RELIABLE: conjugate gradients on the FP32 operator with reliable updates (the
+ *   recursive residual is replaced by the FP64 true residual whenever it has fallen by the factor
+ *   reliable_delta since the last replacement, keeping the search direction); the default.
+ * FFTPERM_MIXED_MINRES_REFINED: iterative refinement around restarted FP32-operator MINRES sweeps
+ *   (fftperm_set_refinement). */
+enum { FFTPERM_MIXED_CG_RELIABLE = 0, FFTPERM_MIXED_MINRES_REFINED = 1 };
 
 typedef struct fftperm_ctx fftperm_ctx;
 
@@ -118,11 +125,18 @@ int fftperm_rhs(fftperm_ctx* ctx, const double grad[3], double* b);
 /* VariationalOperator::apply (src/bignonnet/VariationalOperator.cpp:19-41): y = A x (L -> L).
  * FP64 in every precision mode. */
 int fftperm_operator_apply(fftperm_ctx* ctx, const double* x, double* y);
+/* The same operator through the FP32 five-sweep apply that MIXED-mode MINRES sweeps use (FP32
+ * spectrum and Green tensor, FP64 compact vectors and sums).  No counterpart in the reference: a
+ * hook to measure the FP32 operator's error against fftperm_operator_apply.  MIXED contexts only
+ * (FFTPERM_EINVAL otherwise). */
+int fftperm_operator_apply_fp32(fftperm_ctx* ctx, const double* x, double* y);
 
 /* MinresSolver(tol, maxit).solve(op, b, x0) (src/solver/MinresSolver.cpp:46-218) on the variational
  * operator.  x0 may be NULL (the reference's empty guess).  history (maxit + 1 entries) may be NULL.
- * tol <= 0 -> FFTPERM_EINVAL.  In MIXED mode: iterative refinement with FP32-operator MINRES sweeps
- * and FP64 true residuals until ||b - A x|| <= tol ||b|| (or maxit iterations in total). */
+ * tol <= 0 -> FFTPERM_EINVAL.  In MIXED mode the context's mixed solver runs instead (CG with
+ * reliable updates by default) until the FP64 true residual ||b - A x|| <= tol ||b|| (or maxit
+ * iterations in total); history then holds the true residuals at each FP64 check and
+ * info->refinement_sweeps counts those checks after the initial one. */
 int fftperm_minres(fftperm_ctx* ctx, const double* b, const double* x0, double tol, uint64_t maxit,
                    double* x, double* history, fftperm_solve_info* info);
 
@@ -134,6 +148,11 @@ int fftperm_minres(fftperm_ctx* ctx, const double* b, const double* x0, double t
  * most.  Defaults 2 (FP32 throughout), 3e-6, 64.  fp32_below < 0, inner_floor outside (0, 1) or
  * max_sweeps == 0 -> FFTPERM_EINVAL. */
 int fftperm_set_refinement(fftperm_ctx* ctx, double fp32_below, double inner_floor, uint32_t max_sweeps);
+
+/* MIXED-mode solver choice (no counterpart in the reference): FFTPERM_MIXED_CG_RELIABLE (default,
+ * reliable_delta in (0, 1), default 0.1) or FFTPERM_MIXED_MINRES_REFINED (reliable_delta ignored).
+ * Anything else -> FFTPERM_EINVAL. */
+int fftperm_set_mixed_solver(fftperm_ctx* ctx, int solver, double reliable_delta);
 
 /* VelocityRecovery::recover + average_over_domain (src/bignonnet/VelocityRecovery.cpp:12-47):
  * u (3N, may be NULL) and its domain average avg[3]. */

@@ -121,3 +121,61 @@ def test_mixed_precision_k_tensor_64(fp, oracle_mod):
     o.set_medium(mask)
     Ko, _ = o.solve_tensor(1e-10, 20000)
     assert rel(K, Ko) < 1e-6
+
+
+@pytest.mark.parametrize("solver", [0, 1])
+def test_mixed_solvers_k_and_true_residual(fp, oracle_mod, solver):
+    """Both MIXED solvers (CG with reliable updates, refined MINRES sweeps): K of the 32^3 sphere
+    cell within 1e-6 of the FP64 oracle, and the FP64 true residual of each direction at tol."""
+    from fft_method_test_amd import _lib, geometry as G, permeability as P
+
+    mask = G.sphere_cell(32).reshape(-1)
+    g = P.Grid3D(32, 32, 32)
+    green = P.GreenOperator(g, 1.0, 2, P.PRECISION_MIXED)
+    green.set_mixed_solver(solver, 0.1)
+    sup = P.ForceSupport(P.BinaryMedium(g, mask), P.SUPPORT_INTERFACE, green)
+    K, infos = P.solve_permeability_tensor(green, 1e-10, 10000)
+    o = oracle_mod.Oracle((32, 32, 32))
+    o.set_medium(mask)
+    Ko, _ = o.solve_tensor(1e-10, 10000)
+    assert all(i.converged for i in infos)
+    assert rel(K, Ko) < 1e-6
+    op = P.VariationalOperator(sup, green)
+    b = P.VariationalRHS(sup, green, None).build([0.0, 0.0, 1.0])
+    res = P.MinresSolver(1e-9, 10000).solve(op, b)
+    assert res.converged and res.refinement_sweeps >= 1
+    h = np.asarray(res.residual_history)
+    assert h[0] == pytest.approx(1.0) and h[-1] <= 1e-9 and h[-1] == res.final_rel_residual
+    assert P.Diagnostics.residual_norm(op, res.solution, b) <= 1e-9 * np.linalg.norm(b) * 1.01
+
+
+def test_mixed_cg_warm_start_and_maxit(fp, oracle_mod):
+    """CG with reliable updates: a guess at the FP64 solution converges at once; maxit stops an
+    unconverged solve after exactly maxit iterations."""
+    from fft_method_test_amd import geometry as G, permeability as P
+
+    mask = G.boolean_spheres(24, 3.0, 0.6, 5).reshape(-1)
+    g = P.Grid3D(24, 24, 24)
+    green = P.GreenOperator(g, 1.0, 2, P.PRECISION_MIXED)
+    sup = P.ForceSupport(P.BinaryMedium(g, mask), P.SUPPORT_INTERFACE, green)
+    op = P.VariationalOperator(sup, green)
+    o = oracle_mod.Oracle((24, 24, 24))
+    o.set_medium(mask)
+    b = o.rhs([1.0, 0.0, 0.0])
+    xo, _, _ = o.minres(b, 1e-12, 20000)
+    warm = P.MinresSolver(1e-8, 1000).solve(op, b, xo)
+    assert warm.converged and warm.iterations <= 2
+    cold = P.MinresSolver(1e-14, 10).solve(op, b)
+    assert not cold.converged and cold.iterations == 10
+    assert cold.final_rel_residual < 1.0
+
+
+def test_set_mixed_solver_validation(fp):
+    from fft_method_test_amd import permeability as P
+
+    green = P.GreenOperator(P.Grid3D(8, 8, 8), 1.0, 2, P.PRECISION_MIXED)
+    with pytest.raises(ValueError, match="unknown mixed solver"):
+        green.set_mixed_solver(7, 0.1)
+    with pytest.raises(ValueError, match="reliable_delta"):
+        green.set_mixed_solver(0, 1.5)
+    green.set_mixed_solver(1, 5.0)  # delta ignored for refined MINRES

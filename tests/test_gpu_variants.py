@@ -45,12 +45,12 @@ def pinned(green, kind, precision, code):
         _lib.check(_lib.load().fftperm_pin_variant(green._h, kind, precision, 0))
 
 
-def _pair(oracle_mod, shape, seed):
+def _pair(oracle_mod, shape, seed, precision=0):
     from fft_method_test_amd import permeability as P
 
     mask = random_mask(shape, 0.6, seed=seed)
     g = P.Grid3D(*shape)
-    green = P.GreenOperator(g)
+    green = P.GreenOperator(g, 1.0, 2, precision)
     sup = P.ForceSupport(P.BinaryMedium(g, mask), P.SUPPORT_INTERFACE, green)
     o = oracle_mod.Oracle(shape)
     o.set_medium(mask)
@@ -140,3 +140,40 @@ def test_fp32_variants_mixed_k_sphere32(fp, oracle_mod):
         with pinned(green, kind, _lib.PRECISION_MIXED, code):
             K, _ = P.solve_permeability_tensor(green, 1e-9, 10000)
             assert rel(K, k_ref) < 1e-6, (kind, code)
+
+
+# FP32 operator error bound: the five FP32 sweeps round the spectrum at every pass and compute the
+# butterflies in FP32; measured relative errors are 1e-7 .. 4e-7 on random vectors.
+FP32_APPLY_BOUND = 1e-6
+
+
+@pytest.mark.parametrize("n", [32, 64, 128, 256, 512])
+def test_fp32_operator_variants(fp, oracle_mod, n):
+    """Every FP32 tile variant (x forward / inverse, column pass, z pass) pinned in turn: the FP32
+    variational operator of the mixed sweeps within FP32_APPLY_BOUND of the FP64 oracle."""
+    from fft_method_test_amd import _lib, permeability as P
+
+    cases = []
+    for shape, kinds, codes in ((X_SHAPES[n], (_lib.PASS_XFWD, _lib.PASS_XINV), pass_codes(n)),
+                                (Y_SHAPES[n], (_lib.PASS_YPASS,), pass_codes(n)),
+                                (Z_SHAPES[n], (_lib.PASS_ZGREEN,), _lib.ZGREEN_CODES)):
+        green, sup, o = _pair(oracle_mod, shape, seed=n + 5, precision=_lib.PRECISION_MIXED)
+        op = P.VariationalOperator(sup, green)
+        x = np.random.default_rng(n).standard_normal(sup.size())
+        ref = o.apply(x)
+        for kind in kinds:
+            for code in codes:
+                with pinned(green, kind, _lib.PRECISION_MIXED, code):
+                    err = rel(op.apply_fp32(x), ref)
+                    cases.append((kind, code, err))
+                    assert err < FP32_APPLY_BOUND, (kind, code, err)
+        green.close()
+    print("fp32 apply errors", n, max(c[2] for c in cases))
+
+
+def test_fp32_operator_rejected_in_fp64_context(fp, oracle_mod):
+    from fft_method_test_amd import permeability as P
+
+    green, sup, _ = _pair(oracle_mod, (8, 8, 8), seed=3)
+    with pytest.raises(ValueError, match="MIXED contexts only"):
+        P.VariationalOperator(sup, green).apply_fp32(np.zeros(sup.size()))
