@@ -154,7 +154,7 @@ struct fftperm_ctx {
     double mixed_floor = 3e-6;      // inner recurrence-residual floor of a refinement sweep
     uint32_t mixed_sweeps = 64;     // sweeps per solve at most
     uint64_t fp32_iters = 0;        // iterations with the FP32 operator (whole context)
-    int mixed_solver = FFTPERM_MIXED_CG_RELIABLE;
+    int mixed_solver = FFTPERM_MIXED_CR_RELIABLE;
     double reliable_delta = 0.1;    // CG: reliable update when ||r|| falls below this x its last peak
     int device = 0;
     hipStream_t s = nullptr;
@@ -195,6 +195,7 @@ struct fftperm_ctx {
     double *b = nullptr, *tmp = nullptr, *r = nullptr, *dx = nullptr;
     MinresState* st = nullptr;
     CgState* cst = nullptr;
+    CrState* rst = nullptr;
     double* hist = nullptr;
     size_t hist_cap = 0;
 
@@ -202,7 +203,7 @@ struct fftperm_ctx {
     // started), keyed by everything a capture bakes in: precision, buffer addresses, sizes
     struct GraphKey {
         int fp32;
-        int solver;  // 0 MINRES, 1 CG
+        int solver;  // 0 MINRES, 1 CG, 2 CR
         const void* p[9];
         long nb, tuning;
         bool operator==(const GraphKey& o) const {
@@ -282,7 +283,7 @@ struct fftperm_ctx {
         dfree(part_x); dfree(part_z); dfree(part); dfree(small);
         dfree(x); dfree(v_old); dfree(v); dfree(z); dfree(w_prev); dfree(w);
         dfree(b); dfree(tmp); dfree(r); dfree(dx);
-        dfree(st); dfree(cst); dfree(hist); dfree(field); dfree(full_a); dfree(t6);
+        dfree(st); dfree(cst); dfree(rst); dfree(hist); dfree(field); dfree(full_a); dfree(t6);
         for (const ProfRec& r : prof_pending) {
             (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b);
             (void)hipEventDestroy(r.c); (void)hipEventDestroy(r.d);
@@ -886,9 +887,173 @@ MinresOut device_cg_reliable(fftperm_ctx* c, const double* bdev, const double* x
     return out;
 }
 
+// One CR iteration on the device (r = c->r, q = c->z, p = c->v, Ap = c->w, x = c->x group
+// solution); every kernel is a no-op once rst->done is set.
+template <typename R>
+void cr_iteration(fftperm_ctx* c) {
+    const long nb = (long)c->nb;
+    const double inv_nb = 1.0 / (double)nb;
+    CrState* st = c->rst;
+    XSource sr = src_none(c);
+    sr.compact = c->r;
+    sr.mean = st->mean_in;
+    XSink sk = sink_none(c);
+    sk.compact = c->z;
+    sk.partials = c->part_x;
+    // r.A r comes from the fused z pass (Parseval)
+    c->chain<R>(sr, sk, &st->done, c->part_z);
+    launch_cr_beta(c->part_x, (int)c->xparts, c->part_z, (int)c->zparts, st, inv_nb, 1.0 / (double)c->g.N, c->s);
+    launch_cr_direction(c->v, c->w, c->r, c->z, st, nb, c->part, c->s);
+    launch_cr_alpha(c->part, st, c->s);
+    launch_cr_update(c->x, c->r, c->v, c->w, st, nb, c->part, c->s);
+    launch_cr_check(c->part, st, c->hist, inv_nb, c->s);
+}
+
+template <typename R>
+hipGraphExec_t cr_graph(fftperm_ctx* c) {
+    fftperm_ctx::GraphKey k{};
+    k.fp32 = sizeof(R) == 4 ? 1 : 0;
+    k.solver = 2;
+    const void* ptrs[9] = {c->x, c->r, c->v, c->z, c->w, c->rst, c->hist, c->part_x, c->part};
+    for (int i = 0; i < 9; ++i) k.p[i] = ptrs[i];
+    k.nb = (long)c->nb;
+    k.tuning = tuning_generation();
+    for (const auto& g : c->graphs)
+        if (g.key == k) return g.exec;
+    if (c->graphs.size() >= 8) {
+        HIPCHK(hipGraphExecDestroy(c->graphs.front().exec));
+        c->graphs.erase(c->graphs.begin());
+    }
+    hipGraph_t graph = nullptr;
+    HIPCHK(hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal));
+    cr_iteration<R>(c);
+    cr_iteration<R>(c);
+    HIPCHK(hipStreamEndCapture(c->s, &graph));
+    hipGraphExec_t exec = nullptr;
+    const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    HIPCHK(e);
+    c->graphs.push_back({k, exec});
+    return exec;
+}
+
+// MIXED-mode solve by conjugate residuals with reliable updates: the same FP64 accumulator and
+// true-residual replacement as device_cg_reliable, around the CR recurrence (residual-minimising
+// like MINRES, and carrying r explicitly).  The direction p and its image Ap survive the
+// replacements.  Result in c->x; *updates_out = FP64 checks after the initial one; hist_out: the
+// true residuals.
+template <typename R>
+MinresOut device_cr_reliable(fftperm_ctx* c, const double* bdev, const double* x0dev, double tol, uint64_t maxit,
+                             uint64_t* updates_out, std::vector<double>* hist_out) {
+    if (!(tol > 0.0)) throw Err(FFTPERM_EINVAL, "MinresSolver tolerance must be positive");
+    const long L = c->L3();
+    const long nb = (long)c->nb;
+    const double inv_nb = 1.0 / (double)nb;
+    MinresOut out;
+    *updates_out = 0;
+    const double bnorm = device_norm(c, bdev, L);
+    double* xacc = c->dx;
+    if (x0dev) {
+        if (x0dev != xacc)
+            HIPCHK(hipMemcpyAsync(xacc, x0dev, L * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+    } else {
+        HIPCHK(hipMemsetAsync(xacc, 0, L * sizeof(double), c->s));
+    }
+    bool xacc_zero = (x0dev == nullptr);
+    auto finish = [&](double rel) {
+        HIPCHK(hipMemcpyAsync(c->x, xacc, L * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+        c->sync();
+        out.final_rel = rel;
+        out.converged = rel <= tol ? 1 : 0;
+        out.hist_len = hist_out ? hist_out->size() : 0;
+    };
+    if (bnorm == 0.0) {
+        if (hist_out) hist_out->push_back(0.0);
+        finish(0.0);
+        out.converged = 1;
+        return out;
+    }
+    auto true_residual = [&]() {
+        if (!xacc_zero) {
+            device_apply<double>(c, xacc, c->tmp);
+            launch_axpby(1.0, bdev, -1.0, c->tmp, c->r, L, c->s);
+        } else {
+            HIPCHK(hipMemcpyAsync(c->r, bdev, L * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+        }
+        return device_norm(c, c->r, L) / bnorm;
+    };
+    auto r_mean = [&]() {
+        launch_compact_sums(c->r, nb, c->part, nullptr, c->s);
+        launch_finish_sum(c->part, kRedBlocks, 3, inv_nb, c->rst->mean_in, nullptr, c->s);
+    };
+    double rel = true_residual();
+    if (hist_out) hist_out->push_back(rel);
+    if (rel <= tol || maxit == 0) {
+        finish(rel);
+        return out;
+    }
+    c->ensure_hist((size_t)maxit + 2);
+    HIPCHK(hipMemsetAsync(c->x, 0, L * sizeof(double), c->s));
+    CrState h;
+    std::memset(&h, 0, sizeof(h));
+    h.rnorm_max = rel * bnorm;
+    h.rhs_norm = bnorm;
+    h.tol = tol;
+    h.delta = c->reliable_delta;
+    h.maxit = (long long)maxit;
+    h.first = 1;
+    HIPCHK(hipMemcpyAsync(c->rst, &h, sizeof(h), hipMemcpyHostToDevice, c->s));
+    r_mean();
+    const uint64_t chunk = (c->g.N >= (1L << 21)) ? 4 : 32;
+    long long seg_start = 0;
+    for (;;) {
+        int done = 0;
+        long long seg_launch = 0;
+        while (!done) {
+            const bool graph = c->use_graphs && !c->prof;
+            uint64_t i = 0;
+            if (graph)
+                for (; i + 2 <= chunk; i += 2) HIPCHK(hipGraphLaunch(cr_graph<R>(c), c->s));
+            for (; i < chunk; ++i) {
+                c->cur_tag = seg_launch++;
+                cr_iteration<R>(c);
+            }
+            c->cur_tag = -1;
+            HIPCHK(hipMemcpyAsync(&done, &c->rst->done, sizeof(int), hipMemcpyDeviceToHost, c->s));
+            c->sync();
+        }
+        HIPCHK(hipMemcpyAsync(&h, c->rst, sizeof(h), hipMemcpyDeviceToHost, c->s));
+        c->sync();
+        c->prof_resolve(h.iterations - seg_start);
+        seg_start = h.iterations;
+        launch_axpby(1.0, xacc, 1.0, c->x, xacc, L, c->s);
+        xacc_zero = false;
+        HIPCHK(hipMemsetAsync(c->x, 0, L * sizeof(double), c->s));
+        rel = true_residual();
+        if (hist_out) hist_out->push_back(rel);
+        ++*updates_out;
+        if (rel <= tol || !h.pause || h.breakdown || h.iterations >= h.maxit) break;
+        // continue from the true residual; p, Ap and r.A r of the last apply carry over
+        const double rn = rel * bnorm;
+        h.rr = rn * rn;
+        h.rnorm_max = rn;
+        h.done = 0;
+        h.pause = 0;
+        HIPCHK(hipMemcpyAsync(c->rst, &h, sizeof(h), hipMemcpyHostToDevice, c->s));
+        r_mean();
+    }
+    c->prof_resolve(LLONG_MAX);
+    out.iterations = (uint64_t)h.iterations;
+    if (sizeof(R) == 4) c->fp32_iters += (uint64_t)h.iterations;
+    finish(rel);
+    return out;
+}
+
 MinresOut run_minres(fftperm_ctx* c, const double* bdev, const double* x0dev, double tol, uint64_t maxit,
                      uint64_t* sweeps, std::vector<double>* mixed_hist) {
     *sweeps = 0;
+    if (c->precision == FFTPERM_PRECISION_MIXED && c->mixed_solver == FFTPERM_MIXED_CR_RELIABLE)
+        return device_cr_reliable<float>(c, bdev, x0dev, tol, maxit, sweeps, mixed_hist);
     if (c->precision == FFTPERM_PRECISION_MIXED && c->mixed_solver == FFTPERM_MIXED_CG_RELIABLE)
         return device_cg_reliable<float>(c, bdev, x0dev, tol, maxit, sweeps, mixed_hist);
     if (c->precision == FFTPERM_PRECISION_MIXED) return device_minres_mixed(c, bdev, x0dev, tol, maxit, sweeps, mixed_hist);
@@ -1136,6 +1301,7 @@ int fftperm_create(const fftperm_grid* grid, double mu, int p_radius, int precis
         c->small = dalloc<double>(32);
         c->st = dalloc<MinresState>(1);
         c->cst = dalloc<CgState>(1);
+        c->rst = dalloc<CrState>(1);
         c->sync();
         *out = c.release();
     });
@@ -1390,12 +1556,14 @@ int fftperm_set_refinement(fftperm_ctx* ctx, double fp32_below, double inner_flo
 int fftperm_set_mixed_solver(fftperm_ctx* ctx, int solver, double reliable_delta) {
     return guarded([&] {
         check_ctx(ctx);
-        if (solver != FFTPERM_MIXED_CG_RELIABLE && solver != FFTPERM_MIXED_MINRES_REFINED)
+        if (solver != FFTPERM_MIXED_CG_RELIABLE && solver != FFTPERM_MIXED_MINRES_REFINED &&
+            solver != FFTPERM_MIXED_CR_RELIABLE)
             throw Err(FFTPERM_EINVAL, "unknown mixed solver");
-        if (solver == FFTPERM_MIXED_CG_RELIABLE && !(reliable_delta > 0.0 && reliable_delta < 1.0))
+        const bool reliable = solver != FFTPERM_MIXED_MINRES_REFINED;
+        if (reliable && !(reliable_delta > 0.0 && reliable_delta < 1.0))
             throw Err(FFTPERM_EINVAL, "reliable_delta must lie in (0, 1)");
         ctx->mixed_solver = solver;
-        if (solver == FFTPERM_MIXED_CG_RELIABLE) ctx->reliable_delta = reliable_delta;
+        if (reliable) ctx->reliable_delta = reliable_delta;
     });
 }
 

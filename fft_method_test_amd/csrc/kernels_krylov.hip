@@ -498,4 +498,149 @@ void launch_cg_finish_mean_in(const double* partials, CgState* st, double inv_nb
     hipLaunchKernelGGL(k_cg_finish_mean_in, dim3(1), dim3(kRedThreads), 0, s, partials, st, inv_nb);
 }
 
+// ------------------------------------------------------------ CR with reliable updates
+__global__ void k_cr_beta(const double* __restrict__ xpart, int nxparts, const double* __restrict__ zpart,
+                          int nzparts, CrState* st, double inv_nb, double inv_n) {
+    if (st->done) return;
+    double acc[4] = {0, 0, 0, 0};
+    for (int p = threadIdx.x; p < nxparts; p += blockDim.x) {
+        const double* q = xpart + (long)kXinvParts * p;
+        acc[0] += q[0];
+        acc[1] += q[1];
+        acc[2] += q[2];
+    }
+    for (int p = threadIdx.x; p < nzparts; p += blockDim.x) acc[3] += zpart[p];
+    __shared__ double res[4];
+    block_sum<4>(acc, res);
+    if (threadIdx.x == 0) {
+        st->mean_out[0] = res[0] * inv_nb;
+        st->mean_out[1] = res[1] * inv_nb;
+        st->mean_out[2] = res[2] * inv_nb;
+        // r.A r = <scatter(P_B r), G scatter(P_B r)> over the grid (Parseval, from the z pass)
+        const double s_new = res[3] * inv_n;
+        if (!(s_new > 0.0)) {
+            st->breakdown = 1;
+            st->done = 1;
+            return;
+        }
+        st->beta = st->first ? 0.0 : s_new / st->s;
+        st->first = 0;
+        st->s = s_new;
+    }
+}
+
+void launch_cr_beta(const double* xpart, int nxparts, const double* zpart, int nzparts, CrState* st, double inv_nb,
+                    double inv_n, hipStream_t s) {
+    hipLaunchKernelGGL(k_cr_beta, dim3(1), dim3(1024), 0, s, xpart, nxparts, zpart, nzparts, st, inv_nb, inv_n);
+}
+
+__global__ __launch_bounds__(kFlatThreads) void k_cr_direction(double* __restrict__ p, double* __restrict__ ap,
+                                                               const double* __restrict__ r,
+                                                               const double* __restrict__ q, CrState* st, long n,
+                                                               double* __restrict__ partials) {
+    if (st->done) return;
+    const double b = st->beta;
+    const long i0 = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const double m = st->mean_out[i0 % 3];
+    double acc[1] = {0.0};
+    for (long i = i0; i < n; i += (long)gridDim.x * blockDim.x) {
+        p[i] = r[i] + b * p[i];
+        const double a = (q[i] - m) + b * ap[i];
+        ap[i] = a;
+        acc[0] += a * a;
+    }
+    block_sum<1>(acc, partials + blockIdx.x);
+}
+
+void launch_cr_direction(double* p, double* ap, const double* r, const double* q, CrState* st, long nb,
+                         double* partials, hipStream_t s) {
+    hipLaunchKernelGGL(k_cr_direction, dim3(kRedBlocks), dim3(kFlatThreads), 0, s, p, ap, r, q, st, 3 * nb, partials);
+}
+
+__global__ void k_cr_alpha(const double* __restrict__ partials, CrState* st) {
+    if (st->done) return;
+    double acc[1] = {0.0};
+    for (int p = threadIdx.x; p < kRedBlocks; p += blockDim.x) acc[0] += partials[p];
+    __shared__ double res[1];
+    block_sum<1>(acc, res);
+    if (threadIdx.x == 0) {
+        const double apap = res[0];
+        st->apap = apap;
+        if (apap > 0.0) {
+            st->alpha = st->s / apap;
+        } else {
+            st->alpha = 0.0;
+            st->breakdown = 1;
+            st->done = 1;
+        }
+    }
+}
+
+void launch_cr_alpha(const double* partials, CrState* st, hipStream_t s) {
+    hipLaunchKernelGGL(k_cr_alpha, dim3(1), dim3(kRedThreads), 0, s, partials, st);
+}
+
+__global__ __launch_bounds__(kFlatThreads) void k_cr_update(double* __restrict__ x, double* __restrict__ r,
+                                                            const double* __restrict__ p,
+                                                            const double* __restrict__ ap, CrState* st, long n,
+                                                            double* __restrict__ partials) {
+    if (st->done) return;
+    const double a = st->alpha;
+    const long i0 = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double rr = 0.0, sum = 0.0;
+    for (long i = i0; i < n; i += (long)gridDim.x * blockDim.x) {
+        x[i] += a * p[i];
+        const double ri = r[i] - a * ap[i];
+        r[i] = ri;
+        rr += ri * ri;
+        sum += ri;
+    }
+    const int c = (int)(i0 % 3);
+    double acc[4] = {rr, c == 0 ? sum : 0.0, c == 1 ? sum : 0.0, c == 2 ? sum : 0.0};
+    block_sum<4>(acc, partials + 4L * blockIdx.x);
+}
+
+void launch_cr_update(double* x, double* r, const double* p, const double* ap, CrState* st, long nb,
+                      double* partials, hipStream_t s) {
+    hipLaunchKernelGGL(k_cr_update, dim3(kRedBlocks), dim3(kFlatThreads), 0, s, x, r, p, ap, st, 3 * nb, partials);
+}
+
+__global__ void k_cr_check(const double* __restrict__ partials, CrState* st, double* __restrict__ history,
+                           double inv_nb) {
+    if (st->done) return;
+    double acc[4] = {0, 0, 0, 0};
+    for (int p = threadIdx.x; p < kRedBlocks; p += blockDim.x) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[j] += partials[4 * p + j];
+    }
+    __shared__ double res[4];
+    block_sum<4>(acc, res);
+    if (threadIdx.x != 0) return;
+    const double rr = res[0] > 0.0 ? res[0] : 0.0;
+    const double rn = sqrt(rr);
+    const double rel = rn / st->rhs_norm;
+    const long long k = st->iterations + 1;
+    st->iterations = k;
+    st->rr = rr;
+    st->rel_res = rel;
+    st->mean_in[0] = res[1] * inv_nb;
+    st->mean_in[1] = res[2] * inv_nb;
+    st->mean_in[2] = res[3] * inv_nb;
+    if (history != nullptr) history[k] = rel;
+    if (rel <= st->tol || rn < st->delta * st->rnorm_max) {
+        st->pause = 1;
+        st->done = 1;
+        return;
+    }
+    if (k >= st->maxit) {
+        st->done = 1;
+        return;
+    }
+    if (rn > st->rnorm_max) st->rnorm_max = rn;
+}
+
+void launch_cr_check(const double* partials, CrState* st, double* history, double inv_nb, hipStream_t s) {
+    hipLaunchKernelGGL(k_cr_check, dim3(1), dim3(kRedThreads), 0, s, partials, st, history, inv_nb);
+}
+
 }  // namespace fftperm

@@ -85,6 +85,41 @@ void launch_cg_direction(double* p, const double* r, CgState* st, long nb, doubl
 // mean_in = sums / nb
 void launch_cg_finish_mean_in(const double* partials, CgState* st, double inv_nb, hipStream_t s);
 
+// Conjugate residuals with reliable updates: for symmetric A the iterates of CR minimise the residual
+// norm over the Krylov space, as MINRES's do, but CR carries the residual r explicitly, so it can be
+// replaced by the FP64 true residual without a restart.  Per iteration: q = A r (FP32 apply,
+// r.A r from the fused z pass), beta = r.Ar / previous r.Ar, p = r + beta p, Ap = (q - mean) +
+// beta Ap, alpha = r.Ar / Ap.Ap, x += alpha p, r -= alpha Ap.
+struct CrState {
+    double s;            // r.A r of the current residual (beta's denominator next time)
+    double alpha, beta, apap, rr;
+    double mean_in[3];   // support mean of r (input projection of the next apply)
+    double mean_out[3];  // support mean of the raw apply output
+    double rnorm_max;    // largest ||r|| since the last reliable update
+    double rhs_norm, tol, delta;
+    double rel_res;      // ||r|| / ||b|| of the recursive residual
+    long long iterations;
+    long long maxit;
+    int done;            // every CR kernel returns at once while set
+    int pause;           // 1: the host owes a reliable update / convergence check
+    int breakdown;       // r.A r <= 0 or Ap.Ap <= 0
+    int first;           // 1: beta = 0 (p = r, Ap = A r)
+};
+
+// mean_out, s_new = r.A r (Parseval partials / N), beta = first ? 0 : s_new / s
+void launch_cr_beta(const double* xpart, int nxparts, const double* zpart, int nzparts, CrState* st, double inv_nb,
+                    double inv_n, hipStream_t s);
+// p = r + beta p ; Ap = (q - mean_out) + beta Ap ; partials: Ap.Ap
+void launch_cr_direction(double* p, double* ap, const double* r, const double* q, CrState* st, long nb,
+                         double* partials, hipStream_t s);
+// alpha = s / Ap.Ap
+void launch_cr_alpha(const double* partials, CrState* st, hipStream_t s);
+// x += alpha p ; r -= alpha Ap ; partials: r.r and the per-component sums of r
+void launch_cr_update(double* x, double* r, const double* p, const double* ap, CrState* st, long nb,
+                      double* partials, hipStream_t s);
+// rr, relative residual, history, mean_in, pause / stop decisions
+void launch_cr_check(const double* partials, CrState* st, double* history, double inv_nb, hipStream_t s);
+
 // Generic helpers
 void launch_dot_partials(const double* a, const double* b, long n, double* partials, hipStream_t s);
 void launch_axpby(double a, const double* x, double b, const double* y, double* out, long n, hipStream_t s);

@@ -123,9 +123,9 @@ def test_mixed_precision_k_tensor_64(fp, oracle_mod):
     assert rel(K, Ko) < 1e-6
 
 
-@pytest.mark.parametrize("solver", [0, 1])
+@pytest.mark.parametrize("solver", [0, 1, 2])
 def test_mixed_solvers_k_and_true_residual(fp, oracle_mod, solver):
-    """Both MIXED solvers (CG with reliable updates, refined MINRES sweeps): K of the 32^3 sphere
+    """Every MIXED solver (CG / CR with reliable updates, refined MINRES sweeps): K of the 32^3 sphere
     cell within 1e-6 of the FP64 oracle, and the FP64 true residual of each direction at tol."""
     from fft_method_test_amd import _lib, geometry as G, permeability as P
 
@@ -149,14 +149,16 @@ def test_mixed_solvers_k_and_true_residual(fp, oracle_mod, solver):
     assert P.Diagnostics.residual_norm(op, res.solution, b) <= 1e-9 * np.linalg.norm(b) * 1.01
 
 
-def test_mixed_cg_warm_start_and_maxit(fp, oracle_mod):
-    """CG with reliable updates: a guess at the FP64 solution converges at once; maxit stops an
+@pytest.mark.parametrize("solver", [0, 2])
+def test_mixed_reliable_warm_start_and_maxit(fp, oracle_mod, solver):
+    """CG / CR with reliable updates: a guess at the FP64 solution converges at once; maxit stops an
     unconverged solve after exactly maxit iterations."""
     from fft_method_test_amd import geometry as G, permeability as P
 
     mask = G.boolean_spheres(24, 3.0, 0.6, 5).reshape(-1)
     g = P.Grid3D(24, 24, 24)
     green = P.GreenOperator(g, 1.0, 2, P.PRECISION_MIXED)
+    green.set_mixed_solver(solver, 0.1)
     sup = P.ForceSupport(P.BinaryMedium(g, mask), P.SUPPORT_INTERFACE, green)
     op = P.VariationalOperator(sup, green)
     o = oracle_mod.Oracle((24, 24, 24))
@@ -178,4 +180,6 @@ def test_set_mixed_solver_validation(fp):
         green.set_mixed_solver(7, 0.1)
     with pytest.raises(ValueError, match="reliable_delta"):
         green.set_mixed_solver(0, 1.5)
+    with pytest.raises(ValueError, match="reliable_delta"):
+        green.set_mixed_solver(2, 0.0)
     green.set_mixed_solver(1, 5.0)  # delta ignored for refined MINRES

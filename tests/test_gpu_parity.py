@@ -224,8 +224,9 @@ def test_green_apply_128_cube_vs_oracle(fp, oracle_mod):
     assert rel(P.VariationalRHS(sup, green, None).build([0.0, 0.0, 1.0]), b) < 1e-12
 
 
-@pytest.mark.parametrize("shape,precision", [((32, 32, 32), 0), ((24, 20, 16), 0), ((32, 32, 32), 1)])
-def test_minres_graph_replay_matches_eager(fp, shape, precision):
+@pytest.mark.parametrize("shape,precision,solver", [((32, 32, 32), 0, None), ((24, 20, 16), 0, None),
+                                                   ((32, 32, 32), 1, 0), ((32, 32, 32), 1, 2), ((32, 32, 32), 1, 1)])
+def test_minres_graph_replay_matches_eager(fp, shape, precision, solver):
     """Graph-captured MINRES iteration pairs reproduce the eagerly launched iterations bit for bit,
     over several solves in a row (replays with eager work and synchronisation in between)."""
     from fft_method_test_amd import _lib, permeability as P
@@ -233,6 +234,8 @@ def test_minres_graph_replay_matches_eager(fp, shape, precision):
     mask = random_mask(shape, 0.6, seed=17)
     g = P.Grid3D(*shape)
     green = P.GreenOperator(g, 1.0, 2, precision)
+    if solver is not None:
+        green.set_mixed_solver(solver, 0.1)
     sup = P.ForceSupport(P.BinaryMedium(g, mask), P.SUPPORT_INTERFACE, green)
     op = P.VariationalOperator(sup, green)
     runs = {}
