@@ -157,6 +157,32 @@ struct PassShape {
                                   (LINES * N) % THREADS == 0 && LDS <= 100u * 1024u;
 };
 
+// Merged x-pass variants v in [kPassVariants, kXPassVariants): the two two-for-one rounds of a tile
+// of LINES real lines (x + i y of every line, z of line pairs) share one transform of
+// LT = 3 LINES / 2 complex lines, so a tile has one load phase, one transform and one store phase.
+// LINES = the base count, halved, quartered; OS = 1, 2, 4 keeps 1.5 * base * T threads.
+constexpr int kXPassVariants = kPassVariants + 3;
+
+constexpr bool xpass_merged(int v) { return v >= kPassVariants; }
+
+template <int N>
+constexpr int xpass_lines(int v) {
+    return v < kPassVariants ? pass_lines<N>(v) : (pass_base_lines<N>() >> (v - kPassVariants));
+}
+
+constexpr int xpass_os(int v) { return v < kPassVariants ? pass_os(v) : (1 << (v - kPassVariants)); }
+
+template <typename R, int N, int LINES, int OS>
+struct MergedShape {
+    static constexpr int T = threads_per_line<N>();
+    static constexpr int LT = 3 * LINES / 2;
+    static constexpr int LD = N + 1;
+    static constexpr int THREADS = OS * LT * T;
+    static constexpr size_t LDS = (size_t)(N + LT * LD) * sizeof(cx<R>);
+    static constexpr bool valid = LINES >= 2 && LINES % 2 == 0 && THREADS >= 64 && THREADS <= 1024 &&
+                                  THREADS % 64 == 0 && LDS <= 100u * 1024u;
+};
+
 // Block index remap so that consecutive tiles land on the same XCD (workgroups are dealt to
 // the 8 XCDs round-robin; each XCD has its own L2).
 __device__ __forceinline__ long xcd_tile(long b, long nblocks) {

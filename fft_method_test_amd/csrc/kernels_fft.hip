@@ -558,9 +558,9 @@ std::vector<std::pair<int, float>> tune_candidates(int kind, int prec, int n, lo
 
 int zgreen_code_identity(int, int v) { return v; }
 
-std::vector<int> pass_candidates() {
+std::vector<int> pass_candidates(int count = kPassVariants) {
     std::vector<int> v;
-    for (int i = 0; i < kPassVariants; ++i) v.push_back(i);
+    for (int i = 0; i < count; ++i) v.push_back(i);
     return v;
 }
 
@@ -604,7 +604,8 @@ int variant_of_code(int kind, int n, int code) {
             if (c == code) return c;
         return -1;
     }
-    for (int v = 0; v < kPassVariants; ++v)
+    const int nv = (kind == kKindXfwd || kind == kKindXinv) ? kXPassVariants : kPassVariants;
+    for (int v = 0; v < nv; ++v)
         if (pass_variant_code(n, v) == code && code != 0) return v;
     return -1;
 }
@@ -648,10 +649,10 @@ std::vector<std::pair<int, float>> autotune_x(const XSource& src, const XSink& s
                                               const FFTTables<R>& t, hipStream_t st) {
     const long nlines = (long)g.ny * g.nz;
     auto tf = tune_candidates(
-        kKindXfwd, prec_of<R>(), g.nx, nlines, pass_candidates(),
+        kKindXfwd, prec_of<R>(), g.nx, nlines, pass_candidates(kXPassVariants),
         [&](int v) { return static_xfwd<R>(v, src, S, g, t.twx, nullptr, st); }, pass_variant_code, st);
     auto ti = tune_candidates(
-        kKindXinv, prec_of<R>(), g.nx, nlines, pass_candidates(),
+        kKindXinv, prec_of<R>(), g.nx, nlines, pass_candidates(kXPassVariants),
         [&](int v) { return static_xinv<R>(v, S, sink, g, t.twx, nullptr, st); }, pass_variant_code, st);
     tf.insert(tf.end(), ti.begin(), ti.end());
     return tf;

@@ -70,32 +70,23 @@ __device__ __forceinline__ int tile_local(const uint32_t* sb, const int* sp, con
     return sl[line] + sp[line * W + w] + __popc(word & ((1u << b) - 1u));
 }
 
-template <typename R, int N, int LINES, int OS>
-__global__ __launch_bounds__((PassShape<R, N, LINES, OS>::THREADS)) void k_xfwd_s(XSource src, cx<R>* __restrict__ S,
-                                                                                long nlines, long N3, long Nh,
-                                                                                const cx<R>* __restrict__ twg,
-                                                                                const int* skip) {
-    if (skip != nullptr && *skip) return;
-    using SH = PassShape<R, N, LINES, OS>;
-    constexpr int LD = SH::LD, THREADS = SH::THREADS, NXH = N / 2 + 1;
-    constexpr int KE = LINES * N / THREADS;  // elements per thread per component
-    extern __shared__ unsigned char smem_raw[];
-    cx<R>* tw = reinterpret_cast<cx<R>*>(smem_raw);
-    cx<R>* buf = tw + N;
-    for (int i = threadIdx.x; i < N; i += THREADS) tw[i] = twg[i];
-    const long line0 = xcd_tile(blockIdx.x, gridDim.x) * LINES;
-    const int nl = (int)min((long)LINES, nlines - line0);
-    // every global load of the block is issued up front: the support code once per voxel, then
-    // the three compact components (or the three field components) of each owned voxel
-    double v[3][KE];
+// The three source values of the tile's LINES * N voxels, v[c][k] for element
+// idx = threadIdx.x + k * THREADS (tile line idx / N, voxel idx % N; zero past the tile).  Every
+// global load of the block is issued up front: the support words (or codes) once per voxel, then
+// the three compact components (or the three field components) of each owned voxel.  With the
+// line bitmask and no background force, the tile's support indices form one contiguous run
+// [cs, ce) of the compact vector: it is staged through `stage` (cap doubles of LDS, the transform
+// buffer) with coalesced loads, then each voxel reads its three values from there; runs larger
+// than the stage fall back to direct gathers.  `bits`: LDS for the tile's support words.  Returns
+// after a barrier once the stage may be overwritten.
+template <int N, int LINES, int THREADS, int KE>
+__device__ __forceinline__ void xfwd_load(const XSource& src, long line0, int nl, long nlines, long N3, double* stage,
+                                          long cap, unsigned char* bits, double (&v)[3][KE]) {
     const bool no_f0 = src.a_fluid[0] == 0.0 && src.a_fluid[1] == 0.0 && src.a_fluid[2] == 0.0 &&
                        src.a_support[0] == 0.0 && src.a_support[1] == 0.0 && src.a_support[2] == 0.0;
     if (src.field == nullptr && src.lines.bits != nullptr && no_f0) {
-        // the tile's support indices form one contiguous run [cs, ce) of the compact vector: it is
-        // staged through LDS (in the transform buffer) with coalesced loads, then each voxel reads
-        // its three values from there; runs larger than the buffer fall back to direct gathers
         constexpr int W = (N + 31) / 32;
-        uint32_t* sb = reinterpret_cast<uint32_t*>(buf + LINES * LD);
+        uint32_t* sb = reinterpret_cast<uint32_t*>(bits);
         int* sp = reinterpret_cast<int*>(sb + LINES * W);
         int* sl = sp + LINES * W;
         // the support words are requested first, so their latency overlaps the compact-run loads
@@ -104,9 +95,7 @@ __global__ __launch_bounds__((PassShape<R, N, LINES, OS>::THREADS)) void k_xfwd_
         const long cs = src.lines.lstart[line0];
         const long ce = (line0 + nl < nlines) ? (long)src.lines.lstart[line0 + nl] : src.lines.nb;
         const long cnt3 = 3 * (ce - cs);
-        constexpr long CAP = (long)(LINES * LD * sizeof(cx<R>) / sizeof(double));
-        const bool staged = cnt3 <= CAP;
-        double* stage = reinterpret_cast<double*>(buf);
+        const bool staged = cnt3 <= cap;
         if (staged && src.compact != nullptr) {
             const double* cc = src.compact + 3 * cs;
 #pragma unroll 4
@@ -120,7 +109,9 @@ __global__ __launch_bounds__((PassShape<R, N, LINES, OS>::THREADS)) void k_xfwd_
         for (int k = 0; k < KE; ++k) {
             const int idx = threadIdx.x + k * THREADS;
             const int line = idx / N;
-            const int l = (line < nl && src.compact != nullptr) ? tile_local<N>(sb, sp, sl, line, idx % N) : -1;
+            const int l = (idx < LINES * N && line < nl && src.compact != nullptr)
+                              ? tile_local<N>(sb, sp, sl, line, idx % N)
+                              : -1;
 #pragma unroll
             for (int c = 0; c < 3; ++c)
                 v[c][k] = (l < 0) ? 0.0 : (staged ? stage[3 * (l - cs) + c] : src.compact[3L * l + c]) - mc[c];
@@ -132,7 +123,7 @@ __global__ __launch_bounds__((PassShape<R, N, LINES, OS>::THREADS)) void k_xfwd_
             const int idx = threadIdx.x + k * THREADS;
             const int line = idx / N;
             const long vox = (line0 + line) * N + idx % N;
-            const bool ok = line < nl;
+            const bool ok = idx < LINES * N && line < nl;
 #pragma unroll
             for (int c = 0; c < 3; ++c) v[c][k] = ok ? src.field[c * N3 + vox] : 0.0;
         }
@@ -142,7 +133,7 @@ __global__ __launch_bounds__((PassShape<R, N, LINES, OS>::THREADS)) void k_xfwd_
         for (int k = 0; k < KE; ++k) {
             const int idx = threadIdx.x + k * THREADS;
             const int line = idx / N;
-            code[k] = (line < nl) ? src.code[(line0 + line) * N + idx % N] : -2;
+            code[k] = (idx < LINES * N && line < nl) ? src.code[(line0 + line) * N + idx % N] : -2;
         }
         double mc[3];
 #pragma unroll
@@ -159,6 +150,27 @@ __global__ __launch_bounds__((PassShape<R, N, LINES, OS>::THREADS)) void k_xfwd_
             }
         }
     }
+}
+
+template <typename R, int N, int LINES, int OS>
+__global__ __launch_bounds__((PassShape<R, N, LINES, OS>::THREADS)) void k_xfwd_s(XSource src, cx<R>* __restrict__ S,
+                                                                                long nlines, long N3, long Nh,
+                                                                                const cx<R>* __restrict__ twg,
+                                                                                const int* skip) {
+    if (skip != nullptr && *skip) return;
+    using SH = PassShape<R, N, LINES, OS>;
+    constexpr int LD = SH::LD, THREADS = SH::THREADS, NXH = N / 2 + 1;
+    constexpr int KE = LINES * N / THREADS;  // elements per thread per component
+    extern __shared__ unsigned char smem_raw[];
+    cx<R>* tw = reinterpret_cast<cx<R>*>(smem_raw);
+    cx<R>* buf = tw + N;
+    for (int i = threadIdx.x; i < N; i += THREADS) tw[i] = twg[i];
+    const long line0 = xcd_tile(blockIdx.x, gridDim.x) * LINES;
+    const int nl = (int)min((long)LINES, nlines - line0);
+    double v[3][KE];
+    constexpr long CAP = (long)(LINES * LD * sizeof(cx<R>) / sizeof(double));
+    xfwd_load<N, LINES, THREADS, KE>(src, line0, nl, nlines, N3, reinterpret_cast<double*>(buf), CAP,
+                                     reinterpret_cast<unsigned char*>(buf + LINES * LD), v);
     // round A: z = x + i y per line
 #pragma unroll
     for (int k = 0; k < KE; ++k) {
@@ -344,40 +356,249 @@ __global__ __launch_bounds__((PassShape<R, N, LINES, OS>::THREADS)) void k_xinv_
     if (sink.partials != nullptr) block_sum<kXinvParts>(acc, sink.partials + (long)kXinvParts * blockIdx.x);
 }
 
+// Merged-round x forward: the tile's LINES real lines of the three components go through one
+// transform of LT = 3 LINES / 2 complex lines (line l < LINES: x + i y of tile line l; line
+// LINES + m: z of tile lines 2m (real part) and 2m + 1 (imaginary part)), then every output is
+// split and stored in one pass.
+template <typename R, int N, int LINES, int OS>
+__global__ __launch_bounds__((MergedShape<R, N, LINES, OS>::THREADS)) void k_xfwd_m(XSource src,
+                                                                                 cx<R>* __restrict__ S, long nlines,
+                                                                                 long N3, long Nh,
+                                                                                 const cx<R>* __restrict__ twg,
+                                                                                 const int* skip) {
+    if (skip != nullptr && *skip) return;
+    using SH = MergedShape<R, N, LINES, OS>;
+    constexpr int LD = SH::LD, THREADS = SH::THREADS, NXH = N / 2 + 1, LT = SH::LT;
+    constexpr int KE = (LINES * N + THREADS - 1) / THREADS;
+    extern __shared__ unsigned char smem_raw[];
+    cx<R>* tw = reinterpret_cast<cx<R>*>(smem_raw);
+    cx<R>* buf = tw + N;
+    for (int i = threadIdx.x; i < N; i += THREADS) tw[i] = twg[i];
+    const long line0 = xcd_tile(blockIdx.x, gridDim.x) * LINES;
+    const int nl = (int)min((long)LINES, nlines - line0);
+    double v[3][KE];
+    constexpr long CAP = (long)(LT * LD * sizeof(cx<R>) / sizeof(double));
+    xfwd_load<N, LINES, THREADS, KE>(src, line0, nl, nlines, N3, reinterpret_cast<double*>(buf), CAP,
+                                     reinterpret_cast<unsigned char*>(buf + LT * LD), v);
+    R* bufr = reinterpret_cast<R*>(buf);
+#pragma unroll
+    for (int k = 0; k < KE; ++k) {
+        const int idx = threadIdx.x + k * THREADS;
+        if (idx < LINES * N) {
+            const int line = idx / N, i = idx % N;
+            buf[line * LD + i] = mkc<R>((R)v[0][k], (R)v[1][k]);
+            bufr[2 * ((LINES + (line >> 1)) * LD + i) + (line & 1)] = (R)v[2][k];
+        }
+    }
+    __syncthreads();
+    sfft_lines<R, -1, N, LT, SH::T, LD>(buf, tw, LT);
+    cx<R>* Sz = S + 2 * Nh;
+    for (int idx = threadIdx.x; idx < LT * NXH; idx += THREADS) {
+        const int line = idx / NXH;
+        const int kx = idx % NXH;
+        const cx<R> zk = buf[line * LD + kx];
+        const cx<R> zm = buf[line * LD + (N - kx) % N];
+        const cx<R> a = mkc<R>((R)0.5 * (zk.x + zm.x), (R)0.5 * (zk.y - zm.y));
+        const cx<R> b = mkc<R>((R)0.5 * (zk.y + zm.y), (R)0.5 * (zm.x - zk.x));
+        if (line < LINES) {
+            if (line < nl) {
+                const long o = (line0 + line) * NXH + kx;
+                S[o] = a;
+                S[Nh + o] = b;
+            }
+        } else {
+            const int la = 2 * (line - LINES), lb = la + 1;
+            if (la < nl) Sz[(line0 + la) * NXH + kx] = a;
+            if (lb < nl) Sz[(line0 + lb) * NXH + kx] = b;
+        }
+    }
+}
+
+// Writes the three output values of voxel vox / support index cd and accumulates the partial
+// sums {support sums of x, y, z, all-voxel sums of x, y, z}.
+__device__ __forceinline__ void xinv_emit3(const XSink& sink, long N3, long vox, int cd, double a, double b, double c,
+                                           double (&acc)[kXinvParts]) {
+    if (sink.field != nullptr) {
+        sink.field[vox] = a;
+        sink.field[N3 + vox] = b;
+        sink.field[2 * N3 + vox] = c;
+    }
+    if (cd >= 0) {
+        if (sink.compact != nullptr) {
+            sink.compact[3L * cd] = a;
+            sink.compact[3L * cd + 1] = b;
+            sink.compact[3L * cd + 2] = c;
+        }
+        acc[0] += a;
+        acc[1] += b;
+        acc[2] += c;
+    }
+    acc[3] += a;
+    acc[4] += b;
+    acc[5] += c;
+}
+
+// Merged-round x inverse: the x and y half spectra of the tile's lines and the z half spectra of
+// its line pairs fill LT = 3 LINES / 2 transform lines, one inverse transform, and every voxel's
+// three components are written together.
+template <typename R, int N, int LINES, int OS>
+__global__ __launch_bounds__((MergedShape<R, N, LINES, OS>::THREADS)) void k_xinv_m(const cx<R>* __restrict__ S,
+                                                                                 XSink sink, long nlines, long N3,
+                                                                                 long Nh,
+                                                                                 const cx<R>* __restrict__ twg,
+                                                                                 double inv_n, const int* skip) {
+    if (skip != nullptr && *skip) return;
+    using SH = MergedShape<R, N, LINES, OS>;
+    constexpr int LD = SH::LD, THREADS = SH::THREADS, NXH = N / 2 + 1, LT = SH::LT;
+    constexpr int KE = (LINES * N + THREADS - 1) / THREADS;
+    constexpr int KA = (LINES * NXH + THREADS - 1) / THREADS;        // x / y spectrum loads per thread
+    constexpr int KB = ((LINES / 2) * NXH + THREADS - 1) / THREADS;  // z spectrum pair loads per thread
+    extern __shared__ unsigned char smem_raw[];
+    cx<R>* tw = reinterpret_cast<cx<R>*>(smem_raw);
+    cx<R>* buf = tw + N;
+    for (int i = threadIdx.x; i < N; i += THREADS) tw[i] = twg[i];
+    const long line0 = xcd_tile(blockIdx.x, gridDim.x) * LINES;
+    const int nl = (int)min((long)LINES, nlines - line0);
+    const bool use_bits = sink.lines.bits != nullptr;
+    TileBitsRegs<N, LINES, THREADS> tb;
+    if (use_bits) tb.fetch(sink.lines, line0, nl);
+    int code[KE];
+    if (!use_bits) {
+#pragma unroll
+        for (int k = 0; k < KE; ++k) {
+            const int idx = threadIdx.x + k * THREADS;
+            const int line = idx / N;
+            code[k] = (idx < LINES * N && line < nl) ? sink.code[(line0 + line) * N + idx % N] : -2;
+        }
+    }
+    cx<R> ha[KA], hb[KA];
+#pragma unroll
+    for (int k = 0; k < KA; ++k) {
+        const int idx = threadIdx.x + k * THREADS;
+        const int line = idx / NXH;
+        const bool ok = idx < LINES * NXH && line < nl;
+        const long o = (line0 + line) * NXH + idx % NXH;
+        ha[k] = mkc<R>((R)0, (R)0);
+        hb[k] = mkc<R>((R)0, (R)0);
+        if (ok) {
+            ha[k] = S[o];
+            hb[k] = S[Nh + o];
+        }
+    }
+    cx<R> ga[KB], gb[KB];
+#pragma unroll
+    for (int k = 0; k < KB; ++k) {
+        const int idx = threadIdx.x + k * THREADS;
+        const int m = idx / NXH;
+        const int kx = idx % NXH;
+        const bool in = idx < (LINES / 2) * NXH;
+        ga[k] = mkc<R>((R)0, (R)0);
+        gb[k] = mkc<R>((R)0, (R)0);
+        if (in && 2 * m < nl) ga[k] = S[2 * Nh + (line0 + 2 * m) * NXH + kx];
+        if (in && 2 * m + 1 < nl) gb[k] = S[2 * Nh + (line0 + 2 * m + 1) * NXH + kx];
+    }
+    if (use_bits) {
+        constexpr int W = (N + 31) / 32;
+        uint32_t* sb = reinterpret_cast<uint32_t*>(buf + LT * LD);
+        int* sp = reinterpret_cast<int*>(sb + LINES * W);
+        int* sl = sp + LINES * W;
+        tb.store(sb, sp, sl);
+#pragma unroll
+        for (int k = 0; k < KE; ++k) {
+            const int idx = threadIdx.x + k * THREADS;
+            const int line = idx / N;
+            code[k] = (idx < LINES * N && line < nl) ? tile_local<N>(sb, sp, sl, line, idx % N) : -2;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < KA; ++k) {
+        const int idx = threadIdx.x + k * THREADS;
+        if (idx < LINES * NXH) xinv_put<R, N, LD>(buf, idx / NXH, idx % NXH, ha[k], hb[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < KB; ++k) {
+        const int idx = threadIdx.x + k * THREADS;
+        if (idx < (LINES / 2) * NXH) xinv_put<R, N, LD>(buf, LINES + idx / NXH, idx % NXH, ga[k], gb[k]);
+    }
+    __syncthreads();
+    sfft_lines<R, +1, N, LT, SH::T, LD>(buf, tw, LT);
+    const R* bufr = reinterpret_cast<const R*>(buf);
+    double acc[kXinvParts] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < KE; ++k) {
+        const int idx = threadIdx.x + k * THREADS;
+        const int line = idx / N;
+        if (idx < LINES * N && line < nl) {
+            const int i = idx % N;
+            const cx<R> z = buf[line * LD + i];
+            const double zz = (double)bufr[2 * ((LINES + (line >> 1)) * LD + i) + (line & 1)];
+            xinv_emit3(sink, N3, (line0 + line) * N + i, code[k], (double)z.x * inv_n, (double)z.y * inv_n,
+                       zz * inv_n, acc);
+        }
+    }
+    if (sink.partials != nullptr) block_sum<kXinvParts>(acc, sink.partials + (long)kXinvParts * blockIdx.x);
+}
+
 template <typename R, int N, int V>
 long xfwd_variant(const XSource& src, cx<R>* S, const Geometry3& g, const cx<R>* tw, const int* skip,
                   hipStream_t st) {
-    constexpr int LINES = pass_lines<N>(V), OS = pass_os(V);
-    using SH = PassShape<R, N, LINES, OS>;
-    if constexpr (!SH::valid) {
-        return -1;
+    constexpr int LINES = xpass_lines<N>(V), OS = xpass_os(V);
+    const long nlines = (long)g.ny * g.nz;
+    const long blocks = (nlines + LINES - 1) / LINES;
+    if constexpr (xpass_merged(V)) {
+        using SH = MergedShape<R, N, LINES, OS>;
+        if constexpr (!SH::valid) {
+            return -1;
+        } else {
+            constexpr size_t lds = SH::LDS + bits_lds<N, LINES>();
+            if (lds > lds_limit()) return -1;
+            hipLaunchKernelGGL((k_xfwd_m<R, N, LINES, OS>), dim3((unsigned)blocks), dim3(SH::THREADS), lds, st, src,
+                               S, nlines, g.N, g.Nh, tw, skip);
+            return blocks;
+        }
     } else {
-        constexpr size_t lds = SH::LDS + bits_lds<N, LINES>();
-        if (lds > lds_limit()) return -1;
-        const long nlines = (long)g.ny * g.nz;
-        const long blocks = (nlines + LINES - 1) / LINES;
-        hipLaunchKernelGGL((k_xfwd_s<R, N, LINES, OS>), dim3((unsigned)blocks), dim3(SH::THREADS),
-                           lds, st, src, S, nlines, g.N, g.Nh, tw, skip);
-        return blocks;
+        using SH = PassShape<R, N, LINES, OS>;
+        if constexpr (!SH::valid) {
+            return -1;
+        } else {
+            constexpr size_t lds = SH::LDS + bits_lds<N, LINES>();
+            if (lds > lds_limit()) return -1;
+            hipLaunchKernelGGL((k_xfwd_s<R, N, LINES, OS>), dim3((unsigned)blocks), dim3(SH::THREADS), lds, st, src,
+                               S, nlines, g.N, g.Nh, tw, skip);
+            return blocks;
+        }
     }
 }
 
 template <typename R, int N, int V>
 long xinv_variant(const cx<R>* S, const XSink& sink, const Geometry3& g, const cx<R>* tw, const int* skip,
                   hipStream_t st) {
-    constexpr int LINES = pass_lines<N>(V), OS = pass_os(V);
-    using SH = PassShape<R, N, LINES, OS>;
-    if constexpr (!SH::valid) {
-        return -1;
+    constexpr int LINES = xpass_lines<N>(V), OS = xpass_os(V);
+    const long nlines = (long)g.ny * g.nz;
+    const long blocks = (nlines + LINES - 1) / LINES;
+    const double inv_n = 1.0 / (double)g.N;
+    if constexpr (xpass_merged(V)) {
+        using SH = MergedShape<R, N, LINES, OS>;
+        if constexpr (!SH::valid) {
+            return -1;
+        } else {
+            constexpr size_t lds = SH::LDS + bits_lds<N, LINES>();
+            if (lds > lds_limit()) return -1;
+            hipLaunchKernelGGL((k_xinv_m<R, N, LINES, OS>), dim3((unsigned)blocks), dim3(SH::THREADS), lds, st, S,
+                               sink, nlines, g.N, g.Nh, tw, inv_n, skip);
+            return blocks;
+        }
     } else {
-        constexpr size_t lds = SH::LDS + bits_lds<N, LINES>();
-        if (lds > lds_limit()) return -1;
-        const long nlines = (long)g.ny * g.nz;
-        const long blocks = (nlines + LINES - 1) / LINES;
-        const double inv_n = 1.0 / (double)g.N;
-        hipLaunchKernelGGL((k_xinv_s<R, N, LINES, OS>), dim3((unsigned)blocks), dim3(SH::THREADS),
-                           lds, st, S, sink, nlines, g.N, g.Nh, tw, inv_n, skip);
-        return blocks;
+        using SH = PassShape<R, N, LINES, OS>;
+        if constexpr (!SH::valid) {
+            return -1;
+        } else {
+            constexpr size_t lds = SH::LDS + bits_lds<N, LINES>();
+            if (lds > lds_limit()) return -1;
+            hipLaunchKernelGGL((k_xinv_s<R, N, LINES, OS>), dim3((unsigned)blocks), dim3(SH::THREADS), lds, st, S,
+                               sink, nlines, g.N, g.Nh, tw, inv_n, skip);
+            return blocks;
+        }
     }
 }
 
@@ -390,6 +611,9 @@ long xfwd_n(int v, const XSource& src, cx<R>* S, const Geometry3& g, const cx<R>
         case 2: return xfwd_variant<R, N, 2>(src, S, g, tw, skip, st);
         case 3: return xfwd_variant<R, N, 3>(src, S, g, tw, skip, st);
         case 4: return xfwd_variant<R, N, 4>(src, S, g, tw, skip, st);
+        case 5: return xfwd_variant<R, N, 5>(src, S, g, tw, skip, st);
+        case 6: return xfwd_variant<R, N, 6>(src, S, g, tw, skip, st);
+        case 7: return xfwd_variant<R, N, 7>(src, S, g, tw, skip, st);
         default: return -1;
     }
 }
@@ -403,13 +627,17 @@ long xinv_n(int v, const cx<R>* S, const XSink& sink, const Geometry3& g, const 
         case 2: return xinv_variant<R, N, 2>(S, sink, g, tw, skip, st);
         case 3: return xinv_variant<R, N, 3>(S, sink, g, tw, skip, st);
         case 4: return xinv_variant<R, N, 4>(S, sink, g, tw, skip, st);
+        case 5: return xinv_variant<R, N, 5>(S, sink, g, tw, skip, st);
+        case 6: return xinv_variant<R, N, 6>(S, sink, g, tw, skip, st);
+        case 7: return xinv_variant<R, N, 7>(S, sink, g, tw, skip, st);
         default: return -1;
     }
 }
 
+// 10 * lines + threads per butterfly thread; merged-round x variants add 5 (last digit 6, 7 or 9)
 template <int N>
 int code_of(int v) {
-    return 10 * pass_lines<N>(v) + pass_os(v);
+    return 10 * xpass_lines<N>(v) + xpass_os(v) + (xpass_merged(v) ? 5 : 0);
 }
 
 }  // namespace
@@ -441,7 +669,7 @@ long static_xinv(int v, const cx<R>* S, const XSink& sink, const Geometry3& g, c
 }
 
 int pass_variant_code(int n, int v) {
-    if (v < 0 || v >= kPassVariants) return 0;
+    if (v < 0 || v >= kXPassVariants) return 0;
     switch (n) {
         case 32: return code_of<32>(v);
         case 64: return code_of<64>(v);

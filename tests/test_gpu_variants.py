@@ -34,6 +34,23 @@ def pass_codes(n):
     return out
 
 
+def xpass_codes(n, precision=0):
+    """pass_codes(n) plus the merged-round x-pass variants valid at this precision (one transform of
+    3 lines / 2 complex lines per tile; code 10 * lines + threads per butterfly thread + 5)."""
+    t = n // _MAX_RADIX[n]
+    base = max(2, 256 // t)
+    out = pass_codes(n)
+    cx_bytes = 16 if precision == 0 else 8
+    for j in range(3):
+        lines, os_ = base >> j, 1 << j
+        lt = 3 * lines // 2
+        threads = os_ * lt * t
+        lds = (n + lt * (n + 1)) * cx_bytes
+        if lines >= 2 and lines % 2 == 0 and 64 <= threads <= 1024 and threads % 64 == 0 and lds <= 100 * 1024:
+            out.append(10 * lines + os_ + 5)
+    return out
+
+
 @contextlib.contextmanager
 def pinned(green, kind, precision, code):
     from fft_method_test_amd import _lib
@@ -79,7 +96,7 @@ def test_x_pass_variants_fp64(fp, oracle_mod, n):
 
     shape = X_SHAPES[n]
     green, sup, o = _pair(oracle_mod, shape, seed=n)
-    for code in pass_codes(n):
+    for code in xpass_codes(n, 0):
         for kind in (_lib.PASS_XFWD, _lib.PASS_XINV):
             with pinned(green, kind, _lib.PRECISION_FP64, code):
                 _check_apply(green, sup, o, shape, seed=code)
@@ -135,7 +152,8 @@ def test_fp32_variants_mixed_k_sphere32(fp, oracle_mod):
     green = P.GreenOperator(g, 1.0, 2, P.PRECISION_MIXED)
     sup = P.ForceSupport(P.BinaryMedium(g, mask), P.SUPPORT_INTERFACE, green)
     cases = [(_lib.PASS_ZGREEN, c) for c in _lib.ZGREEN_CODES]
-    cases += [(k, c) for k in (_lib.PASS_XFWD, _lib.PASS_XINV, _lib.PASS_YPASS) for c in pass_codes(n)]
+    cases += [(k, c) for k in (_lib.PASS_XFWD, _lib.PASS_XINV) for c in xpass_codes(n, 1)]
+    cases += [(_lib.PASS_YPASS, c) for c in pass_codes(n)]
     for kind, code in cases:
         with pinned(green, kind, _lib.PRECISION_MIXED, code):
             K, _ = P.solve_permeability_tensor(green, 1e-9, 10000)
@@ -154,7 +172,7 @@ def test_fp32_operator_variants(fp, oracle_mod, n):
     from fft_method_test_amd import _lib, permeability as P
 
     cases = []
-    for shape, kinds, codes in ((X_SHAPES[n], (_lib.PASS_XFWD, _lib.PASS_XINV), pass_codes(n)),
+    for shape, kinds, codes in ((X_SHAPES[n], (_lib.PASS_XFWD, _lib.PASS_XINV), xpass_codes(n, 1)),
                                 (Y_SHAPES[n], (_lib.PASS_YPASS,), pass_codes(n)),
                                 (Z_SHAPES[n], (_lib.PASS_ZGREEN,), _lib.ZGREEN_CODES)):
         green, sup, o = _pair(oracle_mod, shape, seed=n + 5, precision=_lib.PRECISION_MIXED)
