@@ -48,13 +48,16 @@ def fp64_bytes_zgreen(Nh, s_c=16, s_r=8):
     return 2 * 3 * Nh * s_c + 6 * Nh * s_r
 
 
-def bytes_apply(N, Nh, L, s_c=16, s_r=8):
-    # x-fwd: support map (int32) + compact vector in, 3 half spectra out; y fwd/inv and z: in+out;
-    # z adds the Green tensor; x-inv: spectra + map in, compact vector out
-    xf = 4 * N + 8 * L + 3 * Nh * s_c
+def bytes_apply(N, Nh, L, s_c=16, s_r=8, nlines=None):
+    # x-fwd: line bitmask of the support (1 bit per voxel, one int32 start per x line) + compact
+    # vector in, 3 half spectra out; y fwd/inv and z: in + out; z adds the Green tensor; x-inv:
+    # spectra + bitmask in, compact vector out
+    nl = nlines if nlines is not None else round(N ** (2.0 / 3.0))
+    bits = N // 8 + 4 * nl
+    xf = bits + 8 * L + 3 * Nh * s_c
     yp = 2 * 3 * Nh * s_c
     zg = fp64_bytes_zgreen(Nh, s_c, s_r)
-    xi = 3 * Nh * s_c + 4 * N + 8 * L
+    xi = 3 * Nh * s_c + bits + 8 * L
     return xf + 2 * yp + zg + xi
 
 
