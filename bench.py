@@ -10,8 +10,8 @@ once per grid and viscosity (reported separately as setup).
 
 The metric is quoted at <= 1e-6 relative K error, so both the plain FP64 path
 (MINRES, the reference's solver) and the mixed-precision path (conjugate residuals
-on the FP32 operator with reliable FP64 residual updates, ending on the FP64 true
-residual) are timed at the same relative-residual tolerance (--tol, default 1e-6)
+on the FP32 operator and FP32 Krylov vectors with reliable FP64 residual updates,
+ending on the FP64 true residual) are timed at the same relative-residual tolerance (--tol, default 1e-6)
 and each one's K error is measured against an untimed FP64 solve of the same
 medium at --ref-tol (1e-10): ||K - K_ref||_F / ||K_ref||_F.  The headline
 value is the mixed-precision path with the learned warm start (the metric's
@@ -253,8 +253,8 @@ def main():
         "config": {
             "workload": (f"{args.n}^3 periodic Boolean model of overlapping radius-{args.radius:g} spheres, porosity "
                          f"{args.porosity:g} (seed {args.seed}); full 3x3 K from 3 load directions; Bignonnet variational "
-                         f"solve to relative residual {args.tol:g} (FP64: MINRES; mixed: CR on the FP32 operator with "
-                         f"reliable FP64 updates), p_radius 2, mu 1; step = support detection + 3 direction solves; K error "
+                         f"solve to relative residual {args.tol:g} (FP64: MINRES; mixed: CR on the FP32 operator and FP32 "
+                         f"vectors with reliable FP64 updates), p_radius 2, mu 1; step = support detection + 3 direction solves; K error "
                          f"against an untimed FP64 solve at tol {args.ref_tol:g}"),
             "n": args.n,
             "tol": args.tol,

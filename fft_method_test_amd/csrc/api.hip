@@ -154,7 +154,7 @@ struct fftperm_ctx {
     double mixed_floor = 3e-6;      // inner recurrence-residual floor of a refinement sweep
     uint32_t mixed_sweeps = 64;     // sweeps per solve at most
     uint64_t fp32_iters = 0;        // iterations with the FP32 operator (whole context)
-    int mixed_solver = FFTPERM_MIXED_CR_RELIABLE;
+    int mixed_solver = FFTPERM_MIXED_CR_FP32_VECTORS;
     double reliable_delta = 0.1;    // CG: reliable update when ||r|| falls below this x its last peak
     int device = 0;
     hipStream_t s = nullptr;
@@ -193,6 +193,9 @@ struct fftperm_ctx {
     size_t Lcap = 0;
     double *x = nullptr, *v_old = nullptr, *v = nullptr, *z = nullptr, *w_prev = nullptr, *w = nullptr;
     double *b = nullptr, *tmp = nullptr, *r = nullptr, *dx = nullptr;
+    // FP32 Krylov vectors of the mixed CR solver (allocated on first use)
+    size_t L32cap = 0;
+    float *x32 = nullptr, *r32 = nullptr, *p32 = nullptr, *ap32 = nullptr, *q32 = nullptr;
     MinresState* st = nullptr;
     CgState* cst = nullptr;
     CrState* rst = nullptr;
@@ -283,6 +286,7 @@ struct fftperm_ctx {
         dfree(part_x); dfree(part_z); dfree(part); dfree(small);
         dfree(x); dfree(v_old); dfree(v); dfree(z); dfree(w_prev); dfree(w);
         dfree(b); dfree(tmp); dfree(r); dfree(dx);
+        dfree(x32); dfree(r32); dfree(p32); dfree(ap32); dfree(q32);
         dfree(st); dfree(cst); dfree(rst); dfree(hist); dfree(field); dfree(full_a); dfree(t6);
         for (const ProfRec& r : prof_pending) {
             (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b);
@@ -313,6 +317,15 @@ struct fftperm_ctx {
         w_prev = dalloc<double>(L); w = dalloc<double>(L); b = dalloc<double>(L); tmp = dalloc<double>(L);
         r = dalloc<double>(L); dx = dalloc<double>(L);
         Lcap = L;
+    }
+
+    void ensure_vectors32() {
+        const size_t L = (size_t)std::max(1L, L3());
+        if (L <= L32cap) return;
+        dfree(x32); dfree(r32); dfree(p32); dfree(ap32); dfree(q32);
+        x32 = dalloc<float>(L); r32 = dalloc<float>(L); p32 = dalloc<float>(L); ap32 = dalloc<float>(L);
+        q32 = dalloc<float>(L);
+        L32cap = L;
     }
 
     void ensure_hist(size_t n) {
@@ -909,12 +922,34 @@ void cr_iteration(fftperm_ctx* c) {
     launch_cr_check(c->part, st, c->hist, inv_nb, c->s);
 }
 
+// The same on FP32 vectors (x32, r32, p32, ap32, q32).
 template <typename R>
+void cr32_iteration(fftperm_ctx* c) {
+    const long nb = (long)c->nb;
+    const double inv_nb = 1.0 / (double)nb;
+    CrState* st = c->rst;
+    XSource sr = src_none(c);
+    sr.compact32 = c->r32;
+    sr.mean = st->mean_in;
+    XSink sk = sink_none(c);
+    sk.compact32 = c->q32;
+    sk.partials = c->part_x;
+    c->chain<R>(sr, sk, &st->done, c->part_z);
+    launch_cr_beta(c->part_x, (int)c->xparts, c->part_z, (int)c->zparts, st, inv_nb, 1.0 / (double)c->g.N, c->s);
+    launch_cr32_direction(c->p32, c->ap32, c->r32, c->q32, st, nb, c->part, c->s);
+    launch_cr_alpha(c->part, st, c->s);
+    launch_cr32_update(c->x32, c->r32, c->p32, c->ap32, st, nb, c->part, c->s);
+    launch_cr_check(c->part, st, c->hist, inv_nb, c->s);
+}
+
+template <typename R, bool V32>
 hipGraphExec_t cr_graph(fftperm_ctx* c) {
     fftperm_ctx::GraphKey k{};
     k.fp32 = sizeof(R) == 4 ? 1 : 0;
-    k.solver = 2;
-    const void* ptrs[9] = {c->x, c->r, c->v, c->z, c->w, c->rst, c->hist, c->part_x, c->part};
+    k.solver = V32 ? 3 : 2;
+    const void* ptrs[9] = {V32 ? (const void*)c->x32 : c->x, V32 ? (const void*)c->r32 : c->r,
+                           V32 ? (const void*)c->p32 : c->v, V32 ? (const void*)c->q32 : c->z,
+                           V32 ? (const void*)c->ap32 : c->w, c->rst, c->hist, c->part_x, c->part};
     for (int i = 0; i < 9; ++i) k.p[i] = ptrs[i];
     k.nb = (long)c->nb;
     k.tuning = tuning_generation();
@@ -926,8 +961,12 @@ hipGraphExec_t cr_graph(fftperm_ctx* c) {
     }
     hipGraph_t graph = nullptr;
     HIPCHK(hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal));
-    cr_iteration<R>(c);
-    cr_iteration<R>(c);
+    for (int i = 0; i < 2; ++i) {
+        if (V32)
+            cr32_iteration<R>(c);
+        else
+            cr_iteration<R>(c);
+    }
     HIPCHK(hipStreamEndCapture(c->s, &graph));
     hipGraphExec_t exec = nullptr;
     const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
@@ -940,9 +979,10 @@ hipGraphExec_t cr_graph(fftperm_ctx* c) {
 // MIXED-mode solve by conjugate residuals with reliable updates: the same FP64 accumulator and
 // true-residual replacement as device_cg_reliable, around the CR recurrence (residual-minimising
 // like MINRES, and carrying r explicitly).  The direction p and its image Ap survive the
-// replacements.  Result in c->x; *updates_out = FP64 checks after the initial one; hist_out: the
-// true residuals.
-template <typename R>
+// replacements.  V32: the iterated vectors (group solution, r, p, Ap, apply output) are FP32; the
+// accumulator, b, the true residual and every scalar stay FP64.  Result in c->x;
+// *updates_out = FP64 checks after the initial one; hist_out: the true residuals.
+template <typename R, bool V32>
 MinresOut device_cr_reliable(fftperm_ctx* c, const double* bdev, const double* x0dev, double tol, uint64_t maxit,
                              uint64_t* updates_out, std::vector<double>* hist_out) {
     if (!(tol > 0.0)) throw Err(FFTPERM_EINVAL, "MinresSolver tolerance must be positive");
@@ -993,7 +1033,13 @@ MinresOut device_cr_reliable(fftperm_ctx* c, const double* bdev, const double* x
         return out;
     }
     c->ensure_hist((size_t)maxit + 2);
-    HIPCHK(hipMemsetAsync(c->x, 0, L * sizeof(double), c->s));
+    if (V32) {
+        c->ensure_vectors32();
+        HIPCHK(hipMemsetAsync(c->x32, 0, L * sizeof(float), c->s));
+        launch_to_f32(c->r, c->r32, L, c->s);
+    } else {
+        HIPCHK(hipMemsetAsync(c->x, 0, L * sizeof(double), c->s));
+    }
     CrState h;
     std::memset(&h, 0, sizeof(h));
     h.rnorm_max = rel * bnorm;
@@ -1013,10 +1059,13 @@ MinresOut device_cr_reliable(fftperm_ctx* c, const double* bdev, const double* x
             const bool graph = c->use_graphs && !c->prof;
             uint64_t i = 0;
             if (graph)
-                for (; i + 2 <= chunk; i += 2) HIPCHK(hipGraphLaunch(cr_graph<R>(c), c->s));
+                for (; i + 2 <= chunk; i += 2) HIPCHK(hipGraphLaunch(cr_graph<R, V32>(c), c->s));
             for (; i < chunk; ++i) {
                 c->cur_tag = seg_launch++;
-                cr_iteration<R>(c);
+                if (V32)
+                    cr32_iteration<R>(c);
+                else
+                    cr_iteration<R>(c);
             }
             c->cur_tag = -1;
             HIPCHK(hipMemcpyAsync(&done, &c->rst->done, sizeof(int), hipMemcpyDeviceToHost, c->s));
@@ -1026,9 +1075,13 @@ MinresOut device_cr_reliable(fftperm_ctx* c, const double* bdev, const double* x
         c->sync();
         c->prof_resolve(h.iterations - seg_start);
         seg_start = h.iterations;
-        launch_axpby(1.0, xacc, 1.0, c->x, xacc, L, c->s);
+        if (V32) {
+            launch_accumulate_f32(xacc, c->x32, L, c->s);
+        } else {
+            launch_axpby(1.0, xacc, 1.0, c->x, xacc, L, c->s);
+            HIPCHK(hipMemsetAsync(c->x, 0, L * sizeof(double), c->s));
+        }
         xacc_zero = false;
-        HIPCHK(hipMemsetAsync(c->x, 0, L * sizeof(double), c->s));
         rel = true_residual();
         if (hist_out) hist_out->push_back(rel);
         ++*updates_out;
@@ -1041,6 +1094,7 @@ MinresOut device_cr_reliable(fftperm_ctx* c, const double* bdev, const double* x
         h.pause = 0;
         HIPCHK(hipMemcpyAsync(c->rst, &h, sizeof(h), hipMemcpyHostToDevice, c->s));
         r_mean();
+        if (V32) launch_to_f32(c->r, c->r32, L, c->s);
     }
     c->prof_resolve(LLONG_MAX);
     out.iterations = (uint64_t)h.iterations;
@@ -1053,7 +1107,9 @@ MinresOut run_minres(fftperm_ctx* c, const double* bdev, const double* x0dev, do
                      uint64_t* sweeps, std::vector<double>* mixed_hist) {
     *sweeps = 0;
     if (c->precision == FFTPERM_PRECISION_MIXED && c->mixed_solver == FFTPERM_MIXED_CR_RELIABLE)
-        return device_cr_reliable<float>(c, bdev, x0dev, tol, maxit, sweeps, mixed_hist);
+        return device_cr_reliable<float, false>(c, bdev, x0dev, tol, maxit, sweeps, mixed_hist);
+    if (c->precision == FFTPERM_PRECISION_MIXED && c->mixed_solver == FFTPERM_MIXED_CR_FP32_VECTORS)
+        return device_cr_reliable<float, true>(c, bdev, x0dev, tol, maxit, sweeps, mixed_hist);
     if (c->precision == FFTPERM_PRECISION_MIXED && c->mixed_solver == FFTPERM_MIXED_CG_RELIABLE)
         return device_cg_reliable<float>(c, bdev, x0dev, tol, maxit, sweeps, mixed_hist);
     if (c->precision == FFTPERM_PRECISION_MIXED) return device_minres_mixed(c, bdev, x0dev, tol, maxit, sweeps, mixed_hist);
@@ -1557,7 +1613,7 @@ int fftperm_set_mixed_solver(fftperm_ctx* ctx, int solver, double reliable_delta
     return guarded([&] {
         check_ctx(ctx);
         if (solver != FFTPERM_MIXED_CG_RELIABLE && solver != FFTPERM_MIXED_MINRES_REFINED &&
-            solver != FFTPERM_MIXED_CR_RELIABLE)
+            solver != FFTPERM_MIXED_CR_RELIABLE && solver != FFTPERM_MIXED_CR_FP32_VECTORS)
             throw Err(FFTPERM_EINVAL, "unknown mixed solver");
         const bool reliable = solver != FFTPERM_MIXED_MINRES_REFINED;
         if (reliable && !(reliable_delta > 0.0 && reliable_delta < 1.0))

@@ -31,6 +31,7 @@ struct LineBits {
 struct XSource {
     const int32_t* code;
     const double* compact;
+    const float* compact32;  // FP32 compact vector (instead of compact; at most one of the two)
     const double* mean;  // device [3], may be null (= 0)
     const double* field;
     double a_fluid[3];
@@ -45,6 +46,7 @@ struct XSource {
 struct XSink {
     const int32_t* code;
     double* compact;
+    float* compact32;  // FP32 compact output (instead of compact; at most one of the two)
     double* field;
     double* partials;
     LineBits lines;  // when set, the specialised inverse pass reads the support from it

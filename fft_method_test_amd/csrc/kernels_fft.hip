@@ -708,6 +708,7 @@ __global__ void k_xfwd(XSource src, cx<R>* __restrict__ S, int nx, long nlines, 
                 const double f0 = (code == -1) ? af : ((code >= 0) ? as : 0.0);
                 double fx = 0.0;
                 if (src.compact != nullptr && code >= 0) fx = src.compact[3L * code + c] - mc;
+                if (src.compact32 != nullptr && code >= 0) fx = (double)src.compact32[3L * code + c] - mc;
                 val = f0 + fx;
             }
             buf[line * ld + i] = mkc<R>((R)val, (R)0);
@@ -777,6 +778,7 @@ __global__ void k_xinv(const cx<R>* __restrict__ S, XSink sink, int nx, long nli
             const int code = sink.code[vox];
             if (code >= 0) {
                 if (sink.compact != nullptr) sink.compact[3L * code + c] = val;
+                if (sink.compact32 != nullptr) sink.compact32[3L * code + c] = (float)val;
                 if (c == 0) acc[0] += val;
                 if (c == 1) acc[1] += val;
                 if (c == 2) acc[2] += val;

@@ -543,9 +543,12 @@ __global__ __launch_bounds__(kFlatThreads) void k_cr_direction(double* __restric
     const long i0 = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const double m = st->mean_out[i0 % 3];
     double acc[1] = {0.0};
+    // beta = 0 on the first iteration: p and Ap are then set, not updated (their buffers hold
+    // whatever an earlier solve left, possibly non-finite values that 0 * x would keep)
+    const bool fresh = (b == 0.0);
     for (long i = i0; i < n; i += (long)gridDim.x * blockDim.x) {
-        p[i] = r[i] + b * p[i];
-        const double a = (q[i] - m) + b * ap[i];
+        p[i] = fresh ? r[i] : r[i] + b * p[i];
+        const double a = fresh ? (q[i] - m) : (q[i] - m) + b * ap[i];
         ap[i] = a;
         acc[0] += a * a;
     }
@@ -641,6 +644,77 @@ __global__ void k_cr_check(const double* __restrict__ partials, CrState* st, dou
 
 void launch_cr_check(const double* partials, CrState* st, double* history, double inv_nb, hipStream_t s) {
     hipLaunchKernelGGL(k_cr_check, dim3(1), dim3(kRedThreads), 0, s, partials, st, history, inv_nb);
+}
+
+// ------------------------------------------------------------ CR on FP32 vectors
+__global__ __launch_bounds__(kFlatThreads) void k_cr32_direction(float* __restrict__ p, float* __restrict__ ap,
+                                                                 const float* __restrict__ r,
+                                                                 const float* __restrict__ q, CrState* st, long n,
+                                                                 double* __restrict__ partials) {
+    if (st->done) return;
+    const double b = st->beta;
+    const long i0 = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const double m = st->mean_out[i0 % 3];
+    double acc[1] = {0.0};
+    const bool fresh = (b == 0.0);  // first iteration: set p and Ap (see k_cr_direction)
+    for (long i = i0; i < n; i += (long)gridDim.x * blockDim.x) {
+        p[i] = fresh ? r[i] : (float)((double)r[i] + b * (double)p[i]);
+        const float a = fresh ? (float)((double)q[i] - m) : (float)(((double)q[i] - m) + b * (double)ap[i]);
+        ap[i] = a;
+        acc[0] += (double)a * (double)a;
+    }
+    block_sum<1>(acc, partials + blockIdx.x);
+}
+
+void launch_cr32_direction(float* p, float* ap, const float* r, const float* q, CrState* st, long nb,
+                           double* partials, hipStream_t s) {
+    hipLaunchKernelGGL(k_cr32_direction, dim3(kRedBlocks), dim3(kFlatThreads), 0, s, p, ap, r, q, st, 3 * nb,
+                       partials);
+}
+
+__global__ __launch_bounds__(kFlatThreads) void k_cr32_update(float* __restrict__ x, float* __restrict__ r,
+                                                              const float* __restrict__ p,
+                                                              const float* __restrict__ ap, CrState* st, long n,
+                                                              double* __restrict__ partials) {
+    if (st->done) return;
+    const double a = st->alpha;
+    const long i0 = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double rr = 0.0, sum = 0.0;
+    for (long i = i0; i < n; i += (long)gridDim.x * blockDim.x) {
+        x[i] = (float)((double)x[i] + a * (double)p[i]);
+        const float ri = (float)((double)r[i] - a * (double)ap[i]);
+        r[i] = ri;
+        rr += (double)ri * (double)ri;
+        sum += (double)ri;
+    }
+    const int c = (int)(i0 % 3);
+    double acc[4] = {rr, c == 0 ? sum : 0.0, c == 1 ? sum : 0.0, c == 2 ? sum : 0.0};
+    block_sum<4>(acc, partials + 4L * blockIdx.x);
+}
+
+void launch_cr32_update(float* x, float* r, const float* p, const float* ap, CrState* st, long nb, double* partials,
+                        hipStream_t s) {
+    hipLaunchKernelGGL(k_cr32_update, dim3(kRedBlocks), dim3(kFlatThreads), 0, s, x, r, p, ap, st, 3 * nb, partials);
+}
+
+__global__ void k_accumulate_f32(double* __restrict__ xacc, float* __restrict__ x32, long n) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        xacc[i] += (double)x32[i];
+        x32[i] = 0.0f;
+    }
+}
+
+void launch_accumulate_f32(double* xacc, float* x32, long n, hipStream_t s) {
+    hipLaunchKernelGGL(k_accumulate_f32, dim3(red_grid(n)), dim3(kRedThreads), 0, s, xacc, x32, n);
+}
+
+__global__ void k_to_f32(const double* __restrict__ a, float* __restrict__ out, long n) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        out[i] = (float)a[i];
+}
+
+void launch_to_f32(const double* a, float* out, long n, hipStream_t s) {
+    hipLaunchKernelGGL(k_to_f32, dim3(red_grid(n)), dim3(kRedThreads), 0, s, a, out, n);
 }
 
 }  // namespace fftperm

@@ -120,6 +120,17 @@ void launch_cr_update(double* x, double* r, const double* p, const double* ap, C
 // rr, relative residual, history, mean_in, pause / stop decisions
 void launch_cr_check(const double* partials, CrState* st, double* history, double inv_nb, hipStream_t s);
 
+// The same CR steps on FP32 Krylov vectors (x, r, p, Ap and the apply output in FP32; scalars,
+// reductions and the arithmetic of each element in FP64).
+void launch_cr32_direction(float* p, float* ap, const float* r, const float* q, CrState* st, long nb,
+                           double* partials, hipStream_t s);
+void launch_cr32_update(float* x, float* r, const float* p, const float* ap, CrState* st, long nb, double* partials,
+                        hipStream_t s);
+// xacc += x32, x32 = 0
+void launch_accumulate_f32(double* xacc, float* x32, long n, hipStream_t s);
+// out = (float) a
+void launch_to_f32(const double* a, float* out, long n, hipStream_t s);
+
 // Generic helpers
 void launch_dot_partials(const double* a, const double* b, long n, double* partials, hipStream_t s);
 void launch_axpby(double a, const double* x, double b, const double* y, double* out, long n, hipStream_t s);

@@ -96,10 +96,15 @@ __device__ __forceinline__ void xfwd_load(const XSource& src, long line0, int nl
         const long ce = (line0 + nl < nlines) ? (long)src.lines.lstart[line0 + nl] : src.lines.nb;
         const long cnt3 = 3 * (ce - cs);
         const bool staged = cnt3 <= cap;
+        const bool has = src.compact != nullptr || src.compact32 != nullptr;
         if (staged && src.compact != nullptr) {
             const double* cc = src.compact + 3 * cs;
 #pragma unroll 4
             for (long e = threadIdx.x; e < cnt3; e += THREADS) stage[e] = cc[e];
+        } else if (staged && src.compact32 != nullptr) {
+            const float* cc = src.compact32 + 3 * cs;
+#pragma unroll 4
+            for (long e = threadIdx.x; e < cnt3; e += THREADS) stage[e] = (double)cc[e];
         }
         tb.store(sb, sp, sl);
         double mc[3];
@@ -109,12 +114,16 @@ __device__ __forceinline__ void xfwd_load(const XSource& src, long line0, int nl
         for (int k = 0; k < KE; ++k) {
             const int idx = threadIdx.x + k * THREADS;
             const int line = idx / N;
-            const int l = (idx < LINES * N && line < nl && src.compact != nullptr)
-                              ? tile_local<N>(sb, sp, sl, line, idx % N)
-                              : -1;
+            const int l = (idx < LINES * N && line < nl && has) ? tile_local<N>(sb, sp, sl, line, idx % N) : -1;
 #pragma unroll
-            for (int c = 0; c < 3; ++c)
-                v[c][k] = (l < 0) ? 0.0 : (staged ? stage[3 * (l - cs) + c] : src.compact[3L * l + c]) - mc[c];
+            for (int c = 0; c < 3; ++c) {
+                double raw = 0.0;
+                if (l >= 0)
+                    raw = staged ? stage[3 * (l - cs) + c]
+                                 : (src.compact != nullptr ? src.compact[3L * l + c]
+                                                           : (double)src.compact32[3L * l + c]);
+                v[c][k] = (l < 0) ? 0.0 : raw - mc[c];
+            }
         }
         __syncthreads();  // the stage is the transform buffer
     } else if (src.field != nullptr) {
@@ -146,6 +155,7 @@ __device__ __forceinline__ void xfwd_load(const XSource& src, long line0, int nl
                 const double f0 = (cd == -1) ? src.a_fluid[c] : ((cd >= 0) ? src.a_support[c] : 0.0);
                 double fx = 0.0;
                 if (src.compact != nullptr && cd >= 0) fx = src.compact[3L * cd + c] - mc[c];
+                if (src.compact32 != nullptr && cd >= 0) fx = (double)src.compact32[3L * cd + c] - mc[c];
                 v[c][k] = f0 + fx;
             }
         }
@@ -234,6 +244,7 @@ __device__ __forceinline__ void xinv_emit(const XSink& sink, long N3, int c, lon
     if (sink.field != nullptr) sink.field[c * N3 + vox] = val;
     if (cd >= 0) {
         if (sink.compact != nullptr) sink.compact[3L * cd + c] = val;
+        if (sink.compact32 != nullptr) sink.compact32[3L * cd + c] = (float)val;
         acc[c] += val;
     }
     acc[3 + c] += val;
@@ -428,6 +439,11 @@ __device__ __forceinline__ void xinv_emit3(const XSink& sink, long N3, long vox,
             sink.compact[3L * cd] = a;
             sink.compact[3L * cd + 1] = b;
             sink.compact[3L * cd + 2] = c;
+        }
+        if (sink.compact32 != nullptr) {
+            sink.compact32[3L * cd] = (float)a;
+            sink.compact32[3L * cd + 1] = (float)b;
+            sink.compact32[3L * cd + 2] = (float)c;
         }
         acc[0] += a;
         acc[1] += b;

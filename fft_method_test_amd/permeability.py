@@ -130,8 +130,9 @@ class GreenOperator:
         the FP64 operator while its recurrence residual is above fp32_below (>= 1: FP32 throughout)."""
         check(_lib.load().fftperm_set_refinement(self._h, float(fp32_below), float(inner_floor), int(max_sweeps)))
 
-    def set_mixed_solver(self, solver=_lib.MIXED_CR_RELIABLE, reliable_delta=0.1):
-        """MIXED mode solver: CR (default) or CG with reliable updates, or refined MINRES sweeps."""
+    def set_mixed_solver(self, solver=_lib.MIXED_CR_FP32_VECTORS, reliable_delta=0.1):
+        """MIXED mode solver: CR with reliable updates on FP32 (default) or FP64 vectors, CG with
+        reliable updates, or refined MINRES sweeps."""
         check(_lib.load().fftperm_set_mixed_solver(self._h, int(solver), float(reliable_delta)))
 
     def close(self):

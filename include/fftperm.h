@@ -53,11 +53,19 @@ enum { FFTPERM_PRECISION_FP64 = 0, FFTPERM_PRECISION_MIXED = 1 };
  * FFTPERM_MIXED_CR_RELIABLE: conjugate residuals (residual-minimising like MINRES, with an explicit
  *   residual) on the FP32 operator with reliable updates: the recursive residual is replaced by the
  *   FP64 true residual whenever it has fallen by the factor reliable_delta since the last
- *   replacement, keeping the search direction; the default.
- * FFTPERM_MIXED_CG_RELIABLE: conjugate gradients with the same reliable updates.
+ *   replacement, keeping the search direction; FP64 Krylov vectors.
+ * FFTPERM_MIXED_CR_FP32_VECTORS: the same with its iterated vectors (group solution, residual,
+ *   direction and its image) in FP32; the accumulated solution, b, true residuals and every scalar
+ *   stay FP64.  The default.
+ * FFTPERM_MIXED_CG_RELIABLE: conjugate gradients with the same reliable updates (FP64 vectors).
  * FFTPERM_MIXED_MINRES_REFINED: iterative refinement around restarted FP32-operator MINRES sweeps
  *   (fftperm_set_refinement). */
-enum { FFTPERM_MIXED_CG_RELIABLE = 0, FFTPERM_MIXED_MINRES_REFINED = 1, FFTPERM_MIXED_CR_RELIABLE = 2 };
+enum {
+    FFTPERM_MIXED_CG_RELIABLE = 0,
+    FFTPERM_MIXED_MINRES_REFINED = 1,
+    FFTPERM_MIXED_CR_RELIABLE = 2,
+    FFTPERM_MIXED_CR_FP32_VECTORS = 3
+};
 
 typedef struct fftperm_ctx fftperm_ctx;
 
@@ -136,7 +144,7 @@ int fftperm_operator_apply_fp32(fftperm_ctx* ctx, const double* x, double* y);
 /* MinresSolver(tol, maxit).solve(op, b, x0) (src/solver/MinresSolver.cpp:46-218) on the variational
  * operator.  x0 may be NULL (the reference's empty guess).  history (maxit + 1 entries) may be NULL.
  * tol <= 0 -> FFTPERM_EINVAL.  In MIXED mode the context's mixed solver runs instead (CR with
- * reliable updates by default) until the FP64 true residual ||b - A x|| <= tol ||b|| (or maxit
+ * reliable updates on FP32 vectors by default) until the FP64 true residual ||b - A x|| <= tol ||b|| (or maxit
  * iterations in total); history then holds the true residuals at each FP64 check and
  * info->refinement_sweeps counts those checks after the initial one. */
 int fftperm_minres(fftperm_ctx* ctx, const double* b, const double* x0, double tol, uint64_t maxit,
@@ -151,9 +159,9 @@ int fftperm_minres(fftperm_ctx* ctx, const double* b, const double* x0, double t
  * max_sweeps == 0 -> FFTPERM_EINVAL. */
 int fftperm_set_refinement(fftperm_ctx* ctx, double fp32_below, double inner_floor, uint32_t max_sweeps);
 
-/* MIXED-mode solver choice (no counterpart in the reference): FFTPERM_MIXED_CR_RELIABLE (default)
- * or FFTPERM_MIXED_CG_RELIABLE (reliable_delta in (0, 1), default 0.1), or
- * FFTPERM_MIXED_MINRES_REFINED (reliable_delta ignored).  Anything else -> FFTPERM_EINVAL. */
+/* MIXED-mode solver choice (no counterpart in the reference): FFTPERM_MIXED_CR_FP32_VECTORS
+ * (default), FFTPERM_MIXED_CR_RELIABLE or FFTPERM_MIXED_CG_RELIABLE (reliable_delta in (0, 1), default 0.1),
+ * or FFTPERM_MIXED_MINRES_REFINED (reliable_delta ignored).  Anything else -> FFTPERM_EINVAL. */
 int fftperm_set_mixed_solver(fftperm_ctx* ctx, int solver, double reliable_delta);
 
 /* VelocityRecovery::recover + average_over_domain (src/bignonnet/VelocityRecovery.cpp:12-47):

@@ -123,9 +123,10 @@ def test_mixed_precision_k_tensor_64(fp, oracle_mod):
     assert rel(K, Ko) < 1e-6
 
 
-@pytest.mark.parametrize("solver", [0, 1, 2])
+@pytest.mark.parametrize("solver", [0, 1, 2, 3])
 def test_mixed_solvers_k_and_true_residual(fp, oracle_mod, solver):
-    """Every MIXED solver (CG / CR with reliable updates, refined MINRES sweeps): K of the 32^3 sphere
+    """Every MIXED solver (CG / CR with reliable updates, CR on FP32 vectors, refined MINRES sweeps):
+    K of the 32^3 sphere
     cell within 1e-6 of the FP64 oracle, and the FP64 true residual of each direction at tol."""
     from fft_method_test_amd import _lib, geometry as G, permeability as P
 
@@ -149,7 +150,7 @@ def test_mixed_solvers_k_and_true_residual(fp, oracle_mod, solver):
     assert P.Diagnostics.residual_norm(op, res.solution, b) <= 1e-9 * np.linalg.norm(b) * 1.01
 
 
-@pytest.mark.parametrize("solver", [0, 2])
+@pytest.mark.parametrize("solver", [0, 2, 3])
 def test_mixed_reliable_warm_start_and_maxit(fp, oracle_mod, solver):
     """CG / CR with reliable updates: a guess at the FP64 solution converges at once; maxit stops an
     unconverged solve after exactly maxit iterations."""

@@ -225,7 +225,8 @@ def test_green_apply_128_cube_vs_oracle(fp, oracle_mod):
 
 
 @pytest.mark.parametrize("shape,precision,solver", [((32, 32, 32), 0, None), ((24, 20, 16), 0, None),
-                                                   ((32, 32, 32), 1, 0), ((32, 32, 32), 1, 2), ((32, 32, 32), 1, 1)])
+                                                   ((32, 32, 32), 1, 0), ((32, 32, 32), 1, 2), ((32, 32, 32), 1, 1),
+                                                   ((32, 32, 32), 1, 3)])
 def test_minres_graph_replay_matches_eager(fp, shape, precision, solver):
     """Graph-captured MINRES iteration pairs reproduce the eagerly launched iterations bit for bit,
     over several solves in a row (replays with eager work and synchronisation in between)."""
