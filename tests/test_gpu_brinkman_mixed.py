@@ -126,8 +126,8 @@ def test_mixed_precision_k_tensor_64(fp, oracle_mod):
 @pytest.mark.parametrize("solver", [0, 1, 2, 3])
 def test_mixed_solvers_k_and_true_residual(fp, oracle_mod, solver):
     """Every MIXED solver (CG / CR with reliable updates, CR on FP32 vectors, refined MINRES sweeps):
-    K of the 32^3 sphere
-    cell within 1e-6 of the FP64 oracle, and the FP64 true residual of each direction at tol."""
+    K of the 32^3 sphere cell within 1e-6 of the FP64 oracle, and the FP64 true residual of each
+    direction at tol."""
     from fft_method_test_amd import _lib, geometry as G, permeability as P
 
     mask = G.sphere_cell(32).reshape(-1)
