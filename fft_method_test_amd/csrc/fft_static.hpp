@@ -183,6 +183,18 @@ struct MergedShape {
                                   THREADS % 64 == 0 && LDS <= 100u * 1024u;
 };
 
+// Column-pass variants v in [kPassVariants, kColVariants): twice the base line count (rows of
+// 2 * base contiguous values, 256 bytes in FP32 at n = 256) with OS = 1 or 2; in FP64 their LDS
+// exceeds the budget and they are skipped.
+constexpr int kColVariants = kPassVariants + 2;
+
+template <int N>
+constexpr int col_lines(int v) {
+    return v < kPassVariants ? pass_lines<N>(v) : 2 * pass_base_lines<N>();
+}
+
+constexpr int col_os(int v) { return v < kPassVariants ? pass_os(v) : (v == kPassVariants ? 1 : 2); }
+
 // Block index remap so that consecutive tiles land on the same XCD (workgroups are dealt to
 // the 8 XCDs round-robin; each XCD has its own L2).
 __device__ __forceinline__ long xcd_tile(long b, long nblocks) {

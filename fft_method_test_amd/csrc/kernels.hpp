@@ -122,7 +122,8 @@ template <typename R>
 long static_col(int v, int n, int sign, cx<R>* S, long ncols, long inner, long outer, long estride, long comp_stride,
                 const cx<R>* tw, const int* skip, hipStream_t st);
 // 10 * lines per workgroup + threads per butterfly thread of variant v at length n (0 if none).
-int pass_variant_code(int n, int v);
+int pass_variant_code(int n, int v);  // x passes (kXPassVariants)
+int col_variant_code(int n, int v);   // column passes (kColVariants)
 
 // Autotuner registry, keyed by (pass kind, precision 0 = FP64 / 1 = FP32, transform length,
 // line or column count).  Values: pass variant index for the x and column passes, variant code

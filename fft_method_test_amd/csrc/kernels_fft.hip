@@ -604,8 +604,12 @@ int variant_of_code(int kind, int n, int code) {
             if (c == code) return c;
         return -1;
     }
-    const int nv = (kind == kKindXfwd || kind == kKindXinv) ? kXPassVariants : kPassVariants;
-    for (int v = 0; v < nv; ++v)
+    if (kind == kKindCol) {
+        for (int v = 0; v < kColVariants; ++v)
+            if (col_variant_code(n, v) == code && code != 0) return v;
+        return -1;
+    }
+    for (int v = 0; v < kXPassVariants; ++v)
         if (pass_variant_code(n, v) == code && code != 0) return v;
     return -1;
 }
@@ -616,7 +620,8 @@ int variant_code(int kind, int prec, int n, long count) {
         const int c = tuned_variant(kind, prec, n, count);
         return c < 0 ? zgreen_default_code(n, prec) : c;
     }
-    return pass_variant_code(n, pass_variant_or_default(kind, prec, n, count));
+    const int v = pass_variant_or_default(kind, prec, n, count);
+    return kind == kKindCol ? col_variant_code(n, v) : pass_variant_code(n, v);
 }
 
 template <typename R>
@@ -636,12 +641,12 @@ std::vector<std::pair<int, float>> autotune_col(cx<R>* S, const Geometry3& g, co
     const long ncols = (long)g.nz * g.nxh;
     (void)hipMemsetAsync(S, 0, 3 * (size_t)g.Nh * sizeof(cx<R>), st);
     return tune_candidates(
-        kKindCol, prec_of<R>(), g.ny, ncols, pass_candidates(),
+        kKindCol, prec_of<R>(), g.ny, ncols, pass_candidates(kColVariants),
         [&](int v) {
             return static_col<R>(v, g.ny, -1, S, ncols, (long)g.nxh, (long)g.ny * g.nxh, (long)g.nxh, g.Nh, t.twy,
                                  nullptr, st);
         },
-        pass_variant_code, st);
+        col_variant_code, st);
 }
 
 template <typename R>

@@ -104,6 +104,7 @@ struct CrState {
     int pause;           // 1: the host owes a reliable update / convergence check
     int breakdown;       // r.A r <= 0 or Ap.Ap <= 0
     int first;           // 1: beta = 0 (p = r, Ap = A r)
+    int pad_;
 };
 
 // mean_out, s_new = r.A r (Parseval partials / N), beta = first ? 0 : s_new / s

@@ -59,7 +59,7 @@ __global__ __launch_bounds__((PassShape<R, N, LINES, OS>::THREADS)) void k_col_s
 template <typename R, int N, int V>
 long col_variant(int sign, cx<R>* S, long ncols, long inner, long outer, long estride, long comp_stride,
                  const cx<R>* tw, const int* skip, hipStream_t st) {
-    constexpr int LINES = pass_lines<N>(V), OS = pass_os(V);
+    constexpr int LINES = col_lines<N>(V), OS = col_os(V);
     using SH = PassShape<R, N, LINES, OS>;
     if constexpr (!SH::valid) {
         return -1;
@@ -86,6 +86,8 @@ long col_n(int v, int sign, cx<R>* S, long ncols, long inner, long outer, long e
         case 2: return col_variant<R, N, 2>(sign, S, ncols, inner, outer, estride, comp_stride, tw, skip, st);
         case 3: return col_variant<R, N, 3>(sign, S, ncols, inner, outer, estride, comp_stride, tw, skip, st);
         case 4: return col_variant<R, N, 4>(sign, S, ncols, inner, outer, estride, comp_stride, tw, skip, st);
+        case 5: return col_variant<R, N, 5>(sign, S, ncols, inner, outer, estride, comp_stride, tw, skip, st);
+        case 6: return col_variant<R, N, 6>(sign, S, ncols, inner, outer, estride, comp_stride, tw, skip, st);
         default: return -1;
     }
 }
@@ -102,6 +104,23 @@ long static_col(int v, int n, int sign, cx<R>* S, long ncols, long inner, long o
         case 256: return col_n<R, 256>(v, sign, S, ncols, inner, outer, estride, comp_stride, tw, skip, st);
         case 512: return col_n<R, 512>(v, sign, S, ncols, inner, outer, estride, comp_stride, tw, skip, st);
         default: return -1;
+    }
+}
+
+template <int N>
+int col_code_of(int v) {
+    return 10 * col_lines<N>(v) + col_os(v);
+}
+
+int col_variant_code(int n, int v) {
+    if (v < 0 || v >= kColVariants) return 0;
+    switch (n) {
+        case 32: return col_code_of<32>(v);
+        case 64: return col_code_of<64>(v);
+        case 128: return col_code_of<128>(v);
+        case 256: return col_code_of<256>(v);
+        case 512: return col_code_of<512>(v);
+        default: return 0;
     }
 }
 

@@ -51,6 +51,22 @@ def xpass_codes(n, precision=0):
     return out
 
 
+def col_codes(n, precision=0):
+    """pass_codes(n) plus the wide column-pass variants (twice the base line count, OS 1 and 2)
+    that fit the LDS budget at this precision."""
+    t = n // _MAX_RADIX[n]
+    base = max(2, 256 // t)
+    out = pass_codes(n)
+    cx_bytes = 16 if precision == 0 else 8
+    lines = 2 * base
+    for os_ in (1, 2):
+        threads = os_ * lines * t
+        lds = (n + lines * (n + 1)) * cx_bytes
+        if 64 <= threads <= 1024 and (lines * n) % threads == 0 and lds <= 100 * 1024:
+            out.append(10 * lines + os_)
+    return out
+
+
 @contextlib.contextmanager
 def pinned(green, kind, precision, code):
     from fft_method_test_amd import _lib
@@ -108,7 +124,7 @@ def test_y_pass_variants_fp64(fp, oracle_mod, n):
 
     shape = Y_SHAPES[n]
     green, sup, o = _pair(oracle_mod, shape, seed=n + 1)
-    for code in pass_codes(n):
+    for code in col_codes(n, 0):
         with pinned(green, _lib.PASS_YPASS, _lib.PRECISION_FP64, code):
             _check_apply(green, sup, o, shape, seed=code)
 
@@ -153,7 +169,7 @@ def test_fp32_variants_mixed_k_sphere32(fp, oracle_mod):
     sup = P.ForceSupport(P.BinaryMedium(g, mask), P.SUPPORT_INTERFACE, green)
     cases = [(_lib.PASS_ZGREEN, c) for c in _lib.ZGREEN_CODES]
     cases += [(k, c) for k in (_lib.PASS_XFWD, _lib.PASS_XINV) for c in xpass_codes(n, 1)]
-    cases += [(_lib.PASS_YPASS, c) for c in pass_codes(n)]
+    cases += [(_lib.PASS_YPASS, c) for c in col_codes(n, 1)]
     for kind, code in cases:
         with pinned(green, kind, _lib.PRECISION_MIXED, code):
             K, _ = P.solve_permeability_tensor(green, 1e-9, 10000)
@@ -173,7 +189,7 @@ def test_fp32_operator_variants(fp, oracle_mod, n):
 
     cases = []
     for shape, kinds, codes in ((X_SHAPES[n], (_lib.PASS_XFWD, _lib.PASS_XINV), xpass_codes(n, 1)),
-                                (Y_SHAPES[n], (_lib.PASS_YPASS,), pass_codes(n)),
+                                (Y_SHAPES[n], (_lib.PASS_YPASS,), col_codes(n, 1)),
                                 (Z_SHAPES[n], (_lib.PASS_ZGREEN,), _lib.ZGREEN_CODES)):
         green, sup, o = _pair(oracle_mod, shape, seed=n + 5, precision=_lib.PRECISION_MIXED)
         op = P.VariationalOperator(sup, green)
