@@ -560,10 +560,10 @@ void launch_cr_direction(double* p, double* ap, const double* r, const double* q
     hipLaunchKernelGGL(k_cr_direction, dim3(kRedBlocks), dim3(kFlatThreads), 0, s, p, ap, r, q, st, 3 * nb, partials);
 }
 
-__global__ void k_cr_alpha(const double* __restrict__ partials, CrState* st) {
-    if (st->done) return;
+// alpha = s / Ap.Ap from the per-block partials; one block.
+__device__ __forceinline__ void cr_alpha_finish(const double* __restrict__ partials, int nparts, CrState* st) {
     double acc[1] = {0.0};
-    for (int p = threadIdx.x; p < kRedBlocks; p += blockDim.x) acc[0] += partials[p];
+    for (int p = threadIdx.x; p < nparts; p += blockDim.x) acc[0] += partials[p];
     __shared__ double res[1];
     block_sum<1>(acc, res);
     if (threadIdx.x == 0) {
@@ -577,6 +577,11 @@ __global__ void k_cr_alpha(const double* __restrict__ partials, CrState* st) {
             st->done = 1;
         }
     }
+}
+
+__global__ void k_cr_alpha(const double* __restrict__ partials, CrState* st) {
+    if (st->done) return;
+    cr_alpha_finish(partials, kRedBlocks, st);
 }
 
 void launch_cr_alpha(const double* partials, CrState* st, hipStream_t s) {
@@ -608,11 +613,12 @@ void launch_cr_update(double* x, double* r, const double* p, const double* ap, C
     hipLaunchKernelGGL(k_cr_update, dim3(kRedBlocks), dim3(kFlatThreads), 0, s, x, r, p, ap, st, 3 * nb, partials);
 }
 
-__global__ void k_cr_check(const double* __restrict__ partials, CrState* st, double* __restrict__ history,
-                           double inv_nb) {
-    if (st->done) return;
+// rr, relative residual, history, mean_in and the pause / stop decisions from the per-block
+// partials {r.r, sums of r}; one block.
+__device__ __forceinline__ void cr_check_finish(const double* __restrict__ partials, int nparts, CrState* st,
+                                                double* __restrict__ history, double inv_nb) {
     double acc[4] = {0, 0, 0, 0};
-    for (int p = threadIdx.x; p < kRedBlocks; p += blockDim.x) {
+    for (int p = threadIdx.x; p < nparts; p += blockDim.x) {
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[j] += partials[4 * p + j];
     }
@@ -642,22 +648,54 @@ __global__ void k_cr_check(const double* __restrict__ partials, CrState* st, dou
     if (rn > st->rnorm_max) st->rnorm_max = rn;
 }
 
+__global__ void k_cr_check(const double* __restrict__ partials, CrState* st, double* __restrict__ history,
+                           double inv_nb) {
+    if (st->done) return;
+    cr_check_finish(partials, kRedBlocks, st, history, inv_nb);
+}
+
 void launch_cr_check(const double* partials, CrState* st, double* history, double inv_nb, hipStream_t s) {
     hipLaunchKernelGGL(k_cr_check, dim3(1), dim3(kRedThreads), 0, s, partials, st, history, inv_nb);
 }
 
 // ------------------------------------------------------------ CR on FP32 vectors
-__global__ __launch_bounds__(kFlatThreads) void k_cr32_direction(float* __restrict__ p, float* __restrict__ ap,
-                                                                 const float* __restrict__ r,
-                                                                 const float* __restrict__ q, CrState* st, long n,
-                                                                 double* __restrict__ partials) {
+// FP32 vector kernels move four consecutive elements per thread and step (one 16-byte load per
+// array); element i has component i % 3.  n4 = n / 4 full quads, then the n % 4 tail.
+constexpr int kVecThreads = 256;
+
+__global__ __launch_bounds__(kVecThreads) void k_cr32_direction(float* __restrict__ p, float* __restrict__ ap,
+                                                                const float* __restrict__ r,
+                                                                const float* __restrict__ q, CrState* st, long n,
+                                                                double* __restrict__ partials) {
     if (st->done) return;
     const double b = st->beta;
-    const long i0 = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const double m = st->mean_out[i0 % 3];
-    double acc[1] = {0.0};
+    const double m3[3] = {st->mean_out[0], st->mean_out[1], st->mean_out[2]};
     const bool fresh = (b == 0.0);  // first iteration: set p and Ap (see k_cr_direction)
-    for (long i = i0; i < n; i += (long)gridDim.x * blockDim.x) {
+    double acc[1] = {0.0};
+    const long n4 = n / 4;
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < n4; v += stride) {
+        const float4 rv = reinterpret_cast<const float4*>(r)[v];
+        const float4 qv = reinterpret_cast<const float4*>(q)[v];
+        float4 pv = reinterpret_cast<float4*>(p)[v];
+        float4 av = reinterpret_cast<float4*>(ap)[v];
+        const int c0 = (int)((4 * v) % 3);
+        float* pe = &pv.x;
+        float* ae = &av.x;
+        const float* re = &rv.x;
+        const float* qe = &qv.x;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const double m = m3[(c0 + j) % 3];
+            pe[j] = fresh ? re[j] : (float)((double)re[j] + b * (double)pe[j]);
+            ae[j] = fresh ? (float)((double)qe[j] - m) : (float)(((double)qe[j] - m) + b * (double)ae[j]);
+            acc[0] += (double)ae[j] * (double)ae[j];
+        }
+        reinterpret_cast<float4*>(p)[v] = pv;
+        reinterpret_cast<float4*>(ap)[v] = av;
+    }
+    for (long i = 4 * n4 + (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const double m = m3[i % 3];
         p[i] = fresh ? r[i] : (float)((double)r[i] + b * (double)p[i]);
         const float a = fresh ? (float)((double)q[i] - m) : (float)(((double)q[i] - m) + b * (double)ap[i]);
         ap[i] = a;
@@ -668,33 +706,59 @@ __global__ __launch_bounds__(kFlatThreads) void k_cr32_direction(float* __restri
 
 void launch_cr32_direction(float* p, float* ap, const float* r, const float* q, CrState* st, long nb,
                            double* partials, hipStream_t s) {
-    hipLaunchKernelGGL(k_cr32_direction, dim3(kRedBlocks), dim3(kFlatThreads), 0, s, p, ap, r, q, st, 3 * nb,
+    hipLaunchKernelGGL(k_cr32_direction, dim3(kRedBlocks), dim3(kVecThreads), 0, s, p, ap, r, q, st, 3 * nb,
                        partials);
 }
 
-__global__ __launch_bounds__(kFlatThreads) void k_cr32_update(float* __restrict__ x, float* __restrict__ r,
-                                                              const float* __restrict__ p,
-                                                              const float* __restrict__ ap, CrState* st, long n,
-                                                              double* __restrict__ partials) {
+__global__ __launch_bounds__(kVecThreads) void k_cr32_update(float* __restrict__ x, float* __restrict__ r,
+                                                             const float* __restrict__ p,
+                                                             const float* __restrict__ ap, CrState* st, long n,
+                                                             double* __restrict__ partials) {
     if (st->done) return;
     const double a = st->alpha;
-    const long i0 = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    double rr = 0.0, sum = 0.0;
-    for (long i = i0; i < n; i += (long)gridDim.x * blockDim.x) {
+    double rr = 0.0, sum[3] = {0.0, 0.0, 0.0};
+    const long n4 = n / 4;
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < n4; v += stride) {
+        float4 xv = reinterpret_cast<float4*>(x)[v];
+        float4 rv = reinterpret_cast<float4*>(r)[v];
+        const float4 pv = reinterpret_cast<const float4*>(p)[v];
+        const float4 av = reinterpret_cast<const float4*>(ap)[v];
+        const int c0 = (int)((4 * v) % 3);
+        float* xe = &xv.x;
+        float* re = &rv.x;
+        const float* pe = &pv.x;
+        const float* ae = &av.x;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            xe[j] = (float)((double)xe[j] + a * (double)pe[j]);
+            re[j] = (float)((double)re[j] - a * (double)ae[j]);
+            rr += (double)re[j] * (double)re[j];
+            const int c = (c0 + j) % 3;
+            sum[0] += c == 0 ? (double)re[j] : 0.0;
+            sum[1] += c == 1 ? (double)re[j] : 0.0;
+            sum[2] += c == 2 ? (double)re[j] : 0.0;
+        }
+        reinterpret_cast<float4*>(x)[v] = xv;
+        reinterpret_cast<float4*>(r)[v] = rv;
+    }
+    for (long i = 4 * n4 + (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         x[i] = (float)((double)x[i] + a * (double)p[i]);
         const float ri = (float)((double)r[i] - a * (double)ap[i]);
         r[i] = ri;
         rr += (double)ri * (double)ri;
-        sum += (double)ri;
+        const int c = (int)(i % 3);
+        sum[0] += c == 0 ? (double)ri : 0.0;
+        sum[1] += c == 1 ? (double)ri : 0.0;
+        sum[2] += c == 2 ? (double)ri : 0.0;
     }
-    const int c = (int)(i0 % 3);
-    double acc[4] = {rr, c == 0 ? sum : 0.0, c == 1 ? sum : 0.0, c == 2 ? sum : 0.0};
+    double acc[4] = {rr, sum[0], sum[1], sum[2]};
     block_sum<4>(acc, partials + 4L * blockIdx.x);
 }
 
 void launch_cr32_update(float* x, float* r, const float* p, const float* ap, CrState* st, long nb, double* partials,
                         hipStream_t s) {
-    hipLaunchKernelGGL(k_cr32_update, dim3(kRedBlocks), dim3(kFlatThreads), 0, s, x, r, p, ap, st, 3 * nb, partials);
+    hipLaunchKernelGGL(k_cr32_update, dim3(kRedBlocks), dim3(kVecThreads), 0, s, x, r, p, ap, st, 3 * nb, partials);
 }
 
 __global__ void k_accumulate_f32(double* __restrict__ xacc, float* __restrict__ x32, long n) {
