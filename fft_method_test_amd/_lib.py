@@ -22,7 +22,7 @@ PRECISION_FP64, PRECISION_MIXED = 0, 1
 PASS_XFWD, PASS_XINV, PASS_YPASS, PASS_ZGREEN = 0, 1, 2, 3
 MIXED_CG_RELIABLE, MIXED_MINRES_REFINED, MIXED_CR_RELIABLE, MIXED_CR_FP32_VECTORS = 0, 1, 2, 3
 # tile variants the specialised passes offer at a length (codes as in fftperm_profile_stats)
-ZGREEN_CODES = (11, 12, 21, 22, 41, 42, 81, 82, 161, 162, 44, 84, 164, 324, 45, 85, 165)
+ZGREEN_CODES = (11, 12, 21, 22, 41, 42, 81, 82, 161, 162, 44, 84, 164, 324, 45, 85, 165, 86, 87, 88, 166, 167, 168, 326, 327)
 
 EXPORTED_SYMBOLS = (
     "fftperm_last_error", "fftperm_device_info", "fftperm_create", "fftperm_destroy",

@@ -1280,6 +1280,9 @@ int fftperm_create(const fftperm_grid* grid, double mu, int p_radius, int precis
         g.nxh = g.nx / 2 + 1;
         g.N = (long)g.nx * g.ny * g.nz;
         g.Nh = (long)g.nxh * g.ny * g.nz;
+        g.rp = spec_row_pitch(g.nxh);
+        g.pz = spec_plane_pitch(g.ny, g.rp);
+        g.Nc = (long)g.nz * g.pz;
         c->h[0] = grid->hx; c->h[1] = grid->hy; c->h[2] = grid->hz;
         // Grid3D: l = h * n unless given (src/core/Grid3D.hpp:19-21)
         c->L[0] = grid->lx > 0.0 ? grid->lx : grid->hx * (double)g.nx;
@@ -1299,8 +1302,12 @@ int fftperm_create(const fftperm_grid* grid, double mu, int p_radius, int precis
         c->t64.twx = make_twiddles<double>(g.nx);
         c->t64.twy = make_twiddles<double>(g.ny);
         c->t64.twz = make_twiddles<double>(g.nz);
-        c->S64 = dalloc<double2>(3 * (size_t)g.Nh);
-        c->G64 = dalloc<double>(6 * (size_t)g.Nh);
+        // padded half spectra: the padding stays zero (the passes read it as zero spectrum and never
+        // write it), so the buffers are cleared once here
+        c->S64 = dalloc<double2>(3 * (size_t)g.Nc);
+        c->G64 = dalloc<double>(6 * (size_t)g.Nc);
+        HIPCHK(hipMemsetAsync(c->S64, 0, 3 * (size_t)g.Nc * sizeof(double2), c->s));
+        HIPCHK(hipMemsetAsync(c->G64, 0, 6 * (size_t)g.Nc * sizeof(double), c->s));
         if (precision == FFTPERM_PRECISION_MIXED) {
             c->t32.px = c->t64.px;
             c->t32.py = c->t64.py;
@@ -1308,8 +1315,10 @@ int fftperm_create(const fftperm_grid* grid, double mu, int p_radius, int precis
             c->t32.twx = make_twiddles<float>(g.nx);
             c->t32.twy = make_twiddles<float>(g.ny);
             c->t32.twz = make_twiddles<float>(g.nz);
-            c->S32 = dalloc<float2>(3 * (size_t)g.Nh);
-            c->G32 = dalloc<float>(6 * (size_t)g.Nh);
+            c->S32 = dalloc<float2>(3 * (size_t)g.Nc);
+            c->G32 = dalloc<float>(6 * (size_t)g.Nc);
+            HIPCHK(hipMemsetAsync(c->S32, 0, 3 * (size_t)g.Nc * sizeof(float2), c->s));
+            HIPCHK(hipMemsetAsync(c->G32, 0, 6 * (size_t)g.Nc * sizeof(float), c->s));
         }
         // per-axis Green tables
         std::vector<double> s2x, qx, s2y, qy, s2z, qz;
@@ -1336,7 +1345,7 @@ int fftperm_create(const fftperm_grid* grid, double mu, int p_radius, int precis
         c->gt.P = c->P;
         c->gt.mu = mu;
         HIPCHK(hipMemcpy(c->gtab, host.data(), tot * sizeof(double), hipMemcpyHostToDevice));
-        launch_green_sym(c->gt, g.nx, g.ny, g.nz, c->G64, c->G32, c->s);
+        launch_green_sym(c->gt, g.nx, g.ny, g.nz, g.rp, g.pz, c->G64, c->G32, c->s);
         HIPCHK(hipGetLastError());
         // autotune the fused z pass tile width on this grid (cached per process)
         c->tune64 = autotune_zgreen<double>(c->S64, c->G64, g, c->t64, c->s);

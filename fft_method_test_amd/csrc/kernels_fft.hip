@@ -1,8 +1,9 @@
 // kernels_fft.hip -- the five sweeps of the Green-operator apply on the R2C half spectrum.
 //
 // Layout in HBM: real fields are SoA by component, flat index (z*ny + y)*nx + x
-// (src/core/Grid3D.hpp:41-43).  The spectrum S holds three components of
-// nz*ny*nxh complex values (nxh = nx/2 + 1), flat index (kz*ny + ky)*nxh + kx.
+// (src/core/Grid3D.hpp:41-43).  The spectrum S holds three components of the
+// nz*ny*nxh frequencies (nxh = nx/2 + 1) in the padded layout of Geometry3:
+// component c, frequency (kz, ky, kx) at c*Nc + kz*pz + ky*rp + kx.
 //
 //   x-forward : real x-lines -> complex lines -> n-point FFT -> first nxh bins
 //               (source values built on the fly from the support map: fused scatter)
@@ -95,8 +96,11 @@ struct ZShape {
     static constexpr int ZL = zl_raw();
 };
 
-// Hermitian weight of column kx of the half spectrum in a sum over the full spectrum.
-__device__ __forceinline__ double herm_weight(int kx, int nx) { return (kx == 0 || 2 * kx == nx) ? 1.0 : 2.0; }
+// Hermitian weight of column kx of the half spectrum in a sum over the full spectrum (0 on the
+// row padding, kx > nx / 2).
+__device__ __forceinline__ double herm_weight(int kx, int nx) {
+    return (2 * kx > nx) ? 0.0 : ((kx == 0 || 2 * kx == nx) ? 1.0 : 2.0);
+}
 
 // Fused z pass: lines 0..3*ZL-1 of the LDS buffer hold components x | y | z of ZL columns.
 // OS = threads per butterfly thread (1 or 2): extra threads only help the memory phases
@@ -108,7 +112,7 @@ constexpr int zgreen_threads() {
 template <typename R, int N, int ZL, int OS>
 __global__ __launch_bounds__((zgreen_threads<R, N, ZL, OS>())) void k_zgreen_s(cx<R>* __restrict__ S,
                                                                                  const R* __restrict__ G6, long ncols,
-                                                                                 long Nh, int nx, int nxh,
+                                                                                 long pitch, long Nc, int nx, int rp,
                                                                                  const cx<R>* __restrict__ twg,
                                                                                  double* __restrict__ qpart,
                                                                                  const int* skip) {
@@ -129,12 +133,12 @@ __global__ __launch_bounds__((zgreen_threads<R, N, ZL, OS>())) void k_zgreen_s(c
     const bool own = ci < ncl;
     constexpr int KL = N / EST;
     static_assert(N % EST == 0, "column elements split evenly over the load threads");
-    const cx<R>* Sc = S + comp * Nh + col0 + ci;
+    const cx<R>* Sc = S + comp * Nc + col0 + ci;
     cx<R> ld[KL];
 #pragma unroll
     for (int k = 0; k < KL; ++k) {
         ld[k] = mkc<R>((R)0, (R)0);
-        if (own) ld[k] = Sc[(long)(threadIdx.x / L3 + k * EST) * ncols];
+        if (own) ld[k] = Sc[(long)(threadIdx.x / L3 + k * EST) * pitch];
     }
     // the Green tensor of the frequencies this thread multiplies, fetched now so that its latency
     // overlaps the column loads and the forward transform
@@ -151,7 +155,7 @@ __global__ __launch_bounds__((zgreen_threads<R, N, ZL, OS>())) void k_zgreen_s(c
 #pragma unroll
             for (int q = 0; q < 6; ++q) gt[m][q] = (R)0;
             if (mown && e < N) {
-                const R* gp = Gc + 6L * e * ncols;
+                const R* gp = Gc + 6L * e * pitch;
 #pragma unroll
                 for (int q = 0; q < 6; ++q) gt[m][q] = gp[q];
             }
@@ -186,17 +190,126 @@ __global__ __launch_bounds__((zgreen_threads<R, N, ZL, OS>())) void k_zgreen_s(c
                         (double)fz.x * gz.x + (double)fz.y * gz.y;
         }
         if (qpart != nullptr) {
-            q[0] *= herm_weight((int)((col0 + mci) % nxh), nx);
+            q[0] *= herm_weight((int)((col0 + mci) % rp), nx);
             block_sum<1>(q, qpart + blockIdx.x);
         }
     }
     __syncthreads();
     sfft_lines<R, +1, N, L3, SH::T, LD>(buf, tw, L3);
-    cx<R>* Sw = S + comp * Nh + col0 + ci;
+    cx<R>* Sw = S + comp * Nc + col0 + ci;
 #pragma unroll
     for (int k = 0; k < KL; ++k) {
         const int e = threadIdx.x / L3 + k * EST;
-        if (own) Sw[(long)e * ncols] = buf[lt * LD + e];
+        if (own) Sw[(long)e * pitch] = buf[lt * LD + e];
+    }
+}
+
+// Component-sequential variant: the LDS holds one component of the ZL-column tile at a time
+// (ZL lines instead of 3 ZL), so wide tiles -- rows of 16 columns (128 B in FP32, 256 B in FP64)
+// -- fit several workgroups per CU.  Every thread keeps its share of all three components in
+// registers across the six transforms (three forward, three inverse) and does the Green
+// multiply on them; the tensor is read at the multiply.
+template <int N, int ZL, int OS>
+constexpr int zgreen_c_threads() {
+    return OS * ZL * threads_per_line<N>();
+}
+
+template <typename R, int N, int ZL, int OS>
+__global__ __launch_bounds__((zgreen_c_threads<N, ZL, OS>())) void k_zgreen_c(cx<R>* __restrict__ S,
+                                                                               const R* __restrict__ G6, long ncols,
+                                                                               long pitch, long Nc, int nx, int rp,
+                                                                               const cx<R>* __restrict__ twg,
+                                                                               double* __restrict__ qpart,
+                                                                               const int* skip) {
+    if (skip != nullptr && *skip) return;
+    constexpr int T = threads_per_line<N>(), LD = N + 1, THREADS = zgreen_c_threads<N, ZL, OS>();
+    constexpr int EST = THREADS / ZL;  // element stride of a thread within its column
+    constexpr int KL = N / EST;        // elements per thread per component
+    static_assert(THREADS % ZL == 0 && N % EST == 0, "column elements split evenly over the threads");
+    extern __shared__ unsigned char smem_raw[];
+    cx<R>* tw = reinterpret_cast<cx<R>*>(smem_raw);
+    cx<R>* buf = tw + N;
+    for (int i = threadIdx.x; i < N; i += THREADS) tw[i] = twg[i];
+    const long col0 = xcd_tile(blockIdx.x, gridDim.x) * ZL;
+    const int ncl = (int)min((long)ZL, ncols - col0);
+    const int ci = threadIdx.x % ZL;
+    const int e0 = threadIdx.x / ZL;
+    const bool own = ci < ncl;
+    const long colg = col0 + ci;
+    cx<R> f[3][KL];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int k = 0; k < KL; ++k) {
+            f[c][k] = mkc<R>((R)0, (R)0);
+            if (own) f[c][k] = S[c * Nc + colg + (long)(e0 + k * EST) * pitch];
+        }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        if (c > 0) __syncthreads();  // the previous component has been read back
+#pragma unroll
+        for (int k = 0; k < KL; ++k) buf[ci * LD + e0 + k * EST] = f[c][k];
+        __syncthreads();
+        sfft_lines<R, -1, N, ZL, T, LD>(buf, tw, ZL);
+#pragma unroll
+        for (int k = 0; k < KL; ++k) f[c][k] = buf[ci * LD + e0 + k * EST];
+    }
+    // symmetric Green multiply (src/bignonnet/GreenOperator.cpp:84-92 with G_sym) and the quadratic
+    // form conj(f) . G f of the forward spectrum when asked for
+    double q[1] = {0.0};
+#pragma unroll
+    for (int k = 0; k < KL; ++k) {
+        const long e = e0 + k * EST;
+        R t6[6];
+        const R* gp = G6 + 6L * (colg + e * pitch);
+#pragma unroll
+        for (int m = 0; m < 6; ++m) t6[m] = own ? gp[m] : (R)0;
+        const R txx = t6[0], txy = t6[1], txz = t6[2], tyy = t6[3], tyz = t6[4], tzz = t6[5];
+        const cx<R> fx = f[0][k], fy = f[1][k], fz = f[2][k];
+        const cx<R> gx = mkc<R>(txx * fx.x + txy * fy.x + txz * fz.x, txx * fx.y + txy * fy.y + txz * fz.y);
+        const cx<R> gy = mkc<R>(txy * fx.x + tyy * fy.x + tyz * fz.x, txy * fx.y + tyy * fy.y + tyz * fz.y);
+        const cx<R> gz = mkc<R>(txz * fx.x + tyz * fy.x + tzz * fz.x, txz * fx.y + tyz * fy.y + tzz * fz.y);
+        if (qpart != nullptr)
+            q[0] += (double)fx.x * gx.x + (double)fx.y * gx.y + (double)fy.x * gy.x + (double)fy.y * gy.y +
+                    (double)fz.x * gz.x + (double)fz.y * gz.y;
+        f[0][k] = gx;
+        f[1][k] = gy;
+        f[2][k] = gz;
+        // two frequencies per scheduling group: the tensor loads are not all hoisted to the top,
+        // which would hold 6 KL tensor values next to the 3 KL spectrum values
+        if (k % 2 == 1) __builtin_amdgcn_sched_barrier(0);
+    }
+    if (qpart != nullptr) {
+        q[0] = own ? q[0] * herm_weight((int)(colg % rp), nx) : 0.0;
+        block_sum<1>(q, qpart + blockIdx.x);
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < KL; ++k) buf[ci * LD + e0 + k * EST] = f[c][k];
+        __syncthreads();
+        sfft_lines<R, +1, N, ZL, T, LD>(buf, tw, ZL);
+        cx<R>* Sc = S + c * Nc + colg;
+#pragma unroll
+        for (int k = 0; k < KL; ++k)
+            if (own) Sc[(long)(e0 + k * EST) * pitch] = buf[ci * LD + e0 + k * EST];
+    }
+}
+
+template <typename R, int N, int ZL, int OS>
+long launch_zgreen_c(cx<R>* S, const R* G6, long ncols, const Geometry3& g, const cx<R>* tw, double* qpart,
+                     const int* skip, hipStream_t st) {
+    constexpr int threads = zgreen_c_threads<N, ZL, OS>();
+    constexpr size_t lds = (size_t)(N + ZL * (N + 1)) * sizeof(cx<R>);
+    if constexpr (threads > 1024 || threads < 64 || lds > 100u * 1024u) {
+        return -1;
+    } else {
+        if (lds > g_lds_max) return -1;
+        const long blocks = (ncols + ZL - 1) / ZL;
+        hipLaunchKernelGGL((k_zgreen_c<R, N, ZL, OS>), dim3((unsigned)blocks), dim3(threads), lds, st, S, G6, ncols,
+                           g.pz, g.Nc, g.nx, g.rp, tw, qpart, skip);
+        return blocks;
     }
 }
 
@@ -218,8 +331,9 @@ constexpr int zgreen_p_threads() {
 
 template <typename R, int N, int ZL>
 __global__ __launch_bounds__((zgreen_p_threads<R, N, ZL>())) void k_zgreen_p(cx<R>* __restrict__ S,
-                                                                      const R* __restrict__ G6, long ncols, long Nh,
-                                                                      int nx, int nxh, const cx<R>* __restrict__ twg,
+                                                                      const R* __restrict__ G6, long ncols, long pitch,
+                                                                      long Nc, int nx, int rp,
+                                                                      const cx<R>* __restrict__ twg,
                                                                       double* __restrict__ qpart, const int* skip) {
     if (skip != nullptr && *skip) return;
     using PLAN = typename SmallRadixPlan<N>::type;
@@ -247,11 +361,11 @@ __global__ __launch_bounds__((zgreen_p_threads<R, N, ZL>())) void k_zgreen_p(cx<
     auto fetch = [&](long tile, cx<R> (&ld)[KL]) {
         const long col0 = tile * ZL;
         const bool own = col0 + ci < ncols;
-        const cx<R>* Sc = S + comp * Nh + col0 + ci;
+        const cx<R>* Sc = S + comp * Nc + col0 + ci;
 #pragma unroll
         for (int k = 0; k < KL; ++k) {
             ld[k] = mkc<R>((R)0, (R)0);
-            if (own) ld[k] = Sc[(long)(threadIdx.x / L3 + k * EST) * ncols];
+            if (own) ld[k] = Sc[(long)(threadIdx.x / L3 + k * EST) * pitch];
         }
     };
     double q[1] = {0.0};
@@ -275,7 +389,7 @@ __global__ __launch_bounds__((zgreen_p_threads<R, N, ZL>())) void k_zgreen_p(cx<
 #pragma unroll
                 for (int qq = 0; qq < 6; ++qq) gt[m][qq] = (R)0;
                 if (mown && e < N) {
-                    const R* gp = Gc + 6L * e * ncols;
+                    const R* gp = Gc + 6L * e * pitch;
 #pragma unroll
                     for (int qq = 0; qq < 6; ++qq) gt[m][qq] = gp[qq];
                 }
@@ -302,18 +416,18 @@ __global__ __launch_bounds__((zgreen_p_threads<R, N, ZL>())) void k_zgreen_p(cx<
             *py = gy;
             *pz = gz;
             if (qpart != nullptr)
-                q[0] += herm_weight((int)((col0 + mci) % nxh), nx) *
+                q[0] += herm_weight((int)((col0 + mci) % rp), nx) *
                         ((double)fx.x * gx.x + (double)fx.y * gx.y + (double)fy.x * gy.x + (double)fy.y * gy.y +
                          (double)fz.x * gz.x + (double)fz.y * gz.y);
         }
         __syncthreads();
         sfft_lines_plan<R, +1, N, L3, T, LD, PLAN>(buf, tw, L3);
-        cx<R>* Sw = S + comp * Nh + col0 + ci;
+        cx<R>* Sw = S + comp * Nc + col0 + ci;
         const bool own = ci < ncl;
 #pragma unroll
         for (int k = 0; k < KL; ++k) {
             const int e = threadIdx.x / L3 + k * EST;
-            if (own) Sw[(long)e * ncols] = buf[lt * LD + e];
+            if (own) Sw[(long)e * pitch] = buf[lt * LD + e];
         }
         __syncthreads();  // the next tile overwrites the buffer
     }
@@ -334,8 +448,8 @@ long launch_zgreen_p(cx<R>* S, const R* G6, long ncols, const Geometry3& g, cons
         const long per_cu = std::max<long>(1, (long)(g_lds_cu / lds));
         long blocks = std::min<long>((long)g_cus * per_cu, ((ntiles + 7) / 8) * 8);
         blocks = std::max<long>(8, blocks / 8 * 8);
-        hipLaunchKernelGGL((k_zgreen_p<R, N, ZL>), dim3((unsigned)blocks), dim3(threads), lds, st, S, G6, ncols, g.Nh,
-                           g.nx, g.nxh, tw, qpart, skip);
+        hipLaunchKernelGGL((k_zgreen_p<R, N, ZL>), dim3((unsigned)blocks), dim3(threads), lds, st, S, G6, ncols, g.pz,
+                           g.Nc, g.nx, g.rp, tw, qpart, skip);
         return blocks;
     }
 }
@@ -343,7 +457,8 @@ long launch_zgreen_p(cx<R>* S, const R* G6, long ncols, const Geometry3& g, cons
 // Register-resident variant (zgreen_reg.hpp): ZL columns, N/4 threads per column.
 template <typename R, int N, int ZL>
 __global__ __launch_bounds__(ZL * N / 4) void k_zgreen_r(cx<R>* __restrict__ S, const R* __restrict__ G6, long ncols,
-                                                         long Nh, int nx, int nxh, const cx<R>* __restrict__ twg,
+                                                         long pitch, long Nc, int nx, int rp,
+                                                         const cx<R>* __restrict__ twg,
                                                          double* __restrict__ qpart, const int* skip) {
     if (skip != nullptr && *skip) return;
     constexpr int THREADS = ZL * N / 4, LD = N + 1, Q = N / 4;
@@ -357,15 +472,15 @@ __global__ __launch_bounds__(ZL * N / 4) void k_zgreen_r(cx<R>* __restrict__ S, 
     const int j = threadIdx.x / ZL;
     const bool own = ci < ncl;
     const long colg = col0 + ci;
-    // element offsets inside one component (Nh < 2^31 for every grid this kernel serves) off a
+    // element offsets inside one component (Nc < 2^31 for every grid this kernel serves) off a
     // wave-uniform component base: one 32-bit offset per lane instead of a 64-bit address per access
-    const int nc = (int)ncols;
+    const int nc = (int)pitch;
     const int off0 = (int)colg + j * nc;
     // element j + r N/4 of the three components
     cx<R> f[3][4];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        const cx<R>* Sc = S + c * Nh;
+        const cx<R>* Sc = S + c * Nc;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             f[c][r] = mkc<R>((R)0, (R)0);
@@ -403,12 +518,12 @@ __global__ __launch_bounds__(ZL * N / 4) void k_zgreen_r(cx<R>* __restrict__ S, 
         if (r % 2 == 1) __builtin_amdgcn_sched_barrier(0);
     }
     if (qpart != nullptr) {
-        qf[0] = own ? qf[0] * herm_weight((int)(colg % nxh), nx) : 0.0;
+        qf[0] = own ? qf[0] * herm_weight((int)(colg % rp), nx) : 0.0;
         block_sum<1>(qf, qpart + blockIdx.x);
     }
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        cx<R>* Sc = S + c * Nh;
+        cx<R>* Sc = S + c * Nc;
         rfft_bwd<R, N>(f[c], col, tw, j, [&](int e, cx<R> v) {
             if (own) Sc[(int)colg + e * nc] = v;
         });
@@ -430,8 +545,8 @@ long launch_zgreen_r(cx<R>* S, const R* G6, long ncols, const Geometry3& g, cons
         constexpr size_t lds = zgreen_r_lds<R, N, ZL>();
         if (lds > g_lds_max) return -1;
         const long blocks = (ncols + ZL - 1) / ZL;
-        hipLaunchKernelGGL((k_zgreen_r<R, N, ZL>), dim3((unsigned)blocks), dim3(threads), lds, st, S, G6, ncols, g.Nh,
-                           g.nx, g.nxh, tw, qpart, skip);
+        hipLaunchKernelGGL((k_zgreen_r<R, N, ZL>), dim3((unsigned)blocks), dim3(threads), lds, st, S, G6, ncols, g.pz,
+                           g.Nc, g.nx, g.rp, tw, qpart, skip);
         return blocks;
     }
 }
@@ -452,14 +567,16 @@ long launch_zgreen_zl(cx<R>* S, const R* G6, long ncols, const Geometry3& g, con
         const long blocks = (ncols + ZL - 1) / ZL;
         constexpr int threads = zgreen_threads<R, N, ZL, OS>();
         hipLaunchKernelGGL((k_zgreen_s<R, N, ZL, OS>), dim3((unsigned)blocks), dim3(threads), lds, st, S, G6, ncols,
-                           g.Nh, g.nx, g.nxh, tw, qpart, skip);
+                           g.pz, g.Nc, g.nx, g.rp, tw, qpart, skip);
         return blocks;
     }
 }
 
 // variant code = 10 * ZL + OS for the all-in-LDS kernel (OS = 1, 2 threads per butterfly thread),
-// 10 * ZL + 4 for the register-resident radix-4 kernel, 10 * ZL + 5 for the persistent kernel
-constexpr int kZgreenCodes[] = {11, 12, 21, 22, 41, 42, 81, 82, 161, 162, 44, 84, 164, 324, 45, 85, 165};
+// 10 * ZL + 4 for the register-resident radix-4 kernel, 10 * ZL + 5 for the persistent kernel,
+// 10 * ZL + 6 / + 7 / + 8 for the component-sequential kernel with OS = 1 / 2 / 4
+constexpr int kZgreenCodes[] = {11, 12, 21, 22, 41, 42, 81, 82, 161, 162, 44, 84, 164, 324, 45, 85, 165,
+                                86, 87, 88, 166, 167, 168, 326, 327};
 
 template <typename R, int N>
 long launch_zgreen_n(int code, cx<R>* S, const R* G6, long ncols, const Geometry3& g, const cx<R>* tw,
@@ -482,6 +599,14 @@ long launch_zgreen_n(int code, cx<R>* S, const R* G6, long ncols, const Geometry
         case 45: return launch_zgreen_p<R, N, 4>(S, G6, ncols, g, tw, qpart, skip, st);
         case 85: return launch_zgreen_p<R, N, 8>(S, G6, ncols, g, tw, qpart, skip, st);
         case 165: return launch_zgreen_p<R, N, 16>(S, G6, ncols, g, tw, qpart, skip, st);
+        case 86: return launch_zgreen_c<R, N, 8, 1>(S, G6, ncols, g, tw, qpart, skip, st);
+        case 87: return launch_zgreen_c<R, N, 8, 2>(S, G6, ncols, g, tw, qpart, skip, st);
+        case 88: return launch_zgreen_c<R, N, 8, 4>(S, G6, ncols, g, tw, qpart, skip, st);
+        case 168: return launch_zgreen_c<R, N, 16, 4>(S, G6, ncols, g, tw, qpart, skip, st);
+        case 166: return launch_zgreen_c<R, N, 16, 1>(S, G6, ncols, g, tw, qpart, skip, st);
+        case 167: return launch_zgreen_c<R, N, 16, 2>(S, G6, ncols, g, tw, qpart, skip, st);
+        case 326: return launch_zgreen_c<R, N, 32, 1>(S, G6, ncols, g, tw, qpart, skip, st);
+        case 327: return launch_zgreen_c<R, N, 32, 2>(S, G6, ncols, g, tw, qpart, skip, st);
         default: return -1;
     }
 }
@@ -521,38 +646,54 @@ constexpr int prec_of() {
 }
 
 // Times `launch` (which returns < 0 when the variant does not exist) for every candidate and
-// records the fastest under (kind, prec, n, count).
+// records the fastest under (kind, prec, n, count).  Every candidate runs once first (existence
+// check and warm-up, so that the clocks are up before any timing); then kTuneRounds rounds visit
+// the candidates in turn, each timing kTuneReps launches, and a candidate's time is its best
+// round -- interleaving spreads clock and cache drift over all candidates instead of charging it
+// to whichever is timed first.
+constexpr int kTuneRounds = 3;
+constexpr int kTuneReps = 3;
+
 template <typename F>
 std::vector<std::pair<int, float>> tune_candidates(int kind, int prec, int n, long count, const std::vector<int>& cands,
                                                    F&& launch, int (*code_of)(int, int), hipStream_t st) {
     std::vector<std::pair<int, float>> timings;
     if (!static_passes_ok() || tuned_variant(kind, prec, n, count) >= 0) return timings;
+    std::vector<int> live;
+    for (int v : cands)
+        if (launch(v) >= 0) live.push_back(v);
+    if (live.empty()) return timings;
     hipEvent_t a, b;
     if (hipEventCreate(&a) != hipSuccess) return timings;
     if (hipEventCreate(&b) != hipSuccess) {
         (void)hipEventDestroy(a);
         return timings;
     }
-    int best = -1;
-    float best_ms = 1e30f;
-    for (int v : cands) {
-        if (launch(v) < 0) continue;  // warm-up (and existence check)
-        (void)hipEventRecord(a, st);
-        for (int rep = 0; rep < 3; ++rep) launch(v);
-        (void)hipEventRecord(b, st);
-        if (hipEventSynchronize(b) != hipSuccess) break;
-        float ms = 0.f;
-        (void)hipEventElapsedTime(&ms, a, b);
-        ms /= 3.f;
-        timings.emplace_back(code_of(n, v), ms);
-        if (ms < best_ms) {
-            best_ms = ms;
-            best = v;
+    std::vector<float> best_of(live.size(), 1e30f);
+    bool ok = true;
+    for (int round = 0; round < kTuneRounds && ok; ++round) {
+        for (size_t i = 0; i < live.size(); ++i) {
+            (void)hipEventRecord(a, st);
+            for (int rep = 0; rep < kTuneReps; ++rep) launch(live[i]);
+            (void)hipEventRecord(b, st);
+            if (hipEventSynchronize(b) != hipSuccess) {
+                ok = false;
+                break;
+            }
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, a, b);
+            best_of[i] = std::min(best_of[i], ms / (float)kTuneReps);
         }
     }
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
-    if (best >= 0) set_tuned_variant(kind, prec, n, count, best);
+    if (!ok) return timings;
+    size_t best = 0;
+    for (size_t i = 0; i < live.size(); ++i) {
+        timings.emplace_back(code_of(n, live[i]), best_of[i]);
+        if (best_of[i] < best_of[best]) best = i;
+    }
+    set_tuned_variant(kind, prec, n, count, live[best]);
     return timings;
 }
 
@@ -627,24 +768,25 @@ int variant_code(int kind, int prec, int n, long count) {
 template <typename R>
 std::vector<std::pair<int, float>> autotune_zgreen(cx<R>* S, const R* G6, const Geometry3& g, const FFTTables<R>& t,
                                                    hipStream_t st) {
-    const long ncols = (long)g.ny * g.nxh;
-    (void)hipMemsetAsync(S, 0, 3 * (size_t)g.Nh * sizeof(cx<R>), st);
+    const long key = (long)g.ny * g.nxh;   // registry count: the frequency columns
+    const long ncols = (long)g.ny * g.rp;  // columns a launch covers: every padded row
+    (void)hipMemsetAsync(S, 0, 3 * (size_t)g.Nc * sizeof(cx<R>), st);
     std::vector<int> cands(std::begin(kZgreenCodes), std::end(kZgreenCodes));
     return tune_candidates(
-        kKindZgreen, prec_of<R>(), g.nz, ncols, cands,
+        kKindZgreen, prec_of<R>(), g.nz, key, cands,
         [&](int code) { return static_zgreen_code<R>(code, S, G6, ncols, g, t.twz, nullptr, nullptr, st); },
         zgreen_code_identity, st);
 }
 
 template <typename R>
 std::vector<std::pair<int, float>> autotune_col(cx<R>* S, const Geometry3& g, const FFTTables<R>& t, hipStream_t st) {
-    const long ncols = (long)g.nz * g.nxh;
-    (void)hipMemsetAsync(S, 0, 3 * (size_t)g.Nh * sizeof(cx<R>), st);
+    const long key = (long)g.nz * g.nxh;   // registry count: the frequency columns
+    const long ncols = (long)g.nz * g.rp;  // columns a launch covers: every padded row
+    (void)hipMemsetAsync(S, 0, 3 * (size_t)g.Nc * sizeof(cx<R>), st);
     return tune_candidates(
-        kKindCol, prec_of<R>(), g.ny, ncols, pass_candidates(kColVariants),
+        kKindCol, prec_of<R>(), g.ny, key, pass_candidates(kColVariants),
         [&](int v) {
-            return static_col<R>(v, g.ny, -1, S, ncols, (long)g.nxh, (long)g.ny * g.nxh, (long)g.nxh, g.Nh, t.twy,
-                                 nullptr, st);
+            return static_col<R>(v, g.ny, -1, S, ncols, (long)g.rp, g.pz, (long)g.rp, g.Nc, t.twy, nullptr, st);
         },
         col_variant_code, st);
 }
@@ -682,11 +824,11 @@ long xinv_max_blocks(const Geometry3& g) {
     return (long)g.ny * g.nz;
 }
 
-long zgreen_max_blocks(const Geometry3& g) { return (long)g.ny * g.nxh; }
+long zgreen_max_blocks(const Geometry3& g) { return (long)g.ny * g.rp; }
 
 // ------------------------------------------------------------------ x forward
 template <typename R>
-__global__ void k_xfwd(XSource src, cx<R>* __restrict__ S, int nx, long nlines, int nxh, long N, long Nh,
+__global__ void k_xfwd(XSource src, cx<R>* __restrict__ S, int nx, long nlines, int nxh, long N, SpecLayout SL,
                        LinePlan pl, const cx<R>* __restrict__ twg, int lpw, const int* skip) {
     if (skip != nullptr && *skip) return;
     extern __shared__ unsigned char smem_raw[];
@@ -721,11 +863,11 @@ __global__ void k_xfwd(XSource src, cx<R>* __restrict__ S, int nx, long nlines, 
         __syncthreads();
         fft_lines<R, -1>(buf, ld, nl, pl, tw);
         const int cnth = nl * nxh;
-        cx<R>* Sc = S + c * Nh;
+        cx<R>* Sc = S + c * SL.nc;
         for (int idx = threadIdx.x; idx < cnth; idx += blockDim.x) {
             const int line = idx / nxh;
             const int kx = idx - line * nxh;
-            Sc[(line0 + line) * nxh + kx] = buf[line * ld + kx];
+            Sc[spec_line(SL, line0 + line) + kx] = buf[line * ld + kx];
         }
         __syncthreads();
     }
@@ -740,12 +882,12 @@ void launch_xfwd(const XSource& src, cx<R>* S, const Geometry3& g, const FFTTabl
         return;
     PassCfg c = cfg_xpass(t.px, nlines);
     hipLaunchKernelGGL(k_xfwd<R>, dim3((unsigned)c.blocks), dim3(c.threads), c.lds, st, src, S, g.nx, nlines,
-                       g.nxh, g.N, g.Nh, t.px, t.twx, c.lines, skip);
+                       g.nxh, g.N, spec_layout(g), t.px, t.twx, c.lines, skip);
 }
 
 // ------------------------------------------------------------------ x inverse
 template <typename R>
-__global__ void k_xinv(const cx<R>* __restrict__ S, XSink sink, int nx, long nlines, int nxh, long N, long Nh,
+__global__ void k_xinv(const cx<R>* __restrict__ S, XSink sink, int nx, long nlines, int nxh, long N, SpecLayout SL,
                        LinePlan pl, const cx<R>* __restrict__ twg, int lpw, double inv_n, const int* skip) {
     if (skip != nullptr && *skip) return;
     extern __shared__ unsigned char smem_raw[];
@@ -757,12 +899,12 @@ __global__ void k_xinv(const cx<R>* __restrict__ S, XSink sink, int nx, long nli
     const int nl = (int)min((long)lpw, nlines - line0);
     double acc[kXinvParts] = {0, 0, 0, 0, 0, 0};
     for (int c = 0; c < 3; ++c) {
-        const cx<R>* Sc = S + c * Nh;
+        const cx<R>* Sc = S + c * SL.nc;
         const int cnth = nl * nxh;
         for (int idx = threadIdx.x; idx < cnth; idx += blockDim.x) {
             const int line = idx / nxh;
             const int kx = idx - line * nxh;
-            buf[line * ld + kx] = Sc[(line0 + line) * nxh + kx];
+            buf[line * ld + kx] = Sc[spec_line(SL, line0 + line) + kx];
         }
         __syncthreads();
         const int nm = nx - nxh;
@@ -811,7 +953,7 @@ long launch_xinv(const cx<R>* S, const XSink& sink, const Geometry3& g, const FF
     PassCfg c = cfg_xpass(t.px, nlines);
     const double inv_n = 1.0 / (double)g.N;
     hipLaunchKernelGGL(k_xinv<R>, dim3((unsigned)c.blocks), dim3(c.threads), c.lds, st, S, sink, g.nx, nlines,
-                       g.nxh, g.N, g.Nh, t.px, t.twx, c.lines, inv_n, skip);
+                       g.nxh, g.N, spec_layout(g), t.px, t.twx, c.lines, inv_n, skip);
     return c.blocks;
 }
 
@@ -853,42 +995,46 @@ __global__ void k_colpass(cx<R>* __restrict__ S, int n, long ncols, long inner, 
 template <typename R>
 void launch_ypass(cx<R>* S, int sign, const Geometry3& g, const FFTTables<R>& t, const int* skip,
                   hipStream_t st) {
-    const long ncols = (long)g.nz * g.nxh;
+    // column (kz, kx) of every padded row, kx < rp: base kz * pz + kx, elements ky * rp.  Covering
+    // the row padding (zeros in, zeros out) keeps every tile of 8 columns on a 64-byte boundary.
+    const long key = (long)g.nz * g.nxh;
+    const long ncols = (long)g.nz * g.rp;
     if (static_passes_ok() &&
-        static_col<R>(pass_variant_or_default(kKindCol, prec_of<R>(), g.ny, ncols), g.ny, sign, S, ncols, (long)g.nxh,
-                      (long)g.ny * g.nxh, (long)g.nxh, g.Nh, t.twy, skip, st) >= 0)
+        static_col<R>(pass_variant_or_default(kKindCol, prec_of<R>(), g.ny, key), g.ny, sign, S, ncols, (long)g.rp,
+                      g.pz, (long)g.rp, g.Nc, t.twy, skip, st) >= 0)
         return;
     PassCfg c = cfg_colpass(t.py, ncols);
     dim3 grid((unsigned)c.blocks, 3);
     if (sign < 0)
-        hipLaunchKernelGGL((k_colpass<R, -1>), grid, dim3(c.threads), c.lds, st, S, g.ny, ncols, (long)g.nxh,
-                           (long)g.ny * g.nxh, (long)g.nxh, g.Nh, t.py, t.twy, c.lines, skip);
+        hipLaunchKernelGGL((k_colpass<R, -1>), grid, dim3(c.threads), c.lds, st, S, g.ny, ncols, (long)g.rp, g.pz,
+                           (long)g.rp, g.Nc, t.py, t.twy, c.lines, skip);
     else
-        hipLaunchKernelGGL((k_colpass<R, +1>), grid, dim3(c.threads), c.lds, st, S, g.ny, ncols, (long)g.nxh,
-                           (long)g.ny * g.nxh, (long)g.nxh, g.Nh, t.py, t.twy, c.lines, skip);
+        hipLaunchKernelGGL((k_colpass<R, +1>), grid, dim3(c.threads), c.lds, st, S, g.ny, ncols, (long)g.rp, g.pz,
+                           (long)g.rp, g.Nc, t.py, t.twy, c.lines, skip);
 }
 
 template <typename R>
 void launch_zpass(cx<R>* S, int sign, const Geometry3& g, const FFTTables<R>& t, const int* skip,
                   hipStream_t st) {
-    const long ncols = (long)g.ny * g.nxh;
-    if (static_passes_ok() && static_col<R>(0, g.nz, sign, S, ncols, ncols, 0L, ncols, g.Nh, t.twz, skip, st) >= 0)
+    // column (ky, kx) of every padded row: base ky * rp + kx, elements kz * pz (padding stays zero)
+    const long ncols = (long)g.ny * g.rp;
+    if (static_passes_ok() && static_col<R>(0, g.nz, sign, S, ncols, ncols, 0L, g.pz, g.Nc, t.twz, skip, st) >= 0)
         return;
     PassCfg c = cfg_colpass(t.pz, ncols);
     dim3 grid((unsigned)c.blocks, 3);
     if (sign < 0)
-        hipLaunchKernelGGL((k_colpass<R, -1>), grid, dim3(c.threads), c.lds, st, S, g.nz, ncols, ncols, 0L,
-                           ncols, g.Nh, t.pz, t.twz, c.lines, skip);
+        hipLaunchKernelGGL((k_colpass<R, -1>), grid, dim3(c.threads), c.lds, st, S, g.nz, ncols, ncols, 0L, g.pz,
+                           g.Nc, t.pz, t.twz, c.lines, skip);
     else
-        hipLaunchKernelGGL((k_colpass<R, +1>), grid, dim3(c.threads), c.lds, st, S, g.nz, ncols, ncols, 0L,
-                           ncols, g.Nh, t.pz, t.twz, c.lines, skip);
+        hipLaunchKernelGGL((k_colpass<R, +1>), grid, dim3(c.threads), c.lds, st, S, g.nz, ncols, ncols, 0L, g.pz,
+                           g.Nc, t.pz, t.twz, c.lines, skip);
 }
 
 // ------------------------------------------------------------- fused z + Green
 template <typename R>
-__global__ void k_zgreen(cx<R>* __restrict__ S, const R* __restrict__ G6, int nz, long ncols, long Nh, int nx,
-                         int nxh, LinePlan pl, const cx<R>* __restrict__ twg, int tl, double* __restrict__ qpart,
-                         const int* skip) {
+__global__ void k_zgreen(cx<R>* __restrict__ S, const R* __restrict__ G6, int nz, long ncols, long pitch, long Nc,
+                         int nx, int rp, LinePlan pl, const cx<R>* __restrict__ twg, int tl,
+                         double* __restrict__ qpart, const int* skip) {
     if (skip != nullptr && *skip) return;
     extern __shared__ unsigned char smem_raw[];
     cx<R>* tw = reinterpret_cast<cx<R>*>(smem_raw);
@@ -899,12 +1045,12 @@ __global__ void k_zgreen(cx<R>* __restrict__ S, const R* __restrict__ G6, int nz
     const int ncl = (int)min((long)tl, ncols - col0);
     const int cnt = ncl * nz;
     for (int c = 0; c < 3; ++c) {
-        const cx<R>* Sc = S + c * Nh;
+        const cx<R>* Sc = S + c * Nc;
         cx<R>* B = buf + c * ncl * ld;
         for (int idx = threadIdx.x; idx < cnt; idx += blockDim.x) {
             const int e = idx / ncl;
             const int ci = idx - e * ncl;
-            B[ci * ld + e] = Sc[col0 + ci + e * ncols];
+            B[ci * ld + e] = Sc[col0 + ci + e * pitch];
         }
     }
     __syncthreads();
@@ -916,7 +1062,7 @@ __global__ void k_zgreen(cx<R>* __restrict__ S, const R* __restrict__ G6, int nz
     for (int idx = threadIdx.x; idx < cnt; idx += blockDim.x) {
         const int e = idx / ncl;
         const int ci = idx - e * ncl;
-        const R* gp = G6 + 6L * (col0 + ci + e * ncols);
+        const R* gp = G6 + 6L * (col0 + ci + e * pitch);
         const R txx = gp[0], txy = gp[1], txz = gp[2];
         const R tyy = gp[3], tyz = gp[4], tzz = gp[5];
         cx<R>* px = buf + (0 * ncl + ci) * ld + e;
@@ -930,7 +1076,7 @@ __global__ void k_zgreen(cx<R>* __restrict__ S, const R* __restrict__ G6, int nz
         *py = gy;
         *pz = gz;
         if (qpart != nullptr)
-            q[0] += herm_weight((int)((col0 + ci) % nxh), nx) *
+            q[0] += herm_weight((int)((col0 + ci) % rp), nx) *
                     ((double)fx.x * gx.x + (double)fx.y * gx.y + (double)fy.x * gy.x + (double)fy.y * gy.y +
                      (double)fz.x * gz.x + (double)fz.y * gz.y);
     }
@@ -938,12 +1084,12 @@ __global__ void k_zgreen(cx<R>* __restrict__ S, const R* __restrict__ G6, int nz
     __syncthreads();
     fft_lines<R, +1>(buf, ld, 3 * ncl, pl, tw);
     for (int c = 0; c < 3; ++c) {
-        cx<R>* Sc = S + c * Nh;
+        cx<R>* Sc = S + c * Nc;
         const cx<R>* B = buf + c * ncl * ld;
         for (int idx = threadIdx.x; idx < cnt; idx += blockDim.x) {
             const int e = idx / ncl;
             const int ci = idx - e * ncl;
-            Sc[col0 + ci + e * ncols] = B[ci * ld + e];
+            Sc[col0 + ci + e * pitch] = B[ci * ld + e];
         }
     }
 }
@@ -951,23 +1097,24 @@ __global__ void k_zgreen(cx<R>* __restrict__ S, const R* __restrict__ G6, int nz
 template <typename R>
 long launch_zgreen(cx<R>* S, const R* G6, const Geometry3& g, const FFTTables<R>& t, double* qpart, const int* skip,
                    hipStream_t st) {
-    const long ncols = (long)g.ny * g.nxh;
+    const long key = (long)g.ny * g.nxh;   // registry count: the frequency columns
+    const long ncols = (long)g.ny * g.rp;  // columns a launch covers: every padded row
     if (static_passes_ok()) {
-        int code = tuned_variant(kKindZgreen, prec_of<R>(), g.nz, ncols);
+        int code = tuned_variant(kKindZgreen, prec_of<R>(), g.nz, key);
         if (code < 0) code = zgreen_default_code(g.nz, prec_of<R>());
         const long b = static_zgreen_code<R>(code, S, G6, ncols, g, t.twz, qpart, skip, st);
         if (b >= 0) return b;
     }
     PassCfg c = cfg_zgreen(t.pz, ncols);
-    hipLaunchKernelGGL(k_zgreen<R>, dim3((unsigned)c.blocks), dim3(c.threads), c.lds, st, S, G6, g.nz, ncols,
-                       g.Nh, g.nx, g.nxh, t.pz, t.twz, c.lines, qpart, skip);
+    hipLaunchKernelGGL(k_zgreen<R>, dim3((unsigned)c.blocks), dim3(c.threads), c.lds, st, S, G6, g.nz, ncols, g.pz,
+                       g.Nc, g.nx, g.rp, t.pz, t.twz, c.lines, qpart, skip);
     return c.blocks;
 }
 
 // ------------------------------------------------------- spectrum utilities
 template <typename R>
 __global__ void k_expand_full(const cx<R>* __restrict__ S, double2* __restrict__ F, int nx, int ny, int nz,
-                              int nxh, long N, long Nh) {
+                              int nxh, long N, SpecLayout SL) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
     const int c = blockIdx.y;
@@ -977,12 +1124,12 @@ __global__ void k_expand_full(const cx<R>* __restrict__ S, double2* __restrict__
     const int kz = (int)(r / ny);
     double2 v;
     if (kx < nxh) {
-        const cx<R> s = S[c * Nh + ((long)kz * ny + ky) * nxh + kx];
+        const cx<R> s = S[c * SL.nc + (long)kz * SL.pz + (long)ky * SL.rp + kx];
         v.x = (double)s.x;
         v.y = (double)s.y;
     } else {
         const int mz = (nz - kz) % nz, my = (ny - ky) % ny, mx = nx - kx;
-        const cx<R> s = S[c * Nh + ((long)mz * ny + my) * nxh + mx];
+        const cx<R> s = S[c * SL.nc + (long)mz * SL.pz + (long)my * SL.rp + mx];
         v.x = (double)s.x;
         v.y = -(double)s.y;
     }
@@ -992,12 +1139,12 @@ __global__ void k_expand_full(const cx<R>* __restrict__ S, double2* __restrict__
 template <typename R>
 void launch_expand_full(const cx<R>* S, double2* F, const Geometry3& g, hipStream_t st) {
     dim3 grid((unsigned)((g.N + 255) / 256), 3);
-    hipLaunchKernelGGL(k_expand_full<R>, grid, dim3(256), 0, st, S, F, g.nx, g.ny, g.nz, g.nxh, g.N, g.Nh);
+    hipLaunchKernelGGL(k_expand_full<R>, grid, dim3(256), 0, st, S, F, g.nx, g.ny, g.nz, g.nxh, g.N, spec_layout(g));
 }
 
 template <typename R>
 __global__ void k_hermitian_half(const double2* __restrict__ X, cx<R>* __restrict__ S, int nx, int ny, int nz,
-                                 int nxh, long N, long Nh) {
+                                 int nxh, long N, long Nh, SpecLayout SL) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= Nh) return;
     const int c = blockIdx.y;
@@ -1008,13 +1155,14 @@ __global__ void k_hermitian_half(const double2* __restrict__ X, cx<R>* __restric
     const double2 a = X[c * N + ((long)kz * ny + ky) * nx + kx];
     const int mz = (nz - kz) % nz, my = (ny - ky) % ny, mx = (nx - kx) % nx;
     const double2 b = X[c * N + ((long)mz * ny + my) * nx + mx];
-    S[c * Nh + i] = mkc<R>((R)(0.5 * (a.x + b.x)), (R)(0.5 * (a.y - b.y)));
+    S[c * SL.nc + (long)kz * SL.pz + (long)ky * SL.rp + kx] = mkc<R>((R)(0.5 * (a.x + b.x)), (R)(0.5 * (a.y - b.y)));
 }
 
 template <typename R>
 void launch_hermitian_half(const double2* X, cx<R>* S, const Geometry3& g, hipStream_t st) {
     dim3 grid((unsigned)((g.Nh + 255) / 256), 3);
-    hipLaunchKernelGGL(k_hermitian_half<R>, grid, dim3(256), 0, st, X, S, g.nx, g.ny, g.nz, g.nxh, g.N, g.Nh);
+    hipLaunchKernelGGL(k_hermitian_half<R>, grid, dim3(256), 0, st, X, S, g.nx, g.ny, g.nz, g.nxh, g.N, g.Nh,
+                       spec_layout(g));
 }
 
 __global__ void k_spectral_green_full(const double* __restrict__ T6, const double2* __restrict__ F,

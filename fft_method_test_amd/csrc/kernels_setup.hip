@@ -224,8 +224,8 @@ __device__ void green_at(const GreenAxisTables& t, int ki, int kj, int kk, doubl
     out[5] = gzz;
 }
 
-__global__ void k_green_sym(GreenAxisTables t, int nx, int ny, int nz, int nxh, long Nh, double* __restrict__ G6,
-                            float* __restrict__ G6f) {
+__global__ void k_green_sym(GreenAxisTables t, int nx, int ny, int nz, int nxh, long Nh, int rp, long pz,
+                            double* __restrict__ G6, float* __restrict__ G6f) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= Nh) return;
     const int kx = (int)(i % nxh);
@@ -235,19 +235,22 @@ __global__ void k_green_sym(GreenAxisTables t, int nx, int ny, int nz, int nxh, 
     double a[6], b[6];
     green_at(t, kx, ky, kz, a);
     green_at(t, (nx - kx) % nx, (ny - ky) % ny, (nz - kz) % nz, b);
-    // AoS: the six components of one frequency are contiguous (one 48-byte run per frequency)
+    // AoS: the six components of one frequency are contiguous (one 48-byte run per frequency), at
+    // the spectrum's padded element index kz * pz + ky * rp + kx
+    const long e = (long)kz * pz + (long)ky * rp + kx;
     for (int g = 0; g < 6; ++g) {
         const double s = 0.5 * (a[g] + b[g]);
-        G6[6 * i + g] = s;
-        if (G6f != nullptr) G6f[6 * i + g] = (float)s;
+        G6[6 * e + g] = s;
+        if (G6f != nullptr) G6f[6 * e + g] = (float)s;
     }
 }
 
-void launch_green_sym(const GreenAxisTables& t, int nx, int ny, int nz, double* G6, float* G6f, hipStream_t st) {
+void launch_green_sym(const GreenAxisTables& t, int nx, int ny, int nz, int rp, long pz, double* G6, float* G6f,
+                      hipStream_t st) {
     const int nxh = nx / 2 + 1;
     const long Nh = (long)nxh * ny * nz;
-    hipLaunchKernelGGL(k_green_sym, dim3((unsigned)((Nh + 127) / 128)), dim3(128), 0, st, t, nx, ny, nz, nxh, Nh,
-                       G6, G6f);
+    hipLaunchKernelGGL(k_green_sym, dim3((unsigned)((Nh + 127) / 128)), dim3(128), 0, st, t, nx, ny, nz, nxh, Nh, rp,
+                       pz, G6, G6f);
 }
 
 __global__ void k_green_full(GreenAxisTables t, int nx, int ny, int nz, long N, double* __restrict__ T6) {

@@ -164,7 +164,7 @@ __device__ __forceinline__ void xfwd_load(const XSource& src, long line0, int nl
 
 template <typename R, int N, int LINES, int OS>
 __global__ __launch_bounds__((PassShape<R, N, LINES, OS>::THREADS)) void k_xfwd_s(XSource src, cx<R>* __restrict__ S,
-                                                                                long nlines, long N3, long Nh,
+                                                                                long nlines, long N3, SpecLayout SL,
                                                                                 const cx<R>* __restrict__ twg,
                                                                                 const int* skip) {
     if (skip != nullptr && *skip) return;
@@ -195,9 +195,9 @@ __global__ __launch_bounds__((PassShape<R, N, LINES, OS>::THREADS)) void k_xfwd_
         if (line < nl) {
             const cx<R> zk = buf[line * LD + kx];
             const cx<R> zm = buf[line * LD + (N - kx) % N];
-            const long o = (line0 + line) * NXH + kx;
+            const long o = spec_line(SL, line0 + line) + kx;
             S[o] = mkc<R>((R)0.5 * (zk.x + zm.x), (R)0.5 * (zk.y - zm.y));
-            S[Nh + o] = mkc<R>((R)0.5 * (zk.y + zm.y), (R)0.5 * (zm.x - zk.x));
+            S[SL.nc + o] = mkc<R>((R)0.5 * (zk.y + zm.y), (R)0.5 * (zm.x - zk.x));
         }
     }
     __syncthreads();
@@ -212,15 +212,15 @@ __global__ __launch_bounds__((PassShape<R, N, LINES, OS>::THREADS)) void k_xfwd_
     __syncthreads();
     const int nl2 = (nl + 1) / 2;
     sfft_lines<R, -1, N, LINES, SH::T, LD>(buf, tw, nl2);
-    cx<R>* Sz = S + 2 * Nh;
+    cx<R>* Sz = S + 2 * SL.nc;
     for (int idx = threadIdx.x; idx < (LINES / 2) * NXH; idx += THREADS) {
         const int m = idx / NXH;
         const int kx = idx % NXH;
         const cx<R> zk = buf[m * LD + kx];
         const cx<R> zm = buf[m * LD + (N - kx) % N];
         const int la = 2 * m, lb = 2 * m + 1;
-        if (la < nl) Sz[(line0 + la) * NXH + kx] = mkc<R>((R)0.5 * (zk.x + zm.x), (R)0.5 * (zk.y - zm.y));
-        if (lb < nl) Sz[(line0 + lb) * NXH + kx] = mkc<R>((R)0.5 * (zk.y + zm.y), (R)0.5 * (zm.x - zk.x));
+        if (la < nl) Sz[spec_line(SL, line0 + la) + kx] = mkc<R>((R)0.5 * (zk.x + zm.x), (R)0.5 * (zk.y - zm.y));
+        if (lb < nl) Sz[spec_line(SL, line0 + lb) + kx] = mkc<R>((R)0.5 * (zk.y + zm.y), (R)0.5 * (zm.x - zk.x));
     }
 }
 
@@ -257,7 +257,7 @@ __device__ __forceinline__ void xinv_emit(const XSink& sink, long N3, int c, lon
 template <typename R, int N, int LINES, int OS>
 __global__ __launch_bounds__((PassShape<R, N, LINES, OS>::THREADS)) void k_xinv_s(const cx<R>* __restrict__ S,
                                                                                 XSink sink, long nlines, long N3,
-                                                                                long Nh,
+                                                                                SpecLayout SL,
                                                                                 const cx<R>* __restrict__ twg,
                                                                                 double inv_n, const int* skip) {
     if (skip != nullptr && *skip) return;
@@ -292,14 +292,14 @@ __global__ __launch_bounds__((PassShape<R, N, LINES, OS>::THREADS)) void k_xinv_
         const int idx = threadIdx.x + k * THREADS;
         const int line = idx / NXH;
         const bool ok = idx < LINES * NXH && line < nl;
-        const long o = (line0 + line) * NXH + idx % NXH;
+        const long o = spec_line(SL, line0 + line) + idx % NXH;
         // predicated loads into zeroed registers (a select between the spectrum and a zero
         // constant in memory would become a flat load through a private stack slot)
         ha[k] = mkc<R>((R)0, (R)0);
         hb[k] = mkc<R>((R)0, (R)0);
         if (ok) {
             ha[k] = S[o];
-            hb[k] = S[Nh + o];
+            hb[k] = S[SL.nc + o];
         }
     }
     cx<R> ga[KB], gb[KB];
@@ -311,8 +311,8 @@ __global__ __launch_bounds__((PassShape<R, N, LINES, OS>::THREADS)) void k_xinv_
         const bool in = idx < (LINES / 2) * NXH;
         ga[k] = mkc<R>((R)0, (R)0);
         gb[k] = mkc<R>((R)0, (R)0);
-        if (in && 2 * m < nl) ga[k] = S[2 * Nh + (line0 + 2 * m) * NXH + kx];
-        if (in && 2 * m + 1 < nl) gb[k] = S[2 * Nh + (line0 + 2 * m + 1) * NXH + kx];
+        if (in && 2 * m < nl) ga[k] = S[2 * SL.nc + spec_line(SL, line0 + 2 * m) + kx];
+        if (in && 2 * m + 1 < nl) gb[k] = S[2 * SL.nc + spec_line(SL, line0 + 2 * m + 1) + kx];
     }
     if (use_bits) {
         constexpr int W = (N + 31) / 32;
@@ -374,7 +374,7 @@ __global__ __launch_bounds__((PassShape<R, N, LINES, OS>::THREADS)) void k_xinv_
 template <typename R, int N, int LINES, int OS>
 __global__ __launch_bounds__((MergedShape<R, N, LINES, OS>::THREADS)) void k_xfwd_m(XSource src,
                                                                                  cx<R>* __restrict__ S, long nlines,
-                                                                                 long N3, long Nh,
+                                                                                 long N3, SpecLayout SL,
                                                                                  const cx<R>* __restrict__ twg,
                                                                                  const int* skip) {
     if (skip != nullptr && *skip) return;
@@ -403,7 +403,7 @@ __global__ __launch_bounds__((MergedShape<R, N, LINES, OS>::THREADS)) void k_xfw
     }
     __syncthreads();
     sfft_lines<R, -1, N, LT, SH::T, LD>(buf, tw, LT);
-    cx<R>* Sz = S + 2 * Nh;
+    cx<R>* Sz = S + 2 * SL.nc;
     for (int idx = threadIdx.x; idx < LT * NXH; idx += THREADS) {
         const int line = idx / NXH;
         const int kx = idx % NXH;
@@ -413,14 +413,14 @@ __global__ __launch_bounds__((MergedShape<R, N, LINES, OS>::THREADS)) void k_xfw
         const cx<R> b = mkc<R>((R)0.5 * (zk.y + zm.y), (R)0.5 * (zm.x - zk.x));
         if (line < LINES) {
             if (line < nl) {
-                const long o = (line0 + line) * NXH + kx;
+                const long o = spec_line(SL, line0 + line) + kx;
                 S[o] = a;
-                S[Nh + o] = b;
+                S[SL.nc + o] = b;
             }
         } else {
             const int la = 2 * (line - LINES), lb = la + 1;
-            if (la < nl) Sz[(line0 + la) * NXH + kx] = a;
-            if (lb < nl) Sz[(line0 + lb) * NXH + kx] = b;
+            if (la < nl) Sz[spec_line(SL, line0 + la) + kx] = a;
+            if (lb < nl) Sz[spec_line(SL, line0 + lb) + kx] = b;
         }
     }
 }
@@ -460,7 +460,7 @@ __device__ __forceinline__ void xinv_emit3(const XSink& sink, long N3, long vox,
 template <typename R, int N, int LINES, int OS>
 __global__ __launch_bounds__((MergedShape<R, N, LINES, OS>::THREADS)) void k_xinv_m(const cx<R>* __restrict__ S,
                                                                                  XSink sink, long nlines, long N3,
-                                                                                 long Nh,
+                                                                                 SpecLayout SL,
                                                                                  const cx<R>* __restrict__ twg,
                                                                                  double inv_n, const int* skip) {
     if (skip != nullptr && *skip) return;
@@ -493,12 +493,12 @@ __global__ __launch_bounds__((MergedShape<R, N, LINES, OS>::THREADS)) void k_xin
         const int idx = threadIdx.x + k * THREADS;
         const int line = idx / NXH;
         const bool ok = idx < LINES * NXH && line < nl;
-        const long o = (line0 + line) * NXH + idx % NXH;
+        const long o = spec_line(SL, line0 + line) + idx % NXH;
         ha[k] = mkc<R>((R)0, (R)0);
         hb[k] = mkc<R>((R)0, (R)0);
         if (ok) {
             ha[k] = S[o];
-            hb[k] = S[Nh + o];
+            hb[k] = S[SL.nc + o];
         }
     }
     cx<R> ga[KB], gb[KB];
@@ -510,8 +510,8 @@ __global__ __launch_bounds__((MergedShape<R, N, LINES, OS>::THREADS)) void k_xin
         const bool in = idx < (LINES / 2) * NXH;
         ga[k] = mkc<R>((R)0, (R)0);
         gb[k] = mkc<R>((R)0, (R)0);
-        if (in && 2 * m < nl) ga[k] = S[2 * Nh + (line0 + 2 * m) * NXH + kx];
-        if (in && 2 * m + 1 < nl) gb[k] = S[2 * Nh + (line0 + 2 * m + 1) * NXH + kx];
+        if (in && 2 * m < nl) ga[k] = S[2 * SL.nc + spec_line(SL, line0 + 2 * m) + kx];
+        if (in && 2 * m + 1 < nl) gb[k] = S[2 * SL.nc + spec_line(SL, line0 + 2 * m + 1) + kx];
     }
     if (use_bits) {
         constexpr int W = (N + 31) / 32;
@@ -569,7 +569,7 @@ long xfwd_variant(const XSource& src, cx<R>* S, const Geometry3& g, const cx<R>*
             constexpr size_t lds = SH::LDS + bits_lds<N, LINES>();
             if (lds > lds_limit()) return -1;
             hipLaunchKernelGGL((k_xfwd_m<R, N, LINES, OS>), dim3((unsigned)blocks), dim3(SH::THREADS), lds, st, src,
-                               S, nlines, g.N, g.Nh, tw, skip);
+                               S, nlines, g.N, spec_layout(g), tw, skip);
             return blocks;
         }
     } else {
@@ -580,7 +580,7 @@ long xfwd_variant(const XSource& src, cx<R>* S, const Geometry3& g, const cx<R>*
             constexpr size_t lds = SH::LDS + bits_lds<N, LINES>();
             if (lds > lds_limit()) return -1;
             hipLaunchKernelGGL((k_xfwd_s<R, N, LINES, OS>), dim3((unsigned)blocks), dim3(SH::THREADS), lds, st, src,
-                               S, nlines, g.N, g.Nh, tw, skip);
+                               S, nlines, g.N, spec_layout(g), tw, skip);
             return blocks;
         }
     }
@@ -601,7 +601,7 @@ long xinv_variant(const cx<R>* S, const XSink& sink, const Geometry3& g, const c
             constexpr size_t lds = SH::LDS + bits_lds<N, LINES>();
             if (lds > lds_limit()) return -1;
             hipLaunchKernelGGL((k_xinv_m<R, N, LINES, OS>), dim3((unsigned)blocks), dim3(SH::THREADS), lds, st, S,
-                               sink, nlines, g.N, g.Nh, tw, inv_n, skip);
+                               sink, nlines, g.N, spec_layout(g), tw, inv_n, skip);
             return blocks;
         }
     } else {
@@ -612,7 +612,7 @@ long xinv_variant(const cx<R>* S, const XSink& sink, const Geometry3& g, const c
             constexpr size_t lds = SH::LDS + bits_lds<N, LINES>();
             if (lds > lds_limit()) return -1;
             hipLaunchKernelGGL((k_xinv_s<R, N, LINES, OS>), dim3((unsigned)blocks), dim3(SH::THREADS), lds, st, S,
-                               sink, nlines, g.N, g.Nh, tw, inv_n, skip);
+                               sink, nlines, g.N, spec_layout(g), tw, inv_n, skip);
             return blocks;
         }
     }
