@@ -34,7 +34,7 @@ EXPORTED_SYMBOLS = (
     "fftperm_solve_direction", "fftperm_solve_tensor", "fftperm_residual_norm", "fftperm_symmetry_gap",
     "fftperm_profile", "fftperm_profile_read", "fftperm_set_graphs", "fftperm_unet_create", "fftperm_unet_destroy",
     "fftperm_unet_num_params", "fftperm_unet_get_params", "fftperm_unet_set_params", "fftperm_unet_loss_grad",
-    "fftperm_unet_predict", "fftperm_solve_tensor_warm", "fftperm_brinkman_solve",
+    "fftperm_unet_predict", "fftperm_solve_tensor_warm", "fftperm_brinkman_solve", "fftperm_set_tensor_batch",
 )
 
 
@@ -53,7 +53,8 @@ class ProfileStats(C.Structure):
     _fields_ = [("zgreen_ms", C.c_double * 2), ("zgreen_launches", C.c_uint64 * 2),
                 ("apply_ms", C.c_double * 2), ("apply_launches", C.c_uint64 * 2),
                 ("N", C.c_uint64), ("Nh", C.c_uint64), ("nb", C.c_uint64), ("zgreen_tile", C.c_int * 2),
-                ("xfwd_tile", C.c_int * 2), ("xinv_tile", C.c_int * 2), ("ypass_tile", C.c_int * 2)]
+                ("xfwd_tile", C.c_int * 2), ("xinv_tile", C.c_int * 2), ("ypass_tile", C.c_int * 2),
+                ("zgreen_rhs", C.c_uint64 * 2), ("apply_rhs", C.c_uint64 * 2)]
 
 
 class BrinkmanInfo(C.Structure):
@@ -101,6 +102,7 @@ def load():
     L.fftperm_set_mixed_solver.argtypes = [_vp, C.c_int, C.c_double]
     L.fftperm_pin_variant.argtypes = [_vp, C.c_int, C.c_int, C.c_int]
     L.fftperm_set_graphs.argtypes = [_vp, C.c_int]
+    L.fftperm_set_tensor_batch.argtypes = [_vp, C.c_int]
     L.fftperm_recover.argtypes = [_vp, _dp, _dp, _vp, _dp]
     L.fftperm_solve_direction.argtypes = [_vp, _dp, C.c_double, C.c_uint64, _vp, _dp, _vp, C.POINTER(SolveInfo)]
     L.fftperm_solve_tensor.argtypes = [_vp, C.c_double, C.c_uint64, _dp, _vp]

@@ -156,6 +156,10 @@ struct fftperm_ctx {
     uint64_t fp32_iters = 0;        // iterations with the FP32 operator (whole context)
     int mixed_solver = FFTPERM_MIXED_CR_FP32_VECTORS;
     double reliable_delta = 0.1;    // CG: reliable update when ||r|| falls below this x its last peak
+    // MIXED CR32 tensor solves: 0 one direction after the other; 1 the three in lockstep, each
+    // direction's five passes run back to back (its spectrum stays in the last-level cache from one
+    // pass to the next); 2 the three in lockstep with one launch per pass for all three
+    int tensor_batch = 1;
     int device = 0;
     hipStream_t s = nullptr;
 
@@ -184,7 +188,7 @@ struct fftperm_ctx {
     // reductions
     double* part_x = nullptr;  // kXinvParts x x-inverse workgroups
     double* part_z = nullptr;  // one quadratic-form partial per fused z-pass workgroup
-    double* part = nullptr;    // 8 x kRedBlocks
+    double* part = nullptr;    // 4 kRedBlocks per right-hand side (kMaxRhs sides)
     double* small = nullptr;   // 32 scalars: [0,3) input mean, [4,10) x-inverse sums, [12,15) average, 16 norm
     long xparts = 0;           // partial rows written by the last x-inverse launch
     long zparts = 0;           // partials written by the last fused z-pass launch
@@ -193,9 +197,14 @@ struct fftperm_ctx {
     size_t Lcap = 0;
     double *x = nullptr, *v_old = nullptr, *v = nullptr, *z = nullptr, *w_prev = nullptr, *w = nullptr;
     double *b = nullptr, *tmp = nullptr, *r = nullptr, *dx = nullptr;
-    // FP32 Krylov vectors of the mixed CR solver (allocated on first use)
+    // FP32 Krylov vectors of the mixed CR solver (allocated on first use), kMaxRhs slots of vs32
+    // elements each (slot 0 serves the single-RHS solves)
     size_t L32cap = 0;
+    long vs32 = 0;
     float *x32 = nullptr, *r32 = nullptr, *p32 = nullptr, *ap32 = nullptr, *q32 = nullptr;
+    // right-hand sides and FP64 accumulated solutions of a multi-RHS solve (kMaxRhs x L each)
+    size_t tcap = 0;
+    double *tb = nullptr, *txa = nullptr;
     MinresState* st = nullptr;
     CgState* cst = nullptr;
     CrState* rst = nullptr;
@@ -206,7 +215,7 @@ struct fftperm_ctx {
     // started), keyed by everything a capture bakes in: precision, buffer addresses, sizes
     struct GraphKey {
         int fp32;
-        int solver;  // 0 MINRES, 1 CG, 2 CR
+        int solver;  // 0 MINRES, 1 CG, 2 CR, 3 CR on FP32 vectors, 10 * sides for the multi-RHS CR
         const void* p[9];
         long nb, tuning;
         bool operator==(const GraphKey& o) const {
@@ -227,14 +236,17 @@ struct fftperm_ctx {
     // fused z-pass and around the whole five-sweep apply
     struct ProfRec {
         hipEvent_t a, b, c, d;  // before x-fwd, before z-green, after z-green, after x-inv
-        long long tag;          // MINRES iteration index, or -1 (always a real apply)
+        long long tag;          // Krylov iteration index, or -1 (always a real apply)
         int fp32;
+        int batch;              // lockstep record of sides 0 .. batch - 1 (0: not a lockstep record)
+        int side;               // per-side record of a multi-RHS iteration (-1: none)
     };
     bool prof = false;
     std::vector<ProfRec> prof_pending;
     std::vector<hipEvent_t> ev_free;
     double prof_z_ms[2] = {0, 0}, prof_apply_ms[2] = {0, 0};
     long long prof_z_n[2] = {0, 0}, prof_apply_n[2] = {0, 0};
+    long long prof_rhs[2] = {0, 0};  // right-hand sides served by the counted applies
     long long cur_tag = -1;
 
     hipEvent_t take_event() {
@@ -247,13 +259,23 @@ struct fftperm_ctx {
         HIPCHK(hipEventCreate(&e));
         return e;
     }
-    // Accumulate recorded applies; MINRES applies launched after convergence (tag >= valid_below)
-    // were no-ops and are dropped.
-    void prof_resolve(long long valid_below) {
+    // Accumulate recorded applies; Krylov applies launched after convergence (tag >= valid_below)
+    // were no-ops and are dropped.  Multi-RHS records: side j ran d[j] iterations of the segment,
+    // so iteration t served the sides with t < d[j] (none: a no-op).
+    void prof_resolve(long long valid_below, const long long* d = nullptr, int nd = 0) {
         if (prof_pending.empty()) return;
         HIPCHK(hipStreamSynchronize(s));
         for (const ProfRec& r : prof_pending) {
-            if (r.tag < 0 || r.tag < valid_below) {
+            long long rhs = 1;
+            if (r.tag >= 0 && r.batch > 0) {
+                rhs = 0;
+                for (int j = 0; j < nd; ++j) rhs += (r.tag < d[j]) ? 1 : 0;
+            } else if (r.tag >= 0 && r.side >= 0) {
+                rhs = (r.side < nd && r.tag < d[r.side]) ? 1 : 0;
+            } else if (r.tag >= valid_below) {
+                rhs = 0;
+            }
+            if (rhs > 0) {
                 float zms = 0.f, ams = 0.f;
                 HIPCHK(hipEventElapsedTime(&zms, r.b, r.c));
                 HIPCHK(hipEventElapsedTime(&ams, r.a, r.d));
@@ -261,6 +283,7 @@ struct fftperm_ctx {
                 prof_apply_ms[r.fp32] += ams;
                 prof_z_n[r.fp32] += 1;
                 prof_apply_n[r.fp32] += 1;
+                prof_rhs[r.fp32] += rhs;
             }
             ev_free.push_back(r.a);
             ev_free.push_back(r.b);
@@ -286,7 +309,7 @@ struct fftperm_ctx {
         dfree(part_x); dfree(part_z); dfree(part); dfree(small);
         dfree(x); dfree(v_old); dfree(v); dfree(z); dfree(w_prev); dfree(w);
         dfree(b); dfree(tmp); dfree(r); dfree(dx);
-        dfree(x32); dfree(r32); dfree(p32); dfree(ap32); dfree(q32);
+        dfree(x32); dfree(r32); dfree(p32); dfree(ap32); dfree(q32); dfree(tb); dfree(txa);
         dfree(st); dfree(cst); dfree(rst); dfree(hist); dfree(field); dfree(full_a); dfree(t6);
         for (const ProfRec& r : prof_pending) {
             (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b);
@@ -323,9 +346,22 @@ struct fftperm_ctx {
         const size_t L = (size_t)std::max(1L, L3());
         if (L <= L32cap) return;
         dfree(x32); dfree(r32); dfree(p32); dfree(ap32); dfree(q32);
-        x32 = dalloc<float>(L); r32 = dalloc<float>(L); p32 = dalloc<float>(L); ap32 = dalloc<float>(L);
-        q32 = dalloc<float>(L);
+        // slots start on 256-byte boundaries (the vector kernels move 16-byte quads)
+        const long vs = (long)((L + 63) / 64 * 64);
+        const size_t n = (size_t)kMaxRhs * (size_t)vs;
+        x32 = dalloc<float>(n); r32 = dalloc<float>(n); p32 = dalloc<float>(n); ap32 = dalloc<float>(n);
+        q32 = dalloc<float>(n);
+        vs32 = vs;
         L32cap = L;
+    }
+
+    void ensure_tensor_vectors() {
+        const size_t L = (size_t)std::max(1L, L3());
+        if (L <= tcap) return;
+        dfree(tb); dfree(txa);
+        tb = dalloc<double>((size_t)kMaxRhs * L);
+        txa = dalloc<double>((size_t)kMaxRhs * L);
+        tcap = L;
     }
 
     void ensure_hist(size_t n) {
@@ -1103,6 +1139,252 @@ MinresOut device_cr_reliable(fftperm_ctx* c, const double* bdev, const double* x
     return out;
 }
 
+// ------------------------------------------------------------------ multi-RHS CR on FP32 vectors
+// The three load directions of a tensor iterate in lockstep (src/app/PermeabilityRunner.cpp:186-198
+// solves them one after the other): side b owns CR state rst[b], FP32 vector slot b (at b * vs32),
+// spectrum slot b (components 3b .. 3b + 2 of S32) and partial rows b; every kernel skips a side
+// whose state is done, so each side runs exactly the single-RHS recurrence.
+
+// int stride between the done flags of consecutive CR states
+constexpr int kCrDoneStride = (int)(sizeof(CrState) / sizeof(int));
+static_assert(sizeof(CrState) % sizeof(int) == 0, "CR states are laid out in whole ints");
+
+long cr_xstride(const fftperm_ctx* c) { return (long)kXinvParts * xinv_max_blocks(c->g); }
+long cr_zstride(const fftperm_ctx* c) { return zgreen_max_blocks(c->g); }
+
+// The FP32 five-sweep apply q_b = gather(G scatter(r_b - mean_b)) of nrhs sides: side after side
+// (tensor_batch 1), or pass after pass over all sides (tensor_batch 2: one y launch for the nine
+// components, one z launch reading the Green tensor once).
+void chain32_batch(fftperm_ctx* c, int nrhs) {
+    const Geometry3& g = c->g;
+    CrState* st = c->rst;
+    const long spec = 3 * g.Nc;
+    auto source = [&](int b) {
+        XSource sr = src_none(c);
+        sr.compact32 = c->r32 + b * c->vs32;
+        sr.mean = st[b].mean_in;
+        return sr;
+    };
+    auto sink = [&](int b) {
+        XSink sk = sink_none(c);
+        sk.compact32 = c->q32 + b * c->vs32;
+        sk.partials = c->part_x + b * cr_xstride(c);
+        return sk;
+    };
+    auto prof_begin = [&](int batch, int side) {
+        fftperm_ctx::ProfRec rec{};
+        rec.a = c->take_event();
+        rec.b = c->take_event();
+        rec.c = c->take_event();
+        rec.d = c->take_event();
+        rec.tag = c->cur_tag;
+        rec.fp32 = 1;
+        rec.batch = batch;
+        rec.side = side;
+        HIPCHK(hipEventRecord(rec.a, c->s));
+        return rec;
+    };
+    if (c->tensor_batch != 2) {
+        for (int b = 0; b < nrhs; ++b) {
+            const int* skip = &st[b].done;
+            cx<float>* S = c->S32 + b * spec;
+            fftperm_ctx::ProfRec rec{};
+            if (c->prof) rec = prof_begin(0, b);
+            launch_xfwd<float>(source(b), S, g, c->t32, skip, c->s);
+            launch_ypass<float>(S, -1, g, c->t32, skip, c->s);
+            if (c->prof) HIPCHK(hipEventRecord(rec.b, c->s));
+            c->zparts = launch_zgreen<float>(S, c->G32, g, c->t32, c->part_z + b * cr_zstride(c), skip, c->s);
+            if (c->prof) HIPCHK(hipEventRecord(rec.c, c->s));
+            launch_ypass<float>(S, +1, g, c->t32, skip, c->s);
+            c->xparts = launch_xinv<float>(S, sink(b), g, c->t32, skip, c->s);
+            if (c->prof) {
+                HIPCHK(hipEventRecord(rec.d, c->s));
+                c->prof_pending.push_back(rec);
+            }
+        }
+        return;
+    }
+    fftperm_ctx::ProfRec rec{};
+    if (c->prof) rec = prof_begin(nrhs, -1);
+    for (int b = 0; b < nrhs; ++b) launch_xfwd<float>(source(b), c->S32 + b * spec, g, c->t32, &st[b].done, c->s);
+    launch_ypass_n<float>(c->S32, -1, g, c->t32, nrhs, &st[0].done, kCrDoneStride, c->s);
+    if (c->prof) HIPCHK(hipEventRecord(rec.b, c->s));
+    c->zparts = launch_zgreen_batch<float>(c->S32, c->G32, g, c->t32, c->part_z, cr_zstride(c), nrhs, &st[0].done,
+                                           kCrDoneStride, c->s);
+    if (c->prof) HIPCHK(hipEventRecord(rec.c, c->s));
+    launch_ypass_n<float>(c->S32, +1, g, c->t32, nrhs, &st[0].done, kCrDoneStride, c->s);
+    for (int b = 0; b < nrhs; ++b)
+        c->xparts = launch_xinv<float>(c->S32 + b * spec, sink(b), g, c->t32, &st[b].done, c->s);
+    if (c->prof) {
+        HIPCHK(hipEventRecord(rec.d, c->s));
+        c->prof_pending.push_back(rec);
+    }
+}
+
+// One lockstep CR iteration of nrhs sides (the steps of cr32_iteration, one launch each; no
+// residual history).
+void cr32_batch_iteration(fftperm_ctx* c, int nrhs) {
+    const long nb = (long)c->nb;
+    const double inv_nb = 1.0 / (double)nb;
+    const long pstride = 4L * kRedBlocks;
+    chain32_batch(c, nrhs);
+    launch_cr_beta_n(c->part_x, (int)c->xparts, cr_xstride(c), c->part_z, (int)c->zparts, cr_zstride(c), c->rst, nrhs,
+                     inv_nb, 1.0 / (double)c->g.N, c->s);
+    launch_cr32_direction_n(c->p32, c->ap32, c->r32, c->q32, c->rst, nb, c->vs32, c->part, pstride, nrhs, c->s);
+    launch_cr_alpha_n(c->part, pstride, c->rst, nrhs, c->s);
+    launch_cr32_update_n(c->x32, c->r32, c->p32, c->ap32, c->rst, nb, c->vs32, c->part, pstride, nrhs, c->s);
+    launch_cr_check_n(c->part, pstride, c->rst, nullptr, 0, nrhs, inv_nb, c->s);
+}
+
+hipGraphExec_t cr32_batch_graph(fftperm_ctx* c, int nrhs) {
+    fftperm_ctx::GraphKey k{};
+    k.fp32 = 1;
+    k.solver = 10 * nrhs + c->tensor_batch;
+    const void* ptrs[9] = {c->x32, c->r32, c->p32, c->q32, c->ap32, c->rst, c->part_z, c->part_x, c->part};
+    for (int i = 0; i < 9; ++i) k.p[i] = ptrs[i];
+    k.nb = (long)c->nb;
+    k.tuning = tuning_generation();
+    for (const auto& g : c->graphs)
+        if (g.key == k) return g.exec;
+    if (c->graphs.size() >= 8) {
+        HIPCHK(hipGraphExecDestroy(c->graphs.front().exec));
+        c->graphs.erase(c->graphs.begin());
+    }
+    hipGraph_t graph = nullptr;
+    HIPCHK(hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal));
+    for (int i = 0; i < 2; ++i) cr32_batch_iteration(c, nrhs);
+    HIPCHK(hipStreamEndCapture(c->s, &graph));
+    hipGraphExec_t exec = nullptr;
+    const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    HIPCHK(e);
+    c->graphs.push_back({k, exec});
+    return exec;
+}
+
+// device_cr_reliable<float, true> for nrhs sides at once: side b solves A x = bdev[b] from the
+// guess already in xacc[b] (has_x0[b]) or from zero, into xacc[b].  Each side pauses, takes its
+// FP64 true residual and resumes on its own; the others keep iterating meanwhile.
+void device_cr32_batch(fftperm_ctx* c, int nrhs, const double* const* bdev, double* const* xacc, const bool* has_x0,
+                       double tol, uint64_t maxit, MinresOut* outs, uint64_t* updates) {
+    if (!(tol > 0.0)) throw Err(FFTPERM_EINVAL, "MinresSolver tolerance must be positive");
+    if (nrhs < 1 || nrhs > kMaxRhs) throw Err(FFTPERM_EINVAL, "unsupported right-hand side count");
+    const long L = c->L3();
+    const long nb = (long)c->nb;
+    const double inv_nb = 1.0 / (double)nb;
+    c->ensure_vectors32();
+    CrState h[kMaxRhs];
+    double bnorm[kMaxRhs];
+    bool active[kMaxRhs];
+    long long seg_start[kMaxRhs];
+    // the FP64 true residual r = b - A xacc of side b into c->r; returns ||r|| / ||b||
+    auto true_residual = [&](int b, bool xzero) {
+        if (!xzero) {
+            device_apply<double>(c, xacc[b], c->tmp);
+            launch_axpby(1.0, bdev[b], -1.0, c->tmp, c->r, L, c->s);
+        } else {
+            HIPCHK(hipMemcpyAsync(c->r, bdev[b], L * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+        }
+        return device_norm(c, c->r, L) / bnorm[b];
+    };
+    // side b continues from c->r: mean of r into its state, r into its FP32 slot
+    auto load_residual = [&](int b) {
+        launch_compact_sums(c->r, nb, c->part, nullptr, c->s);
+        launch_finish_sum(c->part, kRedBlocks, 3, inv_nb, c->rst[b].mean_in, nullptr, c->s);
+        launch_to_f32(c->r, c->r32 + b * c->vs32, L, c->s);
+    };
+    for (int b = 0; b < nrhs; ++b) {
+        updates[b] = 0;
+        outs[b] = MinresOut{};
+        active[b] = false;
+        seg_start[b] = 0;
+        std::memset(&h[b], 0, sizeof(CrState));
+        h[b].done = 1;  // sides that do not iterate stay skipped
+        HIPCHK(hipMemcpyAsync(&c->rst[b], &h[b], sizeof(CrState), hipMemcpyHostToDevice, c->s));
+        if (!has_x0[b]) HIPCHK(hipMemsetAsync(xacc[b], 0, L * sizeof(double), c->s));
+        bnorm[b] = device_norm(c, bdev[b], L);
+        if (bnorm[b] == 0.0) {
+            outs[b].final_rel = 0.0;
+            outs[b].converged = 1;
+            continue;
+        }
+        const double rel = true_residual(b, !has_x0[b]);
+        if (rel <= tol || maxit == 0) {
+            outs[b].final_rel = rel;
+            outs[b].converged = rel <= tol ? 1 : 0;
+            continue;
+        }
+        HIPCHK(hipMemsetAsync(c->x32 + b * c->vs32, 0, L * sizeof(float), c->s));
+        h[b].done = 0;
+        h[b].rnorm_max = rel * bnorm[b];
+        h[b].rhs_norm = bnorm[b];
+        h[b].tol = tol;
+        h[b].delta = c->reliable_delta;
+        h[b].maxit = (long long)maxit;
+        h[b].first = 1;
+        outs[b].final_rel = rel;
+        HIPCHK(hipMemcpyAsync(&c->rst[b], &h[b], sizeof(CrState), hipMemcpyHostToDevice, c->s));
+        load_residual(b);
+        active[b] = true;
+    }
+    const uint64_t chunk = (c->g.N >= (1L << 21)) ? 4 : 32;
+    for (;;) {
+        bool any = false;
+        for (int b = 0; b < nrhs; ++b) any = any || active[b];
+        if (!any) break;
+        // iterate until an active side stops (pause for its reliable update, or its end)
+        bool stop = false;
+        long long seg_launch = 0;
+        while (!stop) {
+            const bool graph = c->use_graphs && !c->prof;
+            uint64_t i = 0;
+            if (graph)
+                for (; i + 2 <= chunk; i += 2) HIPCHK(hipGraphLaunch(cr32_batch_graph(c, nrhs), c->s));
+            for (; i < chunk; ++i) {
+                c->cur_tag = seg_launch++;
+                cr32_batch_iteration(c, nrhs);
+            }
+            c->cur_tag = -1;
+            HIPCHK(hipMemcpyAsync(h, c->rst, nrhs * sizeof(CrState), hipMemcpyDeviceToHost, c->s));
+            c->sync();
+            for (int b = 0; b < nrhs; ++b) stop = stop || (active[b] && h[b].done);
+        }
+        long long d[kMaxRhs];
+        for (int b = 0; b < nrhs; ++b) d[b] = active[b] ? h[b].iterations - seg_start[b] : 0;
+        c->prof_resolve(0, d, nrhs);
+        for (int b = 0; b < nrhs; ++b) seg_start[b] = h[b].iterations;
+        for (int b = 0; b < nrhs; ++b) {
+            if (!active[b] || !h[b].done) continue;
+            launch_accumulate_f32(xacc[b], c->x32 + b * c->vs32, L, c->s);
+            const double rel = true_residual(b, false);
+            outs[b].final_rel = rel;
+            ++updates[b];
+            if (rel <= tol || !h[b].pause || h[b].breakdown || h[b].iterations >= h[b].maxit) {
+                active[b] = false;  // its state stays done: every later kernel skips it
+                outs[b].iterations = (uint64_t)h[b].iterations;
+                outs[b].converged = rel <= tol ? 1 : 0;
+                c->fp32_iters += (uint64_t)h[b].iterations;
+                continue;
+            }
+            // continue from the true residual; p, Ap and r.A r of the last apply carry over
+            const double rn = rel * bnorm[b];
+            h[b].rr = rn * rn;
+            h[b].rnorm_max = rn;
+            h[b].done = 0;
+            h[b].pause = 0;
+            HIPCHK(hipMemcpyAsync(&c->rst[b], &h[b], sizeof(CrState), hipMemcpyHostToDevice, c->s));
+            load_residual(b);
+        }
+    }
+    c->prof_resolve(LLONG_MAX);
+    c->sync();
+}
+
+bool use_tensor_batch(const fftperm_ctx* c) {
+    return c->tensor_batch != 0 && c->precision == FFTPERM_PRECISION_MIXED &&
+           c->mixed_solver == FFTPERM_MIXED_CR_FP32_VECTORS;
+}
+
 MinresOut run_minres(fftperm_ctx* c, const double* bdev, const double* x0dev, double tol, uint64_t maxit,
                      uint64_t* sweeps, std::vector<double>* mixed_hist) {
     *sweeps = 0;
@@ -1315,9 +1597,10 @@ int fftperm_create(const fftperm_grid* grid, double mu, int p_radius, int precis
             c->t32.twx = make_twiddles<float>(g.nx);
             c->t32.twy = make_twiddles<float>(g.ny);
             c->t32.twz = make_twiddles<float>(g.nz);
-            c->S32 = dalloc<float2>(3 * (size_t)g.Nc);
+            // kMaxRhs spectra: the multi-RHS solve transforms the three directions side by side
+            c->S32 = dalloc<float2>(3 * (size_t)kMaxRhs * (size_t)g.Nc);
             c->G32 = dalloc<float>(6 * (size_t)g.Nc);
-            HIPCHK(hipMemsetAsync(c->S32, 0, 3 * (size_t)g.Nc * sizeof(float2), c->s));
+            HIPCHK(hipMemsetAsync(c->S32, 0, 3 * (size_t)kMaxRhs * (size_t)g.Nc * sizeof(float2), c->s));
             HIPCHK(hipMemsetAsync(c->G32, 0, 6 * (size_t)g.Nc * sizeof(float), c->s));
         }
         // per-axis Green tables
@@ -1360,13 +1643,13 @@ int fftperm_create(const fftperm_grid* grid, double mu, int p_radius, int precis
             c->tune32.insert(c->tune32.end(), t.begin(), t.end());
         }
         HIPCHK(hipGetLastError());
-        c->part_x = dalloc<double>((size_t)kXinvParts * xinv_max_blocks(g));
-        c->part_z = dalloc<double>((size_t)zgreen_max_blocks(g));
-        c->part = dalloc<double>(8 * (size_t)kRedBlocks);
+        c->part_x = dalloc<double>((size_t)kMaxRhs * kXinvParts * xinv_max_blocks(g));
+        c->part_z = dalloc<double>((size_t)kMaxRhs * zgreen_max_blocks(g));
+        c->part = dalloc<double>((size_t)kMaxRhs * 4 * kRedBlocks);
         c->small = dalloc<double>(32);
         c->st = dalloc<MinresState>(1);
         c->cst = dalloc<CgState>(1);
-        c->rst = dalloc<CrState>(1);
+        c->rst = dalloc<CrState>(kMaxRhs);
         c->sync();
         *out = c.release();
     });
@@ -1656,10 +1939,47 @@ int fftperm_solve_direction(fftperm_ctx* ctx, const double grad[3], double tol, 
     });
 }
 
+// The three directions of a tensor from the right-hand sides in c->tb (guesses in c->txa where
+// has_x0) by the multi-RHS CR, then recovery and K (row-major, K[r][col] = mu <v_r>^(col)).
+void solve_tensor_batch(fftperm_ctx* c, const bool has_x0[3], double tol, uint64_t maxit, double K[9],
+                        fftperm_solve_info* info) {
+    const long L = c->L3();
+    const double* bptr[3];
+    double* xptr[3];
+    for (int b = 0; b < 3; ++b) {
+        bptr[b] = c->tb + b * L;
+        xptr[b] = c->txa + b * L;
+    }
+    MinresOut outs[3];
+    uint64_t updates[3];
+    device_cr32_batch(c, 3, bptr, xptr, has_x0, tol, maxit, outs, updates);
+    for (int col = 0; col < 3; ++col) {
+        double g[3] = {0.0, 0.0, 0.0};
+        g[col] = 1.0;
+        double avg[3];
+        device_recover(c, g, xptr[col], avg);
+        fill_info(info ? &info[col] : nullptr, outs[col], updates[col]);
+        for (int r = 0; r < 3; ++r) K[r * 3 + col] = c->mu * avg[r];
+    }
+}
+
 int fftperm_solve_tensor(fftperm_ctx* ctx, double tol, uint64_t maxit, double K[9], fftperm_solve_info* info) {
     return guarded([&] {
         check_ctx(ctx);
         if (!K) throw Err(FFTPERM_EINVAL, "null argument");
+        if (use_tensor_batch(ctx)) {
+            ctx->require_support();
+            ctx->ensure_vectors();
+            ctx->ensure_tensor_vectors();
+            for (int col = 0; col < 3; ++col) {
+                double g[3] = {0.0, 0.0, 0.0};
+                g[col] = 1.0;
+                device_rhs(ctx, g, ctx->tb + col * ctx->L3());
+            }
+            const bool none[3] = {false, false, false};
+            solve_tensor_batch(ctx, none, tol, maxit, K, info);
+            return;
+        }
         for (int col = 0; col < 3; ++col) {
             double g[3] = {0.0, 0.0, 0.0};
             g[col] = 1.0;
@@ -1670,6 +1990,14 @@ int fftperm_solve_tensor(fftperm_ctx* ctx, double tol, uint64_t maxit, double K[
                 K[r * 3 + col] = -colK;
             }
         }
+    });
+}
+
+int fftperm_set_tensor_batch(fftperm_ctx* ctx, int mode) {
+    return guarded([&] {
+        check_ctx(ctx);
+        if (mode < 0 || mode > 2) throw Err(FFTPERM_EINVAL, "tensor batch mode must be 0, 1 or 2");
+        ctx->tensor_batch = mode;
     });
 }
 
@@ -1758,6 +2086,7 @@ int fftperm_profile(fftperm_ctx* ctx, int enable) {
         for (int i = 0; i < 2; ++i) {
             ctx->prof_z_ms[i] = ctx->prof_apply_ms[i] = 0.0;
             ctx->prof_z_n[i] = ctx->prof_apply_n[i] = 0;
+            ctx->prof_rhs[i] = 0;
         }
     });
 }
@@ -1772,6 +2101,8 @@ int fftperm_profile_read(fftperm_ctx* ctx, fftperm_profile_stats* out) {
             out->zgreen_launches[i] = (uint64_t)ctx->prof_z_n[i];
             out->apply_ms[i] = ctx->prof_apply_ms[i];
             out->apply_launches[i] = (uint64_t)ctx->prof_apply_n[i];
+            out->zgreen_rhs[i] = (uint64_t)ctx->prof_rhs[i];
+            out->apply_rhs[i] = (uint64_t)ctx->prof_rhs[i];
         }
         out->N = (uint64_t)ctx->g.N;
         out->Nh = (uint64_t)ctx->g.Nh;
@@ -1859,33 +2190,39 @@ int fftperm_solve_tensor_warm(fftperm_ctx* ctx, fftperm_unet* net, double tol, u
         ctx->ensure_vectors();
         auto* u = reinterpret_cast<unet::Net*>(net);
         const long L = ctx->L3();
+        const bool batch = use_tensor_batch(ctx);
+        if (batch) ctx->ensure_tensor_vectors();
+        bool has_x0[3] = {false, false, false};
         for (int col = 0; col < 3; ++col) {
             double grad[3] = {0.0, 0.0, 0.0};
             grad[col] = 1.0;
-            device_rhs(ctx, grad, ctx->b);
+            double* bvec = batch ? ctx->tb + col * L : ctx->b;
+            device_rhs(ctx, grad, bvec);
             ctx->sync();
             // the network predicts the x-direction density of the axis-swapped medium
-            double* x0 = (ctx->precision == FFTPERM_PRECISION_MIXED) ? ctx->dx : ctx->tmp;
+            double* x0 = batch ? ctx->txa + col * L : (ctx->precision == FFTPERM_PRECISION_MIXED) ? ctx->dx : ctx->tmp;
             unet::predict(u, ctx->mask, g.nx, col);
             unet::gather_guess(u, col, ctx->active, (long)ctx->nb, x0);
             // the network fixes the shape of the density, the scale comes from the minimal-residual
             // step along it: x0 = alpha p, alpha = <b, A p> / ||A p||^2, so ||b - A x0|| <= ||b||;
             // the guess is kept only when it lowers the FP64 true residual below ||b||
-            const double bnorm = device_norm(ctx, ctx->b, L);
+            const double bnorm = device_norm(ctx, bvec, L);
             device_apply<double>(ctx, x0, ctx->r);
             auto dot = [&](const double* a, const double* bb) {
                 launch_dot_partials(a, bb, L, ctx->part, ctx->s);
                 launch_finish_sum(ctx->part, kRedBlocks, 1, 1.0, ctx->small + 16, nullptr, ctx->s);
                 return ctx->host_scalar(ctx->small + 16);
             };
-            const double bap = dot(ctx->b, ctx->r);
+            const double bap = dot(bvec, ctx->r);
             const double apap = dot(ctx->r, ctx->r);
             const double alpha = (apap > 0.0 && std::isfinite(apap)) ? bap / apap : 0.0;
             launch_axpby(alpha, x0, 0.0, x0, x0, L, ctx->s);
-            launch_axpby(1.0, ctx->b, -alpha, ctx->r, ctx->r, L, ctx->s);
+            launch_axpby(1.0, bvec, -alpha, ctx->r, ctx->r, L, ctx->s);
             const double rnorm = device_norm(ctx, ctx->r, L);
             const bool use = alpha != 0.0 && std::isfinite(rnorm) && rnorm < bnorm;
             if (used_guess) used_guess[col] = use ? 1 : 0;
+            has_x0[col] = use;
+            if (batch) continue;  // the three directions are solved together below
             uint64_t sweeps = 0;
             MinresOut o = run_minres(ctx, ctx->b, use ? x0 : nullptr, tol, maxit, &sweeps, nullptr);
             double avg[3];
@@ -1893,6 +2230,7 @@ int fftperm_solve_tensor_warm(fftperm_ctx* ctx, fftperm_unet* net, double tol, u
             fill_info(info ? &info[col] : nullptr, o, sweeps);
             for (int r = 0; r < 3; ++r) K[r * 3 + col] = ctx->mu * avg[r];
         }
+        if (batch) solve_tensor_batch(ctx, has_x0, tol, maxit, K, info);
     });
 }
 

@@ -122,6 +122,11 @@ long launch_xinv(const cx<R>* S, const XSink& sink, const Geometry3& g, const FF
 template <typename R>
 void launch_ypass(cx<R>* S, int sign, const Geometry3& g, const FFTTables<R>& t, const int* skip,
                   hipStream_t st);
+// The same over nrhs right-hand sides: side b's three components are spectrum components 3 b ..
+// 3 b + 2 (at (3 b + c) * Nc); skip[b * skip_stride] set skips side b.
+template <typename R>
+void launch_ypass_n(cx<R>* S, int sign, const Geometry3& g, const FFTTables<R>& t, int nrhs, const int* skip,
+                    int skip_stride, hipStream_t st);
 // z-axis transform without the Green multiply (standalone FFT hooks).
 template <typename R>
 void launch_zpass(cx<R>* S, int sign, const Geometry3& g, const FFTTables<R>& t, const int* skip,
@@ -135,6 +140,12 @@ void launch_zpass(cx<R>* S, int sign, const Geometry3& g, const FFTTables<R>& t,
 template <typename R>
 long launch_zgreen(cx<R>* S, const R* G6, const Geometry3& g, const FFTTables<R>& t, double* qpart,
                    const int* skip, hipStream_t st);
+// The same over nrhs right-hand sides laid out as for launch_ypass_n, side b's partials at
+// qpart + b * qstride; the tensor is read once per launch when the tuned kernel serves several
+// sides (else one launch per side).  Returns the workgroup count of one side.
+template <typename R>
+long launch_zgreen_batch(cx<R>* S, const R* G6, const Geometry3& g, const FFTTables<R>& t, double* qpart,
+                         long qstride, int nrhs, const int* skip, int skip_stride, hipStream_t st);
 // Upper bounds of the workgroup counts of launch_xinv / launch_zgreen over every variant.
 long xinv_max_blocks(const Geometry3& g);
 long zgreen_max_blocks(const Geometry3& g);
@@ -148,9 +159,10 @@ long static_xfwd(int v, const XSource& src, cx<R>* S, const Geometry3& g, const 
 template <typename R>
 long static_xinv(int v, const cx<R>* S, const XSink& sink, const Geometry3& g, const cx<R>* tw, const int* skip,
                  hipStream_t st);
+// ncomp components (grid rows); skip[(component / 3) * skip_stride] set skips a component's side.
 template <typename R>
 long static_col(int v, int n, int sign, cx<R>* S, long ncols, long inner, long outer, long estride, long comp_stride,
-                const cx<R>* tw, const int* skip, hipStream_t st);
+                const cx<R>* tw, const int* skip, hipStream_t st, int ncomp = 3, int skip_stride = 0);
 // 10 * lines per workgroup + threads per butterfly thread of variant v at length n (0 if none).
 int pass_variant_code(int n, int v);  // x passes (kXPassVariants)
 int col_variant_code(int n, int v);   // column passes (kColVariants)
@@ -221,6 +233,9 @@ template <typename R>
 void launch_hermitian_half(const double2* X, cx<R>* S, const Geometry3& g, hipStream_t st);
 // Pointwise spectral Green multiply over full spectra (apply_spectral_green_tensor hook).
 void launch_spectral_green_full(const double* T6, const double2* F, double2* V, long N, hipStream_t st);
+
+// Right-hand sides a multi-RHS solve carries at most (the three load directions of a tensor).
+constexpr int kMaxRhs = 3;
 
 // ------------------------------------------------------------ reductions
 constexpr int kRedBlocks = 4096;

@@ -109,21 +109,29 @@ constexpr int zgreen_threads() {
     return OS * 3 * ZL * ZShape<R, N>::T;
 }
 
+// Right-hand sides of one fused z-pass launch: side b's spectrum starts at b * rhs_stride, its
+// quadratic-form partials at b * qstride, and skip[b * skip_stride] set skips it.
+struct ZBatch {
+    int nrhs;
+    int skip_stride;
+    long rhs_stride;
+    long qstride;
+};
+constexpr ZBatch kOneRhs{1, 0, 0, 0};
+
 template <typename R, int N, int ZL, int OS>
 __global__ __launch_bounds__((zgreen_threads<R, N, ZL, OS>())) void k_zgreen_s(cx<R>* __restrict__ S,
                                                                                  const R* __restrict__ G6, long ncols,
                                                                                  long pitch, long Nc, int nx, int rp,
                                                                                  const cx<R>* __restrict__ twg,
                                                                                  double* __restrict__ qpart,
-                                                                                 const int* skip) {
-    if (skip != nullptr && *skip) return;
+                                                                                 const int* skip, ZBatch zb) {
     using SH = ZShape<R, N>;
     constexpr int LD = SH::LD, THREADS = zgreen_threads<R, N, ZL, OS>(), L3 = 3 * ZL;
     constexpr int EST = THREADS / L3;
     extern __shared__ unsigned char smem_raw[];
     cx<R>* tw = reinterpret_cast<cx<R>*>(smem_raw);
     cx<R>* buf = tw + N;
-    for (int i = threadIdx.x; i < N; i += THREADS) tw[i] = twg[i];
     const long col0 = xcd_tile(blockIdx.x, gridDim.x) * ZL;
     const int ncl = (int)min((long)ZL, ncols - col0);
     // loads: thread t serves line t % L3 = comp * ZL + ci, elements t / L3 + k * EST
@@ -133,74 +141,85 @@ __global__ __launch_bounds__((zgreen_threads<R, N, ZL, OS>())) void k_zgreen_s(c
     const bool own = ci < ncl;
     constexpr int KL = N / EST;
     static_assert(N % EST == 0, "column elements split evenly over the load threads");
-    const cx<R>* Sc = S + comp * Nc + col0 + ci;
-    cx<R> ld[KL];
-#pragma unroll
-    for (int k = 0; k < KL; ++k) {
-        ld[k] = mkc<R>((R)0, (R)0);
-        if (own) ld[k] = Sc[(long)(threadIdx.x / L3 + k * EST) * pitch];
-    }
-    // the Green tensor of the frequencies this thread multiplies, fetched now so that its latency
-    // overlaps the column loads and the forward transform
     constexpr int MT = THREADS / ZL;
     constexpr int KM = (N + MT - 1) / MT;
     const int mci = threadIdx.x % ZL;
     const bool mown = mci < ncl;
+    // the Green tensor of the frequencies this thread multiplies stays in registers for every
+    // right-hand side of the launch: one read of G per launch
+    bool gloaded = false;
     R gt[KM][6];
-    {
-        const R* Gc = G6 + 6L * (col0 + mci);
+    for (int b = 0; b < zb.nrhs; ++b) {
+        if (skip != nullptr && skip[b * zb.skip_stride]) continue;
+        cx<R>* Sb = S + b * zb.rhs_stride;
+        const cx<R>* Sc = Sb + comp * Nc + col0 + ci;
+        cx<R> ld[KL];
 #pragma unroll
-        for (int m = 0; m < KM; ++m) {
-            const int e = threadIdx.x / ZL + m * MT;
+        for (int k = 0; k < KL; ++k) {
+            ld[k] = mkc<R>((R)0, (R)0);
+            if (own) ld[k] = Sc[(long)(threadIdx.x / L3 + k * EST) * pitch];
+        }
+        if (!gloaded) {
+            // fetched after the first side's column loads are issued, so that its latency overlaps
+            // them and the forward transform; the twiddles go to LDS meanwhile
+            const R* Gc = G6 + 6L * (col0 + mci);
 #pragma unroll
-            for (int q = 0; q < 6; ++q) gt[m][q] = (R)0;
-            if (mown && e < N) {
-                const R* gp = Gc + 6L * e * pitch;
+            for (int m = 0; m < KM; ++m) {
+                const int e = threadIdx.x / ZL + m * MT;
 #pragma unroll
-                for (int q = 0; q < 6; ++q) gt[m][q] = gp[q];
+                for (int q = 0; q < 6; ++q) gt[m][q] = (R)0;
+                if (mown && e < N) {
+                    const R* gp = Gc + 6L * e * pitch;
+#pragma unroll
+                    for (int q = 0; q < 6; ++q) gt[m][q] = gp[q];
+                }
+            }
+            for (int i = threadIdx.x; i < N; i += THREADS) tw[i] = twg[i];
+            gloaded = true;
+        } else {
+            __syncthreads();  // the previous side's stores have read the buffer
+        }
+#pragma unroll
+        for (int k = 0; k < KL; ++k) buf[lt * LD + threadIdx.x / L3 + k * EST] = ld[k];
+        __syncthreads();
+        sfft_lines<R, -1, N, L3, SH::T, LD>(buf, tw, L3);
+        // symmetric Green multiply (src/bignonnet/GreenOperator.cpp:84-92 with G_sym), and the
+        // quadratic form conj(f) . G f of the forward spectrum when asked for
+        {
+            double q[1] = {0.0};
+#pragma unroll
+            for (int m = 0; m < KM; ++m) {
+                const int e = threadIdx.x / ZL + m * MT;
+                if (!mown || e >= N) continue;
+                const R txx = gt[m][0], txy = gt[m][1], txz = gt[m][2];
+                const R tyy = gt[m][3], tyz = gt[m][4], tzz = gt[m][5];
+                cx<R>* px = buf + (0 * ZL + mci) * LD + e;
+                cx<R>* py = buf + (1 * ZL + mci) * LD + e;
+                cx<R>* pz = buf + (2 * ZL + mci) * LD + e;
+                const cx<R> fx = *px, fy = *py, fz = *pz;
+                const cx<R> gx = mkc<R>(txx * fx.x + txy * fy.x + txz * fz.x, txx * fx.y + txy * fy.y + txz * fz.y);
+                const cx<R> gy = mkc<R>(txy * fx.x + tyy * fy.x + tyz * fz.x, txy * fx.y + tyy * fy.y + tyz * fz.y);
+                const cx<R> gz = mkc<R>(txz * fx.x + tyz * fy.x + tzz * fz.x, txz * fx.y + tyz * fy.y + tzz * fz.y);
+                *px = gx;
+                *py = gy;
+                *pz = gz;
+                if (qpart != nullptr)
+                    q[0] += (double)fx.x * gx.x + (double)fx.y * gx.y + (double)fy.x * gy.x + (double)fy.y * gy.y +
+                            (double)fz.x * gz.x + (double)fz.y * gz.y;
+            }
+            if (qpart != nullptr) {
+                q[0] *= herm_weight((int)((col0 + mci) % rp), nx);
+                block_sum<1>(q, qpart + b * zb.qstride + blockIdx.x);
             }
         }
-    }
+        __syncthreads();
+        sfft_lines<R, +1, N, L3, SH::T, LD>(buf, tw, L3);
+        cx<R>* Sw = Sb + comp * Nc + col0 + ci;
 #pragma unroll
-    for (int k = 0; k < KL; ++k) buf[lt * LD + threadIdx.x / L3 + k * EST] = ld[k];
-    __syncthreads();
-    sfft_lines<R, -1, N, L3, SH::T, LD>(buf, tw, L3);
-    // symmetric Green multiply (src/bignonnet/GreenOperator.cpp:84-92 with G_sym), and the
-    // quadratic form conj(f) . G f of the forward spectrum when asked for
-    {
-        double q[1] = {0.0};
-#pragma unroll
-        for (int m = 0; m < KM; ++m) {
-            const int e = threadIdx.x / ZL + m * MT;
-            if (!mown || e >= N) continue;
-            const R txx = gt[m][0], txy = gt[m][1], txz = gt[m][2];
-            const R tyy = gt[m][3], tyz = gt[m][4], tzz = gt[m][5];
-            cx<R>* px = buf + (0 * ZL + mci) * LD + e;
-            cx<R>* py = buf + (1 * ZL + mci) * LD + e;
-            cx<R>* pz = buf + (2 * ZL + mci) * LD + e;
-            const cx<R> fx = *px, fy = *py, fz = *pz;
-            const cx<R> gx = mkc<R>(txx * fx.x + txy * fy.x + txz * fz.x, txx * fx.y + txy * fy.y + txz * fz.y);
-            const cx<R> gy = mkc<R>(txy * fx.x + tyy * fy.x + tyz * fz.x, txy * fx.y + tyy * fy.y + tyz * fz.y);
-            const cx<R> gz = mkc<R>(txz * fx.x + tyz * fy.x + tzz * fz.x, txz * fx.y + tyz * fy.y + tzz * fz.y);
-            *px = gx;
-            *py = gy;
-            *pz = gz;
-            if (qpart != nullptr)
-                q[0] += (double)fx.x * gx.x + (double)fx.y * gx.y + (double)fy.x * gy.x + (double)fy.y * gy.y +
-                        (double)fz.x * gz.x + (double)fz.y * gz.y;
+        for (int k = 0; k < KL; ++k) {
+            const int e = threadIdx.x / L3 + k * EST;
+            if (own) Sw[(long)e * pitch] = buf[lt * LD + e];
         }
-        if (qpart != nullptr) {
-            q[0] *= herm_weight((int)((col0 + mci) % rp), nx);
-            block_sum<1>(q, qpart + blockIdx.x);
-        }
-    }
-    __syncthreads();
-    sfft_lines<R, +1, N, L3, SH::T, LD>(buf, tw, L3);
-    cx<R>* Sw = S + comp * Nc + col0 + ci;
-#pragma unroll
-    for (int k = 0; k < KL; ++k) {
-        const int e = threadIdx.x / L3 + k * EST;
-        if (own) Sw[(long)e * pitch] = buf[lt * LD + e];
     }
 }
 
@@ -558,7 +577,7 @@ constexpr size_t zgreen_lds() {
 
 template <typename R, int N, int ZL, int OS>
 long launch_zgreen_zl(cx<R>* S, const R* G6, long ncols, const Geometry3& g, const cx<R>* tw, double* qpart,
-                      const int* skip, hipStream_t st) {
+                      const int* skip, hipStream_t st, ZBatch zb = kOneRhs) {
     if constexpr (zgreen_threads<R, N, ZL, OS>() > 1024) {
         return -1;
     } else {
@@ -567,7 +586,7 @@ long launch_zgreen_zl(cx<R>* S, const R* G6, long ncols, const Geometry3& g, con
         const long blocks = (ncols + ZL - 1) / ZL;
         constexpr int threads = zgreen_threads<R, N, ZL, OS>();
         hipLaunchKernelGGL((k_zgreen_s<R, N, ZL, OS>), dim3((unsigned)blocks), dim3(threads), lds, st, S, G6, ncols,
-                           g.pz, g.Nc, g.nx, g.rp, tw, qpart, skip);
+                           g.pz, g.Nc, g.nx, g.rp, tw, qpart, skip, zb);
         return blocks;
     }
 }
@@ -578,20 +597,23 @@ long launch_zgreen_zl(cx<R>* S, const R* G6, long ncols, const Geometry3& g, con
 constexpr int kZgreenCodes[] = {11, 12, 21, 22, 41, 42, 81, 82, 161, 162, 44, 84, 164, 324, 45, 85, 165,
                                 86, 87, 88, 166, 167, 168, 326, 327};
 
+// zb.nrhs > 1: only the all-in-LDS kernel serves several right-hand sides in one launch; the
+// other codes return -1 for it.
 template <typename R, int N>
 long launch_zgreen_n(int code, cx<R>* S, const R* G6, long ncols, const Geometry3& g, const cx<R>* tw,
-                     double* qpart, const int* skip, hipStream_t st) {
+                     double* qpart, const int* skip, hipStream_t st, ZBatch zb = kOneRhs) {
+    if (zb.nrhs > 1 && code % 10 != 1 && code % 10 != 2) return -1;
     switch (code) {
-        case 11: return launch_zgreen_zl<R, N, 1, 1>(S, G6, ncols, g, tw, qpart, skip, st);
-        case 12: return launch_zgreen_zl<R, N, 1, 2>(S, G6, ncols, g, tw, qpart, skip, st);
-        case 21: return launch_zgreen_zl<R, N, 2, 1>(S, G6, ncols, g, tw, qpart, skip, st);
-        case 22: return launch_zgreen_zl<R, N, 2, 2>(S, G6, ncols, g, tw, qpart, skip, st);
-        case 41: return launch_zgreen_zl<R, N, 4, 1>(S, G6, ncols, g, tw, qpart, skip, st);
-        case 42: return launch_zgreen_zl<R, N, 4, 2>(S, G6, ncols, g, tw, qpart, skip, st);
-        case 81: return launch_zgreen_zl<R, N, 8, 1>(S, G6, ncols, g, tw, qpart, skip, st);
-        case 82: return launch_zgreen_zl<R, N, 8, 2>(S, G6, ncols, g, tw, qpart, skip, st);
-        case 161: return launch_zgreen_zl<R, N, 16, 1>(S, G6, ncols, g, tw, qpart, skip, st);
-        case 162: return launch_zgreen_zl<R, N, 16, 2>(S, G6, ncols, g, tw, qpart, skip, st);
+        case 11: return launch_zgreen_zl<R, N, 1, 1>(S, G6, ncols, g, tw, qpart, skip, st, zb);
+        case 12: return launch_zgreen_zl<R, N, 1, 2>(S, G6, ncols, g, tw, qpart, skip, st, zb);
+        case 21: return launch_zgreen_zl<R, N, 2, 1>(S, G6, ncols, g, tw, qpart, skip, st, zb);
+        case 22: return launch_zgreen_zl<R, N, 2, 2>(S, G6, ncols, g, tw, qpart, skip, st, zb);
+        case 41: return launch_zgreen_zl<R, N, 4, 1>(S, G6, ncols, g, tw, qpart, skip, st, zb);
+        case 42: return launch_zgreen_zl<R, N, 4, 2>(S, G6, ncols, g, tw, qpart, skip, st, zb);
+        case 81: return launch_zgreen_zl<R, N, 8, 1>(S, G6, ncols, g, tw, qpart, skip, st, zb);
+        case 82: return launch_zgreen_zl<R, N, 8, 2>(S, G6, ncols, g, tw, qpart, skip, st, zb);
+        case 161: return launch_zgreen_zl<R, N, 16, 1>(S, G6, ncols, g, tw, qpart, skip, st, zb);
+        case 162: return launch_zgreen_zl<R, N, 16, 2>(S, G6, ncols, g, tw, qpart, skip, st, zb);
         case 44: return launch_zgreen_r<R, N, 4>(S, G6, ncols, g, tw, qpart, skip, st);
         case 84: return launch_zgreen_r<R, N, 8>(S, G6, ncols, g, tw, qpart, skip, st);
         case 164: return launch_zgreen_r<R, N, 16>(S, G6, ncols, g, tw, qpart, skip, st);
@@ -613,13 +635,13 @@ long launch_zgreen_n(int code, cx<R>* S, const R* G6, long ncols, const Geometry
 
 template <typename R>
 long static_zgreen_code(int code, cx<R>* S, const R* G6, long ncols, const Geometry3& g, const cx<R>* tw,
-                        double* qpart, const int* skip, hipStream_t st) {
+                        double* qpart, const int* skip, hipStream_t st, ZBatch zb = kOneRhs) {
     switch (g.nz) {
-        case 32: return launch_zgreen_n<R, 32>(code, S, G6, ncols, g, tw, qpart, skip, st);
-        case 64: return launch_zgreen_n<R, 64>(code, S, G6, ncols, g, tw, qpart, skip, st);
-        case 128: return launch_zgreen_n<R, 128>(code, S, G6, ncols, g, tw, qpart, skip, st);
-        case 256: return launch_zgreen_n<R, 256>(code, S, G6, ncols, g, tw, qpart, skip, st);
-        case 512: return launch_zgreen_n<R, 512>(code, S, G6, ncols, g, tw, qpart, skip, st);
+        case 32: return launch_zgreen_n<R, 32>(code, S, G6, ncols, g, tw, qpart, skip, st, zb);
+        case 64: return launch_zgreen_n<R, 64>(code, S, G6, ncols, g, tw, qpart, skip, st, zb);
+        case 128: return launch_zgreen_n<R, 128>(code, S, G6, ncols, g, tw, qpart, skip, st, zb);
+        case 256: return launch_zgreen_n<R, 256>(code, S, G6, ncols, g, tw, qpart, skip, st, zb);
+        case 512: return launch_zgreen_n<R, 512>(code, S, G6, ncols, g, tw, qpart, skip, st, zb);
         default: return -1;
     }
 }
@@ -963,8 +985,8 @@ long launch_xinv(const cx<R>* S, const XSink& sink, const Geometry3& g, const FF
 template <typename R, int SIGN>
 __global__ void k_colpass(cx<R>* __restrict__ S, int n, long ncols, long inner, long outer_stride, long estride,
                           long comp_stride, LinePlan pl, const cx<R>* __restrict__ twg, int tl,
-                          const int* skip) {
-    if (skip != nullptr && *skip) return;
+                          const int* skip, int skip_stride) {
+    if (skip != nullptr && skip[(blockIdx.y / 3) * skip_stride]) return;
     extern __shared__ unsigned char smem_raw[];
     cx<R>* tw = reinterpret_cast<cx<R>*>(smem_raw);
     cx<R>* buf = tw + n;
@@ -995,22 +1017,30 @@ __global__ void k_colpass(cx<R>* __restrict__ S, int n, long ncols, long inner, 
 template <typename R>
 void launch_ypass(cx<R>* S, int sign, const Geometry3& g, const FFTTables<R>& t, const int* skip,
                   hipStream_t st) {
+    launch_ypass_n<R>(S, sign, g, t, 1, skip, 0, st);
+}
+
+template <typename R>
+void launch_ypass_n(cx<R>* S, int sign, const Geometry3& g, const FFTTables<R>& t, int nrhs, const int* skip,
+                    int skip_stride, hipStream_t st) {
     // column (kz, kx) of every padded row, kx < rp: base kz * pz + kx, elements ky * rp.  Covering
     // the row padding (zeros in, zeros out) keeps every tile of 8 columns on a 64-byte boundary.
+    // Component c of right-hand side b is spectrum component 3 b + c.
     const long key = (long)g.nz * g.nxh;
     const long ncols = (long)g.nz * g.rp;
+    const int ncomp = 3 * nrhs;
     if (static_passes_ok() &&
         static_col<R>(pass_variant_or_default(kKindCol, prec_of<R>(), g.ny, key), g.ny, sign, S, ncols, (long)g.rp,
-                      g.pz, (long)g.rp, g.Nc, t.twy, skip, st) >= 0)
+                      g.pz, (long)g.rp, g.Nc, t.twy, skip, st, ncomp, skip_stride) >= 0)
         return;
     PassCfg c = cfg_colpass(t.py, ncols);
-    dim3 grid((unsigned)c.blocks, 3);
+    dim3 grid((unsigned)c.blocks, (unsigned)ncomp);
     if (sign < 0)
         hipLaunchKernelGGL((k_colpass<R, -1>), grid, dim3(c.threads), c.lds, st, S, g.ny, ncols, (long)g.rp, g.pz,
-                           (long)g.rp, g.Nc, t.py, t.twy, c.lines, skip);
+                           (long)g.rp, g.Nc, t.py, t.twy, c.lines, skip, skip_stride);
     else
         hipLaunchKernelGGL((k_colpass<R, +1>), grid, dim3(c.threads), c.lds, st, S, g.ny, ncols, (long)g.rp, g.pz,
-                           (long)g.rp, g.Nc, t.py, t.twy, c.lines, skip);
+                           (long)g.rp, g.Nc, t.py, t.twy, c.lines, skip, skip_stride);
 }
 
 template <typename R>
@@ -1024,10 +1054,10 @@ void launch_zpass(cx<R>* S, int sign, const Geometry3& g, const FFTTables<R>& t,
     dim3 grid((unsigned)c.blocks, 3);
     if (sign < 0)
         hipLaunchKernelGGL((k_colpass<R, -1>), grid, dim3(c.threads), c.lds, st, S, g.nz, ncols, ncols, 0L, g.pz,
-                           g.Nc, t.pz, t.twz, c.lines, skip);
+                           g.Nc, t.pz, t.twz, c.lines, skip, 0);
     else
         hipLaunchKernelGGL((k_colpass<R, +1>), grid, dim3(c.threads), c.lds, st, S, g.nz, ncols, ncols, 0L, g.pz,
-                           g.Nc, t.pz, t.twz, c.lines, skip);
+                           g.Nc, t.pz, t.twz, c.lines, skip, 0);
 }
 
 // ------------------------------------------------------------- fused z + Green
@@ -1109,6 +1139,28 @@ long launch_zgreen(cx<R>* S, const R* G6, const Geometry3& g, const FFTTables<R>
     hipLaunchKernelGGL(k_zgreen<R>, dim3((unsigned)c.blocks), dim3(c.threads), c.lds, st, S, G6, g.nz, ncols, g.pz,
                        g.Nc, g.nx, g.rp, t.pz, t.twz, c.lines, qpart, skip);
     return c.blocks;
+}
+
+template <typename R>
+long launch_zgreen_batch(cx<R>* S, const R* G6, const Geometry3& g, const FFTTables<R>& t, double* qpart,
+                         long qstride, int nrhs, const int* skip, int skip_stride, hipStream_t st) {
+    if (nrhs == 1) return launch_zgreen<R>(S, G6, g, t, qpart, skip, st);
+    const long key = (long)g.ny * g.nxh;
+    const long ncols = (long)g.ny * g.rp;
+    const long rhs_stride = 3 * g.Nc;
+    if (static_passes_ok()) {
+        int code = tuned_variant(kKindZgreen, prec_of<R>(), g.nz, key);
+        if (code < 0) code = zgreen_default_code(g.nz, prec_of<R>());
+        const ZBatch zb{nrhs, skip_stride, rhs_stride, qstride};
+        const long b = static_zgreen_code<R>(code, S, G6, ncols, g, t.twz, qpart, skip, st, zb);
+        if (b >= 0) return b;
+    }
+    // the tuned kernel serves one side per launch: one launch per side
+    long blocks = 0;
+    for (int b = 0; b < nrhs; ++b)
+        blocks = launch_zgreen<R>(S + b * rhs_stride, G6, g, t, qpart != nullptr ? qpart + b * qstride : nullptr,
+                                  skip != nullptr ? skip + b * skip_stride : nullptr, st);
+    return blocks;
 }
 
 // ------------------------------------------------------- spectrum utilities
@@ -1199,6 +1251,10 @@ void launch_spectral_green_full(const double* T6, const double2* F, double2* V, 
                                   hipStream_t);                                                               \
     template long launch_zgreen<R>(cx<R>*, const R*, const Geometry3&, const FFTTables<R>&, double*,          \
                                    const int*, hipStream_t);                                                  \
+    template long launch_zgreen_batch<R>(cx<R>*, const R*, const Geometry3&, const FFTTables<R>&, double*,    \
+                                         long, int, const int*, int, hipStream_t);                            \
+    template void launch_ypass_n<R>(cx<R>*, int, const Geometry3&, const FFTTables<R>&, int, const int*, int, \
+                                    hipStream_t);                                                             \
     template void launch_expand_full<R>(const cx<R>*, double2*, const Geometry3&, hipStream_t);               \
     template void launch_hermitian_half<R>(const double2*, cx<R>*, const Geometry3&, hipStream_t);
 

@@ -499,8 +499,15 @@ void launch_cg_finish_mean_in(const double* partials, CgState* st, double inv_nb
 }
 
 // ------------------------------------------------------------ CR with reliable updates
-__global__ void k_cr_beta(const double* __restrict__ xpart, int nxparts, const double* __restrict__ zpart,
-                          int nzparts, CrState* st, double inv_nb, double inv_n) {
+// Batched launches serve nrhs independent right-hand sides at once: RHS b owns st[b] and the
+// partial rows at b * (its stride); the finish kernels take one workgroup per RHS, the vector
+// kernels one grid row (blockIdx.y) per RHS.  With nrhs = 1 they are the single-RHS kernels.
+__global__ void k_cr_beta(const double* __restrict__ xpart, int nxparts, long xstride,
+                          const double* __restrict__ zpart, int nzparts, long zstride, CrState* st, double inv_nb,
+                          double inv_n) {
+    st += blockIdx.x;
+    xpart += blockIdx.x * xstride;
+    zpart += blockIdx.x * zstride;
     if (st->done) return;
     double acc[4] = {0, 0, 0, 0};
     for (int p = threadIdx.x; p < nxparts; p += blockDim.x) {
@@ -531,7 +538,13 @@ __global__ void k_cr_beta(const double* __restrict__ xpart, int nxparts, const d
 
 void launch_cr_beta(const double* xpart, int nxparts, const double* zpart, int nzparts, CrState* st, double inv_nb,
                     double inv_n, hipStream_t s) {
-    hipLaunchKernelGGL(k_cr_beta, dim3(1), dim3(1024), 0, s, xpart, nxparts, zpart, nzparts, st, inv_nb, inv_n);
+    launch_cr_beta_n(xpart, nxparts, 0, zpart, nzparts, 0, st, 1, inv_nb, inv_n, s);
+}
+
+void launch_cr_beta_n(const double* xpart, int nxparts, long xstride, const double* zpart, int nzparts, long zstride,
+                      CrState* st, int nrhs, double inv_nb, double inv_n, hipStream_t s) {
+    hipLaunchKernelGGL(k_cr_beta, dim3(nrhs), dim3(1024), 0, s, xpart, nxparts, xstride, zpart, nzparts, zstride, st,
+                       inv_nb, inv_n);
 }
 
 __global__ __launch_bounds__(kFlatThreads) void k_cr_direction(double* __restrict__ p, double* __restrict__ ap,
@@ -579,13 +592,16 @@ __device__ __forceinline__ void cr_alpha_finish(const double* __restrict__ parti
     }
 }
 
-__global__ void k_cr_alpha(const double* __restrict__ partials, CrState* st) {
+__global__ void k_cr_alpha(const double* __restrict__ partials, long pstride, CrState* st) {
+    st += blockIdx.x;
     if (st->done) return;
-    cr_alpha_finish(partials, kRedBlocks, st);
+    cr_alpha_finish(partials + blockIdx.x * pstride, kRedBlocks, st);
 }
 
-void launch_cr_alpha(const double* partials, CrState* st, hipStream_t s) {
-    hipLaunchKernelGGL(k_cr_alpha, dim3(1), dim3(kRedThreads), 0, s, partials, st);
+void launch_cr_alpha(const double* partials, CrState* st, hipStream_t s) { launch_cr_alpha_n(partials, 0, st, 1, s); }
+
+void launch_cr_alpha_n(const double* partials, long pstride, CrState* st, int nrhs, hipStream_t s) {
+    hipLaunchKernelGGL(k_cr_alpha, dim3(nrhs), dim3(kRedThreads), 0, s, partials, pstride, st);
 }
 
 __global__ __launch_bounds__(kFlatThreads) void k_cr_update(double* __restrict__ x, double* __restrict__ r,
@@ -648,14 +664,22 @@ __device__ __forceinline__ void cr_check_finish(const double* __restrict__ parti
     if (rn > st->rnorm_max) st->rnorm_max = rn;
 }
 
-__global__ void k_cr_check(const double* __restrict__ partials, CrState* st, double* __restrict__ history,
-                           double inv_nb) {
+__global__ void k_cr_check(const double* __restrict__ partials, long pstride, CrState* st,
+                           double* __restrict__ history, long hstride, double inv_nb) {
+    st += blockIdx.x;
     if (st->done) return;
-    cr_check_finish(partials, kRedBlocks, st, history, inv_nb);
+    cr_check_finish(partials + blockIdx.x * pstride, kRedBlocks, st,
+                    history != nullptr ? history + blockIdx.x * hstride : nullptr, inv_nb);
 }
 
 void launch_cr_check(const double* partials, CrState* st, double* history, double inv_nb, hipStream_t s) {
-    hipLaunchKernelGGL(k_cr_check, dim3(1), dim3(kRedThreads), 0, s, partials, st, history, inv_nb);
+    launch_cr_check_n(partials, 0, st, history, 0, 1, inv_nb, s);
+}
+
+void launch_cr_check_n(const double* partials, long pstride, CrState* st, double* history, long hstride, int nrhs,
+                       double inv_nb, hipStream_t s) {
+    hipLaunchKernelGGL(k_cr_check, dim3(nrhs), dim3(kRedThreads), 0, s, partials, pstride, st, history, hstride,
+                       inv_nb);
 }
 
 // ------------------------------------------------------------ CR on FP32 vectors
@@ -666,7 +690,15 @@ constexpr int kVecThreads = 256;
 __global__ __launch_bounds__(kVecThreads) void k_cr32_direction(float* __restrict__ p, float* __restrict__ ap,
                                                                 const float* __restrict__ r,
                                                                 const float* __restrict__ q, CrState* st, long n,
-                                                                double* __restrict__ partials) {
+                                                                long vstride, double* __restrict__ partials,
+                                                                long pstride) {
+    const long voff = blockIdx.y * vstride;
+    p += voff;
+    ap += voff;
+    r += voff;
+    q += voff;
+    st += blockIdx.y;
+    partials += blockIdx.y * pstride;
     if (st->done) return;
     const double b = st->beta;
     const double m3[3] = {st->mean_out[0], st->mean_out[1], st->mean_out[2]};
@@ -706,14 +738,27 @@ __global__ __launch_bounds__(kVecThreads) void k_cr32_direction(float* __restric
 
 void launch_cr32_direction(float* p, float* ap, const float* r, const float* q, CrState* st, long nb,
                            double* partials, hipStream_t s) {
-    hipLaunchKernelGGL(k_cr32_direction, dim3(kRedBlocks), dim3(kVecThreads), 0, s, p, ap, r, q, st, 3 * nb,
-                       partials);
+    launch_cr32_direction_n(p, ap, r, q, st, nb, 0, partials, 0, 1, s);
+}
+
+void launch_cr32_direction_n(float* p, float* ap, const float* r, const float* q, CrState* st, long nb, long vstride,
+                             double* partials, long pstride, int nrhs, hipStream_t s) {
+    hipLaunchKernelGGL(k_cr32_direction, dim3(kRedBlocks, nrhs), dim3(kVecThreads), 0, s, p, ap, r, q, st, 3 * nb,
+                       vstride, partials, pstride);
 }
 
 __global__ __launch_bounds__(kVecThreads) void k_cr32_update(float* __restrict__ x, float* __restrict__ r,
                                                              const float* __restrict__ p,
                                                              const float* __restrict__ ap, CrState* st, long n,
-                                                             double* __restrict__ partials) {
+                                                             long vstride, double* __restrict__ partials,
+                                                             long pstride) {
+    const long voff = blockIdx.y * vstride;
+    x += voff;
+    r += voff;
+    p += voff;
+    ap += voff;
+    st += blockIdx.y;
+    partials += blockIdx.y * pstride;
     if (st->done) return;
     const double a = st->alpha;
     double rr = 0.0, sum[3] = {0.0, 0.0, 0.0};
@@ -758,7 +803,13 @@ __global__ __launch_bounds__(kVecThreads) void k_cr32_update(float* __restrict__
 
 void launch_cr32_update(float* x, float* r, const float* p, const float* ap, CrState* st, long nb, double* partials,
                         hipStream_t s) {
-    hipLaunchKernelGGL(k_cr32_update, dim3(kRedBlocks), dim3(kVecThreads), 0, s, x, r, p, ap, st, 3 * nb, partials);
+    launch_cr32_update_n(x, r, p, ap, st, nb, 0, partials, 0, 1, s);
+}
+
+void launch_cr32_update_n(float* x, float* r, const float* p, const float* ap, CrState* st, long nb, long vstride,
+                          double* partials, long pstride, int nrhs, hipStream_t s) {
+    hipLaunchKernelGGL(k_cr32_update, dim3(kRedBlocks, nrhs), dim3(kVecThreads), 0, s, x, r, p, ap, st, 3 * nb,
+                       vstride, partials, pstride);
 }
 
 __global__ void k_accumulate_f32(double* __restrict__ xacc, float* __restrict__ x32, long n) {

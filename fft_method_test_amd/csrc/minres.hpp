@@ -127,6 +127,18 @@ void launch_cr32_direction(float* p, float* ap, const float* r, const float* q, 
                            double* partials, hipStream_t s);
 void launch_cr32_update(float* x, float* r, const float* p, const float* ap, CrState* st, long nb, double* partials,
                         hipStream_t s);
+// Batched forms over nrhs right-hand sides: RHS b uses st[b], the vectors at b * vstride (a
+// multiple of 4 elements), the partial rows at b * pstride / xstride / zstride and the history at
+// b * hstride.  The single-RHS launchers above are the nrhs = 1 case.
+void launch_cr_beta_n(const double* xpart, int nxparts, long xstride, const double* zpart, int nzparts, long zstride,
+                      CrState* st, int nrhs, double inv_nb, double inv_n, hipStream_t s);
+void launch_cr_alpha_n(const double* partials, long pstride, CrState* st, int nrhs, hipStream_t s);
+void launch_cr_check_n(const double* partials, long pstride, CrState* st, double* history, long hstride, int nrhs,
+                       double inv_nb, hipStream_t s);
+void launch_cr32_direction_n(float* p, float* ap, const float* r, const float* q, CrState* st, long nb, long vstride,
+                             double* partials, long pstride, int nrhs, hipStream_t s);
+void launch_cr32_update_n(float* x, float* r, const float* p, const float* ap, CrState* st, long nb, long vstride,
+                          double* partials, long pstride, int nrhs, hipStream_t s);
 // xacc += x32, x32 = 0
 void launch_accumulate_f32(double* xacc, float* x32, long n, hipStream_t s);
 // out = (float) a

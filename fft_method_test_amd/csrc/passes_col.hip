@@ -5,7 +5,8 @@
 // Column col has base (col / inner) * outer_stride + col % inner and element e at
 // base + e * estride; a workgroup takes LINES consecutive columns, so each row of its tile is
 // LINES contiguous complex values in HBM.  Thread t always serves column t % LINES in the load
-// and store loops.
+// and store loops.  Grid row y transforms component y (at y * comp_stride); a launch over several
+// right-hand sides has 3 components per side and skips side b when skip[b * skip_stride] is set.
 #include "fft_static.hpp"
 #include "kernels.hpp"
 
@@ -18,8 +19,9 @@ __global__ __launch_bounds__((PassShape<R, N, LINES, OS>::THREADS)) void k_col_s
                                                                                long inner, long outer_stride,
                                                                                long estride, long comp_stride,
                                                                                const cx<R>* __restrict__ twg,
-                                                                               const int* skip) {
-    if (skip != nullptr && *skip) return;
+                                                                               const int* skip, int skip_stride) {
+    // grid row blockIdx.y = component; components 3b .. 3b + 2 belong to right-hand side b
+    if (skip != nullptr && skip[(blockIdx.y / 3) * skip_stride]) return;
     using SH = PassShape<R, N, LINES, OS>;
     constexpr int LD = SH::LD, THREADS = SH::THREADS;
     constexpr int EST = THREADS / LINES;
@@ -58,7 +60,7 @@ __global__ __launch_bounds__((PassShape<R, N, LINES, OS>::THREADS)) void k_col_s
 
 template <typename R, int N, int V>
 long col_variant(int sign, cx<R>* S, long ncols, long inner, long outer, long estride, long comp_stride,
-                 const cx<R>* tw, const int* skip, hipStream_t st) {
+                 const cx<R>* tw, const int* skip, int ncomp, int skip_stride, hipStream_t st) {
     constexpr int LINES = col_lines<N>(V), OS = col_os(V);
     using SH = PassShape<R, N, LINES, OS>;
     if constexpr (!SH::valid) {
@@ -66,28 +68,42 @@ long col_variant(int sign, cx<R>* S, long ncols, long inner, long outer, long es
     } else {
         if (SH::LDS > lds_limit()) return -1;
         const long blocks = (ncols + LINES - 1) / LINES;
-        const dim3 grid((unsigned)blocks, 3);
+        const dim3 grid((unsigned)blocks, (unsigned)ncomp);
         if (sign < 0)
             hipLaunchKernelGGL((k_col_s<R, N, LINES, OS, -1>), grid, dim3(SH::THREADS), SH::LDS, st, S, ncols, inner,
-                               outer, estride, comp_stride, tw, skip);
+                               outer, estride, comp_stride, tw, skip, skip_stride);
         else
             hipLaunchKernelGGL((k_col_s<R, N, LINES, OS, +1>), grid, dim3(SH::THREADS), SH::LDS, st, S, ncols, inner,
-                               outer, estride, comp_stride, tw, skip);
+                               outer, estride, comp_stride, tw, skip, skip_stride);
         return blocks;
     }
 }
 
 template <typename R, int N>
 long col_n(int v, int sign, cx<R>* S, long ncols, long inner, long outer, long estride, long comp_stride,
-           const cx<R>* tw, const int* skip, hipStream_t st) {
+           const cx<R>* tw, const int* skip, int ncomp, int skip_stride, hipStream_t st) {
     switch (v) {
-        case 0: return col_variant<R, N, 0>(sign, S, ncols, inner, outer, estride, comp_stride, tw, skip, st);
-        case 1: return col_variant<R, N, 1>(sign, S, ncols, inner, outer, estride, comp_stride, tw, skip, st);
-        case 2: return col_variant<R, N, 2>(sign, S, ncols, inner, outer, estride, comp_stride, tw, skip, st);
-        case 3: return col_variant<R, N, 3>(sign, S, ncols, inner, outer, estride, comp_stride, tw, skip, st);
-        case 4: return col_variant<R, N, 4>(sign, S, ncols, inner, outer, estride, comp_stride, tw, skip, st);
-        case 5: return col_variant<R, N, 5>(sign, S, ncols, inner, outer, estride, comp_stride, tw, skip, st);
-        case 6: return col_variant<R, N, 6>(sign, S, ncols, inner, outer, estride, comp_stride, tw, skip, st);
+        case 0:
+            return col_variant<R, N, 0>(sign, S, ncols, inner, outer, estride, comp_stride, tw, skip, ncomp,
+                                             skip_stride, st);
+        case 1:
+            return col_variant<R, N, 1>(sign, S, ncols, inner, outer, estride, comp_stride, tw, skip, ncomp,
+                                             skip_stride, st);
+        case 2:
+            return col_variant<R, N, 2>(sign, S, ncols, inner, outer, estride, comp_stride, tw, skip, ncomp,
+                                             skip_stride, st);
+        case 3:
+            return col_variant<R, N, 3>(sign, S, ncols, inner, outer, estride, comp_stride, tw, skip, ncomp,
+                                             skip_stride, st);
+        case 4:
+            return col_variant<R, N, 4>(sign, S, ncols, inner, outer, estride, comp_stride, tw, skip, ncomp,
+                                             skip_stride, st);
+        case 5:
+            return col_variant<R, N, 5>(sign, S, ncols, inner, outer, estride, comp_stride, tw, skip, ncomp,
+                                             skip_stride, st);
+        case 6:
+            return col_variant<R, N, 6>(sign, S, ncols, inner, outer, estride, comp_stride, tw, skip, ncomp,
+                                             skip_stride, st);
         default: return -1;
     }
 }
@@ -96,13 +112,23 @@ long col_n(int v, int sign, cx<R>* S, long ncols, long inner, long outer, long e
 
 template <typename R>
 long static_col(int v, int n, int sign, cx<R>* S, long ncols, long inner, long outer, long estride, long comp_stride,
-                const cx<R>* tw, const int* skip, hipStream_t st) {
+                const cx<R>* tw, const int* skip, hipStream_t st, int ncomp, int skip_stride) {
     switch (n) {
-        case 32: return col_n<R, 32>(v, sign, S, ncols, inner, outer, estride, comp_stride, tw, skip, st);
-        case 64: return col_n<R, 64>(v, sign, S, ncols, inner, outer, estride, comp_stride, tw, skip, st);
-        case 128: return col_n<R, 128>(v, sign, S, ncols, inner, outer, estride, comp_stride, tw, skip, st);
-        case 256: return col_n<R, 256>(v, sign, S, ncols, inner, outer, estride, comp_stride, tw, skip, st);
-        case 512: return col_n<R, 512>(v, sign, S, ncols, inner, outer, estride, comp_stride, tw, skip, st);
+        case 32:
+            return col_n<R, 32>(v, sign, S, ncols, inner, outer, estride, comp_stride, tw, skip, ncomp, skip_stride,
+                                    st);
+        case 64:
+            return col_n<R, 64>(v, sign, S, ncols, inner, outer, estride, comp_stride, tw, skip, ncomp, skip_stride,
+                                    st);
+        case 128:
+            return col_n<R, 128>(v, sign, S, ncols, inner, outer, estride, comp_stride, tw, skip, ncomp, skip_stride,
+                                    st);
+        case 256:
+            return col_n<R, 256>(v, sign, S, ncols, inner, outer, estride, comp_stride, tw, skip, ncomp, skip_stride,
+                                    st);
+        case 512:
+            return col_n<R, 512>(v, sign, S, ncols, inner, outer, estride, comp_stride, tw, skip, ncomp, skip_stride,
+                                    st);
         default: return -1;
     }
 }
@@ -125,8 +151,8 @@ int col_variant_code(int n, int v) {
 }
 
 template long static_col<double>(int, int, int, cx<double>*, long, long, long, long, long, const cx<double>*,
-                                 const int*, hipStream_t);
+                                 const int*, hipStream_t, int, int);
 template long static_col<float>(int, int, int, cx<float>*, long, long, long, long, long, const cx<float>*,
-                                const int*, hipStream_t);
+                                const int*, hipStream_t, int, int);
 
 }  // namespace fftperm

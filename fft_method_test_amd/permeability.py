@@ -135,6 +135,12 @@ class GreenOperator:
         reliable updates, or refined MINRES sweeps."""
         check(_lib.load().fftperm_set_mixed_solver(self._h, int(solver), float(reliable_delta)))
 
+    def set_tensor_batch(self, mode=1):
+        """Iterate the three load directions of a MIXED tensor solve in lockstep, each direction's
+        FFT passes back to back (1, the default) or one launch per pass for the three (2), or one
+        direction after the other (0); the results are identical."""
+        check(_lib.load().fftperm_set_tensor_batch(self._h, int(mode)))
+
     def close(self):
         if getattr(self, "_h", None) and self._h.value:
             _lib.load().fftperm_destroy(self._h)

@@ -174,8 +174,22 @@ int fftperm_solve_direction(fftperm_ctx* ctx, const double grad[3], double tol, 
                             const double* x0, double avg[3], double* u, fftperm_solve_info* info);
 
 /* PermeabilityRunner::run K assembly for gradient=all (src/app/PermeabilityRunner.cpp:740-777 with
- * src/post/PermeabilityTensor.cpp:11-18): K (row-major 9) = +mu <v_r>^(c).  info[3] may be NULL. */
+ * src/post/PermeabilityTensor.cpp:11-18): K (row-major 9) = +mu <v_r>^(c).  info[3] may be NULL.
+ * The reference solves the three load directions one after the other (:186-198); with the MIXED
+ * default solver (FFTPERM_MIXED_CR_FP32_VECTORS) this library iterates them together (see
+ * fftperm_set_tensor_batch), with identical results. */
 int fftperm_solve_tensor(fftperm_ctx* ctx, double tol, uint64_t maxit, double K[9], fftperm_solve_info* info);
+
+/* Multi-RHS tensor solves (no counterpart in the reference).  With the MIXED solver
+ * FFTPERM_MIXED_CR_FP32_VECTORS, fftperm_solve_tensor and fftperm_solve_tensor_warm run the three
+ * directions' CR iterations in lockstep: one launch of each vector and scalar kernel serves the
+ * three.  mode 1 (the default): each direction's five FFT passes run back to back (its spectrum
+ * stays in the last-level cache from one pass to the next); mode 2: one launch per pass for the
+ * three (one y launch for the nine spectrum components, one fused z launch reading the Green tensor
+ * once); mode 0: one direction after the other.  Every direction keeps its own recurrence, reliable
+ * updates and stopping test, so iterations, residuals and K are bit-identical in all modes.  Other
+ * precision modes and solvers ignore it.  mode outside 0 .. 2 -> FFTPERM_EINVAL. */
+int fftperm_set_tensor_batch(fftperm_ctx* ctx, int mode);
 
 /* Diagnostics::residual_norm / relative_symmetry_gap (src/post/Diagnostics.cpp:26-64), FP64. */
 int fftperm_residual_norm(fftperm_ctx* ctx, const double* x, const double* b, double* out);
@@ -183,8 +197,9 @@ int fftperm_symmetry_gap(fftperm_ctx* ctx, const double* x, const double* y, dou
 
 /* Live kernel timing (no counterpart in the reference; a measurement hook).  While enabled, the
  * context records HIP events on its own stream around every fused z-pass launch (z forward FFT,
- * Green multiply, z backward FFT) and around every five-sweep Green apply; MINRES applies that ran
- * after convergence (no-ops) are not counted.  fftperm_profile(ctx, 1) resets and enables. */
+ * Green multiply, z backward FFT) and around every five-sweep Green apply (of one right-hand side,
+ * or of all sides of a multi-RHS iteration); Krylov applies that ran after convergence (no-ops) are
+ * not counted.  fftperm_profile(ctx, 1) resets and enables. */
 typedef struct {
     double zgreen_ms[2];         /* [0] FP64, [1] FP32 (mixed) : summed durations */
     uint64_t zgreen_launches[2];
@@ -195,6 +210,8 @@ typedef struct {
     int xfwd_tile[2];            /* x-forward variant in use: 10*lines + threads per butterfly [FP64, FP32]; 0 = runtime kernel */
     int xinv_tile[2];            /* x-inverse variant in use, same code */
     int ypass_tile[2];           /* y-pass variant in use, same code */
+    uint64_t zgreen_rhs[2];      /* right-hand sides the counted z-pass launches transformed (a multi-RHS launch counts each side it served) */
+    uint64_t apply_rhs[2];       /* right-hand sides the counted applies served */
 } fftperm_profile_stats;
 
 /* Pin the tile variant a pass uses on this context's grid shape, process-wide (no counterpart in the

@@ -1,0 +1,133 @@
+"""GPU tests of the multi-RHS tensor solve (the three load directions iterated in lockstep by the
+MIXED CR solver on FP32 vectors).  Each direction keeps its own recurrence, reliable updates and
+stopping test, so the batched solve must reproduce the one-direction-at-a-time solve bit for bit;
+the sequential solve is itself checked against the FP64 oracle by the mixed-precision tests."""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import random_mask, rel
+
+pytestmark = pytest.mark.gpu
+
+NET_PATH = os.path.join(os.path.dirname(__file__), "..", "fft_method_test_amd", "data", "warmstart_unet.npz")
+
+
+def _green(shape, mask, mode):
+    from fft_method_test_amd import permeability as P
+
+    g = P.Grid3D(*shape)
+    green = P.GreenOperator(g, 1.0, 2, P.PRECISION_MIXED)
+    green.set_tensor_batch(mode)
+    P.ForceSupport(P.BinaryMedium(g, mask), P.SUPPORT_INTERFACE, green)
+    return green
+
+
+def _fields(infos):
+    return [(int(i.iterations), float(i.final_rel_residual), int(i.converged), int(i.refinement_sweeps))
+            for i in infos]
+
+
+def _media():
+    from fft_method_test_amd import geometry as G
+
+    return [
+        ("sphere32", (32, 32, 32), G.sphere_cell(32).reshape(-1)),
+        ("boolean64", (64, 64, 64), G.boolean_spheres(64, 4.0, 0.6, 17).reshape(-1)),
+        # lengths without specialised kernels: the generic column and z-pass kernels, one z launch
+        # per direction
+        ("random48x32x40", (48, 32, 40), random_mask((48, 32, 40), 0.65, seed=11)),
+    ]
+
+
+@pytest.mark.parametrize("case", range(3))
+@pytest.mark.parametrize("tol,maxit", [(1e-10, 20000), (1e-14, 25)])
+def test_tensor_batch_matches_sequential(fp, case, tol, maxit):
+    from fft_method_test_amd import permeability as P
+
+    name, shape, mask = _media()[case]
+    print(f"[tensor batch] {name} tol={tol:g} maxit={maxit}", flush=True)
+    res = {}
+    for mode in (0, 1, 2):
+        green = _green(shape, mask, mode)
+        K, infos = P.solve_permeability_tensor(green, tol, maxit)
+        res[mode] = (K, _fields(infos))
+        green.close()
+    Ks, fs = res[0]
+    for mode in (1, 2):
+        Kb, fb = res[mode]
+        assert fb == fs
+        assert np.array_equal(Kb, Ks)
+    if maxit == 25:
+        assert all(f[0] == 25 and f[2] == 0 for f in fs)
+    else:
+        assert all(f[2] == 1 for f in fs)
+
+
+def test_tensor_batch_repeat_and_other_solvers(fp, oracle_mod):
+    """A second batched solve on the same context reproduces the first (state, spectra and vectors
+    of the first do not leak); the switch leaves the other MIXED solvers on their own path; K of the
+    batched solve is within 1e-6 of the FP64 oracle."""
+    from fft_method_test_amd import _lib, geometry as G, permeability as P
+
+    mask = G.sphere_cell(32).reshape(-1)
+    green = _green((32, 32, 32), mask, 1)
+    K1, i1 = P.solve_permeability_tensor(green, 1e-10, 10000)
+    K2, i2 = P.solve_permeability_tensor(green, 1e-10, 10000)
+    assert np.array_equal(K1, K2) and _fields(i1) == _fields(i2)
+    o = oracle_mod.Oracle((32, 32, 32))
+    o.set_medium(mask)
+    Ko, _ = o.solve_tensor(1e-10, 10000)
+    assert rel(K1, Ko) < 1e-6
+    green.set_mixed_solver(_lib.MIXED_CR_RELIABLE, 0.1)
+    K3, i3 = P.solve_permeability_tensor(green, 1e-10, 10000)
+    assert all(i.converged for i in i3) and rel(K3, Ko) < 1e-6
+    with pytest.raises(ValueError, match="tensor batch mode"):
+        green.set_tensor_batch(3)
+    green.close()
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_tensor_batch_profile_counts(fp, mode):
+    """Live timing of the batched solve: every FP32 apply of every direction is counted once
+    (a lockstep launch counts the directions it served)."""
+    import ctypes as C
+
+    from fft_method_test_amd import _lib, geometry as G, permeability as P
+
+    mask = G.boolean_spheres(64, 4.0, 0.6, 17).reshape(-1)
+    green = _green((64, 64, 64), mask, mode)
+    L = _lib.load()
+    _lib.check(L.fftperm_profile(green._h, 1))
+    K, infos = P.solve_permeability_tensor(green, 1e-8, 20000)
+    pr = _lib.ProfileStats()
+    _lib.check(L.fftperm_profile_read(green._h, C.byref(pr)))
+    its = sum(int(i.iterations) for i in infos)
+    assert pr.apply_rhs[1] == its
+    assert pr.zgreen_rhs[1] == its
+    if mode == 2:  # one record per lockstep iteration in which some direction progressed
+        assert max(int(i.iterations) for i in infos) <= pr.zgreen_launches[1] <= its
+    else:  # one record per direction and iteration
+        assert pr.zgreen_launches[1] == its
+    assert pr.apply_launches[0] == pr.apply_rhs[0] > 0  # FP64 true residuals, one side each
+    green.close()
+
+
+@pytest.mark.skipif(not os.path.exists(NET_PATH), reason="no trained warm-start network")
+def test_tensor_batch_warm_matches_sequential(fp):
+    from fft_method_test_amd import geometry as G, warmstart as W
+
+    mask = G.boolean_spheres(64, 4.0, 0.6, 23).reshape(-1)
+    net = W.WarmStartNet(0)
+    net.load(NET_PATH)
+    res = {}
+    for mode in (0, 1):
+        green = _green((64, 64, 64), mask, mode)
+        K, infos, used = W.solve_tensor_warm(green, net, 1e-8, 20000)
+        res[mode] = (K, _fields(infos), used)
+        green.close()
+    net.close()
+    assert res[1][2] == res[0][2]
+    assert res[1][1] == res[0][1]
+    assert np.array_equal(res[1][0], res[0][0])
