@@ -35,6 +35,7 @@ EXPORTED_SYMBOLS = (
     "fftperm_profile", "fftperm_profile_read", "fftperm_set_graphs", "fftperm_unet_create", "fftperm_unet_destroy",
     "fftperm_unet_num_params", "fftperm_unet_get_params", "fftperm_unet_set_params", "fftperm_unet_loss_grad",
     "fftperm_unet_predict", "fftperm_solve_tensor_warm", "fftperm_brinkman_solve", "fftperm_set_tensor_batch",
+    "fftperm_time_apply",
 )
 
 
@@ -103,6 +104,7 @@ def load():
     L.fftperm_pin_variant.argtypes = [_vp, C.c_int, C.c_int, C.c_int]
     L.fftperm_set_graphs.argtypes = [_vp, C.c_int]
     L.fftperm_set_tensor_batch.argtypes = [_vp, C.c_int]
+    L.fftperm_time_apply.argtypes = [_vp, C.c_int, C.c_int, C.POINTER(C.c_double)]
     L.fftperm_recover.argtypes = [_vp, _dp, _dp, _vp, _dp]
     L.fftperm_solve_direction.argtypes = [_vp, _dp, C.c_double, C.c_uint64, _vp, _dp, _vp, C.POINTER(SolveInfo)]
     L.fftperm_solve_tensor.argtypes = [_vp, C.c_double, C.c_uint64, _dp, _vp]

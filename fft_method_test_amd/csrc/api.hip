@@ -8,8 +8,11 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
+#include <tuple>
+#include <set>
 #include <vector>
 
 #include "../../include/fftperm.h"
@@ -1487,6 +1490,92 @@ void tune_x_passes(fftperm_ctx* c) {
     c->sync();
 }
 
+// Second tuning stage, once per grid shape and precision in a process, on the first medium: the
+// standalone timings above see each pass alone on a buffer it has just touched, while in the
+// solver a pass reads what the previous pass left in the last-level cache.  So every launchable
+// variant of each pass kind is timed again inside back-to-back five-sweep applies of the solver's
+// own form (FP32 compact source and sink for the FP32 operator, Parseval partials on), one kind
+// after the other with the others at their current choice, best of two interleaved rounds.
+std::mutex g_chain_mu;
+std::set<std::tuple<int, int, int, int>> g_chain_tuned;
+
+// Milliseconds per five-sweep apply of the solver's form (FP32 compact source and sink for the FP32
+// operator, FP64 for FP64; Parseval partials on) over reps back-to-back applies on a zero input,
+// after one untimed apply.
+template <typename R>
+float time_chain(fftperm_ctx* c, int reps) {
+    const long L = c->L3();
+    c->ensure_vectors();
+    XSource sr = src_none(c);
+    XSink sk = sink_none(c);
+    sk.partials = c->part_x;
+    if (sizeof(R) == 4) {
+        c->ensure_vectors32();
+        HIPCHK(hipMemsetAsync(c->x32, 0, L * sizeof(float), c->s));
+        sr.compact32 = c->x32;
+        sk.compact32 = c->q32;
+    } else {
+        HIPCHK(hipMemsetAsync(c->x, 0, L * sizeof(double), c->s));
+        sr.compact = c->x;
+        sk.compact = c->tmp;
+    }
+    hipEvent_t a = c->take_event(), b = c->take_event();
+    c->chain<R>(sr, sk, nullptr, c->part_z);
+    HIPCHK(hipEventRecord(a, c->s));
+    for (int rep = 0; rep < reps; ++rep) c->chain<R>(sr, sk, nullptr, c->part_z);
+    HIPCHK(hipEventRecord(b, c->s));
+    HIPCHK(hipEventSynchronize(b));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, a, b));
+    c->ev_free.push_back(a);
+    c->ev_free.push_back(b);
+    return ms / (float)std::max(1, reps);
+}
+
+template <typename R>
+void tune_chain_prec(fftperm_ctx* c) {
+    const Geometry3& g = c->g;
+    const int prec = sizeof(R) == 4 ? 1 : 0;
+    {
+        std::lock_guard<std::mutex> lk(g_chain_mu);
+        if (!g_chain_tuned.insert(std::make_tuple(g.nx, g.ny, g.nz, prec)).second) return;
+    }
+    struct Key {
+        int kind, n;
+        long count;
+    };
+    const Key keys[4] = {{kKindZgreen, g.nz, (long)g.ny * g.nxh},
+                         {kKindCol, g.ny, (long)g.nz * g.nxh},
+                         {kKindXfwd, g.nx, (long)g.ny * g.nz},
+                         {kKindXinv, g.nx, (long)g.ny * g.nz}};
+    constexpr int kRounds = 2, kReps = 2;
+    for (const Key& k : keys) {
+        const std::vector<int> cands = tuned_candidates(k.kind, prec, k.n, k.count);
+        if (cands.size() < 2) continue;
+        std::vector<float> best_of(cands.size(), 1e30f);
+        for (int round = 0; round < kRounds; ++round) {
+            for (size_t i = 0; i < cands.size(); ++i) {
+                set_tuned_variant(k.kind, prec, k.n, k.count, cands[i]);
+                best_of[i] = std::min(best_of[i], time_chain<R>(c, kReps));
+            }
+        }
+        size_t best = 0;
+        for (size_t i = 1; i < cands.size(); ++i)
+            if (best_of[i] < best_of[best]) best = i;
+        set_tuned_variant(k.kind, prec, k.n, k.count, cands[best]);
+    }
+    c->sync();
+}
+
+void tune_chain(fftperm_ctx* c) {
+    if (!static_passes_ok() || c->nb == 0) return;
+    const bool prof = c->prof;
+    c->prof = false;  // the tuning applies are not the solver's
+    tune_chain_prec<double>(c);
+    if (c->precision == FFTPERM_PRECISION_MIXED) tune_chain_prec<float>(c);
+    c->prof = prof;
+}
+
 void set_medium_impl(fftperm_ctx* c, const uint8_t* mask, bool host, int mode, uint64_t* nb_out) {
     if (mode != FFTPERM_SUPPORT_INTERFACE && mode != FFTPERM_SUPPORT_FULLSOLID)
         throw Err(FFTPERM_EINVAL, "unknown support mode");
@@ -1513,7 +1602,10 @@ void set_medium_impl(fftperm_ctx* c, const uint8_t* mask, bool host, int mode, u
     c->nf = cnt[1];
     c->has_medium = true;
     if (nb_out) *nb_out = c->nb;
-    if (c->nb > 0) tune_x_passes(c);
+    if (c->nb > 0) {
+        tune_x_passes(c);
+        tune_chain(c);
+    }
 }
 
 }  // namespace
@@ -2075,6 +2167,25 @@ int fftperm_set_graphs(fftperm_ctx* ctx, int enable) {
     return guarded([&] {
         check_ctx(ctx);
         ctx->use_graphs = enable != 0;
+    });
+}
+
+int fftperm_time_apply(fftperm_ctx* ctx, int precision, int reps, double* ms_per_apply) {
+    return guarded([&] {
+        check_ctx(ctx);
+        if (!ms_per_apply) throw Err(FFTPERM_EINVAL, "null output");
+        if (reps < 1) throw Err(FFTPERM_EINVAL, "reps must be >= 1");
+        if (precision != FFTPERM_PRECISION_FP64 && precision != FFTPERM_PRECISION_MIXED)
+            throw Err(FFTPERM_EINVAL, "unknown precision mode");
+        if (precision == FFTPERM_PRECISION_MIXED && ctx->precision != FFTPERM_PRECISION_MIXED)
+            throw Err(FFTPERM_EINVAL, "the FP32 operator needs a MIXED context");
+        ctx->require_support();
+        const bool prof = ctx->prof;
+        ctx->prof = false;
+        const float ms = precision == FFTPERM_PRECISION_MIXED ? time_chain<float>(ctx, reps)
+                                                               : time_chain<double>(ctx, reps);
+        ctx->prof = prof;
+        *ms_per_apply = (double)ms;
     });
 }
 

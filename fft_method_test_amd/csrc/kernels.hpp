@@ -174,6 +174,9 @@ enum PassKind { kKindXfwd = 0, kKindXinv = 1, kKindCol = 2, kKindZgreen = 3 };
 int tuned_variant(int kind, int prec, int n, long count);
 void set_tuned_variant(int kind, int prec, int n, long count, int v);
 void clear_tuned_variant(int kind, int prec, int n, long count);
+// Registry values of the variants the autotuner could launch for this key (empty: not tuned in
+// this process, or no specialised kernel).
+std::vector<int> tuned_candidates(int kind, int prec, int n, long count);
 // Incremented by every change of the registry (captured launch sequences depend on it).
 long tuning_generation();
 // Registry value of a report code (pass variant index or z-pass code), -1 when unknown.

@@ -125,7 +125,106 @@ __global__ __launch_bounds__((zgreen_threads<R, N, ZL, OS>())) void k_zgreen_s(c
                                                                                  long pitch, long Nc, int nx, int rp,
                                                                                  const cx<R>* __restrict__ twg,
                                                                                  double* __restrict__ qpart,
-                                                                                 const int* skip, ZBatch zb) {
+                                                                                 const int* skip) {
+    if (skip != nullptr && *skip) return;
+    using SH = ZShape<R, N>;
+    constexpr int LD = SH::LD, THREADS = zgreen_threads<R, N, ZL, OS>(), L3 = 3 * ZL;
+    constexpr int EST = THREADS / L3;
+    extern __shared__ unsigned char smem_raw[];
+    cx<R>* tw = reinterpret_cast<cx<R>*>(smem_raw);
+    cx<R>* buf = tw + N;
+    for (int i = threadIdx.x; i < N; i += THREADS) tw[i] = twg[i];
+    const long col0 = xcd_tile(blockIdx.x, gridDim.x) * ZL;
+    const int ncl = (int)min((long)ZL, ncols - col0);
+    // loads: thread t serves line t % L3 = comp * ZL + ci, elements t / L3 + k * EST
+    const int lt = threadIdx.x % L3;
+    const int comp = lt / ZL;
+    const int ci = lt % ZL;
+    const bool own = ci < ncl;
+    constexpr int KL = N / EST;
+    static_assert(N % EST == 0, "column elements split evenly over the load threads");
+    const cx<R>* Sc = S + comp * Nc + col0 + ci;
+    cx<R> ld[KL];
+#pragma unroll
+    for (int k = 0; k < KL; ++k) {
+        ld[k] = mkc<R>((R)0, (R)0);
+        if (own) ld[k] = Sc[(long)(threadIdx.x / L3 + k * EST) * pitch];
+    }
+    // the Green tensor of the frequencies this thread multiplies, fetched now so that its latency
+    // overlaps the column loads and the forward transform
+    constexpr int MT = THREADS / ZL;
+    constexpr int KM = (N + MT - 1) / MT;
+    const int mci = threadIdx.x % ZL;
+    const bool mown = mci < ncl;
+    R gt[KM][6];
+    {
+        const R* Gc = G6 + 6L * (col0 + mci);
+#pragma unroll
+        for (int m = 0; m < KM; ++m) {
+            const int e = threadIdx.x / ZL + m * MT;
+#pragma unroll
+            for (int q = 0; q < 6; ++q) gt[m][q] = (R)0;
+            if (mown && e < N) {
+                const R* gp = Gc + 6L * e * pitch;
+#pragma unroll
+                for (int q = 0; q < 6; ++q) gt[m][q] = gp[q];
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < KL; ++k) buf[lt * LD + threadIdx.x / L3 + k * EST] = ld[k];
+    __syncthreads();
+    sfft_lines<R, -1, N, L3, SH::T, LD>(buf, tw, L3);
+    // symmetric Green multiply (src/bignonnet/GreenOperator.cpp:84-92 with G_sym), and the
+    // quadratic form conj(f) . G f of the forward spectrum when asked for
+    {
+        double q[1] = {0.0};
+#pragma unroll
+        for (int m = 0; m < KM; ++m) {
+            const int e = threadIdx.x / ZL + m * MT;
+            if (!mown || e >= N) continue;
+            const R txx = gt[m][0], txy = gt[m][1], txz = gt[m][2];
+            const R tyy = gt[m][3], tyz = gt[m][4], tzz = gt[m][5];
+            cx<R>* px = buf + (0 * ZL + mci) * LD + e;
+            cx<R>* py = buf + (1 * ZL + mci) * LD + e;
+            cx<R>* pz = buf + (2 * ZL + mci) * LD + e;
+            const cx<R> fx = *px, fy = *py, fz = *pz;
+            const cx<R> gx = mkc<R>(txx * fx.x + txy * fy.x + txz * fz.x, txx * fx.y + txy * fy.y + txz * fz.y);
+            const cx<R> gy = mkc<R>(txy * fx.x + tyy * fy.x + tyz * fz.x, txy * fx.y + tyy * fy.y + tyz * fz.y);
+            const cx<R> gz = mkc<R>(txz * fx.x + tyz * fy.x + tzz * fz.x, txz * fx.y + tyz * fy.y + tzz * fz.y);
+            *px = gx;
+            *py = gy;
+            *pz = gz;
+            if (qpart != nullptr)
+                q[0] += (double)fx.x * gx.x + (double)fx.y * gx.y + (double)fy.x * gy.x + (double)fy.y * gy.y +
+                        (double)fz.x * gz.x + (double)fz.y * gz.y;
+        }
+        if (qpart != nullptr) {
+            q[0] *= herm_weight((int)((col0 + mci) % rp), nx);
+            block_sum<1>(q, qpart + blockIdx.x);
+        }
+    }
+    __syncthreads();
+    sfft_lines<R, +1, N, L3, SH::T, LD>(buf, tw, L3);
+    cx<R>* Sw = S + comp * Nc + col0 + ci;
+#pragma unroll
+    for (int k = 0; k < KL; ++k) {
+        const int e = threadIdx.x / L3 + k * EST;
+        if (own) Sw[(long)e * pitch] = buf[lt * LD + e];
+    }
+}
+
+// Several right-hand sides per launch (the multi-RHS solve's one-launch z pass): the tile's Green
+// tensor stays in registers across the sides, read once per launch.  A separate kernel: holding
+// the tensor across the loop raises the register count (72 -> 118 VGPRs at 256, FP32, 8 x 2), which
+// would cost the single-side kernel half its occupancy.
+template <typename R, int N, int ZL, int OS>
+__global__ __launch_bounds__((zgreen_threads<R, N, ZL, OS>())) void k_zgreen_sb(cx<R>* __restrict__ S,
+                                                                                  const R* __restrict__ G6, long ncols,
+                                                                                  long pitch, long Nc, int nx, int rp,
+                                                                                  const cx<R>* __restrict__ twg,
+                                                                                  double* __restrict__ qpart,
+                                                                                  const int* skip, ZBatch zb) {
     using SH = ZShape<R, N>;
     constexpr int LD = SH::LD, THREADS = zgreen_threads<R, N, ZL, OS>(), L3 = 3 * ZL;
     constexpr int EST = THREADS / L3;
@@ -585,8 +684,15 @@ long launch_zgreen_zl(cx<R>* S, const R* G6, long ncols, const Geometry3& g, con
         if (lds > g_lds_max) return -1;
         const long blocks = (ncols + ZL - 1) / ZL;
         constexpr int threads = zgreen_threads<R, N, ZL, OS>();
-        hipLaunchKernelGGL((k_zgreen_s<R, N, ZL, OS>), dim3((unsigned)blocks), dim3(threads), lds, st, S, G6, ncols,
-                           g.pz, g.Nc, g.nx, g.rp, tw, qpart, skip, zb);
+        if (zb.nrhs == 1) {
+            hipLaunchKernelGGL((k_zgreen_s<R, N, ZL, OS>), dim3((unsigned)blocks), dim3(threads), lds, st, S, G6,
+                               ncols, g.pz, g.Nc, g.nx, g.rp, tw, qpart, skip);
+        } else if constexpr (sizeof(R) == 4) {  // the multi-RHS solve runs the FP32 operator
+            hipLaunchKernelGGL((k_zgreen_sb<R, N, ZL, OS>), dim3((unsigned)blocks), dim3(threads), lds, st, S, G6,
+                               ncols, g.pz, g.Nc, g.nx, g.rp, tw, qpart, skip, zb);
+        } else {
+            return -1;
+        }
         return blocks;
     }
 }
@@ -660,6 +766,7 @@ int zgreen_default_code(int n, int prec) {
 
 std::mutex g_tune_mu;
 std::map<std::tuple<int, int, int, long>, int> g_tuned;
+std::map<std::tuple<int, int, int, long>, std::vector<int>> g_live;  // variants that launched
 long g_tune_gen = 0;
 
 template <typename R>
@@ -716,6 +823,10 @@ std::vector<std::pair<int, float>> tune_candidates(int kind, int prec, int n, lo
         if (best_of[i] < best_of[best]) best = i;
     }
     set_tuned_variant(kind, prec, n, count, live[best]);
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_live[std::make_tuple(kind, prec, n, count)] = live;
+    }
     return timings;
 }
 
@@ -748,6 +859,12 @@ void set_tuned_variant(int kind, int prec, int n, long count, int v) {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     g_tuned[std::make_tuple(kind, prec, n, count)] = v;
     ++g_tune_gen;
+}
+
+std::vector<int> tuned_candidates(int kind, int prec, int n, long count) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_live.find(std::make_tuple(kind, prec, n, count));
+    return it == g_live.end() ? std::vector<int>() : it->second;
 }
 
 long tuning_generation() {
