@@ -161,16 +161,22 @@ struct PassShape {
 // of LINES real lines (x + i y of every line, z of line pairs) share one transform of
 // LT = 3 LINES / 2 complex lines, so a tile has one load phase, one transform and one store phase.
 // LINES = the base count, halved, quartered; OS = 1, 2, 4 keeps 1.5 * base * T threads.
-constexpr int kXPassVariants = kPassVariants + 3;
+// The last x variant is the wave-per-line-pair pass (passes_x.hip): 8 lines per workgroup, one
+// wave per line pair, code 88.
+constexpr int kXPassVariants = kPassVariants + 4;
 
-constexpr bool xpass_merged(int v) { return v >= kPassVariants; }
+constexpr bool xpass_wave(int v) { return v == kPassVariants + 3; }
+constexpr bool xpass_merged(int v) { return v >= kPassVariants && !xpass_wave(v); }
 
 template <int N>
 constexpr int xpass_lines(int v) {
-    return v < kPassVariants ? pass_lines<N>(v) : (pass_base_lines<N>() >> (v - kPassVariants));
+    return xpass_wave(v) ? 8
+                         : (v < kPassVariants ? pass_lines<N>(v) : (pass_base_lines<N>() >> (v - kPassVariants)));
 }
 
-constexpr int xpass_os(int v) { return v < kPassVariants ? pass_os(v) : (1 << (v - kPassVariants)); }
+constexpr int xpass_os(int v) {
+    return xpass_wave(v) ? 8 : (v < kPassVariants ? pass_os(v) : (1 << (v - kPassVariants)));
+}
 
 template <typename R, int N, int LINES, int OS>
 struct MergedShape {

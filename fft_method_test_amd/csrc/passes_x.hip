@@ -555,13 +555,312 @@ __global__ __launch_bounds__((MergedShape<R, N, LINES, OS>::THREADS)) void k_xin
     if (sink.partials != nullptr) block_sum<kXinvParts>(acc, sink.partials + (long)kXinvParts * blockIdx.x);
 }
 
+// ------------------------------------------------------- wave-per-line-pair x passes
+// One wave transforms the three complex lines of an x-line pair (x + i y of each line, then z of
+// the pair) one after the other.  Lane j holds elements j + 64 r (r < RAD = N / 64) in registers
+// and the radix-RAD Stockham stages (N = RAD^S: 128 = 2^7, 256 = 4^4, 512 = 8^3) exchange through
+// a per-wave LDS line, so after the twiddle table is loaded the waves never wait for each other.
+// The support of a line is read as uniform words (one x line per wave at a time) and the compact
+// vector is gathered straight into registers.
+constexpr int kWaveLines = 8;  // x lines per workgroup: 4 waves x 2
+
+template <int N>
+constexpr bool wave_pass_ok() {
+    return N == 128 || N == 256 || N == 512;
+}
+
+// Orders the wave's LDS writes before its later reads (a wave executes its LDS operations in
+// order; the fences keep the compiler from moving them across).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <typename R, int SIGN, int N, int NS>
+__device__ __forceinline__ void wave_fft_stage(cx<R> (&v)[N / 64], cx<R>* L, const cx<R>* tw, int lane) {
+    constexpr int RAD = N / 64;
+    if constexpr (NS < N) {
+        const int k = lane % NS;
+        if constexpr (NS > 1) {
+#pragma unroll
+            for (int r = 0; r < RAD; ++r) v[r] = L[lane + r * 64];
+            constexpr int TS = N / (NS * RAD);
+#pragma unroll
+            for (int r = 1; r < RAD; ++r) {
+                cx<R> w = tw[r * k * TS];
+                if (SIGN > 0) w.y = -w.y;
+                v[r] = cmul<R>(v[r], w);
+            }
+            wave_lds_sync();
+        }
+        dft_small<R, SIGN, RAD>(v);
+        const int base = (lane / NS) * NS * RAD + k;
+#pragma unroll
+        for (int r = 0; r < RAD; ++r) L[base + r * NS] = v[r];
+        wave_lds_sync();
+        wave_fft_stage<R, SIGN, N, NS * RAD>(v, L, tw, lane);
+    }
+}
+
+// Transform of one line given as v[r] = x[lane + 64 r]; the spectrum is left in L in natural order.
+template <typename R, int SIGN, int N>
+__device__ __forceinline__ void wave_fft(cx<R> (&v)[N / 64], cx<R>* L, const cx<R>* tw, int lane) {
+    wave_fft_stage<R, SIGN, N, 1>(v, L, tw, lane);
+}
+
+// Support words of x line `line` (uniform across the wave) and their in-line prefix counts;
+// loc(r) = support index of voxel lane + 64 r, or -1.
+template <int N>
+struct WaveLineBits {
+    static constexpr int W = N / 32;
+    uint32_t word[W];
+    int pre[W];
+    int start;
+    __device__ __forceinline__ void load(const LineBits& lb, long line) {
+        const uint32_t* wb = lb.bits + line * W;
+#pragma unroll
+        for (int u = 0; u < W; ++u) word[u] = wb[u];
+        start = lb.lstart[line];
+        int acc = 0;
+#pragma unroll
+        for (int u = 0; u < W; ++u) {
+            pre[u] = acc;
+            acc += __popc(word[u]);
+        }
+    }
+    __device__ __forceinline__ int loc(int lane, int r) const {
+        // voxel lane + 64 r lies in word 2 r (lanes 0..31) or 2 r + 1 (lanes 32..63)
+        const bool hi = lane >= 32;
+        const int b = lane & 31;
+        uint32_t w = 0u;
+        int p = 0;
+#pragma unroll
+        for (int u = 0; u < W; u += 2) {
+            if (u == 2 * r) {
+                w = hi ? word[u + 1] : word[u];
+                p = hi ? pre[u + 1] : pre[u];
+            }
+        }
+        if (((w >> b) & 1u) == 0u) return -1;
+        return start + p + __popc(w & ((1u << b) - 1u));
+    }
+};
+
+// Two-for-one split of the wave's spectrum Z (natural order in L) into the half spectra of its
+// real and imaginary parts, stored at rows oa (real part) and ob (imaginary part; ob < 0: none).
+template <typename R, int N>
+__device__ __forceinline__ void wave_store_halves(const cx<R>* L, cx<R>* Sa, cx<R>* Sb, bool has_b, int lane) {
+    constexpr int NXH = N / 2 + 1;
+#pragma unroll
+    for (int q = 0; q < (NXH + 63) / 64; ++q) {
+        const int kx = lane + 64 * q;
+        if (kx < NXH) {
+            const cx<R> zk = L[kx];
+            const cx<R> zm = L[(N - kx) % N];
+            Sa[kx] = mkc<R>((R)0.5 * (zk.x + zm.x), (R)0.5 * (zk.y - zm.y));
+            if (has_b) Sb[kx] = mkc<R>((R)0.5 * (zk.y + zm.y), (R)0.5 * (zm.x - zk.x));
+        }
+    }
+}
+
+template <typename R, int N>
+__global__ __launch_bounds__(256) void k_xfwd_w(XSource src, cx<R>* __restrict__ S, long nlines, SpecLayout SL,
+                                                const cx<R>* __restrict__ twg, const int* skip) {
+    if (skip != nullptr && *skip) return;
+    constexpr int RAD = N / 64;
+    extern __shared__ unsigned char smem_raw[];
+    cx<R>* tw = reinterpret_cast<cx<R>*>(smem_raw);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    cx<R>* L = tw + N + wave * N;
+    for (int i = threadIdx.x; i < N; i += 256) tw[i] = twg[i];
+    __syncthreads();
+    const long line0 = __builtin_amdgcn_readfirstlane((int)(xcd_tile(blockIdx.x, gridDim.x) * kWaveLines)) + 2 * wave;
+    if (line0 >= nlines) return;
+    const bool two = line0 + 1 < nlines;
+    double mc[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) mc[c] = (src.mean != nullptr) ? src.mean[c] : 0.0;
+    // The pair's support voxels are one contiguous run [cs, ce) of the compact vector: it is staged
+    // through the wave's LDS line with coalesced loads when it fits (else gathered directly).
+    const long nb = src.lines.nb;
+    const long cs = src.lines.lstart[line0];
+    const long ce = (line0 + 2 < nlines) ? (long)src.lines.lstart[line0 + 2] : nb;
+    const long cnt3 = 3 * (ce - cs);
+    float* st32 = reinterpret_cast<float*>(L);
+    double* st64 = reinterpret_cast<double*>(L);
+    constexpr long cap32 = (long)(N * sizeof(cx<R>) / sizeof(float));
+    constexpr long cap64 = (long)(N * sizeof(cx<R>) / sizeof(double));
+    const bool staged32 = src.compact32 != nullptr && cnt3 <= cap32;
+    const bool staged64 = src.compact32 == nullptr && src.compact != nullptr && cnt3 <= cap64;
+    if (staged32) {
+        const float* cc = src.compact32 + 3 * cs;
+        for (long e = lane; e < cnt3; e += 64) st32[e] = cc[e];
+    } else if (staged64) {
+        const double* cc = src.compact + 3 * cs;
+        for (long e = lane; e < cnt3; e += 64) st64[e] = cc[e];
+    }
+    wave_lds_sync();
+    // every source value of the pair first: v[a][c][r] of voxel lane + 64 r of line line0 + a
+    R val[2][3][RAD];
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+        WaveLineBits<N> wb;
+        const bool ok = a == 0 || two;
+        if (ok) wb.load(src.lines, line0 + a);
+#pragma unroll
+        for (int r = 0; r < RAD; ++r) {
+            const int l = ok ? wb.loc(lane, r) : -1;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                double raw = 0.0;
+                if (l >= 0) {
+                    const long e = 3L * (l - cs) + c;
+                    if (staged32)
+                        raw = (double)st32[e];
+                    else if (staged64)
+                        raw = st64[e];
+                    else if (src.compact32 != nullptr)
+                        raw = (double)src.compact32[3L * l + c];
+                    else if (src.compact != nullptr)
+                        raw = src.compact[3L * l + c];
+                }
+                val[a][c][r] = (l < 0) ? (R)0 : (R)(raw - mc[c]);
+            }
+        }
+    }
+    wave_lds_sync();  // the stage is the transform line
+    const long oa = spec_line(SL, line0), ob = spec_line(SL, line0 + (two ? 1 : 0));
+    cx<R> v[RAD];
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+        if (a == 1 && !two) break;
+#pragma unroll
+        for (int r = 0; r < RAD; ++r) v[r] = mkc<R>(val[a][0][r], val[a][1][r]);
+        wave_fft<R, -1, N>(v, L, tw, lane);
+        const long o = a == 0 ? oa : ob;
+        wave_store_halves<R, N>(L, S + o, S + SL.nc + o, true, lane);
+        wave_lds_sync();
+    }
+#pragma unroll
+    for (int r = 0; r < RAD; ++r) v[r] = mkc<R>(val[0][2][r], val[1][2][r]);
+    wave_fft<R, -1, N>(v, L, tw, lane);
+    wave_store_halves<R, N>(L, S + 2 * SL.nc + oa, S + 2 * SL.nc + ob, two, lane);
+}
+
+// Z = A + i B over the Hermitian completions of the half-spectrum rows A and B at element
+// lane + 64 r (self-conjugate bins keep their real parts only).
+template <typename R, int N>
+__device__ __forceinline__ void wave_load_pair(const cx<R>* __restrict__ Ar, const cx<R>* __restrict__ Br, bool has_b,
+                                               cx<R> (&v)[N / 64], int lane) {
+#pragma unroll
+    for (int r = 0; r < N / 64; ++r) {
+        const int k = lane + 64 * r;
+        const bool upper = 2 * k > N;
+        const int kk = upper ? N - k : k;
+        cx<R> a = Ar[kk];
+        cx<R> b = mkc<R>((R)0, (R)0);
+        if (has_b) b = Br[kk];
+        if (kk == 0 || 2 * kk == N) {
+            a.y = (R)0;
+            b.y = (R)0;
+        }
+        v[r] = upper ? mkc<R>(a.x + b.y, b.x - a.y) : mkc<R>(a.x - b.y, a.y + b.x);
+    }
+}
+
+template <typename R, int N>
+__global__ __launch_bounds__(256) void k_xinv_w(const cx<R>* __restrict__ S, XSink sink, long nlines, SpecLayout SL,
+                                                const cx<R>* __restrict__ twg, double inv_n, const int* skip) {
+    if (skip != nullptr && *skip) return;
+    constexpr int RAD = N / 64;
+    extern __shared__ unsigned char smem_raw[];
+    cx<R>* tw = reinterpret_cast<cx<R>*>(smem_raw);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    cx<R>* L = tw + N + wave * N;
+    for (int i = threadIdx.x; i < N; i += 256) tw[i] = twg[i];
+    __syncthreads();
+    const long line0 = __builtin_amdgcn_readfirstlane((int)(xcd_tile(blockIdx.x, gridDim.x) * kWaveLines)) + 2 * wave;
+    const bool one = line0 < nlines;
+    const bool two = line0 + 1 < nlines;
+    double acc[kXinvParts] = {0, 0, 0, 0, 0, 0};
+    if (one) {
+        const long oa = spec_line(SL, line0), ob = spec_line(SL, line0 + (two ? 1 : 0));
+        // every spectrum value of the pair first
+        cx<R> z[3][RAD];
+        wave_load_pair<R, N>(S + oa, S + SL.nc + oa, true, z[0], lane);
+        if (two) wave_load_pair<R, N>(S + ob, S + SL.nc + ob, true, z[1], lane);
+        wave_load_pair<R, N>(S + 2 * SL.nc + oa, S + 2 * SL.nc + ob, two, z[2], lane);
+        int loc[2][RAD];
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+            WaveLineBits<N> wb;
+            const bool ok = a == 0 || two;
+            if (ok) wb.load(sink.lines, line0 + a);
+#pragma unroll
+            for (int r = 0; r < RAD; ++r) loc[a][r] = ok ? wb.loc(lane, r) : -1;
+        }
+        auto emit = [&](int c, int l, double val) {
+            if (l >= 0) {
+                if (sink.compact != nullptr) sink.compact[3L * l + c] = val;
+                if (sink.compact32 != nullptr) sink.compact32[3L * l + c] = (float)val;
+                acc[c] += val;
+            }
+            acc[3 + c] += val;
+        };
+#pragma unroll
+        for (int f = 0; f < 3; ++f) {
+            if (f == 1 && !two) continue;
+            wave_fft<R, +1, N>(z[f], L, tw, lane);
+#pragma unroll
+            for (int r = 0; r < RAD; ++r) {
+                const cx<R> t = L[lane + 64 * r];
+                if (f < 2) {
+                    emit(0, loc[f][r], (double)t.x * inv_n);
+                    emit(1, loc[f][r], (double)t.y * inv_n);
+                } else {
+                    emit(2, loc[0][r], (double)t.x * inv_n);
+                    if (two) emit(2, loc[1][r], (double)t.y * inv_n);
+                }
+            }
+            wave_lds_sync();
+        }
+    }
+    if (sink.partials != nullptr) block_sum<kXinvParts>(acc, sink.partials + (long)kXinvParts * blockIdx.x);
+}
+
+template <typename R, int N>
+constexpr size_t wave_pass_lds() {
+    return (size_t)(N + 4 * N) * sizeof(cx<R>);
+}
+
+// The wave passes serve the solver's applies: a compact (FP64 or FP32) source without background
+// force and a compact sink, both over the line bitmask; other calls take the 8 x 2 tile.
+inline bool wave_src_ok(const XSource& src) {
+    const bool no_f0 = src.a_fluid[0] == 0.0 && src.a_fluid[1] == 0.0 && src.a_fluid[2] == 0.0 &&
+                       src.a_support[0] == 0.0 && src.a_support[1] == 0.0 && src.a_support[2] == 0.0;
+    return src.field == nullptr && src.lines.bits != nullptr && no_f0;
+}
+inline bool wave_sink_ok(const XSink& sink) { return sink.field == nullptr && sink.lines.bits != nullptr; }
+
 template <typename R, int N, int V>
 long xfwd_variant(const XSource& src, cx<R>* S, const Geometry3& g, const cx<R>* tw, const int* skip,
                   hipStream_t st) {
     constexpr int LINES = xpass_lines<N>(V), OS = xpass_os(V);
     const long nlines = (long)g.ny * g.nz;
     const long blocks = (nlines + LINES - 1) / LINES;
-    if constexpr (xpass_merged(V)) {
+    if constexpr (xpass_wave(V)) {
+        if constexpr (!wave_pass_ok<N>()) {
+            return -1;
+        } else {
+            if (!wave_src_ok(src)) return xfwd_variant<R, N, 2>(src, S, g, tw, skip, st);
+            constexpr size_t lds = wave_pass_lds<R, N>();
+            if (lds > lds_limit()) return -1;
+            hipLaunchKernelGGL((k_xfwd_w<R, N>), dim3((unsigned)blocks), dim3(256), lds, st, src, S, nlines,
+                               spec_layout(g), tw, skip);
+            return blocks;
+        }
+    } else if constexpr (xpass_merged(V)) {
         using SH = MergedShape<R, N, LINES, OS>;
         if constexpr (!SH::valid) {
             return -1;
@@ -593,7 +892,18 @@ long xinv_variant(const cx<R>* S, const XSink& sink, const Geometry3& g, const c
     const long nlines = (long)g.ny * g.nz;
     const long blocks = (nlines + LINES - 1) / LINES;
     const double inv_n = 1.0 / (double)g.N;
-    if constexpr (xpass_merged(V)) {
+    if constexpr (xpass_wave(V)) {
+        if constexpr (!wave_pass_ok<N>()) {
+            return -1;
+        } else {
+            if (!wave_sink_ok(sink)) return xinv_variant<R, N, 2>(S, sink, g, tw, skip, st);
+            constexpr size_t lds = wave_pass_lds<R, N>();
+            if (lds > lds_limit()) return -1;
+            hipLaunchKernelGGL((k_xinv_w<R, N>), dim3((unsigned)blocks), dim3(256), lds, st, S, sink, nlines,
+                               spec_layout(g), tw, inv_n, skip);
+            return blocks;
+        }
+    } else if constexpr (xpass_merged(V)) {
         using SH = MergedShape<R, N, LINES, OS>;
         if constexpr (!SH::valid) {
             return -1;
@@ -630,6 +940,7 @@ long xfwd_n(int v, const XSource& src, cx<R>* S, const Geometry3& g, const cx<R>
         case 5: return xfwd_variant<R, N, 5>(src, S, g, tw, skip, st);
         case 6: return xfwd_variant<R, N, 6>(src, S, g, tw, skip, st);
         case 7: return xfwd_variant<R, N, 7>(src, S, g, tw, skip, st);
+        case 8: return xfwd_variant<R, N, 8>(src, S, g, tw, skip, st);
         default: return -1;
     }
 }
@@ -646,13 +957,16 @@ long xinv_n(int v, const cx<R>* S, const XSink& sink, const Geometry3& g, const 
         case 5: return xinv_variant<R, N, 5>(S, sink, g, tw, skip, st);
         case 6: return xinv_variant<R, N, 6>(S, sink, g, tw, skip, st);
         case 7: return xinv_variant<R, N, 7>(S, sink, g, tw, skip, st);
+        case 8: return xinv_variant<R, N, 8>(S, sink, g, tw, skip, st);
         default: return -1;
     }
 }
 
-// 10 * lines + threads per butterfly thread; merged-round x variants add 5 (last digit 6, 7 or 9)
+// 10 * lines + threads per butterfly thread; merged-round x variants add 5 (last digit 6, 7 or 9);
+// the wave-per-line-pair variant is 88, for the lengths it serves
 template <int N>
 int code_of(int v) {
+    if (xpass_wave(v)) return wave_pass_ok<N>() ? 88 : 0;
     return 10 * xpass_lines<N>(v) + xpass_os(v) + (xpass_merged(v) ? 5 : 0);
 }
 

@@ -36,7 +36,8 @@ def pass_codes(n):
 
 def xpass_codes(n, precision=0):
     """pass_codes(n) plus the merged-round x-pass variants valid at this precision (one transform of
-    3 lines / 2 complex lines per tile; code 10 * lines + threads per butterfly thread + 5)."""
+    3 lines / 2 complex lines per tile; code 10 * lines + threads per butterfly thread + 5) and the
+    wave-per-line-pair variant (code 88) at the lengths it serves."""
     t = n // _MAX_RADIX[n]
     base = max(2, 256 // t)
     out = pass_codes(n)
@@ -48,6 +49,8 @@ def xpass_codes(n, precision=0):
         lds = (n + lt * (n + 1)) * cx_bytes
         if lines >= 2 and lines % 2 == 0 and 64 <= threads <= 1024 and threads % 64 == 0 and lds <= 100 * 1024:
             out.append(10 * lines + os_ + 5)
+    if n in (128, 256, 512):
+        out.append(88)  # one wave per line pair, 8 lines per workgroup
     return out
 
 
@@ -116,6 +119,27 @@ def test_x_pass_variants_fp64(fp, oracle_mod, n):
         for kind in (_lib.PASS_XFWD, _lib.PASS_XINV):
             with pinned(green, kind, _lib.PRECISION_FP64, code):
                 _check_apply(green, sup, o, shape, seed=code)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_x_wave_variant_odd_line_count(fp, oracle_mod, precision):
+    """The wave-per-line-pair x passes with an odd number of x lines (the last wave holds one line)
+    and a workgroup with idle waves: FP64 operator within 1e-12, FP32 operator within
+    FP32_APPLY_BOUND of the oracle."""
+    from fft_method_test_amd import _lib, permeability as P
+
+    shape = (256, 5, 3)
+    green, sup, o = _pair(oracle_mod, shape, seed=77, precision=precision)
+    op = P.VariationalOperator(sup, green)
+    x = np.random.default_rng(78).standard_normal(sup.size())
+    ref = o.apply(x)
+    for kind in (_lib.PASS_XFWD, _lib.PASS_XINV):
+        with pinned(green, kind, precision, 88):
+            if precision == 0:
+                assert rel(op.apply(x), ref) < 1e-12
+            else:
+                assert rel(op.apply_fp32(x), ref) < FP32_APPLY_BOUND
+    green.close()
 
 
 @pytest.mark.parametrize("n", [32, 64, 128, 256, 512])
