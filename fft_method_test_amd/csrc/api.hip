@@ -241,8 +241,8 @@ struct fftperm_ctx {
         hipEvent_t a, b, c, d;  // before x-fwd, before z-green, after z-green, after x-inv
         long long tag;          // Krylov iteration index, or -1 (always a real apply)
         int fp32;
-        int batch;              // lockstep record of sides 0 .. batch - 1 (0: not a lockstep record)
-        int side;               // per-side record of a multi-RHS iteration (-1: none)
+        int batch = 0;          // lockstep record of sides 0 .. batch - 1 (0: not a lockstep record)
+        int side = -1;          // per-side record of a multi-RHS iteration (-1: none)
     };
     bool prof = false;
     std::vector<ProfRec> prof_pending;

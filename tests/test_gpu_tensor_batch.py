@@ -88,7 +88,7 @@ def test_tensor_batch_repeat_and_other_solvers(fp, oracle_mod):
     green.close()
 
 
-@pytest.mark.parametrize("mode", [1, 2])
+@pytest.mark.parametrize("mode", [0, 1, 2])
 def test_tensor_batch_profile_counts(fp, mode):
     """Live timing of the batched solve: every FP32 apply of every direction is counted once
     (a lockstep launch counts the directions it served)."""
@@ -108,7 +108,7 @@ def test_tensor_batch_profile_counts(fp, mode):
     assert pr.zgreen_rhs[1] == its
     if mode == 2:  # one record per lockstep iteration in which some direction progressed
         assert max(int(i.iterations) for i in infos) <= pr.zgreen_launches[1] <= its
-    else:  # one record per direction and iteration
+    else:  # one record per direction and iteration (mode 0: the single-direction solver)
         assert pr.zgreen_launches[1] == its
     assert pr.apply_launches[0] == pr.apply_rhs[0] > 0  # FP64 true residuals, one side each
     green.close()
