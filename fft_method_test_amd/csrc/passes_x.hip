@@ -569,6 +569,15 @@ constexpr bool wave_pass_ok() {
     return N == 128 || N == 256 || N == 512;
 }
 
+// LDS index of element e of a wave's line or of the twiddle table: one pad slot after every 16
+// elements keeps the Stockham writes (element strides 4 and 16 across lanes at N = 256), the
+// consecutive reads and the twiddle reads of stride r < 4 free of bank conflicts.
+__device__ __forceinline__ int lpad(int e) { return e + (e >> 4); }
+template <int N>
+constexpr int lpad_size() {
+    return N + N / 16;
+}
+
 // Orders the wave's LDS writes before its later reads (a wave executes its LDS operations in
 // order; the fences keep the compiler from moving them across).
 __device__ __forceinline__ void wave_lds_sync() {
@@ -584,11 +593,11 @@ __device__ __forceinline__ void wave_fft_stage(cx<R> (&v)[N / 64], cx<R>* L, con
         const int k = lane % NS;
         if constexpr (NS > 1) {
 #pragma unroll
-            for (int r = 0; r < RAD; ++r) v[r] = L[lane + r * 64];
+            for (int r = 0; r < RAD; ++r) v[r] = L[lpad(lane + r * 64)];
             constexpr int TS = N / (NS * RAD);
 #pragma unroll
             for (int r = 1; r < RAD; ++r) {
-                cx<R> w = tw[r * k * TS];
+                cx<R> w = tw[lpad(r * k * TS)];
                 if (SIGN > 0) w.y = -w.y;
                 v[r] = cmul<R>(v[r], w);
             }
@@ -597,13 +606,14 @@ __device__ __forceinline__ void wave_fft_stage(cx<R> (&v)[N / 64], cx<R>* L, con
         dft_small<R, SIGN, RAD>(v);
         const int base = (lane / NS) * NS * RAD + k;
 #pragma unroll
-        for (int r = 0; r < RAD; ++r) L[base + r * NS] = v[r];
+        for (int r = 0; r < RAD; ++r) L[lpad(base + r * NS)] = v[r];
         wave_lds_sync();
         wave_fft_stage<R, SIGN, N, NS * RAD>(v, L, tw, lane);
     }
 }
 
-// Transform of one line given as v[r] = x[lane + 64 r]; the spectrum is left in L in natural order.
+// Transform of one line given as v[r] = x[lane + 64 r]; the spectrum is left in L in natural order
+// (element k at L[lpad(k)]; the twiddle table padded the same way).
 template <typename R, int SIGN, int N>
 __device__ __forceinline__ void wave_fft(cx<R> (&v)[N / 64], cx<R>* L, const cx<R>* tw, int lane) {
     wave_fft_stage<R, SIGN, N, 1>(v, L, tw, lane);
@@ -656,8 +666,8 @@ __device__ __forceinline__ void wave_store_halves(const cx<R>* L, cx<R>* Sa, cx<
     for (int q = 0; q < (NXH + 63) / 64; ++q) {
         const int kx = lane + 64 * q;
         if (kx < NXH) {
-            const cx<R> zk = L[kx];
-            const cx<R> zm = L[(N - kx) % N];
+            const cx<R> zk = L[lpad(kx)];
+            const cx<R> zm = L[lpad((N - kx) % N)];
             Sa[kx] = mkc<R>((R)0.5 * (zk.x + zm.x), (R)0.5 * (zk.y - zm.y));
             if (has_b) Sb[kx] = mkc<R>((R)0.5 * (zk.y + zm.y), (R)0.5 * (zm.x - zk.x));
         }
@@ -672,8 +682,8 @@ __global__ __launch_bounds__(256) void k_xfwd_w(XSource src, cx<R>* __restrict__
     extern __shared__ unsigned char smem_raw[];
     cx<R>* tw = reinterpret_cast<cx<R>*>(smem_raw);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    cx<R>* L = tw + N + wave * N;
-    for (int i = threadIdx.x; i < N; i += 256) tw[i] = twg[i];
+    cx<R>* L = tw + lpad_size<N>() + wave * lpad_size<N>();
+    for (int i = threadIdx.x; i < N; i += 256) tw[lpad(i)] = twg[i];
     __syncthreads();
     const long line0 = __builtin_amdgcn_readfirstlane((int)(xcd_tile(blockIdx.x, gridDim.x) * kWaveLines)) + 2 * wave;
     if (line0 >= nlines) return;
@@ -689,8 +699,8 @@ __global__ __launch_bounds__(256) void k_xfwd_w(XSource src, cx<R>* __restrict__
     const long cnt3 = 3 * (ce - cs);
     float* st32 = reinterpret_cast<float*>(L);
     double* st64 = reinterpret_cast<double*>(L);
-    constexpr long cap32 = (long)(N * sizeof(cx<R>) / sizeof(float));
-    constexpr long cap64 = (long)(N * sizeof(cx<R>) / sizeof(double));
+    constexpr long cap32 = (long)(lpad_size<N>() * sizeof(cx<R>) / sizeof(float));
+    constexpr long cap64 = (long)(lpad_size<N>() * sizeof(cx<R>) / sizeof(double));
     const bool staged32 = src.compact32 != nullptr && cnt3 <= cap32;
     const bool staged64 = src.compact32 == nullptr && src.compact != nullptr && cnt3 <= cap64;
     if (staged32) {
@@ -777,8 +787,8 @@ __global__ __launch_bounds__(256) void k_xinv_w(const cx<R>* __restrict__ S, XSi
     extern __shared__ unsigned char smem_raw[];
     cx<R>* tw = reinterpret_cast<cx<R>*>(smem_raw);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    cx<R>* L = tw + N + wave * N;
-    for (int i = threadIdx.x; i < N; i += 256) tw[i] = twg[i];
+    cx<R>* L = tw + lpad_size<N>() + wave * lpad_size<N>();
+    for (int i = threadIdx.x; i < N; i += 256) tw[lpad(i)] = twg[i];
     __syncthreads();
     const long line0 = __builtin_amdgcn_readfirstlane((int)(xcd_tile(blockIdx.x, gridDim.x) * kWaveLines)) + 2 * wave;
     const bool one = line0 < nlines;
@@ -814,7 +824,7 @@ __global__ __launch_bounds__(256) void k_xinv_w(const cx<R>* __restrict__ S, XSi
             wave_fft<R, +1, N>(z[f], L, tw, lane);
 #pragma unroll
             for (int r = 0; r < RAD; ++r) {
-                const cx<R> t = L[lane + 64 * r];
+                const cx<R> t = L[lpad(lane + 64 * r)];
                 if (f < 2) {
                     emit(0, loc[f][r], (double)t.x * inv_n);
                     emit(1, loc[f][r], (double)t.y * inv_n);
@@ -831,7 +841,7 @@ __global__ __launch_bounds__(256) void k_xinv_w(const cx<R>* __restrict__ S, XSi
 
 template <typename R, int N>
 constexpr size_t wave_pass_lds() {
-    return (size_t)(N + 4 * N) * sizeof(cx<R>);
+    return (size_t)(5 * lpad_size<N>()) * sizeof(cx<R>);
 }
 
 // The wave passes serve the solver's applies: a compact (FP64 or FP32) source without background
