@@ -131,3 +131,30 @@ def test_tensor_batch_warm_matches_sequential(fp):
     assert res[1][2] == res[0][2]
     assert res[1][1] == res[0][1]
     assert np.array_equal(res[1][0], res[0][0])
+
+
+def test_time_apply_hook(fp):
+    """fftperm_time_apply: positive device time per apply for both operators of a MIXED context;
+    the FP32 operator is refused on an FP64 context, reps < 1 and a context without medium too."""
+    import ctypes as C
+
+    from fft_method_test_amd import _lib, geometry as G, permeability as P
+
+    L = _lib.load()
+    mask = G.sphere_cell(32).reshape(-1)
+    green = _green((32, 32, 32), mask, 1)
+    ms = C.c_double()
+    for prec in (_lib.PRECISION_FP64, _lib.PRECISION_MIXED):
+        _lib.check(L.fftperm_time_apply(green._h, prec, 3, C.byref(ms)))
+        assert 0.0 < ms.value < 100.0
+    with pytest.raises(ValueError, match="reps"):
+        _lib.check(L.fftperm_time_apply(green._h, _lib.PRECISION_MIXED, 0, C.byref(ms)))
+    green.close()
+    g64 = P.GreenOperator(P.Grid3D(16, 16, 16), 1.0, 2, P.PRECISION_FP64)
+    with pytest.raises(RuntimeError, match="no medium"):
+        _lib.check(L.fftperm_time_apply(g64._h, _lib.PRECISION_FP64, 1, C.byref(ms)))
+    P.ForceSupport(P.BinaryMedium(P.Grid3D(16, 16, 16), random_mask((16, 16, 16), 0.6, seed=4)),
+                   P.SUPPORT_INTERFACE, g64)
+    with pytest.raises(ValueError, match="MIXED context"):
+        _lib.check(L.fftperm_time_apply(g64._h, _lib.PRECISION_MIXED, 1, C.byref(ms)))
+    g64.close()
