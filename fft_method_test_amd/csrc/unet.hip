@@ -282,12 +282,31 @@ void conv_fwd_t(const uint16_t* in0, int c0, const uint16_t* in1, int c1, int n,
 #undef UNET_FWD
 }
 
+// The halo-tiled kernel where the shape allows it (n a multiple of 64, 16 | c0, c1, cout <= 64).
+bool conv3_tile(const uint16_t* in0, int c0, const uint16_t* in1, int c1, int n, const uint16_t* wt, int Kp,
+                int cout, const float* bias, bool relu, bool out_f32, void* out, hipStream_t st) {
+    if (n % kTileX != 0 || c0 % 16 != 0 || c1 % 16 != 0 || c0 == 0) return false;
+    const long blocks = (long)(n / kTileX) * (n / 2) * (n / 2);
+#define UNET_TILE(NT)                                                                                        \
+    case NT:                                                                                                 \
+        hipLaunchKernelGGL((k_conv3_tile<NT>), dim3((unsigned)blocks), dim3(256), 0, st, in0, c0, in1, c1, n, wt, \
+                           Kp, bias, relu ? 1 : 0, out_f32 ? 1 : 0, out);                                      \
+        return true;
+    switch (cout / 16) {
+        UNET_TILE(1) UNET_TILE(2) UNET_TILE(4)
+        default: return false;
+    }
+#undef UNET_TILE
+}
+
 void conv_fwd(int taps, const uint16_t* in0, int c0, const uint16_t* in1, int c1, int n, const uint16_t* wt, int Kp,
               int cout, const float* bias, bool relu, bool out_f32, void* out, hipStream_t st) {
-    if (taps == 27)
+    if (taps == 27) {
+        if (cout % 16 == 0 && conv3_tile(in0, c0, in1, c1, n, wt, Kp, cout, bias, relu, out_f32, out, st)) return;
         conv_fwd_t<27>(in0, c0, in1, c1, n, wt, Kp, cout, bias, relu, out_f32, out, st);
-    else
+    } else {
         conv_fwd_t<1>(in0, c0, in1, c1, n, wt, Kp, cout, bias, relu, out_f32, out, st);
+    }
 }
 
 void conv_wgrad(int taps, const uint16_t* in0, int c0, const uint16_t* in1, int c1, int n, const uint16_t* dz,

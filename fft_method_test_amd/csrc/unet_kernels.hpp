@@ -106,6 +106,104 @@ __global__ __launch_bounds__(256) void k_conv_fwd(const uint16_t* __restrict__ i
     }
 }
 
+// Halo-tiled 3x3x3 convolution: a workgroup computes a brick of 64 (x) x 2 (y) x 2 (z) voxels, one
+// wave per x row of 64 voxels (four 16-voxel MFMA tiles, so each weight fragment serves four
+// MFMAs).  The brick's neighbourhood (4 x 4 rows of 66 voxels) is staged in LDS 16 channels at a
+// time and every tap reads it from there, instead of each wave gathering its 27 neighbours through
+// the caches.  K runs over channel chunks, then tap pairs (taps t, t + 1 x 16 channels = one K-32
+// MFMA step).  Needs n % 64 == 0 and 16 | c0, c1 (conv_fwd falls back to k_conv_fwd otherwise).
+constexpr int kTileX = 64;
+constexpr int kTileRows = 16;  // (z - 1 .. z + 2) x (y - 1 .. y + 2)
+constexpr int kTileW = kTileX + 2;
+
+template <int NT>
+__global__ __launch_bounds__(256) void k_conv3_tile(const uint16_t* __restrict__ in0, int c0,
+                                                    const uint16_t* __restrict__ in1, int c1, int n,
+                                                    const uint16_t* __restrict__ wt, int Kp,
+                                                    const float* __restrict__ bias, int relu, int out_f32,
+                                                    void* __restrict__ out) {
+    constexpr int COUT = 16 * NT;
+    __shared__ uint4 st[kTileRows * kTileW * 2];  // 16 bf16 channels per voxel = two 16-byte words
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int row = lane & 15;  // A: voxel within an M tile; B: output channel within an N tile
+    const int g = lane >> 4;    // K group of 8: tap t + (g >> 1), channels 8 (g & 1) .. + 7
+    const int ntx = n / kTileX;
+    const int bx = blockIdx.x % ntx;
+    const int byz = blockIdx.x / ntx;
+    const int i0 = bx * kTileX;
+    const int j0 = (byz % (n / 2)) * 2;
+    const int k0 = (byz / (n / 2)) * 2;
+    const int jj = wave & 1, kk = wave >> 1;  // this wave's output row (j0 + jj, k0 + kk)
+    const int cin = c0 + c1;
+    v4f acc[4][NT];
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) acc[m][nt] = (v4f){0.f, 0.f, 0.f, 0.f};
+    for (int cc = 0; cc < cin; cc += 16) {
+        // stage channels cc .. cc + 15 of the neighbourhood (periodic)
+        const uint16_t* src = cc < c0 ? in0 : in1;
+        const int cs = cc < c0 ? c0 : c1;
+        const int co = cc < c0 ? cc : cc - c0;
+        if (cc > 0) __syncthreads();  // every wave has read the previous chunk
+        for (int e = threadIdx.x; e < kTileRows * kTileW * 2; e += 256) {
+            const int half = e & 1;
+            const int xi = (e >> 1) % kTileW;
+            const int r = (e >> 1) / kTileW;
+            const int x = wrapn(i0 - 1 + xi, n);
+            const int y = wrapn(j0 - 1 + (r & 3), n);
+            const int z = wrapn(k0 - 1 + (r >> 2), n);
+            const long v = ((long)z * n + y) * n + x;
+            st[e] = *reinterpret_cast<const uint4*>(src + v * cs + co + 8 * half);
+        }
+        __syncthreads();
+        for (int t0 = 0; t0 < 27; t0 += 2) {
+            const int t = t0 + (g >> 1);
+            const bool tap = t < 27;
+            const int dk = t / 9, dj = (t / 3) % 3, di = t % 3;  // offsets + 1
+            v8bf a[4];
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                a[m] = zero8();
+                if (tap) {
+                    const int r = (kk + dk) * 4 + (jj + dj);
+                    const int xi = m * 16 + row + di;
+                    a[m] = __builtin_bit_cast(v8bf, st[(r * kTileW + xi) * 2 + (g & 1)]);
+                }
+            }
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) {
+                v8bf b = zero8();
+                if (tap) b = load8(wt + (long)(nt * 16 + row) * Kp + (long)t * cin + cc + 8 * (g & 1));
+#pragma unroll
+                for (int m = 0; m < 4; ++m)
+                    acc[m][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m], b, acc[m][nt], 0, 0, 0);
+            }
+        }
+    }
+    // D[r][c]: lane holds rows 4g + q (voxels of M tile m), column row (channel nt*16 + row)
+    const long vrow = ((long)(k0 + kk) * n + (j0 + jj)) * n + i0;
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) {
+        const int cout = nt * 16 + row;
+        const float bb = bias != nullptr ? bias[cout] : 0.f;
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const long vo = vrow + m * 16 + 4 * g + q;
+                float y = acc[m][nt][q] + bb;
+                if (relu && y < 0.f) y = 0.f;
+                if (out_f32)
+                    static_cast<float*>(out)[vo * COUT + cout] = y;
+                else
+                    static_cast<uint16_t*>(out)[vo * COUT + cout] = f2bf(y);
+            }
+        }
+    }
+}
+
 // Weight gradient: dW[co][k] += sum_v dZ[v][co] A[v][k] over a chunk of voxels per workgroup.
 // One wave per workgroup computes the 16x16 tile (co0.., k0..) with the voxel sum as the MFMA K
 // dimension (32 voxels per step); FP32 atomics add the chunk's tile into dw.
