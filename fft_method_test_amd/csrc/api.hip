@@ -1562,6 +1562,17 @@ void tune_chain_prec(fftperm_ctx* c) {
         size_t best = 0;
         for (size_t i = 1; i < cands.size(); ++i)
             if (best_of[i] < best_of[best]) best = i;
+        if (k.kind == kKindCol) {
+            // column tiles within 1% of the fastest are a tie here (the timing noise of back-to-back
+            // applies); the widest of them reads the longest contiguous row segments, which is what
+            // counts in the solver, where the spectrum comes from HBM more often than between
+            // back-to-back applies (at 256^3 the 32-column tile is ~5 us per pass faster there while
+            // the chain timing cannot tell it from the 16-column one)
+            const float bound = best_of[best] * 1.01f;
+            for (size_t i = 0; i < cands.size(); ++i)
+                if (best_of[i] <= bound && col_variant_code(k.n, cands[i]) / 10 > col_variant_code(k.n, cands[best]) / 10)
+                    best = i;
+        }
         set_tuned_variant(k.kind, prec, k.n, k.count, cands[best]);
     }
     c->sync();
