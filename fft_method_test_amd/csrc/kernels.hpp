@@ -42,7 +42,7 @@ struct XSource {
 // Sink of the x-inverse pass.  value = Re(line) * inv_n.
 //   compact  != null : compact[3*l + c] = value on support voxels
 //   field    != null : field[c*N + v] = value
-//   partials != null : per-block sums (kXinvParts wide): {support sums x,y,z, all-voxel sums x,y,z}
+//   partials != null : per-block sums (kXinvParts wide): the support sums of x, y, z
 struct XSink {
     const int32_t* code;
     double* compact;
@@ -51,7 +51,7 @@ struct XSink {
     double* partials;
     LineBits lines;  // when set, the specialised inverse pass reads the support from it
 };
-constexpr int kXinvParts = 6;
+constexpr int kXinvParts = 3;
 
 struct PassCfg {
     int lines;

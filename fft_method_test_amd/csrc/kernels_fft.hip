@@ -1036,7 +1036,7 @@ __global__ void k_xinv(const cx<R>* __restrict__ S, XSink sink, int nx, long nli
     for (int i = threadIdx.x; i < nx; i += blockDim.x) tw[i] = twg[i];
     const long line0 = (long)blockIdx.x * lpw;
     const int nl = (int)min((long)lpw, nlines - line0);
-    double acc[kXinvParts] = {0, 0, 0, 0, 0, 0};
+    double acc[kXinvParts] = {0, 0, 0};
     for (int c = 0; c < 3; ++c) {
         const cx<R>* Sc = S + c * SL.nc;
         const int cnth = nl * nxh;
@@ -1069,9 +1069,6 @@ __global__ void k_xinv(const cx<R>* __restrict__ S, XSink sink, int nx, long nli
                 if (c == 1) acc[1] += val;
                 if (c == 2) acc[2] += val;
             }
-            if (c == 0) acc[3] += val;
-            if (c == 1) acc[4] += val;
-            if (c == 2) acc[5] += val;
         }
         __syncthreads();
     }

@@ -238,7 +238,7 @@ __device__ __forceinline__ void xinv_put(cx<R>* buf, int line, int kx, cx<R> a, 
 }
 
 // Writes one real output value of component c at voxel vox / support index cd and accumulates
-// the partial sums {support sum of c, all-voxel sum of c}.
+// the support sum of c.
 __device__ __forceinline__ void xinv_emit(const XSink& sink, long N3, int c, long vox, int cd, double val,
                                           double (&acc)[kXinvParts]) {
     if (sink.field != nullptr) sink.field[c * N3 + vox] = val;
@@ -247,7 +247,6 @@ __device__ __forceinline__ void xinv_emit(const XSink& sink, long N3, int c, lon
         if (sink.compact32 != nullptr) sink.compact32[3L * cd + c] = (float)val;
         acc[c] += val;
     }
-    acc[3 + c] += val;
 }
 
 // C2R of two half spectra at once.  The imaginary parts of the self-conjugate bins (0 and N/2)
@@ -327,7 +326,7 @@ __global__ __launch_bounds__((PassShape<R, N, LINES, OS>::THREADS)) void k_xinv_
             code[k] = (line < nl) ? tile_local<N>(sb, sp, sl, line, idx % N) : -2;
         }
     }
-    double acc[kXinvParts] = {0, 0, 0, 0, 0, 0};
+    double acc[kXinvParts] = {0, 0, 0};
     // round A: components x and y
 #pragma unroll
     for (int k = 0; k < KA; ++k) {
@@ -425,8 +424,8 @@ __global__ __launch_bounds__((MergedShape<R, N, LINES, OS>::THREADS)) void k_xfw
     }
 }
 
-// Writes the three output values of voxel vox / support index cd and accumulates the partial
-// sums {support sums of x, y, z, all-voxel sums of x, y, z}.
+// Writes the three output values of voxel vox / support index cd and accumulates the support
+// sums of x, y, z.
 __device__ __forceinline__ void xinv_emit3(const XSink& sink, long N3, long vox, int cd, double a, double b, double c,
                                            double (&acc)[kXinvParts]) {
     if (sink.field != nullptr) {
@@ -449,9 +448,6 @@ __device__ __forceinline__ void xinv_emit3(const XSink& sink, long N3, long vox,
         acc[1] += b;
         acc[2] += c;
     }
-    acc[3] += a;
-    acc[4] += b;
-    acc[5] += c;
 }
 
 // Merged-round x inverse: the x and y half spectra of the tile's lines and the z half spectra of
@@ -539,7 +535,7 @@ __global__ __launch_bounds__((MergedShape<R, N, LINES, OS>::THREADS)) void k_xin
     __syncthreads();
     sfft_lines<R, +1, N, LT, SH::T, LD>(buf, tw, LT);
     const R* bufr = reinterpret_cast<const R*>(buf);
-    double acc[kXinvParts] = {0, 0, 0, 0, 0, 0};
+    double acc[kXinvParts] = {0, 0, 0};
 #pragma unroll
     for (int k = 0; k < KE; ++k) {
         const int idx = threadIdx.x + k * THREADS;
@@ -791,7 +787,7 @@ __global__ __launch_bounds__(256) void k_xinv_w(const cx<R>* __restrict__ S, XSi
     const long line0 = __builtin_amdgcn_readfirstlane((int)(xcd_tile(blockIdx.x, gridDim.x) * kWaveLines)) + 2 * wave;
     const bool one = line0 < nlines;
     const bool two = line0 + 1 < nlines;
-    double acc[kXinvParts] = {0, 0, 0, 0, 0, 0};
+    double acc[kXinvParts] = {0, 0, 0};
     if (one) {
         const long oa = spec_line(SL, line0), ob = spec_line(SL, line0 + (two ? 1 : 0));
         // every spectrum value of the pair first
@@ -808,13 +804,21 @@ __global__ __launch_bounds__(256) void k_xinv_w(const cx<R>* __restrict__ S, XSi
 #pragma unroll
             for (int r = 0; r < RAD; ++r) loc[a][r] = ok ? wb.loc(lane, r) : -1;
         }
-        auto emit = [&](int c, int l, double val) {
+        // an FP32 sink takes the value scaled in the transform's precision (one multiply, no
+        // round trip through FP64); the support sums stay FP64
+        const R inv_r = (R)inv_n;
+        auto emit = [&](int c, int l, R t) {
             if (l >= 0) {
-                if (sink.compact != nullptr) sink.compact[3L * l + c] = val;
-                if (sink.compact32 != nullptr) sink.compact32[3L * l + c] = (float)val;
-                acc[c] += val;
+                if (sink.compact32 != nullptr) {
+                    const float v = (float)(t * inv_r);
+                    sink.compact32[3L * l + c] = v;
+                    acc[c] += (double)v;
+                } else {
+                    const double v = (double)t * inv_n;
+                    if (sink.compact != nullptr) sink.compact[3L * l + c] = v;
+                    acc[c] += v;
+                }
             }
-            acc[3 + c] += val;
         };
 #pragma unroll
         for (int f = 0; f < 3; ++f) {
@@ -824,11 +828,11 @@ __global__ __launch_bounds__(256) void k_xinv_w(const cx<R>* __restrict__ S, XSi
             for (int r = 0; r < RAD; ++r) {
                 const cx<R> t = L[lpad(lane + 64 * r)];
                 if (f < 2) {
-                    emit(0, loc[f][r], (double)t.x * inv_n);
-                    emit(1, loc[f][r], (double)t.y * inv_n);
+                    emit(0, loc[f][r], t.x);
+                    emit(1, loc[f][r], t.y);
                 } else {
-                    emit(2, loc[0][r], (double)t.x * inv_n);
-                    if (two) emit(2, loc[1][r], (double)t.y * inv_n);
+                    emit(2, loc[0][r], t.x);
+                    if (two) emit(2, loc[1][r], t.y);
                 }
             }
             wave_lds_sync();
