@@ -670,6 +670,21 @@ __device__ __forceinline__ void wave_store_halves(const cx<R>* L, cx<R>* Sa, cx<
     }
 }
 
+// Copies the run src[0, cnt) (cnt <= CAP, uniform across the wave) into the wave's LDS line with
+// every global load issued before the first LDS store waits on one.
+template <typename T, int CAP>
+__device__ __forceinline__ void wave_stage_run(const T* __restrict__ src, int cnt, T* dst, int lane) {
+    constexpr int K = (CAP + 63) / 64;
+    T t[K];
+    const int it = (cnt + 63) >> 6;
+#pragma unroll
+    for (int j = 0; j < K; ++j)
+        if (j < it) t[j] = src[min(lane + 64 * j, cnt - 1)];
+#pragma unroll
+    for (int j = 0; j < K; ++j)
+        if (j < it && lane + 64 * j < cnt) dst[lane + 64 * j] = t[j];
+}
+
 template <typename R, int N>
 __global__ __launch_bounds__(256) void k_xfwd_w(XSource src, cx<R>* __restrict__ S, long nlines, SpecLayout SL,
                                                 const cx<R>* __restrict__ twg, const int* skip) {
@@ -677,58 +692,77 @@ __global__ __launch_bounds__(256) void k_xfwd_w(XSource src, cx<R>* __restrict__
     constexpr int RAD = N / 64;
     extern __shared__ unsigned char smem_raw[];
     cx<R>* tw = reinterpret_cast<cx<R>*>(smem_raw);
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    // the wave index is uniform: the line numbers, support words and run bounds below are too
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
     cx<R>* L = tw + lpad_size<N>() + wave * lpad_size<N>();
     for (int i = threadIdx.x; i < N; i += 256) tw[lpad(i)] = twg[i];
     __syncthreads();
     const long line0 = __builtin_amdgcn_readfirstlane((int)(xcd_tile(blockIdx.x, gridDim.x) * kWaveLines)) + 2 * wave;
     if (line0 >= nlines) return;
     const bool two = line0 + 1 < nlines;
+    WaveLineBits<N> wb[2];
+    wb[0].load(src.lines, line0);
+    if (two) wb[1].load(src.lines, line0 + 1);
     double mc[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) mc[c] = (src.mean != nullptr) ? src.mean[c] : 0.0;
     // The pair's support voxels are one contiguous run [cs, ce) of the compact vector: it is staged
     // through the wave's LDS line with coalesced loads when it fits (else gathered directly).
     const long nb = src.lines.nb;
-    const long cs = src.lines.lstart[line0];
+    const long cs = wb[0].start;
     const long ce = (line0 + 2 < nlines) ? (long)src.lines.lstart[line0 + 2] : nb;
     const long cnt3 = 3 * (ce - cs);
     float* st32 = reinterpret_cast<float*>(L);
     double* st64 = reinterpret_cast<double*>(L);
-    constexpr long cap32 = (long)(lpad_size<N>() * sizeof(cx<R>) / sizeof(float));
-    constexpr long cap64 = (long)(lpad_size<N>() * sizeof(cx<R>) / sizeof(double));
+    constexpr int cap32 = (int)(lpad_size<N>() * sizeof(cx<R>) / sizeof(float));
+    constexpr int cap64 = (int)(lpad_size<N>() * sizeof(cx<R>) / sizeof(double));
     const bool staged32 = src.compact32 != nullptr && cnt3 <= cap32;
     const bool staged64 = src.compact32 == nullptr && src.compact != nullptr && cnt3 <= cap64;
-    if (staged32) {
-        const float* cc = src.compact32 + 3 * cs;
-        for (long e = lane; e < cnt3; e += 64) st32[e] = cc[e];
-    } else if (staged64) {
-        const double* cc = src.compact + 3 * cs;
-        for (long e = lane; e < cnt3; e += 64) st64[e] = cc[e];
-    }
+    if (staged32)
+        wave_stage_run<float, cap32>(src.compact32 + 3 * cs, (int)cnt3, st32, lane);
+    else if (staged64)
+        wave_stage_run<double, cap64>(src.compact + 3 * cs, (int)cnt3, st64, lane);
     wave_lds_sync();
     // every source value of the pair first: v[a][c][r] of voxel lane + 64 r of line line0 + a
     R val[2][3][RAD];
+    // from the stage: every lane reads an in-range slot and off-support voxels select zero
+    auto gather_staged = [&](auto&& read) {
 #pragma unroll
-    for (int a = 0; a < 2; ++a) {
-        WaveLineBits<N> wb;
-        const bool ok = a == 0 || two;
-        if (ok) wb.load(src.lines, line0 + a);
+        for (int a = 0; a < 2; ++a) {
 #pragma unroll
-        for (int r = 0; r < RAD; ++r) {
-            const int l = ok ? wb.loc(lane, r) : -1;
+            for (int r = 0; r < RAD; ++r) {
+                const int l = (a == 0 || two) ? wb[a].loc(lane, r) : -1;
+                const int e = l >= 0 ? 3 * (l - (int)cs) : 0;
 #pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                // an FP32 source has its mean subtracted in FP32 (fewer conversions and registers)
-                const long e = 3L * (l - cs) + c;
-                if (src.compact32 != nullptr) {
-                    float raw = 0.0f;
-                    if (l >= 0) raw = staged32 ? st32[e] : src.compact32[3L * l + c];
-                    val[a][c][r] = (l < 0) ? (R)0 : (R)raw - (R)mc[c];
-                } else {
-                    double raw = 0.0;
-                    if (l >= 0 && src.compact != nullptr) raw = staged64 ? st64[e] : src.compact[3L * l + c];
-                    val[a][c][r] = (l < 0) ? (R)0 : (R)(raw - mc[c]);
+                for (int c = 0; c < 3; ++c) {
+                    const R x = read(e + c, c);
+                    val[a][c][r] = l >= 0 ? x : (R)0;
+                }
+            }
+        }
+    };
+    if (staged32) {
+        // an FP32 source has its mean subtracted in FP32 (fewer conversions and registers)
+        const R mr[3] = {(R)mc[0], (R)mc[1], (R)mc[2]};
+        gather_staged([&](int e, int c) { return (R)st32[e] - mr[c]; });
+    } else if (staged64) {
+        gather_staged([&](int e, int c) { return (R)(st64[e] - mc[c]); });
+    } else {
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+#pragma unroll
+            for (int r = 0; r < RAD; ++r) {
+                const int l = (a == 0 || two) ? wb[a].loc(lane, r) : -1;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    if (src.compact32 != nullptr) {
+                        const float raw = (l >= 0) ? src.compact32[3L * l + c] : 0.0f;
+                        val[a][c][r] = (l < 0) ? (R)0 : (R)raw - (R)mc[c];
+                    } else {
+                        double raw = 0.0;
+                        if (l >= 0 && src.compact != nullptr) raw = src.compact[3L * l + c];
+                        val[a][c][r] = (l < 0) ? (R)0 : (R)(raw - mc[c]);
+                    }
                 }
             }
         }
@@ -780,7 +814,7 @@ __global__ __launch_bounds__(256) void k_xinv_w(const cx<R>* __restrict__ S, XSi
     constexpr int RAD = N / 64;
     extern __shared__ unsigned char smem_raw[];
     cx<R>* tw = reinterpret_cast<cx<R>*>(smem_raw);
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
     cx<R>* L = tw + lpad_size<N>() + wave * lpad_size<N>();
     for (int i = threadIdx.x; i < N; i += 256) tw[lpad(i)] = twg[i];
     __syncthreads();
