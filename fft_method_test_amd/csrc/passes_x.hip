@@ -807,6 +807,12 @@ __device__ __forceinline__ void wave_load_pair(const cx<R>* __restrict__ Ar, con
     }
 }
 
+// Bytes of a wave's output stage in the x-inverse pass: one transform line.
+template <typename R, int N>
+constexpr int wave_ostage_bytes() {
+    return (int)(lpad_size<N>() * sizeof(cx<R>));
+}
+
 template <typename R, int N>
 __global__ __launch_bounds__(256) void k_xinv_w(const cx<R>* __restrict__ S, XSink sink, long nlines, SpecLayout SL,
                                                 const cx<R>* __restrict__ twg, double inv_n, const int* skip) {
@@ -814,8 +820,11 @@ __global__ __launch_bounds__(256) void k_xinv_w(const cx<R>* __restrict__ S, XSi
     constexpr int RAD = N / 64;
     extern __shared__ unsigned char smem_raw[];
     cx<R>* tw = reinterpret_cast<cx<R>*>(smem_raw);
+    // the wave index is uniform: the line numbers, support words and run bounds below are too
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
     cx<R>* L = tw + lpad_size<N>() + wave * lpad_size<N>();
+    R* ost = reinterpret_cast<R*>(reinterpret_cast<unsigned char*>(tw + 5 * lpad_size<N>()) +
+                                  wave * wave_ostage_bytes<R, N>());
     for (int i = threadIdx.x; i < N; i += 256) tw[lpad(i)] = twg[i];
     __syncthreads();
     const long line0 = __builtin_amdgcn_readfirstlane((int)(xcd_tile(blockIdx.x, gridDim.x) * kWaveLines)) + 2 * wave;
@@ -830,46 +839,101 @@ __global__ __launch_bounds__(256) void k_xinv_w(const cx<R>* __restrict__ S, XSi
         if (two) wave_load_pair<R, N>(S + ob, S + SL.nc + ob, true, z[1], lane);
         wave_load_pair<R, N>(S + 2 * SL.nc + oa, S + 2 * SL.nc + ob, two, z[2], lane);
         int loc[2][RAD];
+        int cs = 0;
 #pragma unroll
         for (int a = 0; a < 2; ++a) {
             WaveLineBits<N> wb;
             const bool ok = a == 0 || two;
             if (ok) wb.load(sink.lines, line0 + a);
+            if (a == 0) cs = wb.start;
 #pragma unroll
             for (int r = 0; r < RAD; ++r) loc[a][r] = ok ? wb.loc(lane, r) : -1;
         }
-        // an FP32 sink takes the value scaled in the transform's precision (one multiply, no
-        // round trip through FP64); the support sums stay FP64
+        // The pair's support voxels are one contiguous run [cs, ce) of the compact vector.  A sink of
+        // the transform's precision whose run fits the wave's output stage takes the values there
+        // (off-support voxels write a dummy slot) and then gets one coalesced copy of the run; the
+        // support sums are taken during that copy.  Other sinks are written voxel by voxel.
+        R* sdst = sizeof(R) == 4 ? reinterpret_cast<R*>(sink.compact32) : reinterpret_cast<R*>(sink.compact);
+        if (sizeof(R) == 8 && sink.compact32 != nullptr) sdst = nullptr;
+        const int ce = (line0 + 2 < nlines) ? sink.lines.lstart[line0 + 2] : (int)sink.lines.nb;
+        const int cnt3 = 3 * (ce - cs);
+        constexpr int cap = wave_ostage_bytes<R, N>() / (int)sizeof(R) - 3;  // slots cap..cap+2: dummy
         const R inv_r = (R)inv_n;
-        auto emit = [&](int c, int l, R t) {
-            if (l >= 0) {
-                if (sink.compact32 != nullptr) {
-                    const float v = (float)(t * inv_r);
-                    sink.compact32[3L * l + c] = v;
-                    acc[c] += (double)v;
-                } else {
-                    const double v = (double)t * inv_n;
-                    if (sink.compact != nullptr) sink.compact[3L * l + c] = v;
-                    acc[c] += v;
+        if (sdst != nullptr && cnt3 <= cap) {
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int r = 0; r < RAD; ++r) loc[a][r] = loc[a][r] >= 0 ? 3 * (loc[a][r] - cs) : cap;
+#pragma unroll
+            for (int f = 0; f < 3; ++f) {
+                if (f == 1 && !two) continue;
+                wave_fft<R, +1, N>(z[f], L, tw, lane);
+#pragma unroll
+                for (int r = 0; r < RAD; ++r) {
+                    const cx<R> t = L[lpad(lane + 64 * r)];
+                    if (f < 2) {
+                        ost[loc[f][r]] = t.x * inv_r;
+                        ost[loc[f][r] + 1] = t.y * inv_r;
+                    } else {
+                        ost[loc[0][r] + 2] = t.x * inv_r;
+                        if (two) ost[loc[1][r] + 2] = t.y * inv_r;
+                    }
+                }
+                wave_lds_sync();
+            }
+            // element e of the run is component e % 3; lane + 64 j has component (lane + j) % 3
+            constexpr int K = (cap + 63) / 64;
+            const int it = (cnt3 + 63) >> 6;
+            R* dst = sdst + 3L * cs;
+            double a3[3] = {0, 0, 0};
+#pragma unroll
+            for (int j = 0; j < K; ++j) {
+                const int e = lane + 64 * j;
+                if (j < it && e < cnt3) {
+                    const R v = ost[e];
+                    dst[e] = v;
+                    a3[j % 3] += (double)v;
                 }
             }
-        };
+            const int c0 = lane % 3;
 #pragma unroll
-        for (int f = 0; f < 3; ++f) {
-            if (f == 1 && !two) continue;
-            wave_fft<R, +1, N>(z[f], L, tw, lane);
-#pragma unroll
-            for (int r = 0; r < RAD; ++r) {
-                const cx<R> t = L[lpad(lane + 64 * r)];
-                if (f < 2) {
-                    emit(0, loc[f][r], t.x);
-                    emit(1, loc[f][r], t.y);
-                } else {
-                    emit(2, loc[0][r], t.x);
-                    if (two) emit(2, loc[1][r], t.y);
-                }
+            for (int c = 0; c < 3; ++c) {
+                const int k = (c + 3 - c0) % 3;
+                acc[c] += k == 0 ? a3[0] : (k == 1 ? a3[1] : a3[2]);
             }
-            wave_lds_sync();
+        } else {
+            // an FP32 sink takes the value scaled in the transform's precision (one multiply, no
+            // round trip through FP64); the support sums stay FP64
+            auto emit = [&](int c, int l, R t) {
+                if (l >= 0) {
+                    if (sink.compact32 != nullptr) {
+                        const float v = (float)(t * inv_r);
+                        sink.compact32[3L * l + c] = v;
+                        acc[c] += (double)v;
+                    } else {
+                        const double v = (double)t * inv_n;
+                        if (sink.compact != nullptr) sink.compact[3L * l + c] = v;
+                        acc[c] += v;
+                    }
+                }
+            };
+#pragma unroll
+            for (int f = 0; f < 3; ++f) {
+                if (f == 1 && !two) continue;
+                wave_fft<R, +1, N>(z[f], L, tw, lane);
+#pragma unroll
+                for (int r = 0; r < RAD; ++r) {
+                    const cx<R> t = L[lpad(lane + 64 * r)];
+                    if (f < 2) {
+                        emit(0, loc[f][r], t.x);
+                        emit(1, loc[f][r], t.y);
+                    } else {
+                        emit(2, loc[0][r], t.x);
+                        if (two) emit(2, loc[1][r], t.y);
+                    }
+                }
+                wave_lds_sync();
+            }
         }
     }
     if (sink.partials != nullptr) block_sum<kXinvParts>(acc, sink.partials + (long)kXinvParts * blockIdx.x);
@@ -878,6 +942,11 @@ __global__ __launch_bounds__(256) void k_xinv_w(const cx<R>* __restrict__ S, XSi
 template <typename R, int N>
 constexpr size_t wave_pass_lds() {
     return (size_t)(5 * lpad_size<N>()) * sizeof(cx<R>);
+}
+// the x-inverse adds an output stage per wave
+template <typename R, int N>
+constexpr size_t wave_xinv_lds() {
+    return wave_pass_lds<R, N>() + 4 * (size_t)wave_ostage_bytes<R, N>();
 }
 
 // The wave passes serve the solver's applies: a compact (FP64 or FP32) source without background
@@ -943,7 +1012,7 @@ long xinv_variant(const cx<R>* S, const XSink& sink, const Geometry3& g, const c
             return -1;
         } else {
             if (!wave_sink_ok(sink)) return xinv_variant<R, N, 2>(S, sink, g, tw, skip, st);
-            constexpr size_t lds = wave_pass_lds<R, N>();
+            constexpr size_t lds = wave_xinv_lds<R, N>();
             if (lds > lds_limit()) return -1;
             hipLaunchKernelGGL((k_xinv_w<R, N>), dim3((unsigned)blocks), dim3(256), lds, st, S, sink, nlines,
                                spec_layout(g), tw, inv_n, skip);

@@ -142,6 +142,58 @@ def test_x_wave_variant_odd_line_count(fp, oracle_mod, precision):
     green.close()
 
 
+def _run_mix_mask(n, ny, nz, seed):
+    """Fluid mask whose x-line pairs (the unit of the wave passes) have support runs of three
+    kinds: dense pairs (two solid voxels in three, all on the interface: longer than a wave's LDS
+    line), pairs with one short solid block (runs that fit) and all-fluid pairs (empty runs)."""
+    rng = np.random.default_rng(seed)
+    m = np.ones((nz, ny, n), np.uint8)
+    x = np.arange(n)
+    for z in range(nz):
+        for y in range(ny):
+            kind = ((y + ny * z) // 2) % 3
+            if kind == 0:
+                m[z, y] = x % 3 == 0
+            elif kind == 1:
+                s = rng.integers(0, n - n // 8)
+                m[z, y, s:s + n // 8] = 0
+    return m.reshape(-1)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("n", [128, 256, 512])
+@pytest.mark.parametrize("ny,nz", [(4, 6), (5, 3)])
+def test_x_wave_staged_and_direct_runs(fp, oracle_mod, precision, n, ny, nz):
+    """The wave x passes stage a line pair's compact run through LDS when it fits one transform
+    line (forward: source run; inverse: output run) and read / write it voxel by voxel otherwise.
+    A medium with long, short and empty runs (and an odd line count) exercises both paths in
+    one apply: FP64 operator within 1e-12, FP32 operator within FP32_APPLY_BOUND of the oracle."""
+    from fft_method_test_amd import _lib, permeability as P
+
+    shape = (n, ny, nz)
+    mask = _run_mix_mask(n, ny, nz, seed=n + ny)
+    o = oracle_mod.Oracle(shape)
+    o.set_medium(mask)
+    act, _ = o.support()
+    per_line = np.bincount(act.astype(np.int64) // n, minlength=ny * nz)
+    pairs = per_line[0::2] + np.append(per_line[1::2], [0] * (len(per_line) % 2))
+    line_floats = (n + n // 16) * 2  # one padded transform line of FP32 complex values, in floats
+    assert (pairs == 0).any() and ((pairs > 0) & (3 * pairs <= line_floats - 3)).any()
+    assert (3 * pairs > line_floats).any()
+    g = P.Grid3D(*shape)
+    green = P.GreenOperator(g, 1.0, 2, precision)
+    sup = P.ForceSupport(P.BinaryMedium(g, mask), P.SUPPORT_INTERFACE, green)
+    op = P.VariationalOperator(sup, green)
+    x = np.random.default_rng(n).standard_normal(sup.size())
+    ref = o.apply(x)
+    with pinned(green, _lib.PASS_XFWD, precision, 88), pinned(green, _lib.PASS_XINV, precision, 88):
+        if precision == 0:
+            assert rel(op.apply(x), ref) < 1e-12
+        else:
+            assert rel(op.apply_fp32(x), ref) < FP32_APPLY_BOUND
+    green.close()
+
+
 @pytest.mark.parametrize("n", [32, 64, 128, 256, 512])
 def test_y_pass_variants_fp64(fp, oracle_mod, n):
     from fft_method_test_amd import _lib
