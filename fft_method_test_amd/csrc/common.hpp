@@ -21,13 +21,32 @@ __device__ __forceinline__ cx<R> mkc(R re, R im) {
     c.y = im;
     return c;
 }
+// The arithmetic of the transforms and of the Green multiply fixes its own rounding: products that
+// are fused say so (fmar), and contraction is off for everything else, so that two kernels built
+// from the same helpers round identically (the compiler would otherwise choose per kernel which
+// product of a * b - c * d to fuse).
 template <typename R>
-__device__ __forceinline__ cx<R> cadd(cx<R> a, cx<R> b) { return mkc<R>(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ R fmar(R a, R b, R c);
+template <>
+__device__ __forceinline__ float fmar<float>(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+template <>
+__device__ __forceinline__ double fmar<double>(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
 template <typename R>
-__device__ __forceinline__ cx<R> csub(cx<R> a, cx<R> b) { return mkc<R>(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ cx<R> cadd(cx<R> a, cx<R> b) {
+#pragma clang fp contract(off)
+    return mkc<R>(a.x + b.x, a.y + b.y);
+}
+template <typename R>
+__device__ __forceinline__ cx<R> csub(cx<R> a, cx<R> b) {
+#pragma clang fp contract(off)
+    return mkc<R>(a.x - b.x, a.y - b.y);
+}
+// (a.x b.x - a.y b.y, a.x b.y + a.y b.x) with the first product of each part fused
 template <typename R>
 __device__ __forceinline__ cx<R> cmul(cx<R> a, cx<R> b) {
-    return mkc<R>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+#pragma clang fp contract(off)
+    return mkc<R>(fmar<R>(a.x, b.x, -(a.y * b.y)), fmar<R>(a.x, b.y, a.y * b.x));
 }
 template <typename R>
 __device__ __forceinline__ cx<R> conjc(cx<R> a) { return mkc<R>(a.x, -a.y); }

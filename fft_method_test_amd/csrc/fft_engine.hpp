@@ -68,7 +68,7 @@ __device__ __forceinline__ cx<R> tw16(cx<R> v) {
     } else {
         const R c = (R)kCos16[e];
         const R s = (R)(SIGN < 0 ? -kSin16[e] : kSin16[e]);
-        return mkc<R>(v.x * c - v.y * s, v.x * s + v.y * c);
+        return cmul<R>(v, mkc<R>(c, s));
     }
 }
 

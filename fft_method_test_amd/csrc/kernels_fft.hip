@@ -102,6 +102,36 @@ __device__ __forceinline__ double herm_weight(int kx, int nx) {
     return (2 * kx > nx) ? 0.0 : ((kx == 0 || 2 * kx == nx) ? 1.0 : 2.0);
 }
 
+// Symmetric Green multiply g = T f of one frequency (src/bignonnet/GreenOperator.cpp:84-92 with
+// G_sym; T = {xx, xy, xz, yy, yz, zz}) and its quadratic form conj(f) . g in FP64, both in a fixed
+// operation order (g_i = T_iz f_z + (T_iy f_y + T_ix f_x), products fused in turn), so that every
+// z-pass kernel rounds them alike.
+template <typename R>
+__device__ __forceinline__ void green_mul(R txx, R txy, R txz, R tyy, R tyz, R tzz, cx<R> fx, cx<R> fy, cx<R> fz,
+                                          cx<R>& gx, cx<R>& gy, cx<R>& gz) {
+#pragma clang fp contract(off)
+    gx = mkc<R>(fmar<R>(txz, fz.x, fmar<R>(txy, fy.x, txx * fx.x)), fmar<R>(txz, fz.y, fmar<R>(txy, fy.y, txx * fx.y)));
+    gy = mkc<R>(fmar<R>(tyz, fz.x, fmar<R>(tyy, fy.x, txy * fx.x)), fmar<R>(tyz, fz.y, fmar<R>(tyy, fy.y, txy * fx.y)));
+    gz = mkc<R>(fmar<R>(tzz, fz.x, fmar<R>(tyz, fy.x, txz * fx.x)), fmar<R>(tzz, fz.y, fmar<R>(tyz, fy.y, txz * fx.y)));
+}
+template <typename R>
+__device__ __forceinline__ double green_quad(cx<R> fx, cx<R> fy, cx<R> fz, cx<R> gx, cx<R> gy, cx<R> gz) {
+#pragma clang fp contract(off)
+    double q = (double)fx.x * (double)gx.x;
+    q = fmar<double>((double)fx.y, (double)gx.y, q);
+    q = fmar<double>((double)fy.x, (double)gy.x, q);
+    q = fmar<double>((double)fy.y, (double)gy.y, q);
+    q = fmar<double>((double)fz.x, (double)gz.x, q);
+    q = fmar<double>((double)fz.y, (double)gz.y, q);
+    return q;
+}
+template <typename R>
+__device__ __forceinline__ void green_quad_add(double& acc, double w, cx<R> fx, cx<R> fy, cx<R> fz, cx<R> gx,
+                                               cx<R> gy, cx<R> gz) {
+#pragma clang fp contract(off)
+    acc = acc + w * green_quad<R>(fx, fy, fz, gx, gy, gz);
+}
+
 // Fused z pass: lines 0..3*ZL-1 of the LDS buffer hold components x | y | z of ZL columns.
 // OS = threads per butterfly thread (1 or 2): extra threads only help the memory phases
 template <typename R, int N, int ZL, int OS>
@@ -189,15 +219,13 @@ __global__ __launch_bounds__((zgreen_threads<R, N, ZL, OS>())) void k_zgreen_s(c
             cx<R>* py = buf + (1 * ZL + mci) * LD + e;
             cx<R>* pz = buf + (2 * ZL + mci) * LD + e;
             const cx<R> fx = *px, fy = *py, fz = *pz;
-            const cx<R> gx = mkc<R>(txx * fx.x + txy * fy.x + txz * fz.x, txx * fx.y + txy * fy.y + txz * fz.y);
-            const cx<R> gy = mkc<R>(txy * fx.x + tyy * fy.x + tyz * fz.x, txy * fx.y + tyy * fy.y + tyz * fz.y);
-            const cx<R> gz = mkc<R>(txz * fx.x + tyz * fy.x + tzz * fz.x, txz * fx.y + tyz * fy.y + tzz * fz.y);
+            cx<R> gx, gy, gz;
+            green_mul<R>(txx, txy, txz, tyy, tyz, tzz, fx, fy, fz, gx, gy, gz);
             *px = gx;
             *py = gy;
             *pz = gz;
             if (qpart != nullptr)
-                q[0] += (double)fx.x * gx.x + (double)fx.y * gx.y + (double)fy.x * gy.x + (double)fy.y * gy.y +
-                        (double)fz.x * gz.x + (double)fz.y * gz.y;
+                green_quad_add<R>(q[0], 1.0, fx, fy, fz, gx, gy, gz);
         }
         if (qpart != nullptr) {
             q[0] *= herm_weight((int)((col0 + mci) % rp), nx);
@@ -296,15 +324,13 @@ __global__ __launch_bounds__((zgreen_threads<R, N, ZL, OS>())) void k_zgreen_sb(
                 cx<R>* py = buf + (1 * ZL + mci) * LD + e;
                 cx<R>* pz = buf + (2 * ZL + mci) * LD + e;
                 const cx<R> fx = *px, fy = *py, fz = *pz;
-                const cx<R> gx = mkc<R>(txx * fx.x + txy * fy.x + txz * fz.x, txx * fx.y + txy * fy.y + txz * fz.y);
-                const cx<R> gy = mkc<R>(txy * fx.x + tyy * fy.x + tyz * fz.x, txy * fx.y + tyy * fy.y + tyz * fz.y);
-                const cx<R> gz = mkc<R>(txz * fx.x + tyz * fy.x + tzz * fz.x, txz * fx.y + tyz * fy.y + tzz * fz.y);
+                cx<R> gx, gy, gz;
+                green_mul<R>(txx, txy, txz, tyy, tyz, tzz, fx, fy, fz, gx, gy, gz);
                 *px = gx;
                 *py = gy;
                 *pz = gz;
                 if (qpart != nullptr)
-                    q[0] += (double)fx.x * gx.x + (double)fx.y * gx.y + (double)fy.x * gy.x + (double)fy.y * gy.y +
-                            (double)fz.x * gz.x + (double)fz.y * gz.y;
+                    green_quad_add<R>(q[0], 1.0, fx, fy, fz, gx, gy, gz);
             }
             if (qpart != nullptr) {
                 q[0] *= herm_weight((int)((col0 + mci) % rp), nx);
@@ -384,12 +410,10 @@ __global__ __launch_bounds__((zgreen_c_threads<N, ZL, OS>())) void k_zgreen_c(cx
         for (int m = 0; m < 6; ++m) t6[m] = own ? gp[m] : (R)0;
         const R txx = t6[0], txy = t6[1], txz = t6[2], tyy = t6[3], tyz = t6[4], tzz = t6[5];
         const cx<R> fx = f[0][k], fy = f[1][k], fz = f[2][k];
-        const cx<R> gx = mkc<R>(txx * fx.x + txy * fy.x + txz * fz.x, txx * fx.y + txy * fy.y + txz * fz.y);
-        const cx<R> gy = mkc<R>(txy * fx.x + tyy * fy.x + tyz * fz.x, txy * fx.y + tyy * fy.y + tyz * fz.y);
-        const cx<R> gz = mkc<R>(txz * fx.x + tyz * fy.x + tzz * fz.x, txz * fx.y + tyz * fy.y + tzz * fz.y);
+        cx<R> gx, gy, gz;
+        green_mul<R>(txx, txy, txz, tyy, tyz, tzz, fx, fy, fz, gx, gy, gz);
         if (qpart != nullptr)
-            q[0] += (double)fx.x * gx.x + (double)fx.y * gx.y + (double)fy.x * gy.x + (double)fy.y * gy.y +
-                    (double)fz.x * gz.x + (double)fz.y * gz.y;
+            green_quad_add<R>(q[0], 1.0, fx, fy, fz, gx, gy, gz);
         f[0][k] = gx;
         f[1][k] = gy;
         f[2][k] = gz;
@@ -527,16 +551,13 @@ __global__ __launch_bounds__((zgreen_p_threads<R, N, ZL>())) void k_zgreen_p(cx<
             cx<R>* py = buf + (1 * ZL + mci) * LD + e;
             cx<R>* pz = buf + (2 * ZL + mci) * LD + e;
             const cx<R> fx = *px, fy = *py, fz = *pz;
-            const cx<R> gx = mkc<R>(txx * fx.x + txy * fy.x + txz * fz.x, txx * fx.y + txy * fy.y + txz * fz.y);
-            const cx<R> gy = mkc<R>(txy * fx.x + tyy * fy.x + tyz * fz.x, txy * fx.y + tyy * fy.y + tyz * fz.y);
-            const cx<R> gz = mkc<R>(txz * fx.x + tyz * fy.x + tzz * fz.x, txz * fx.y + tyz * fy.y + tzz * fz.y);
+            cx<R> gx, gy, gz;
+            green_mul<R>(txx, txy, txz, tyy, tyz, tzz, fx, fy, fz, gx, gy, gz);
             *px = gx;
             *py = gy;
             *pz = gz;
             if (qpart != nullptr)
-                q[0] += herm_weight((int)((col0 + mci) % rp), nx) *
-                        ((double)fx.x * gx.x + (double)fx.y * gx.y + (double)fy.x * gy.x + (double)fy.y * gy.y +
-                         (double)fz.x * gz.x + (double)fz.y * gz.y);
+                green_quad_add<R>(q[0], herm_weight((int)((col0 + mci) % rp), nx), fx, fy, fz, gx, gy, gz);
         }
         __syncthreads();
         sfft_lines_plan<R, +1, N, L3, T, LD, PLAN>(buf, tw, L3);
@@ -621,12 +642,10 @@ __global__ __launch_bounds__(ZL * N / 4) void k_zgreen_r(cx<R>* __restrict__ S, 
 #pragma unroll
         for (int q = 0; q < 6; ++q) t6[q] = own ? gp[q] : (R)0;
         const R txx = t6[0], txy = t6[1], txz = t6[2], tyy = t6[3], tyz = t6[4], tzz = t6[5];
-        const cx<R> gx = mkc<R>(txx * fx.x + txy * fy.x + txz * fz.x, txx * fx.y + txy * fy.y + txz * fz.y);
-        const cx<R> gy = mkc<R>(txy * fx.x + tyy * fy.x + tyz * fz.x, txy * fx.y + tyy * fy.y + tyz * fz.y);
-        const cx<R> gz = mkc<R>(txz * fx.x + tyz * fy.x + tzz * fz.x, txz * fx.y + tyz * fy.y + tzz * fz.y);
+        cx<R> gx, gy, gz;
+        green_mul<R>(txx, txy, txz, tyy, tyz, tzz, fx, fy, fz, gx, gy, gz);
         if (qpart != nullptr)
-            qf[0] += (double)fx.x * gx.x + (double)fx.y * gx.y + (double)fy.x * gy.x + (double)fy.y * gy.y +
-                     (double)fz.x * gz.x + (double)fz.y * gz.y;
+            green_quad_add<R>(qf[0], 1.0, fx, fy, fz, gx, gy, gz);
         f[0][r] = gx;
         f[1][r] = gy;
         f[2][r] = gz;
@@ -1213,16 +1232,13 @@ __global__ void k_zgreen(cx<R>* __restrict__ S, const R* __restrict__ G6, int nz
         cx<R>* py = buf + (1 * ncl + ci) * ld + e;
         cx<R>* pz = buf + (2 * ncl + ci) * ld + e;
         const cx<R> fx = *px, fy = *py, fz = *pz;
-        const cx<R> gx = mkc<R>(txx * fx.x + txy * fy.x + txz * fz.x, txx * fx.y + txy * fy.y + txz * fz.y);
-        const cx<R> gy = mkc<R>(txy * fx.x + tyy * fy.x + tyz * fz.x, txy * fx.y + tyy * fy.y + tyz * fz.y);
-        const cx<R> gz = mkc<R>(txz * fx.x + tyz * fy.x + tzz * fz.x, txz * fx.y + tyz * fy.y + tzz * fz.y);
+        cx<R> gx, gy, gz;
+        green_mul<R>(txx, txy, txz, tyy, tyz, tzz, fx, fy, fz, gx, gy, gz);
         *px = gx;
         *py = gy;
         *pz = gz;
         if (qpart != nullptr)
-            q[0] += herm_weight((int)((col0 + ci) % rp), nx) *
-                    ((double)fx.x * gx.x + (double)fx.y * gx.y + (double)fy.x * gy.x + (double)fy.y * gy.y +
-                     (double)fz.x * gz.x + (double)fz.y * gz.y);
+            green_quad_add<R>(q[0], herm_weight((int)((col0 + ci) % rp), nx), fx, fy, fz, gx, gy, gz);
     }
     if (qpart != nullptr) block_sum<1>(q, qpart + blockIdx.x);
     __syncthreads();

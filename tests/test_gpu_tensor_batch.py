@@ -65,6 +65,31 @@ def test_tensor_batch_matches_sequential(fp, case, tol, maxit):
         assert all(f[2] == 1 for f in fs)
 
 
+@pytest.mark.parametrize("zcode", [42, 81])
+def test_tensor_batch_lockstep_z_kernel_matches(fp, zcode):
+    """Mode 2 serves the three directions with one z launch (a separate multi-side kernel) when the
+    tuned z tile has that form; the single-direction solve uses the single-side kernel.  With such a
+    tile pinned, the two solves still agree bit for bit: both kernels take their arithmetic from the
+    same fixed-order helpers."""
+    from fft_method_test_amd import _lib, geometry as G, permeability as P
+
+    L = _lib.load()
+    mask = G.sphere_cell(32).reshape(-1)
+    res = {}
+    green = None
+    for mode in (0, 2):
+        green = _green((32, 32, 32), mask, mode)
+        _lib.check(L.fftperm_pin_variant(green._h, _lib.PASS_ZGREEN, _lib.PRECISION_MIXED, zcode))
+        K, infos = P.solve_permeability_tensor(green, 1e-10, 20000)
+        res[mode] = (K, _fields(infos))
+        if mode == 0:
+            green.close()
+    _lib.check(L.fftperm_pin_variant(green._h, _lib.PASS_ZGREEN, _lib.PRECISION_MIXED, 0))
+    green.close()
+    assert res[2][1] == res[0][1]
+    assert np.array_equal(res[2][0], res[0][0])
+
+
 def test_tensor_batch_repeat_and_other_solvers(fp, oracle_mod):
     """A second batched solve on the same context reproduces the first (state, spectra and vectors
     of the first do not leak); the switch leaves the other MIXED solvers on their own path; K of the
