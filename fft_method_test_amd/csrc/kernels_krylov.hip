@@ -498,6 +498,28 @@ void launch_cg_finish_mean_in(const double* partials, CgState* st, double inv_nb
     hipLaunchKernelGGL(k_cg_finish_mean_in, dim3(1), dim3(kRedThreads), 0, s, partials, st, inv_nb);
 }
 
+// Column sums of n rows of NV doubles (row p at base + stride * p) over the block's threads, each
+// thread adding its rows in the order p = t, t + B, t + 2B, ... with UNR rows' loads in flight (a
+// plain strided loop waits on every load in turn; the finishes are single-workgroup kernels that
+// read thousands of partial rows, so that latency is their whole cost).
+template <int NV, int UNR>
+__device__ __forceinline__ void sum_rows(const double* __restrict__ base, int stride, int n, double (&acc)[NV]) {
+    for (int p0 = threadIdx.x; p0 < n; p0 += UNR * blockDim.x) {
+        double v[UNR][NV];
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) {
+            const int p = p0 + u * (int)blockDim.x;
+#pragma unroll
+            for (int j = 0; j < NV; ++j) v[u][j] = p < n ? base[(long)stride * p + j] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < UNR; ++u)
+#pragma unroll
+            for (int j = 0; j < NV; ++j) acc[j] += v[u][j];
+    }
+}
+constexpr int kFinishThreads = 1024;
+
 // ------------------------------------------------------------ CR with reliable updates
 // Batched launches serve nrhs independent right-hand sides at once: RHS b owns st[b] and the
 // partial rows at b * (its stride); the finish kernels take one workgroup per RHS, the vector
@@ -509,14 +531,10 @@ __global__ void k_cr_beta(const double* __restrict__ xpart, int nxparts, long xs
     xpart += blockIdx.x * xstride;
     zpart += blockIdx.x * zstride;
     if (st->done) return;
-    double acc[4] = {0, 0, 0, 0};
-    for (int p = threadIdx.x; p < nxparts; p += blockDim.x) {
-        const double* q = xpart + (long)kXinvParts * p;
-        acc[0] += q[0];
-        acc[1] += q[1];
-        acc[2] += q[2];
-    }
-    for (int p = threadIdx.x; p < nzparts; p += blockDim.x) acc[3] += zpart[p];
+    double ax[3] = {0, 0, 0}, az[1] = {0};
+    sum_rows<3, 4>(xpart, kXinvParts, nxparts, ax);
+    sum_rows<1, 4>(zpart, 1, nzparts, az);
+    double acc[4] = {ax[0], ax[1], ax[2], az[0]};
     __shared__ double res[4];
     block_sum<4>(acc, res);
     if (threadIdx.x == 0) {
@@ -543,7 +561,7 @@ void launch_cr_beta(const double* xpart, int nxparts, const double* zpart, int n
 
 void launch_cr_beta_n(const double* xpart, int nxparts, long xstride, const double* zpart, int nzparts, long zstride,
                       CrState* st, int nrhs, double inv_nb, double inv_n, hipStream_t s) {
-    hipLaunchKernelGGL(k_cr_beta, dim3(nrhs), dim3(1024), 0, s, xpart, nxparts, xstride, zpart, nzparts, zstride, st,
+    hipLaunchKernelGGL(k_cr_beta, dim3(nrhs), dim3(kFinishThreads), 0, s, xpart, nxparts, xstride, zpart, nzparts, zstride, st,
                        inv_nb, inv_n);
 }
 
@@ -576,7 +594,7 @@ void launch_cr_direction(double* p, double* ap, const double* r, const double* q
 // alpha = s / Ap.Ap from the per-block partials; one block.
 __device__ __forceinline__ void cr_alpha_finish(const double* __restrict__ partials, int nparts, CrState* st) {
     double acc[1] = {0.0};
-    for (int p = threadIdx.x; p < nparts; p += blockDim.x) acc[0] += partials[p];
+    sum_rows<1, 4>(partials, 1, nparts, acc);
     __shared__ double res[1];
     block_sum<1>(acc, res);
     if (threadIdx.x == 0) {
@@ -601,7 +619,7 @@ __global__ void k_cr_alpha(const double* __restrict__ partials, long pstride, Cr
 void launch_cr_alpha(const double* partials, CrState* st, hipStream_t s) { launch_cr_alpha_n(partials, 0, st, 1, s); }
 
 void launch_cr_alpha_n(const double* partials, long pstride, CrState* st, int nrhs, hipStream_t s) {
-    hipLaunchKernelGGL(k_cr_alpha, dim3(nrhs), dim3(kRedThreads), 0, s, partials, pstride, st);
+    hipLaunchKernelGGL(k_cr_alpha, dim3(nrhs), dim3(kFinishThreads), 0, s, partials, pstride, st);
 }
 
 __global__ __launch_bounds__(kFlatThreads) void k_cr_update(double* __restrict__ x, double* __restrict__ r,
@@ -634,10 +652,7 @@ void launch_cr_update(double* x, double* r, const double* p, const double* ap, C
 __device__ __forceinline__ void cr_check_finish(const double* __restrict__ partials, int nparts, CrState* st,
                                                 double* __restrict__ history, double inv_nb) {
     double acc[4] = {0, 0, 0, 0};
-    for (int p = threadIdx.x; p < nparts; p += blockDim.x) {
-#pragma unroll
-        for (int j = 0; j < 4; ++j) acc[j] += partials[4 * p + j];
-    }
+    sum_rows<4, 4>(partials, 4, nparts, acc);
     __shared__ double res[4];
     block_sum<4>(acc, res);
     if (threadIdx.x != 0) return;
@@ -678,7 +693,7 @@ void launch_cr_check(const double* partials, CrState* st, double* history, doubl
 
 void launch_cr_check_n(const double* partials, long pstride, CrState* st, double* history, long hstride, int nrhs,
                        double inv_nb, hipStream_t s) {
-    hipLaunchKernelGGL(k_cr_check, dim3(nrhs), dim3(kRedThreads), 0, s, partials, pstride, st, history, hstride,
+    hipLaunchKernelGGL(k_cr_check, dim3(nrhs), dim3(kFinishThreads), 0, s, partials, pstride, st, history, hstride,
                        inv_nb);
 }
 
