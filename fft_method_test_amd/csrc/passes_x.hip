@@ -582,7 +582,7 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-template <typename R, int SIGN, int N, int NS>
+template <typename R, int SIGN, int N, int NS, bool KEEP>
 __device__ __forceinline__ void wave_fft_stage(cx<R> (&v)[N / 64], cx<R>* L, const cx<R>* tw, int lane) {
     constexpr int RAD = N / 64;
     if constexpr (NS < N) {
@@ -600,19 +600,23 @@ __device__ __forceinline__ void wave_fft_stage(cx<R> (&v)[N / 64], cx<R>* L, con
             wave_lds_sync();
         }
         dft_small<R, SIGN, RAD>(v);
+        // the last stage (NS = N / RAD = 64) writes element lane + 64 r from v[r]: with KEEP the
+        // spectrum stays in the registers, already in that order
+        if constexpr (KEEP && NS * RAD == N) return;
         const int base = (lane / NS) * NS * RAD + k;
 #pragma unroll
         for (int r = 0; r < RAD; ++r) L[lpad(base + r * NS)] = v[r];
         wave_lds_sync();
-        wave_fft_stage<R, SIGN, N, NS * RAD>(v, L, tw, lane);
+        wave_fft_stage<R, SIGN, N, NS * RAD, KEEP>(v, L, tw, lane);
     }
 }
 
 // Transform of one line given as v[r] = x[lane + 64 r]; the spectrum is left in L in natural order
-// (element k at L[lpad(k)]; the twiddle table padded the same way).
-template <typename R, int SIGN, int N>
+// (element k at L[lpad(k)]; the twiddle table padded the same way), or with KEEP in the registers
+// (v[r] = X[lane + 64 r]; L then holds nothing of it).
+template <typename R, int SIGN, int N, bool KEEP = false>
 __device__ __forceinline__ void wave_fft(cx<R> (&v)[N / 64], cx<R>* L, const cx<R>* tw, int lane) {
-    wave_fft_stage<R, SIGN, N, 1>(v, L, tw, lane);
+    wave_fft_stage<R, SIGN, N, 1, KEEP>(v, L, tw, lane);
 }
 
 // Support words of x line `line` (uniform across the wave) and their in-line prefix counts;
@@ -867,10 +871,10 @@ __global__ __launch_bounds__(256) void k_xinv_w(const cx<R>* __restrict__ S, XSi
 #pragma unroll
             for (int f = 0; f < 3; ++f) {
                 if (f == 1 && !two) continue;
-                wave_fft<R, +1, N>(z[f], L, tw, lane);
+                wave_fft<R, +1, N, true>(z[f], L, tw, lane);
 #pragma unroll
                 for (int r = 0; r < RAD; ++r) {
-                    const cx<R> t = L[lpad(lane + 64 * r)];
+                    const cx<R> t = z[f][r];
                     if (f < 2) {
                         ost[loc[f][r]] = t.x * inv_r;
                         ost[loc[f][r] + 1] = t.y * inv_r;
@@ -920,10 +924,10 @@ __global__ __launch_bounds__(256) void k_xinv_w(const cx<R>* __restrict__ S, XSi
 #pragma unroll
             for (int f = 0; f < 3; ++f) {
                 if (f == 1 && !two) continue;
-                wave_fft<R, +1, N>(z[f], L, tw, lane);
+                wave_fft<R, +1, N, true>(z[f], L, tw, lane);
 #pragma unroll
                 for (int r = 0; r < RAD; ++r) {
-                    const cx<R> t = L[lpad(lane + 64 * r)];
+                    const cx<R> t = z[f][r];
                     if (f < 2) {
                         emit(0, loc[f][r], t.x);
                         emit(1, loc[f][r], t.y);
