@@ -657,16 +657,19 @@ struct WaveLineBits {
     }
 };
 
-// Two-for-one split of the wave's spectrum Z (natural order in L) into the half spectra of its
-// real and imaginary parts, stored at rows oa (real part) and ob (imaginary part; ob < 0: none).
+// Two-for-one split of the wave's spectrum Z (natural order in L, and v[r] = Z[lane + 64 r] as the
+// last stage left it) into the half spectra of its real and imaginary parts, stored at rows oa
+// (real part) and ob (imaginary part; ob < 0: none).
 template <typename R, int N>
-__device__ __forceinline__ void wave_store_halves(const cx<R>* L, cx<R>* Sa, cx<R>* Sb, bool has_b, int lane) {
+__device__ __forceinline__ void wave_store_halves(const cx<R>* L, const cx<R> (&v)[N / 64], cx<R>* Sa, cx<R>* Sb,
+                                                  bool has_b, int lane) {
     constexpr int NXH = N / 2 + 1;
+    static_assert((NXH + 63) / 64 <= N / 64, "the kept half lies in the registers");
 #pragma unroll
     for (int q = 0; q < (NXH + 63) / 64; ++q) {
         const int kx = lane + 64 * q;
         if (kx < NXH) {
-            const cx<R> zk = L[lpad(kx)];
+            const cx<R> zk = v[q];
             const cx<R> zm = L[lpad((N - kx) % N)];
             Sa[kx] = mkc<R>((R)0.5 * (zk.x + zm.x), (R)0.5 * (zk.y - zm.y));
             if (has_b) Sb[kx] = mkc<R>((R)0.5 * (zk.y + zm.y), (R)0.5 * (zm.x - zk.x));
@@ -781,13 +784,13 @@ __global__ __launch_bounds__(256) void k_xfwd_w(XSource src, cx<R>* __restrict__
         for (int r = 0; r < RAD; ++r) v[r] = mkc<R>(val[a][0][r], val[a][1][r]);
         wave_fft<R, -1, N>(v, L, tw, lane);
         const long o = a == 0 ? oa : ob;
-        wave_store_halves<R, N>(L, S + o, S + SL.nc + o, true, lane);
+        wave_store_halves<R, N>(L, v, S + o, S + SL.nc + o, true, lane);
         wave_lds_sync();
     }
 #pragma unroll
     for (int r = 0; r < RAD; ++r) v[r] = mkc<R>(val[0][2][r], val[1][2][r]);
     wave_fft<R, -1, N>(v, L, tw, lane);
-    wave_store_halves<R, N>(L, S + 2 * SL.nc + oa, S + 2 * SL.nc + ob, two, lane);
+    wave_store_halves<R, N>(L, v, S + 2 * SL.nc + oa, S + 2 * SL.nc + ob, two, lane);
 }
 
 // Z = A + i B over the Hermitian completions of the half-spectrum rows A and B at element
