@@ -820,8 +820,15 @@ constexpr int wave_ostage_bytes() {
     return (int)(lpad_size<N>() * sizeof(cx<R>));
 }
 
+// Waves per SIMD the compiler must allow: the FP32 256 x-inverse fits 80 VGPRs without spilling
+// (6 waves instead of 5); the others keep the compiler's choice.
 template <typename R, int N>
-__global__ __launch_bounds__(256) void k_xinv_w(const cx<R>* __restrict__ S, XSink sink, long nlines, SpecLayout SL,
+constexpr int xinv_w_min_waves() {
+    return sizeof(R) == 4 && N == 256 ? 6 : 1;
+}
+
+template <typename R, int N>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(xinv_w_min_waves<R, N>(), 8))) void k_xinv_w(const cx<R>* __restrict__ S, XSink sink, long nlines, SpecLayout SL,
                                                 const cx<R>* __restrict__ twg, double inv_n, const int* skip) {
     if (skip != nullptr && *skip) return;
     constexpr int RAD = N / 64;
