@@ -91,7 +91,9 @@ def run_precision(args, P, _lib, mask_dev, precision, tol=None, steps=None, warm
     for w in range(warmup):
         step()
         log(f"[bench] {label} warmup {w + 1}/{warmup} iterations={[int(i.iterations) for i in infos]}")
-    _lib.check(_lib.load().fftperm_profile(green._h, 1))
+    # sampled live timing: one chunk of Krylov iterations in eight records HIP events around its
+    # launches; the others replay graphs (timing every launch would cost ~3% of the solve)
+    _lib.check(_lib.load().fftperm_profile(green._h, 2))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for s in range(steps):

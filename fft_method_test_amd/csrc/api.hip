@@ -245,6 +245,8 @@ struct fftperm_ctx {
         int side = -1;          // per-side record of a multi-RHS iteration (-1: none)
     };
     bool prof = false;
+    bool prof_sampled = false;  // fftperm_profile(ctx, 2): one chunk of iterations in kProfSamplePeriod is timed
+    long long prof_chunk = 0;
     std::vector<ProfRec> prof_pending;
     std::vector<hipEvent_t> ev_free;
     double prof_z_ms[2] = {0, 0}, prof_apply_ms[2] = {0, 0};
@@ -454,6 +456,24 @@ XSource src_none(const fftperm_ctx* c) {
     return sr;
 }
 
+// Sampled live timing: with fftperm_profile(ctx, 2) one chunk of Krylov iterations in
+// kProfSamplePeriod runs eagerly with events and the others replay graphs without them (events
+// cost ~3% of a 256^3 solve and rule out graph replay); with fftperm_profile(ctx, 1) every chunk is
+// timed.  ChunkTiming decides for the chunk about to run and switches the event recording off for
+// an untimed one (graph capture included) until it goes out of scope.
+constexpr long long kProfSamplePeriod = 8;
+struct ChunkTiming {
+    fftperm_ctx* c;
+    bool saved;
+    bool timed;
+    explicit ChunkTiming(fftperm_ctx* ctx) : c(ctx), saved(ctx->prof) {
+        timed = ctx->prof && (!ctx->prof_sampled || (ctx->prof_chunk++ % kProfSamplePeriod) == 0);
+        if (!timed) ctx->prof = false;
+    }
+    ~ChunkTiming() { c->prof = saved; }
+    bool graphs() const { return c->use_graphs && !timed; }
+};
+
 XSink sink_none(const fftperm_ctx* c) {
     XSink sk;
     std::memset(&sk, 0, sizeof(sk));
@@ -656,7 +676,8 @@ MinresOut device_minres(fftperm_ctx* c, const double* bdev, const double* x0dev,
     while (!done && launched < maxit) {
         const uint64_t k = std::min<uint64_t>(chunk, maxit - launched);
         // pairs of iterations replay a captured graph (not while live timing records events)
-        const bool graph = c->use_graphs && !c->prof;
+        const ChunkTiming timing(c);
+        const bool graph = timing.graphs();
         uint64_t i = 0;
         if (graph) {
             for (; i + 2 <= k; i += 2) {
@@ -897,10 +918,11 @@ MinresOut device_cg_reliable(fftperm_ctx* c, const double* bdev, const double* x
         int done = 0;
         long long seg_launch = 0;
         while (!done) {
-            const bool graph = c->use_graphs && !c->prof;
+            const ChunkTiming timing(c);
+            const bool graph = timing.graphs();
             uint64_t i = 0;
             if (graph)
-                for (; i + 2 <= chunk; i += 2) HIPCHK(hipGraphLaunch(cg_graph<R>(c), c->s));
+                for (; i + 2 <= chunk; i += 2, seg_launch += 2) HIPCHK(hipGraphLaunch(cg_graph<R>(c), c->s));
             for (; i < chunk; ++i) {
                 c->cur_tag = seg_launch++;
                 cg_iteration<R>(c);
@@ -1095,10 +1117,11 @@ MinresOut device_cr_reliable(fftperm_ctx* c, const double* bdev, const double* x
         int done = 0;
         long long seg_launch = 0;
         while (!done) {
-            const bool graph = c->use_graphs && !c->prof;
+            const ChunkTiming timing(c);
+            const bool graph = timing.graphs();
             uint64_t i = 0;
             if (graph)
-                for (; i + 2 <= chunk; i += 2) HIPCHK(hipGraphLaunch(cr_graph<R, V32>(c), c->s));
+                for (; i + 2 <= chunk; i += 2, seg_launch += 2) HIPCHK(hipGraphLaunch(cr_graph<R, V32>(c), c->s));
             for (; i < chunk; ++i) {
                 c->cur_tag = seg_launch++;
                 if (V32)
@@ -1339,10 +1362,12 @@ void device_cr32_batch(fftperm_ctx* c, int nrhs, const double* const* bdev, doub
         bool stop = false;
         long long seg_launch = 0;
         while (!stop) {
-            const bool graph = c->use_graphs && !c->prof;
+            const ChunkTiming timing(c);
+            const bool graph = timing.graphs();
             uint64_t i = 0;
             if (graph)
-                for (; i + 2 <= chunk; i += 2) HIPCHK(hipGraphLaunch(cr32_batch_graph(c, nrhs), c->s));
+                for (; i + 2 <= chunk; i += 2, seg_launch += 2)
+                    HIPCHK(hipGraphLaunch(cr32_batch_graph(c, nrhs), c->s));
             for (; i < chunk; ++i) {
                 c->cur_tag = seg_launch++;
                 cr32_batch_iteration(c, nrhs);
@@ -2203,8 +2228,11 @@ int fftperm_time_apply(fftperm_ctx* ctx, int precision, int reps, double* ms_per
 int fftperm_profile(fftperm_ctx* ctx, int enable) {
     return guarded([&] {
         check_ctx(ctx);
+        if (enable < 0 || enable > 2) throw Err(FFTPERM_EINVAL, "profile mode must be 0, 1 or 2");
         ctx->prof_resolve(LLONG_MAX);
         ctx->prof = enable != 0;
+        ctx->prof_sampled = enable == 2;
+        ctx->prof_chunk = 0;
         for (int i = 0; i < 2; ++i) {
             ctx->prof_z_ms[i] = ctx->prof_apply_ms[i] = 0.0;
             ctx->prof_z_n[i] = ctx->prof_apply_n[i] = 0;

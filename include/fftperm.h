@@ -199,7 +199,9 @@ int fftperm_symmetry_gap(fftperm_ctx* ctx, const double* x, const double* y, dou
  * context records HIP events on its own stream around every fused z-pass launch (z forward FFT,
  * Green multiply, z backward FFT) and around every five-sweep Green apply (of one right-hand side,
  * or of all sides of a multi-RHS iteration); Krylov applies that ran after convergence (no-ops) are
- * not counted.  fftperm_profile(ctx, 1) resets and enables. */
+ * not counted.  fftperm_profile(ctx, mode) resets and sets the mode: 0 off; 1 every Krylov
+ * iteration is timed (launched eagerly, no graph replay); 2 sampled: one chunk of iterations in
+ * eight is timed and the others replay graphs without events.  Other modes -> FFTPERM_EINVAL. */
 typedef struct {
     double zgreen_ms[2];         /* [0] FP64, [1] FP32 (mixed) : summed durations */
     uint64_t zgreen_launches[2];
@@ -225,7 +227,7 @@ int fftperm_pin_variant(fftperm_ctx* ctx, int kind, int precision, int code);
 
 /* MINRES iterations replay hipGraphs of two captured iterations (default on; no counterpart in the
  * reference).  enable = 0 launches every kernel from the host instead; results are identical.
- * While fftperm_profile is on, iterations launch eagerly (the live timing records events). */
+ * Iterations that fftperm_profile times launch eagerly (the live timing records events). */
 int fftperm_set_graphs(fftperm_ctx* ctx, int enable);
 
 /* Device time of the operator apply (no counterpart in the reference; a measurement hook): reps

@@ -139,6 +139,31 @@ def test_tensor_batch_profile_counts(fp, mode):
     green.close()
 
 
+def test_sampled_profile(fp):
+    """fftperm_profile(ctx, 2) times one chunk of Krylov iterations in eight and replays graphs for
+    the rest: the solve is bit-identical to an untimed one, some but not all applies are counted,
+    the z-pass time is positive; an unknown mode is refused."""
+    import ctypes as C
+
+    from fft_method_test_amd import _lib, geometry as G, permeability as P
+
+    L = _lib.load()
+    mask = G.boolean_spheres(64, 4.0, 0.6, 17).reshape(-1)
+    green = _green((64, 64, 64), mask, 1)
+    K0, i0 = P.solve_permeability_tensor(green, 1e-8, 20000)
+    _lib.check(L.fftperm_profile(green._h, 2))
+    K2, i2 = P.solve_permeability_tensor(green, 1e-8, 20000)
+    pr = _lib.ProfileStats()
+    _lib.check(L.fftperm_profile_read(green._h, C.byref(pr)))
+    assert np.array_equal(K2, K0) and _fields(i2) == _fields(i0)
+    its = sum(int(i.iterations) for i in i2)
+    assert 0 < pr.apply_rhs[1] < its
+    assert pr.zgreen_launches[1] > 0 and pr.zgreen_ms[1] > 0.0
+    with pytest.raises(ValueError, match="profile mode"):
+        _lib.check(L.fftperm_profile(green._h, 3))
+    green.close()
+
+
 @pytest.mark.skipif(not os.path.exists(NET_PATH), reason="no trained warm-start network")
 def test_tensor_batch_warm_matches_sequential(fp):
     from fft_method_test_amd import geometry as G, warmstart as W
