@@ -80,8 +80,17 @@ def lib():
         L.orc_brinkman_solve.argtypes = [C.c_size_t] * 3 + [_dp, _dp, C.c_double, C.c_double, C.c_int,
                                                              _dp, _dp, C.c_void_p,
                                                              C.POINTER(BrinkmanInfo)]
+        L.orc_set_reduction_chunks.argtypes = [C.c_int]
+        L.orc_spectral_green.argtypes = [C.c_size_t, _dp, _dp, _dp]
         _lib = L
     return _lib
+
+
+def set_reduction_chunks(chunks: int) -> None:
+    """Order of the MINRES dots and support means: 0/1 serial, T > 1 that of T OpenMP threads
+    (static schedule), as in the reference built with OpenMP (MinresSolver.cpp:13-22,
+    TrialForceField.cpp:42-64).  Process-wide."""
+    lib().orc_set_reduction_chunks(int(chunks))
 
 
 def ref_available() -> bool:
@@ -243,6 +252,18 @@ class Oracle:
     def symmetry_gap(self, x, y):
         return lib().orc_symmetry_gap(self._h, np.ascontiguousarray(x, dtype=np.float64),
                                       np.ascontiguousarray(y, dtype=np.float64))
+
+
+def spectral_green(t6, force_spec):
+    """V = T F per frequency (GreenOperator.cpp:80-92): t6 = 6 real arrays of N, force_spec 3N complex."""
+    t6 = np.ascontiguousarray(t6, dtype=np.float64).reshape(-1)
+    F = np.ascontiguousarray(force_spec, dtype=np.complex128).reshape(-1)
+    N = t6.size // 6
+    if F.size != 3 * N:
+        raise ValueError("spectrum size mismatch")
+    V = np.zeros(6 * N)
+    lib().orc_spectral_green(N, t6, F.view(np.float64).copy(), V)
+    return V.view(np.complex128)
 
 
 def fft3d(data, sign=-1):

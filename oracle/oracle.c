@@ -500,18 +500,19 @@ static void backend_inverse(orc_ctx* c, double* spec3, double* real3) {
     }
 }
 
-/* GreenOperator::apply + apply_spectral_green_tensor, src/bignonnet/GreenOperator.cpp:54-93. */
-void orc_green_apply(orc_ctx* c, const double* f, double* u) {
-    const size_t N = c->N;
-    backend_forward(c, f, c->spec_f);
-    const double* fxs = c->spec_f;
-    const double* fys = c->spec_f + 2 * N;
-    const double* fzs = c->spec_f + 4 * N;
-    double* vx = c->spec_v;
-    double* vy = c->spec_v + 2 * N;
-    double* vz = c->spec_v + 4 * N;
-    const double *txx = c->G, *txy = c->G + N, *txz = c->G + 2 * N;
-    const double *tyy = c->G + 3 * N, *tyz = c->G + 4 * N, *tzz = c->G + 5 * N;
+/* GreenOperator::apply_spectral_green_tensor's CPU loop, src/bignonnet/GreenOperator.cpp:80-92:
+ * per frequency, V = T F with T real symmetric (6 arrays of N: xx, xy, xz, yy, yz, zz) and F, V
+ * complex 3-vectors (3 interleaved spectra of N).  A real T times a complex component is the
+ * product with the real and imaginary parts separately (std::complex<double> * double). */
+void orc_spectral_green(size_t N, const double* t6, const double* F, double* V) {
+    const double* fxs = F;
+    const double* fys = F + 2 * N;
+    const double* fzs = F + 4 * N;
+    double* vx = V;
+    double* vy = V + 2 * N;
+    double* vz = V + 4 * N;
+    const double *txx = t6, *txy = t6 + N, *txz = t6 + 2 * N;
+    const double *tyy = t6 + 3 * N, *tyz = t6 + 4 * N, *tzz = t6 + 5 * N;
     for (size_t i = 0; i < N; ++i) {
         for (int part = 0; part < 2; ++part) {
             const double fx = fxs[2 * i + part], fy = fys[2 * i + part], fz = fzs[2 * i + part];
@@ -520,6 +521,12 @@ void orc_green_apply(orc_ctx* c, const double* f, double* u) {
             vz[2 * i + part] = txz[i] * fx + tyz[i] * fy + tzz[i] * fz;
         }
     }
+}
+
+/* GreenOperator::apply + apply_spectral_green_tensor, src/bignonnet/GreenOperator.cpp:54-93. */
+void orc_green_apply(orc_ctx* c, const double* f, double* u) {
+    backend_forward(c, f, c->spec_f);
+    orc_spectral_green(c->N, c->G, c->spec_f, c->spec_v);
     backend_inverse(c, c->spec_v, u);
 }
 
@@ -547,14 +554,45 @@ void orc_background_force(const orc_ctx* c, const double grad[3], double* f0) {
     }
 }
 
+/* Reduction order of the reference's OpenMP reductions (MinresSolver.cpp:13-22 dot,
+ * TrialForceField.cpp:42-64 mean_vector, both `#pragma omp parallel for reduction(+:...)`).
+ * 0 or 1: serial order (a build without PERMEABILITY_USE_OPENMP, the default here).  T > 1:
+ * the order of T threads under the static schedule -- contiguous blocks, the first n % T of
+ * length n / T + 1, each summed in index order, the partial sums then added in thread order.
+ * Used to measure how far two correct FP64 runs of the reference drift apart when only the
+ * thread count changes. */
+static int g_red_chunks = 0;
+
+void orc_set_reduction_chunks(int chunks) { g_red_chunks = chunks > 1 ? chunks : 0; }
+
+static void chunk_bounds(size_t n, int t, int T, size_t* lo, size_t* hi) {
+    const size_t q = n / (size_t)T, r = n % (size_t)T, ut = (size_t)t;
+    *lo = ut * q + (ut < r ? ut : r);
+    *hi = *lo + q + (ut < r ? 1 : 0);
+}
+
 /* TrialForceField::mean_vector / subtract_mean, src/bignonnet/TrialForceField.cpp:42-80. */
 static void subtract_mean(double* values, size_t nb) {
     if (nb == 0) return;
     double sx = 0.0, sy = 0.0, sz = 0.0;
-    for (size_t n = 0; n < nb; ++n) {
-        sx += values[3 * n + 0];
-        sy += values[3 * n + 1];
-        sz += values[3 * n + 2];
+    if (g_red_chunks > 1) {
+        for (int t = 0; t < g_red_chunks; ++t) {
+            size_t lo, hi;
+            chunk_bounds(nb, t, g_red_chunks, &lo, &hi);
+            double px = 0.0, py = 0.0, pz = 0.0;
+            for (size_t n = lo; n < hi; ++n) {
+                px += values[3 * n + 0];
+                py += values[3 * n + 1];
+                pz += values[3 * n + 2];
+            }
+            sx += px; sy += py; sz += pz;
+        }
+    } else {
+        for (size_t n = 0; n < nb; ++n) {
+            sx += values[3 * n + 0];
+            sy += values[3 * n + 1];
+            sz += values[3 * n + 2];
+        }
     }
     const double inv = 1.0 / (double)nb;
     const double m0 = sx * inv, m1 = sy * inv, m2 = sz * inv;
@@ -617,6 +655,16 @@ typedef void (*orc_apply_fn)(void* op, const double* in, double* out);
 
 static double vdot(const double* a, const double* b, size_t n) {
     double s = 0.0;
+    if (g_red_chunks > 1) {
+        for (int t = 0; t < g_red_chunks; ++t) {
+            size_t lo, hi;
+            chunk_bounds(n, t, g_red_chunks, &lo, &hi);
+            double p = 0.0;
+            for (size_t i = lo; i < hi; ++i) p += a[i] * b[i];
+            s += p;
+        }
+        return s;
+    }
     for (size_t i = 0; i < n; ++i) s += a[i] * b[i];
     return s;
 }

@@ -59,6 +59,8 @@ int orc_fft3d(size_t nx, size_t ny, size_t nz, double* data, int sign);
 
 /* GreenOperator::apply: u = Re(IFFT(G . FFT f)) (3N SoA in/out). */
 void orc_green_apply(orc_ctx* c, const double* f, double* u);
+/* GreenOperator::apply_spectral_green_tensor CPU loop (GreenOperator.cpp:80-92): V = T F. */
+void orc_spectral_green(size_t N, const double* t6, const double* F, double* V);
 /* BackgroundForce::build (3N SoA out). */
 void orc_background_force(const orc_ctx* c, const double grad[3], double* f0);
 /* VariationalRHS::build (L = 3 NB AoS out). */
@@ -75,6 +77,10 @@ typedef struct {
 
 /* MinresSolver::solve on the variational operator.  x0 may be NULL (empty guess).
  * history (maxit + 1 entries) may be NULL. */
+/* Order of the MINRES dot products and support means: 0/1 serial, T > 1 the order of T
+ * OpenMP threads (static schedule) as in the reference built with OpenMP.  Process-wide. */
+void orc_set_reduction_chunks(int chunks);
+
 int orc_minres(orc_ctx* c, const double* b, const double* x0, double tol, uint64_t maxit,
                double* x, double* history, orc_solve_info* info);
 
