@@ -1,0 +1,233 @@
+// ref_dropin_check.cpp -- TEST INFRASTRUCTURE: the reference's own classes driving libfftperm.so
+// through the C++ binding of fft_method_test_amd/host/*.hpp.
+//
+// oracle/Makefile builds this file together with the reference sources as they lie under
+// /root/reference into oracle/_ref/dropin_check_f64 (-DFFT_METHOD_PRECISION_DOUBLE) and
+// oracle/_ref/dropin_check_f32 (the reference's default Scalar = float), linked against the
+// library.  tests/test_gpu_dropin.py runs them on a GPU and compares their output with the oracle.
+//
+//   dropin_check_fXX <mask.u8> nx ny nz tol maxit mu p_radius brinkman_penalty brinkman_maxit
+//
+// mask.u8: nx*ny*nz bytes, 1 = fluid.  Prints one JSON object:
+//   reference_over_hip_fft  K from the reference's GreenOperator / VariationalRHS /
+//                           VariationalOperator / MinresSolver / VelocityRecovery with the FFT
+//                           backend replaced by HipFFTBackend (src/app/PermeabilityRunner.cpp:173-242)
+//   reference_minres_over_hip_operator  K from the reference's MinresSolver over
+//                           HipVariationalOperator (RHS and recovery as above)
+//   hip_tensor              K from hip::solve_permeability_tensor (everything on the device)
+//   brinkman                the reference's solveBrinkman (host) and hip::solve_brinkman_fields for
+//                           force e_x: superficial velocities and iteration counts
+//   errors                  the exception types the binding throws for bad arguments
+#include <array>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <iostream>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "fftperm_reference.hpp"
+
+#include "bignonnet/BackgroundForce.hpp"
+#include "bignonnet/ForceSupport.hpp"
+#include "bignonnet/GreenOperator.hpp"
+#include "bignonnet/VariationalOperator.hpp"
+#include "bignonnet/VariationalRHS.hpp"
+#include "bignonnet/VelocityRecovery.hpp"
+#include "solver/MinresSolver.hpp"
+
+#include "fftperm_brinkman.hpp"
+
+using namespace permeability;
+
+namespace {
+
+std::string json_array(const double* v, std::size_t n) {
+    std::ostringstream o;
+    o.precision(17);
+    o << "[";
+    for (std::size_t i = 0; i < n; ++i) o << (i ? ", " : "") << v[i];
+    o << "]";
+    return o.str();
+}
+
+std::string k_json(const PermeabilityTensor& K, const std::array<std::size_t, 3>& its) {
+    double k[9];
+    for (int i = 0; i < 9; ++i) k[i] = static_cast<double>(K.data()[i]);
+    std::ostringstream o;
+    o << "{\"K\": " << json_array(k, 9) << ", \"iterations\": [" << its[0] << ", " << its[1] << ", " << its[2]
+      << "]}";
+    return o.str();
+}
+
+template <class E, class F>
+bool throws(F&& f) {
+    try {
+        f();
+    } catch (const E&) {
+        return true;
+    } catch (...) {
+        return false;
+    }
+    return false;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    if (argc != 11) {
+        std::fprintf(stderr, "usage: %s mask.u8 nx ny nz tol maxit mu p_radius brinkman_penalty brinkman_maxit\n",
+                     argv[0]);
+        return 2;
+    }
+    try {
+        const std::size_t nx = std::stoul(argv[2]), ny = std::stoul(argv[3]), nz = std::stoul(argv[4]);
+        const Scalar tol = static_cast<Scalar>(std::stod(argv[5]));
+        const std::size_t maxit = std::stoul(argv[6]);
+        const Scalar mu = static_cast<Scalar>(std::stod(argv[7]));
+        const int p_radius = std::stoi(argv[8]);
+        const double penalty = std::stod(argv[9]);
+        const int b_maxit = std::stoi(argv[10]);
+        const std::size_t n = nx * ny * nz;
+        std::vector<unsigned char> mask(n);
+        {
+            std::ifstream f(argv[1], std::ios::binary);
+            f.read(reinterpret_cast<char*>(mask.data()), static_cast<std::streamsize>(n));
+            if (!f) throw std::runtime_error("cannot read the mask file");
+        }
+        const Grid3D grid(nx, ny, nz);
+        const BinaryMedium medium(grid, mask);
+        const ForceSupport support(medium, SupportMode::InterfaceOnly);
+        std::cout << "{\"scalar_bytes\": " << sizeof(Scalar) << ", \"nb\": " << support.num_active_voxels();
+
+        // 1. The reference path with its FFT backend replaced (create_fft_backend's GPU case).
+        {
+            PermeabilityTensor K;
+            std::array<std::size_t, 3> its{};
+            for (std::size_t col = 0; col < 3; ++col) {
+                std::unique_ptr<IFFTBackend> backend = hip::make_gpu_fft_backend();
+                GreenOperator green(grid, GreenDiscretization{}, mu, *backend, 1, p_radius);
+                BackgroundForce f0(medium, support);
+                VariationalRHS rhs(support, green, f0);
+                VariationalOperator op(support, green);
+                VelocityRecovery recovery(support, green, f0);
+                Real3 grad{Scalar(0), Scalar(0), Scalar(0)};
+                grad[col] = Scalar(1);
+                const SolverResult r = MinresSolver(tol, maxit).solve(op, rhs.build(grad));
+                const Real3 avg = recovery.average_over_domain(recovery.recover(grad, r.solution));
+                K.set_column(col, Real3{-mu * avg[0], -mu * avg[1], -mu * avg[2]});
+                its[col] = r.iterations;
+                if (col == 0) std::cout << ", \"device_info\": \"" << backend->device_info() << "\"";
+            }
+            std::cout << ", \"reference_over_hip_fft\": " << k_json(K, its);
+        }
+
+        // 2. The reference's MinresSolver over the device operator.
+        hip::Context ctx(grid, mu, p_radius);
+        ctx.set_medium(medium, SupportMode::InterfaceOnly);
+        {
+            const std::vector<std::size_t> act = ctx.active_voxels();
+            std::cout << ", \"support_equal\": " << (act == support.active_voxels() ? "true" : "false");
+            hip::HipVariationalOperator hop(ctx);
+            std::unique_ptr<IFFTBackend> backend = hip::make_gpu_fft_backend();
+            GreenOperator green(grid, GreenDiscretization{}, mu, *backend, 1, p_radius);
+            BackgroundForce f0(medium, support);
+            VariationalRHS rhs(support, green, f0);
+            VelocityRecovery recovery(support, green, f0);
+            PermeabilityTensor K;
+            std::array<std::size_t, 3> its{};
+            for (std::size_t col = 0; col < 3; ++col) {
+                Real3 grad{Scalar(0), Scalar(0), Scalar(0)};
+                grad[col] = Scalar(1);
+                const SolverResult r = MinresSolver(tol, maxit).solve(hop, rhs.build(grad));
+                const Real3 avg = recovery.average_over_domain(recovery.recover(grad, r.solution));
+                K.set_column(col, Real3{-mu * avg[0], -mu * avg[1], -mu * avg[2]});
+                its[col] = r.iterations;
+            }
+            std::cout << ", \"reference_minres_over_hip_operator\": " << k_json(K, its);
+        }
+
+        // 3. Everything on the device through the binding.
+        {
+            std::array<hip::DirectionSolve, 3> dirs;
+            const PermeabilityTensor K = hip::solve_permeability_tensor(ctx, tol, maxit, &dirs);
+            std::cout << ", \"hip_tensor\": "
+                      << k_json(K, {dirs[0].solve_result.iterations, dirs[1].solve_result.iterations,
+                                    dirs[2].solve_result.iterations});
+            // the device solve's history reproduces the reference's SolverResult layout
+            std::cout << ", \"hip_history_len\": " << dirs[0].solve_result.residual_history.size();
+        }
+
+        // 4. Brinkman: the reference's host loop against the GPU branch, same fields.
+        {
+            common::Geometry geo;
+            geo.nx = nx;
+            geo.ny = ny;
+            geo.nz = nz;
+            geo.materialIds.resize(n);
+            for (std::size_t i = 0; i < n; ++i) geo.materialIds[i] = mask[i] ? 1U : 0U;
+            common::MaterialDatabase db;
+            common::MaterialProperties props;
+            props.values["viscosity"] = static_cast<double>(mu);
+            db.insert(1U, std::move(props));
+            common::AppConfig cfg;
+            cfg.fluid.fluidMaterialId = 1U;
+            cfg.fluid.solidPenalty = penalty;
+            cfg.fluid.maxIterations = b_maxit;
+            cfg.fluid.tolerance = 1e-8;
+            cfg.fluid.relaxation = 0.9;
+            const std::array<double, 3> force{1.0, 0.0, 0.0};
+            const common::fluid::SolveResult ref = common::fluid::solveBrinkman(cfg, geo, db, force);
+            // buildCoefficientFields (src/brinkman/FluidBrinkman.cpp:152-191) for one fluid id
+            std::vector<double> alpha(n, penalty), visc(n, 1.0);
+            std::size_t fluid = 0;
+            for (std::size_t i = 0; i < n; ++i) {
+                if (mask[i]) {
+                    alpha[i] = 0.0;
+                    visc[i] = static_cast<double>(mu);
+                    ++fluid;
+                }
+            }
+            const common::fluid::SolveResult gpu = common::fluid::hip::solve_brinkman_fields(
+                cfg, geo, alpha, visc, static_cast<double>(fluid) / static_cast<double>(n), static_cast<double>(mu),
+                force);
+            std::cout << ", \"brinkman\": {\"reference_U\": " << json_array(ref.superficialVelocity.data(), 3)
+                      << ", \"reference_iterations\": " << ref.diagnostics.iterations
+                      << ", \"reference_residual\": " << ref.diagnostics.residual
+                      << ", \"hip_U\": " << json_array(gpu.superficialVelocity.data(), 3)
+                      << ", \"hip_iterations\": " << gpu.diagnostics.iterations
+                      << ", \"hip_residual\": " << gpu.diagnostics.residual
+                      << ", \"porosity_equal\": " << (gpu.porosity == ref.porosity ? "true" : "false") << "}";
+        }
+
+        // 5. Error mapping of the binding.
+        {
+            const bool bad_tol = throws<std::invalid_argument>([] { hip::HipMinresSolver(Scalar(0), 10); });
+            const bool bad_mu = throws<std::invalid_argument>([&] { hip::Context(grid, Scalar(-1), p_radius); });
+            const bool bad_size = throws<std::invalid_argument>([&] {
+                hip::HipVariationalOperator hop(ctx);
+                std::vector<Scalar> in(3), out;
+                hop.apply(in, out);
+            });
+            const bool not_ready = throws<std::runtime_error>([&] {
+                hip::HipFFTBackend b;
+                VectorField3D f(grid);
+                ComplexVectorField3D s(grid);
+                b.forward(f, s);
+            });
+            std::cout << ", \"errors\": {\"bad_tol\": " << (bad_tol ? "true" : "false")
+                      << ", \"bad_mu\": " << (bad_mu ? "true" : "false")
+                      << ", \"bad_size\": " << (bad_size ? "true" : "false")
+                      << ", \"not_initialized\": " << (not_ready ? "true" : "false") << "}";
+        }
+        std::cout << "}" << std::endl;
+    } catch (const std::exception& e) {
+        std::cout << std::endl;
+        std::fprintf(stderr, "dropin_check: %s\n", e.what());
+        return 1;
+    }
+    return 0;
+}

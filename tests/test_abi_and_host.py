@@ -116,3 +116,31 @@ def test_c4_batch_generator_is_seeded():
     assert all(np.array_equal(x[1], y[1]) for x, y in zip(a, b))
     assert all(s[0] in (64, 80, 96, 112, 128, 144, 160) for s in a)
     assert all(0.3 <= s[3] <= 0.8 for s in a)
+
+
+REF_SRC = "/root/reference/src"
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_SRC), reason="reference sources not present")
+@pytest.mark.parametrize("precision", ["float", "double"])
+def test_cpp_binding_compiles_against_reference_headers(precision):
+    """fft_method_test_amd/host/*.hpp implement the reference's IFFTBackend, ILinearOperator,
+    MinresSolver::solve, solve_one_direction and solveBrinkman plug points: compile them against
+    the reference's own headers in both of its precisions (src/core/Types.hpp:14-21)."""
+    import subprocess
+    import tempfile
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    src = ('#include "fftperm_reference.hpp"\n#include "fftperm_brinkman.hpp"\n'
+           "static_assert(!std::is_abstract_v<permeability::hip::HipFFTBackend>);\n"
+           "static_assert(!std::is_abstract_v<permeability::hip::HipVariationalOperator>);\n"
+           "int main() { return 0; }\n")
+    with tempfile.NamedTemporaryFile("w", suffix=".cpp", delete=False) as f:
+        f.write(src)
+    flags = ["-DFFT_METHOD_PRECISION_DOUBLE"] if precision == "double" else []
+    cmd = ["g++", "-std=c++20", "-Wall", "-Wextra", "-Werror", "-fsyntax-only", *flags, "-I" + REF_SRC,
+           "-I/root/reference/include", "-I" + os.path.join(root, "include"),
+           "-I" + os.path.join(root, "fft_method_test_amd", "host"), f.name]
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    os.unlink(f.name)
+    assert r.returncode == 0, r.stdout
