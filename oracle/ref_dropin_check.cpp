@@ -19,6 +19,8 @@
 //                           force e_x: superficial velocities and iteration counts
 //   errors                  the exception types the binding throws for bad arguments
 #include <array>
+#include <cmath>
+#include <random>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
@@ -148,6 +150,31 @@ int main(int argc, char** argv) {
                 its[col] = r.iterations;
             }
             std::cout << ", \"reference_minres_over_hip_operator\": " << k_json(K, its);
+        }
+
+        // 2b. One operator apply, reference VariationalOperator (host gather/scatter over
+        // HipFFTBackend) against HipVariationalOperator, on the reference's symmetry-test input
+        // (tests/test_variational_operator_symmetry.cpp: mt19937_64(42), uniform(-1, 1)).
+        {
+            std::unique_ptr<IFFTBackend> backend = hip::make_gpu_fft_backend();
+            GreenOperator green(grid, GreenDiscretization{}, mu, *backend, 1, p_radius);
+            VariationalOperator op(support, green);
+            hip::HipVariationalOperator hop(ctx);
+            std::mt19937_64 rng(42);
+            std::uniform_real_distribution<double> dist(-1.0, 1.0);
+            std::vector<Scalar> x(op.size());
+            for (auto& v : x) v = static_cast<Scalar>(dist(rng));
+            std::vector<Scalar> y_ref, y_hip;
+            op.apply(x, y_ref);
+            hop.apply(x, y_hip);
+            double num = 0.0, den = 0.0;
+            for (std::size_t i = 0; i < y_ref.size(); ++i) {
+                const double d = static_cast<double>(y_hip[i]) - static_cast<double>(y_ref[i]);
+                num += d * d;
+                den += static_cast<double>(y_ref[i]) * static_cast<double>(y_ref[i]);
+            }
+            std::cout << ", \"operator_apply_rel_diff\": " << std::sqrt(num / den)
+                      << ", \"operator_sizes_equal\": " << (y_hip.size() == y_ref.size() ? "true" : "false");
         }
 
         // 3. Everything on the device through the binding.
