@@ -15,7 +15,7 @@ ARCH = os.environ.get("FFTPERM_ARCH", "gfx950")
 
 SOURCES = ["api.hip", "kernels_fft.hip", "passes_x.hip", "passes_col.hip", "kernels_setup.hip", "kernels_krylov.hip",
            "unet.hip",
-           "brinkman.hip"]
+           "brinkman.hip", "noalloc.hip"]
 
 HIPCC_FLAGS = [
     "-O3",

@@ -39,6 +39,13 @@ struct Err : std::runtime_error {
             throw Err(FFTPERM_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));               \
     } while (0)
 
+}  // namespace
+
+// The calling thread's message for fftperm_last_error(), for entry points outside this file.
+void fftperm::set_last_error(const std::string& msg) { g_last_error = msg; }
+
+namespace {
+
 template <typename F>
 int guarded(F&& f) {
     try {

@@ -3,6 +3,8 @@
 
 #include "common.hpp"
 
+#include <string>
+
 #include <utility>
 #include <vector>
 
@@ -209,6 +211,14 @@ void launch_line_bits(const int32_t* code, int nx, long nlines, uint32_t* bits, 
 void launch_support(const uint8_t* mask, int nx, int ny, int nz, int interface_only, int32_t* code,
                     int64_t* active, unsigned long long* counts /*[2]: nb, nf*/, void* scratch,
                     hipStream_t st);
+// The same classification writing the reference's voxel_to_local (int64, -1 off the support; may be
+// null) and at most `capacity` entries of the active list.  counts[2] (nb, nf) are cleared by the
+// launch itself (a kernel, no memset); bsum holds support_scan_blocks(N) ints.
+long support_scan_blocks(long N);
+// Sets the calling thread's fftperm_last_error() message (api.hip).
+void set_last_error(const std::string& msg);
+void launch_support_v2l(const uint8_t* mask, int nx, int ny, int nz, int interface_only, int64_t* voxel_to_local,
+                        int64_t* active, long capacity, unsigned long long* counts, int* bsum, hipStream_t st);
 
 // Green tensor tables: per axis, for k in [0,n) and p in [-P,P]: s2 = sinc^2, q.
 struct GreenAxisTables {

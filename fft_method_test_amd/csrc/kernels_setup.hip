@@ -129,6 +129,43 @@ __global__ void k_support_write(const uint8_t* __restrict__ mask, int nx, int ny
     }
 }
 
+// The support map in the reference's own form: voxel_to_local (int64, -1 = kInvalid,
+// src/bignonnet/ForceSupport.hpp:22-29) instead of the int32 code map, and the ascending active list
+// bounded by the caller's capacity (entries at and beyond it are not written; the caller compares
+// the count against the capacity).
+__global__ void k_support_write_v2l(const uint8_t* __restrict__ mask, int nx, int ny, int nz, long N,
+                                    int interface_only, const int* __restrict__ bsum,
+                                    int64_t* __restrict__ voxel_to_local, int64_t* __restrict__ active,
+                                    long capacity) {
+    const long base = (long)blockIdx.x * kScanChunk + (long)threadIdx.x * kScanItems;
+    int f[kScanItems];
+    int cnt = 0;
+    for (int q = 0; q < kScanItems; ++q) {
+        const long v = base + q;
+        f[q] = (v < N) ? support_flag(mask, nx, ny, nz, v, interface_only) : 0;
+        cnt += f[q];
+    }
+    int excl;
+    block_exclusive_scan(cnt, &excl);
+    long local = (long)bsum[blockIdx.x] + excl;
+    for (int q = 0; q < kScanItems; ++q) {
+        const long v = base + q;
+        if (v >= N) break;
+        if (f[q]) {
+            if (voxel_to_local) voxel_to_local[v] = local;
+            if (local < capacity) active[local] = v;
+            ++local;
+        } else if (voxel_to_local) {
+            voxel_to_local[v] = -1;
+        }
+    }
+}
+
+__global__ void k_fill_u64(unsigned long long* __restrict__ p, int n, unsigned long long value) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = value;
+}
+
 __global__ void k_line_bits(const int32_t* __restrict__ code, int nx, long nlines, int W, uint32_t* __restrict__ bits,
                             int* __restrict__ counts) {
     // one thread per word; the W threads of a line add their popcounts into counts[line]
@@ -170,6 +207,22 @@ void launch_support(const uint8_t* mask, int nx, int ny, int nz, int interface_o
     hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(kScanThreads), 0, st, bsum, nblocks, counts);
     hipLaunchKernelGGL(k_support_write, dim3((unsigned)nblocks), dim3(kScanThreads), 0, st, mask, nx, ny, nz, N,
                        interface_only, bsum, code, active);
+}
+
+long support_scan_blocks(long N) { return (N + kScanChunk - 1) / kScanChunk; }
+
+void launch_support_v2l(const uint8_t* mask, int nx, int ny, int nz, int interface_only, int64_t* voxel_to_local,
+                        int64_t* active, long capacity, unsigned long long* counts, int* bsum, hipStream_t st) {
+    const long N = (long)nx * ny * nz;
+    const long nblocks = support_scan_blocks(N);
+    // the counters start at zero on every call: a fill kernel, not a memset node, so that the
+    // sequence can sit inside a captured graph
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, counts, 2, 0ull);
+    hipLaunchKernelGGL(k_support_count, dim3((unsigned)nblocks), dim3(kScanThreads), 0, st, mask, nx, ny, nz, N,
+                       interface_only, bsum, counts + 1);
+    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(kScanThreads), 0, st, bsum, nblocks, counts);
+    hipLaunchKernelGGL(k_support_write_v2l, dim3((unsigned)nblocks), dim3(kScanThreads), 0, st, mask, nx, ny, nz, N,
+                       interface_only, bsum, voxel_to_local, active, capacity);
 }
 
 // ------------------------------------------------------------ Green tensor

@@ -106,6 +106,27 @@ int fftperm_destroy(fftperm_ctx* ctx);
 int fftperm_set_medium(fftperm_ctx* ctx, const uint8_t* fluid_mask, int support_mode, uint64_t* nb_out);
 int fftperm_set_medium_dev(fftperm_ctx* ctx, const uint8_t* fluid_mask_dev, int support_mode, uint64_t* nb_out);
 
+/* No counterpart in the reference.  The support map of fftperm_set_medium_dev + fftperm_get_support
+ * (ForceSupport, src/bignonnet/ForceSupport.cpp:9-47; InterfaceVoxels::compute_interface_only,
+ * src/geometry/InterfaceVoxels.cpp:7-43) for callers that cannot allocate device memory: code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc.  It needs no context and
+ * allocates nothing; its scan scratch (two counters and one int per 1024 voxels, 4 MiB + 256 B) is a
+ * static device array in the library, one copy per GPU held until the process ends, which bounds the
+ * grid to FFTPERM_NOALLOC_MAX_VOXELS voxels (FFTPERM_ERANGE beyond).
+ *   fluid_mask_dev     N = nx*ny*nz bytes on the current device, 1 = fluid
+ *   voxel_to_local_dev N int64 (may be NULL): local index on the support, -1 elsewhere (kInvalid)
+ *   active_dev         the ascending support voxels; at most active_capacity entries are written
+ *   counts_dev         2 uint64 on the device (may be NULL): [support voxels, fluid voxels]
+ *   stream             hipStream_t to launch on (NULL: the default stream)
+ *   nb_out, nfluid_out host (may be NULL): when either is given the call waits for the stream, reads
+ *                      the counts and returns FFTPERM_ERANGE if the support exceeds active_capacity
+ * Every call on a GPU shares that GPU's scratch and clears its counters itself; calls that overlap
+ * on one GPU (two host threads, or two streams in flight) race and must be serialised by the caller. */
+#define FFTPERM_NOALLOC_MAX_VOXELS 1073741824ull /* 1024^3 */
+int fftperm_support_noalloc(uint64_t nx, uint64_t ny, uint64_t nz, const uint8_t* fluid_mask_dev, int support_mode,
+                            int64_t* voxel_to_local_dev, int64_t* active_dev, uint64_t active_capacity,
+                            uint64_t* counts_dev, void* stream, uint64_t* nb_out, uint64_t* nfluid_out);
+
 /* BinaryMedium::porosity (src/geometry/BinaryMedium.cpp:34-43). */
 int fftperm_porosity(fftperm_ctx* ctx, double* porosity);
 
