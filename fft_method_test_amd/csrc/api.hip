@@ -422,9 +422,9 @@ static void run_chain(fftperm_ctx* c, S_* S, const G_* G, const T_& t, const XSo
     }
     launch_xfwd<R>(src, S, c->g, t, skip, c->s);
     launch_ypass<R>(S, -1, c->g, t, skip, c->s);
-    if (c->prof) HIPCHK(hipEventRecord(rec.b, c->s));
+    // the z-pass launch records rec.b / rec.c at its own start and end
+    if (c->prof) zpass_events() = KernelEvents{rec.b, rec.c};
     c->zparts = launch_zgreen<R>(S, G, c->g, t, qpart, skip, c->s);
-    if (c->prof) HIPCHK(hipEventRecord(rec.c, c->s));
     launch_ypass<R>(S, +1, c->g, t, skip, c->s);
     c->xparts = launch_xinv<R>(S, sink, c->g, t, skip, c->s);
     if (c->prof) {
@@ -1225,9 +1225,8 @@ void chain32_batch(fftperm_ctx* c, int nrhs) {
             if (c->prof) rec = prof_begin(0, b);
             launch_xfwd<float>(source(b), S, g, c->t32, skip, c->s);
             launch_ypass<float>(S, -1, g, c->t32, skip, c->s);
-            if (c->prof) HIPCHK(hipEventRecord(rec.b, c->s));
+            if (c->prof) zpass_events() = KernelEvents{rec.b, rec.c};
             c->zparts = launch_zgreen<float>(S, c->G32, g, c->t32, c->part_z + b * cr_zstride(c), skip, c->s);
-            if (c->prof) HIPCHK(hipEventRecord(rec.c, c->s));
             launch_ypass<float>(S, +1, g, c->t32, skip, c->s);
             c->xparts = launch_xinv<float>(S, sink(b), g, c->t32, skip, c->s);
             if (c->prof) {
@@ -2330,7 +2329,7 @@ int fftperm_unet_predict(fftperm_unet* net, const uint8_t* fluid_mask_dev, uint3
         if (!net || !fluid_mask_dev || !field_dev) throw Err(FFTPERM_EINVAL, "null argument");
         check_unet_edge(n);
         auto* u = reinterpret_cast<unet::Net*>(net);
-        unet::predict(u, fluid_mask_dev, (int)n, 0);
+        unet::predict(u, fluid_mask_dev, (int)n);
         unet::pred_field(u, field_dev);
     });
 }
@@ -2350,15 +2349,15 @@ int fftperm_solve_tensor_warm(fftperm_ctx* ctx, fftperm_unet* net, double tol, u
         const bool batch = use_tensor_batch(ctx);
         if (batch) ctx->ensure_tensor_vectors();
         bool has_x0[3] = {false, false, false};
+        // one forward pass predicts the force densities of the three load directions
+        unet::predict(u, ctx->mask, g.nx);
         for (int col = 0; col < 3; ++col) {
             double grad[3] = {0.0, 0.0, 0.0};
             grad[col] = 1.0;
             double* bvec = batch ? ctx->tb + col * L : ctx->b;
             device_rhs(ctx, grad, bvec);
             ctx->sync();
-            // the network predicts the x-direction density of the axis-swapped medium
             double* x0 = batch ? ctx->txa + col * L : (ctx->precision == FFTPERM_PRECISION_MIXED) ? ctx->dx : ctx->tmp;
-            unet::predict(u, ctx->mask, g.nx, col);
             unet::gather_guess(u, col, ctx->active, (long)ctx->nb, x0);
             // the network fixes the shape of the density, the scale comes from the minimal-residual
             // step along it: x0 = alpha p, alpha = <b, A p> / ||A p||^2, so ||b - A x0|| <= ||b||;

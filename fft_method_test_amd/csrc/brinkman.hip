@@ -6,6 +6,15 @@
 // z*nx*ny + y*nx + x (:24-26); neighbours wrap periodically (:28-33).  In mixed mode the
 // Poisson sweeps run on FP32 copies of div and phi; velocity, pressure, divergence and
 // the residual stay FP64.
+//
+// Layout of the work: the first sweep (from phi = 0) is fused into the divergence kernel, the
+// other 34 run two at a time on LDS-staged bricks (phi with a two-voxel halo, the intermediate
+// sweep with a one-voxel halo), so a pair of sweeps reads phi and div once and writes phi once.
+// Every sweep keeps the reference's per-voxel arithmetic, (sum of the six neighbours in the order
+// -x, +x, -y, +y, -z, +z  minus div) / 6, so the result is independent of the blocking.  The host
+// does not wait for each iteration: a device-side check after each projection records the
+// residual and sets a stop flag when the reference would break (:469-499), later iterations of
+// the chunk skip all their work, and chunks of iterations replay a captured hipGraph.
 #include "brinkman.hpp"
 #include "common.hpp"
 
@@ -49,7 +58,8 @@ __global__ void k_momentum(const double* __restrict__ ux, const double* __restri
                            const double* __restrict__ uz, const double* __restrict__ p,
                            const double* __restrict__ alpha, const double* __restrict__ visc, double* __restrict__ sx,
                            double* __restrict__ sy, double* __restrict__ sz, double f0, double f1, double f2,
-                           double relax, int nx, int ny, int nz, long N) {
+                           double relax, int nx, int ny, int nz, long N, const int* __restrict__ done) {
+    if (*done) return;
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
         const Nbr n = neighbours(i, nx, ny, nz);
         const double mu = visc[i];
@@ -70,32 +80,94 @@ __global__ void k_momentum(const double* __restrict__ ux, const double* __restri
     }
 }
 
-// :371-400
+// :371-400, plus the first pressure sweep (:402-433 from phi = 0): phi1 = (0 - div) / 6
 template <typename P>
 __global__ void k_divergence(const double* __restrict__ sx, const double* __restrict__ sy,
-                             const double* __restrict__ sz, double* __restrict__ div, P* __restrict__ divp, int nx,
-                             int ny, int nz, long N, double* __restrict__ partials) {
+                             const double* __restrict__ sz, double* __restrict__ div, P* __restrict__ divp,
+                             P* __restrict__ phi1, int nx, int ny, int nz, long N, double* __restrict__ partials,
+                             const int* __restrict__ done) {
+    if (*done) return;
     double acc[1] = {0.0};
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
         const Nbr n = neighbours(i, nx, ny, nz);
         const double d = 0.5 * (sx[n.xp] - sx[n.xm]) + 0.5 * (sy[n.yp] - sy[n.ym]) + 0.5 * (sz[n.zp] - sz[n.zm]);
         div[i] = d;
-        if (divp != nullptr) divp[i] = (P)d;
+        const P dp = (P)d;
+        if (divp != nullptr) divp[i] = dp;
+        phi1[i] = ((P)0 - dp) / (P)6.0;
         acc[0] += d * d;
     }
     block_sum<1>(acc, partials + blockIdx.x);
 }
 
-// :402-433; `first` = the sweep from phi = 0
+// Two Jacobi sweeps (:402-433) on a brick of BX x BY x BZ voxels: phi (two-voxel halo) and div
+// (one-voxel halo) are staged in LDS, the first sweep fills the one-voxel-halo region, the second
+// the brick.  Every LDS slot holds the periodic value of its wrapped voxel, so bricks that stick
+// out of small grids stay exact (their outside voxels are not written).
+template <int BX, int BY, int BZ>
+struct JBrick {
+    static constexpr int AX = BX + 4, AY = BY + 4, AZ = BZ + 4;  // phi with the two-voxel halo
+    static constexpr int CX = BX + 2, CY = BY + 2, CZ = BZ + 2;  // first sweep and div
+    static constexpr int NA = AX * AY * AZ, NC = CX * CY * CZ, NB = BX * BY * BZ;
+};
+
+__device__ __forceinline__ int wrapm(int v, int n) {
+    v %= n;
+    return v < 0 ? v + n : v;
+}
+
+template <typename P, int BX, int BY, int BZ>
+__global__ __launch_bounds__(256) void k_jacobi2(const P* __restrict__ phi, const P* __restrict__ div,
+                                                 P* __restrict__ phin, int nx, int ny, int nz, int tiles_x,
+                                                 int tiles_y, const int* __restrict__ done) {
+    if (*done) return;
+    using J = JBrick<BX, BY, BZ>;
+    __shared__ P s0[J::NA];
+    __shared__ P s1[J::NC];
+    __shared__ P sd[J::NC];
+    const int bx = (int)(blockIdx.x % tiles_x);
+    const int by = (int)((blockIdx.x / tiles_x) % tiles_y);
+    const int bz = (int)(blockIdx.x / ((unsigned)tiles_x * tiles_y));
+    const int x0 = bx * BX, y0 = by * BY, z0 = bz * BZ;
+    const long plane = (long)nx * ny;
+    for (int t = threadIdx.x; t < J::NA; t += blockDim.x) {
+        const int lx = t % J::AX, ly = (t / J::AX) % J::AY, lz = t / (J::AX * J::AY);
+        const int gx = wrapm(x0 - 2 + lx, nx), gy = wrapm(y0 - 2 + ly, ny), gz = wrapm(z0 - 2 + lz, nz);
+        s0[t] = phi[(long)gz * plane + (long)gy * nx + gx];
+    }
+    for (int t = threadIdx.x; t < J::NC; t += blockDim.x) {
+        const int lx = t % J::CX, ly = (t / J::CX) % J::CY, lz = t / (J::CX * J::CY);
+        const int gx = wrapm(x0 - 1 + lx, nx), gy = wrapm(y0 - 1 + ly, ny), gz = wrapm(z0 - 1 + lz, nz);
+        sd[t] = div[(long)gz * plane + (long)gy * nx + gx];
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < J::NC; t += blockDim.x) {
+        const int lx = t % J::CX, ly = (t / J::CX) % J::CY, lz = t / (J::CX * J::CY);
+        const int a = ((lz + 1) * J::AY + (ly + 1)) * J::AX + (lx + 1);
+        const P nb = s0[a - 1] + s0[a + 1] + s0[a - J::AX] + s0[a + J::AX] + s0[a - J::AX * J::AY] +
+                     s0[a + J::AX * J::AY];
+        s1[t] = (nb - sd[t]) / (P)6.0;
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < J::NB; t += blockDim.x) {
+        const int lx = t % BX, ly = (t / BX) % BY, lz = t / (BX * BY);
+        const int gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
+        if (gx >= nx || gy >= ny || gz >= nz) continue;
+        const int c = ((lz + 1) * J::CY + (ly + 1)) * J::CX + (lx + 1);
+        const P nb = s1[c - 1] + s1[c + 1] + s1[c - J::CX] + s1[c + J::CX] + s1[c - J::CX * J::CY] +
+                     s1[c + J::CX * J::CY];
+        phin[(long)gz * plane + (long)gy * nx + gx] = (nb - sd[c]) / (P)6.0;
+    }
+}
+
+// One Jacobi sweep (:402-433), for an odd count of sweeps after the fused first one.
 template <typename P>
-__global__ void k_jacobi(const P* __restrict__ phi, const P* __restrict__ div, P* __restrict__ phin, int first,
-                         int nx, int ny, int nz, long N) {
+__global__ void k_jacobi(const P* __restrict__ phi, const P* __restrict__ div, P* __restrict__ phin, int nx, int ny,
+                         int nz, long N, const int* __restrict__ done) {
+    if (*done) return;
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
-        P nb = (P)0;
-        if (!first) {
-            const Nbr n = neighbours(i, nx, ny, nz);
-            nb = phi[n.xm] + phi[n.xp] + phi[n.ym] + phi[n.yp] + phi[n.zm] + phi[n.zp];
-        }
+        const Nbr n = neighbours(i, nx, ny, nz);
+        const P nb = phi[n.xm] + phi[n.xp] + phi[n.ym] + phi[n.yp] + phi[n.zm] + phi[n.zp];
         phin[i] = (nb - div[i]) / (P)6.0;
     }
 }
@@ -105,7 +177,8 @@ template <typename P>
 __global__ void k_project(const double* __restrict__ sx, const double* __restrict__ sy,
                           const double* __restrict__ sz, const P* __restrict__ phi, double* __restrict__ ux,
                           double* __restrict__ uy, double* __restrict__ uz, double* __restrict__ p, int nx, int ny,
-                          int nz, long N) {
+                          int nz, long N, const int* __restrict__ done) {
+    if (*done) return;
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
         const Nbr n = neighbours(i, nx, ny, nz);
         const double dx = 0.5 * ((double)phi[n.xp] - (double)phi[n.xm]);
@@ -118,7 +191,29 @@ __global__ void k_project(const double* __restrict__ sx, const double* __restric
     }
 }
 
-__global__ void k_bfinish(const double* __restrict__ partials, int nparts, int nv, double* __restrict__ out) {
+// Device-side stop rule of one iteration (:468-499): after the projection, the residual
+// sqrt(sum div^2 / N) of this iteration is recorded; a non-finite residual (state 2) or one below
+// tol (state 1) stops the loop -- later kernels of the chunk see done != 0 and return.
+struct BState {
+    int done;        // 0 running, 1 converged, 2 non-finite residual
+    int iterations;  // iterations completed
+    double residual;
+};
+
+__global__ void k_bcheck(const double* __restrict__ l2, BState* __restrict__ st, double tol, double inv_n) {
+    if (threadIdx.x != 0 || st->done) return;
+    const double r = sqrt(*l2 * inv_n);
+    st->iterations += 1;
+    st->residual = r;
+    if (!isfinite(r))
+        st->done = 2;
+    else if (r < tol)
+        st->done = 1;
+}
+
+__global__ void k_bfinish(const double* __restrict__ partials, int nparts, int nv, double* __restrict__ out,
+                          const int* __restrict__ done) {
+    if (done != nullptr && *done) return;
     double acc[3] = {0, 0, 0};
     for (int q = threadIdx.x; q < nparts; q += blockDim.x)
         for (int j = 0; j < nv; ++j) acc[j] += partials[(long)q * nv + j];
@@ -153,6 +248,9 @@ struct DevBuf {
     }
 };
 
+constexpr int kJBX = 32, kJBY = 8, kJBZ = 4;  // brick of the fused sweep pairs
+constexpr int kChunk = 8;                     // iterations per captured graph / host check
+
 template <typename P>
 int run(int nx, int ny, int nz, const double* alpha_h, const double* visc_h, double relax, double tol, int maxit,
         const double force[3], double U[3], double* u_h, BrinkmanResult* res) {
@@ -171,15 +269,21 @@ int run(int nx, int ny, int nz, const double* alpha_h, const double* visc_h, dou
     double* visc = db.alloc<double>(N, err);
     double* partials = db.alloc<double>(3L * kBRedBlocks, err);
     double* scal = db.alloc<double>(4, err);
+    BState* bst = db.alloc<BState>(1, err);
     const bool mixed = sizeof(P) == sizeof(float);
     P* divp = mixed ? db.alloc<P>(N, err) : reinterpret_cast<P*>(div);
     P* phi = db.alloc<P>(N, err);
     P* phin = db.alloc<P>(N, err);
     hipStream_t s = nullptr;
+    hipGraphExec_t chunk_exec = nullptr;
     if (err == hipSuccess) err = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
     auto fail = [&](hipError_t e) {
         res->message = std::string("HIP error in Brinkman solver: ") + hipGetErrorString(e);
-        if (s) (void)hipStreamDestroy(s);
+        if (chunk_exec) (void)hipGraphExecDestroy(chunk_exec);
+        if (s) {
+            (void)hipStreamSynchronize(s);
+            (void)hipStreamDestroy(s);
+        }
         return 1;
     };
     if (err != hipSuccess) return fail(err);
@@ -187,40 +291,71 @@ int run(int nx, int ny, int nz, const double* alpha_h, const double* visc_h, dou
     if ((err = hipMemcpyAsync(visc, visc_h, N * sizeof(double), hipMemcpyHostToDevice, s)) != hipSuccess) return fail(err);
     for (double* a : {ux, uy, uz, p})
         if ((err = hipMemsetAsync(a, 0, N * sizeof(double), s)) != hipSuccess) return fail(err);
+    if ((err = hipMemsetAsync(bst, 0, sizeof(BState), s)) != hipSuccess) return fail(err);
     const unsigned grid = (unsigned)std::min<long>((N + kBThreads - 1) / kBThreads, 65536);
-    const int pressureIterationsPerStep = 35;
-    int rc = 0;
-    for (int iter = 0; iter < maxit; ++iter) {
+    const int tiles_x = (nx + kJBX - 1) / kJBX, tiles_y = (ny + kJBY - 1) / kJBY, tiles_z = (nz + kJBZ - 1) / kJBZ;
+    const unsigned jgrid = (unsigned)((long)tiles_x * tiles_y * tiles_z);
+    const int pressureIterationsPerStep = 35;  // the first one fused into the divergence kernel
+    const int pairs = (pressureIterationsPerStep - 1) / 2, single = (pressureIterationsPerStep - 1) % 2;
+    const int* done = &bst->done;
+    const P* divsrc = mixed ? divp : reinterpret_cast<const P*>(div);
+    // one iteration; the phi buffers alternate the same way every iteration, so the launches of a
+    // chunk replay unchanged
+    auto iteration = [&]() {
         hipLaunchKernelGGL(k_momentum, dim3(grid), dim3(kBThreads), 0, s, ux, uy, uz, p, alpha, visc, sx, sy, sz,
-                           force[0], force[1], force[2], relax, nx, ny, nz, N);
+                           force[0], force[1], force[2], relax, nx, ny, nz, N, done);
         hipLaunchKernelGGL(k_divergence<P>, dim3(kBRedBlocks), dim3(kBThreads), 0, s, sx, sy, sz, div,
-                           mixed ? divp : (P*)nullptr, nx, ny, nz, N, partials);
-        hipLaunchKernelGGL(k_bfinish, dim3(1), dim3(kBThreads), 0, s, partials, kBRedBlocks, 1, scal);
-        const P* divsrc = mixed ? divp : reinterpret_cast<const P*>(div);
-        for (int k = 0; k < pressureIterationsPerStep; ++k) {
-            hipLaunchKernelGGL(k_jacobi<P>, dim3(grid), dim3(kBThreads), 0, s, phi, divsrc, phin, k == 0 ? 1 : 0,
-                               nx, ny, nz, N);
-            std::swap(phi, phin);
+                           mixed ? divp : (P*)nullptr, phi, nx, ny, nz, N, partials, done);
+        hipLaunchKernelGGL(k_bfinish, dim3(1), dim3(kBThreads), 0, s, partials, kBRedBlocks, 1, scal, done);
+        P* a = phi;
+        P* b = phin;
+        for (int k = 0; k < pairs; ++k) {
+            hipLaunchKernelGGL((k_jacobi2<P, kJBX, kJBY, kJBZ>), dim3(jgrid), dim3(256), 0, s, a, divsrc, b, nx, ny, nz,
+                               tiles_x, tiles_y, done);
+            std::swap(a, b);
         }
-        hipLaunchKernelGGL(k_project<P>, dim3(grid), dim3(kBThreads), 0, s, sx, sy, sz, phi, ux, uy, uz, p, nx, ny,
-                           nz, N);
-        double l2 = 0.0;
-        if ((err = hipMemcpyAsync(&l2, scal, sizeof(double), hipMemcpyDeviceToHost, s)) != hipSuccess) return fail(err);
-        if ((err = hipStreamSynchronize(s)) != hipSuccess) return fail(err);
-        res->iterations = iter + 1;
-        res->residual = std::sqrt(l2 / (double)N);
-        if (!std::isfinite(res->residual)) {
-            rc = 2;
-            break;
+        if (single) {
+            hipLaunchKernelGGL(k_jacobi<P>, dim3(grid), dim3(kBThreads), 0, s, a, divsrc, b, nx, ny, nz, N, done);
+            std::swap(a, b);
         }
-        if (res->residual < tol) {
-            res->converged = 1;
-            break;
+        hipLaunchKernelGGL(k_project<P>, dim3(grid), dim3(kBThreads), 0, s, sx, sy, sz, a, ux, uy, uz, p, nx, ny,
+                           nz, N, done);
+        hipLaunchKernelGGL(k_bcheck, dim3(1), dim3(64), 0, s, scal, bst, tol, 1.0 / (double)N);
+    };
+    BState hs{0, 0, 0.0};
+    auto read_state = [&]() {
+        if ((err = hipMemcpyAsync(&hs, bst, sizeof(BState), hipMemcpyDeviceToHost, s)) != hipSuccess) return false;
+        return (err = hipStreamSynchronize(s)) == hipSuccess;
+    };
+    int it = 0;
+    const int full = maxit / kChunk;
+    if (full > 0) {
+        hipGraph_t graph = nullptr;
+        if ((err = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal)) != hipSuccess) return fail(err);
+        for (int c = 0; c < kChunk; ++c) iteration();
+        if ((err = hipStreamEndCapture(s, &graph)) != hipSuccess) return fail(err);
+        err = hipGraphInstantiate(&chunk_exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (err != hipSuccess) return fail(err);
+        for (int c = 0; c < full; ++c) {
+            if ((err = hipGraphLaunch(chunk_exec, s)) != hipSuccess) return fail(err);
+            it += kChunk;
+            if (!read_state()) return fail(err);
+            if (hs.done) break;
         }
     }
+    if (!hs.done) {
+        for (; it < maxit; ++it) iteration();
+        if (!read_state()) return fail(err);
+    }
+    if ((err = hipGetLastError()) != hipSuccess) return fail(err);
+    res->iterations = hs.iterations;
+    res->residual = hs.residual;
+    res->converged = hs.done == 1 ? 1 : 0;
+    int rc = hs.done == 2 ? 2 : 0;
     if (rc == 0) {
         hipLaunchKernelGGL(k_vel_sums, dim3(kBRedBlocks), dim3(kBThreads), 0, s, ux, uy, uz, N, partials);
-        hipLaunchKernelGGL(k_bfinish, dim3(1), dim3(kBThreads), 0, s, partials, kBRedBlocks, 3, scal);
+        hipLaunchKernelGGL(k_bfinish, dim3(1), dim3(kBThreads), 0, s, partials, kBRedBlocks, 3, scal, (const int*)nullptr);
         double sums[3];
         if ((err = hipMemcpyAsync(sums, scal, sizeof(sums), hipMemcpyDeviceToHost, s)) != hipSuccess) return fail(err);
         if (u_h) {
@@ -233,6 +368,7 @@ int run(int nx, int ny, int nz, const double* alpha_h, const double* visc_h, dou
         if (!std::isfinite(U[0]) || !std::isfinite(U[1]) || !std::isfinite(U[2])) rc = 3;
     }
     if ((err = hipGetLastError()) != hipSuccess) return fail(err);
+    if (chunk_exec) (void)hipGraphExecDestroy(chunk_exec);
     (void)hipStreamDestroy(s);
     return rc;
 }

@@ -148,6 +148,12 @@ long launch_zgreen(cx<R>* S, const R* G6, const Geometry3& g, const FFTTables<R>
 template <typename R>
 long launch_zgreen_batch(cx<R>* S, const R* G6, const Geometry3& g, const FFTTables<R>& t, double* qpart,
                          long qstride, int nrhs, const int* skip, int skip_stride, hipStream_t st);
+// Events that the next fused z-pass launch of the calling thread records at the kernel's own start
+// and end (armed by a caller that times it; consumed by that launch).
+struct KernelEvents {
+    hipEvent_t start = nullptr, stop = nullptr;
+};
+KernelEvents& zpass_events();
 // Upper bounds of the workgroup counts of launch_xinv / launch_zgreen over every variant.
 long xinv_max_blocks(const Geometry3& g);
 long zgreen_max_blocks(const Geometry3& g);

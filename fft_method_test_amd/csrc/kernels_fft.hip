@@ -18,6 +18,8 @@
 #include "kernels.hpp"
 #include "zgreen_reg.hpp"
 
+#include <hip/hip_ext.h>
+
 #include <algorithm>
 #include <map>
 #include <mutex>
@@ -25,6 +27,27 @@
 #include <vector>
 
 namespace fftperm {
+
+KernelEvents& zpass_events() {
+    thread_local KernelEvents ev;
+    return ev;
+}
+
+namespace {
+// Launches a fused z-pass kernel; when the caller armed zpass_events(), the launch itself records
+// them at the kernel's start and end (hipExtLaunchKernelGGL), so the timing holds the kernel alone,
+// without the launch gaps that events recorded around it would add.  The events are consumed.
+template <typename... KArgs, typename... Args>
+void zlaunch(void (*kernel)(KArgs...), dim3 grid, dim3 block, size_t lds, hipStream_t st, Args... args) {
+    KernelEvents& ev = zpass_events();
+    if (ev.start == nullptr && ev.stop == nullptr) {
+        hipLaunchKernelGGL(kernel, grid, block, lds, st, args...);
+    } else {
+        hipExtLaunchKernelGGL(kernel, grid, block, (std::uint32_t)lds, st, ev.start, ev.stop, 0, args...);
+        ev = KernelEvents{};
+    }
+}
+}  // namespace
 
 namespace {
 
@@ -449,7 +472,7 @@ long launch_zgreen_c(cx<R>* S, const R* G6, long ncols, const Geometry3& g, cons
     } else {
         if (lds > g_lds_max) return -1;
         const long blocks = (ncols + ZL - 1) / ZL;
-        hipLaunchKernelGGL((k_zgreen_c<R, N, ZL, OS>), dim3((unsigned)blocks), dim3(threads), lds, st, S, G6, ncols,
+        zlaunch((k_zgreen_c<R, N, ZL, OS>), dim3((unsigned)blocks), dim3(threads), lds, st, S, G6, ncols,
                            g.pz, g.Nc, g.nx, g.rp, tw, qpart, skip);
         return blocks;
     }
@@ -587,7 +610,7 @@ long launch_zgreen_p(cx<R>* S, const R* G6, long ncols, const Geometry3& g, cons
         const long per_cu = std::max<long>(1, (long)(g_lds_cu / lds));
         long blocks = std::min<long>((long)g_cus * per_cu, ((ntiles + 7) / 8) * 8);
         blocks = std::max<long>(8, blocks / 8 * 8);
-        hipLaunchKernelGGL((k_zgreen_p<R, N, ZL>), dim3((unsigned)blocks), dim3(threads), lds, st, S, G6, ncols, g.pz,
+        zlaunch((k_zgreen_p<R, N, ZL>), dim3((unsigned)blocks), dim3(threads), lds, st, S, G6, ncols, g.pz,
                            g.Nc, g.nx, g.rp, tw, qpart, skip);
         return blocks;
     }
@@ -682,7 +705,7 @@ long launch_zgreen_r(cx<R>* S, const R* G6, long ncols, const Geometry3& g, cons
         constexpr size_t lds = zgreen_r_lds<R, N, ZL>();
         if (lds > g_lds_max) return -1;
         const long blocks = (ncols + ZL - 1) / ZL;
-        hipLaunchKernelGGL((k_zgreen_r<R, N, ZL>), dim3((unsigned)blocks), dim3(threads), lds, st, S, G6, ncols, g.pz,
+        zlaunch((k_zgreen_r<R, N, ZL>), dim3((unsigned)blocks), dim3(threads), lds, st, S, G6, ncols, g.pz,
                            g.Nc, g.nx, g.rp, tw, qpart, skip);
         return blocks;
     }
@@ -704,10 +727,10 @@ long launch_zgreen_zl(cx<R>* S, const R* G6, long ncols, const Geometry3& g, con
         const long blocks = (ncols + ZL - 1) / ZL;
         constexpr int threads = zgreen_threads<R, N, ZL, OS>();
         if (zb.nrhs == 1) {
-            hipLaunchKernelGGL((k_zgreen_s<R, N, ZL, OS>), dim3((unsigned)blocks), dim3(threads), lds, st, S, G6,
+            zlaunch((k_zgreen_s<R, N, ZL, OS>), dim3((unsigned)blocks), dim3(threads), lds, st, S, G6,
                                ncols, g.pz, g.Nc, g.nx, g.rp, tw, qpart, skip);
         } else if constexpr (sizeof(R) == 4) {  // the multi-RHS solve runs the FP32 operator
-            hipLaunchKernelGGL((k_zgreen_sb<R, N, ZL, OS>), dim3((unsigned)blocks), dim3(threads), lds, st, S, G6,
+            zlaunch((k_zgreen_sb<R, N, ZL, OS>), dim3((unsigned)blocks), dim3(threads), lds, st, S, G6,
                                ncols, g.pz, g.Nc, g.nx, g.rp, tw, qpart, skip, zb);
         } else {
             return -1;
@@ -1266,7 +1289,7 @@ long launch_zgreen(cx<R>* S, const R* G6, const Geometry3& g, const FFTTables<R>
         if (b >= 0) return b;
     }
     PassCfg c = cfg_zgreen(t.pz, ncols);
-    hipLaunchKernelGGL(k_zgreen<R>, dim3((unsigned)c.blocks), dim3(c.threads), c.lds, st, S, G6, g.nz, ncols, g.pz,
+    zlaunch(k_zgreen<R>, dim3((unsigned)c.blocks), dim3(c.threads), c.lds, st, S, G6, g.nz, ncols, g.pz,
                        g.Nc, g.nx, g.rp, t.pz, t.twz, c.lines, qpart, skip);
     return c.blocks;
 }

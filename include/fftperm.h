@@ -217,10 +217,12 @@ int fftperm_residual_norm(fftperm_ctx* ctx, const double* x, const double* b, do
 int fftperm_symmetry_gap(fftperm_ctx* ctx, const double* x, const double* y, double* out);
 
 /* Live kernel timing (no counterpart in the reference; a measurement hook).  While enabled, the
- * context records HIP events on its own stream around every fused z-pass launch (z forward FFT,
- * Green multiply, z backward FFT) and around every five-sweep Green apply (of one right-hand side,
- * or of all sides of a multi-RHS iteration); Krylov applies that ran after convergence (no-ops) are
- * not counted.  fftperm_profile(ctx, mode) resets and sets the mode: 0 off; 1 every Krylov
+ * context times every fused z-pass launch (z forward FFT, Green multiply, z backward FFT) with HIP
+ * events that the launch itself records at the kernel's start and end (hipExtLaunchKernelGGL; the
+ * one-launch z pass of tensor-batch mode 2 with events recorded around it), and every five-sweep
+ * Green apply (of one right-hand side, or of all sides of a multi-RHS iteration) with events
+ * around it on the context's stream; Krylov applies that ran after convergence (no-ops) are not
+ * counted.  fftperm_profile(ctx, mode) resets and sets the mode: 0 off; 1 every Krylov
  * iteration is timed (launched eagerly, no graph replay); 2 sampled: one chunk of iterations in
  * eight is timed and the others replay graphs without events.  Other modes -> FFTPERM_EINVAL. */
 typedef struct {
@@ -264,8 +266,8 @@ int fftperm_profile_read(fftperm_ctx* ctx, fftperm_profile_stats* out);
 /* ------------------------------------------------------------------ learned warm start
  * No counterpart in the reference.  A 4-level 3D U-Net (16 base channels, BF16 weights and
  * activations, FP32 accumulation on the matrix cores) maps a periodic fluid mask to the interface
- * force density of the x-direction variational problem; the y and z directions use the axis-swapped
- * medium (the Stokes problem is isotropic and the cell cubic) with the components exchanged back.
+ * force densities of the three unit load directions of the variational problem in one forward pass
+ * (output channels 3d, 3d + 1, 3d + 2: the x, y, z components for load direction d).
  * Grids: cubic, edge a positive multiple of 8. */
 typedef struct fftperm_unet fftperm_unet;
 
@@ -275,17 +277,18 @@ int fftperm_unet_num_params(fftperm_unet* net, uint64_t* count);
 /* flat FP32 parameters: per layer the weights [cout][27 or 1 taps][cin] then the biases [cout] */
 int fftperm_unet_get_params(fftperm_unet* net, float* params);
 int fftperm_unet_set_params(fftperm_unet* net, const float* params);
-/* Forward, mean squared error over the support voxels of the three predicted components, backward;
+/* Forward, mean squared error over the support voxels of the nine predicted components, backward;
  * with lr > 0 one Adam step (beta 0.9 / 0.999).  fluid_mask_dev (n^3 uint8, fluid = 1) and
- * target_dev (n^3 x 3 FP32, the force density on support voxels) are device pointers.  loss and
- * grads (num_params FP32, may be NULL) receive the loss and its gradient before the step. */
+ * target_dev (n^3 x 9 FP32: per voxel the force densities of load directions x, y, z, three
+ * components each, on support voxels) are device pointers.  loss and grads (num_params FP32, may
+ * be NULL) receive the loss and its gradient before the step. */
 int fftperm_unet_loss_grad(fftperm_unet* net, const uint8_t* fluid_mask_dev, const float* target_dev, uint32_t n,
                            float lr, double* loss, float* grads);
-/* Prediction for the x direction: field_dev (device) n^3 x 3 FP32. */
+/* Prediction for the three load directions: field_dev (device) n^3 x 9 FP32, layout as the target. */
 int fftperm_unet_predict(fftperm_unet* net, const uint8_t* fluid_mask_dev, uint32_t n, float* field_dev);
-/* fftperm_solve_tensor with the network's prediction p (gathered on the support) seeding each
- * direction's MINRES: x0 = alpha p with alpha = <b, A p> / ||A p||^2 (the minimal-residual scale of
- * the predicted shape, one FP64 apply).  MINRES stays the judge: a guess whose FP64 true residual is
+/* fftperm_solve_tensor with the network's prediction (one forward pass for the three directions;
+ * direction d's density p gathered on the support) seeding each direction's solve: x0 = alpha p with
+ * alpha = <b, A p> / ||A p||^2 (the minimal-residual scale of the predicted shape, one FP64 apply).  MINRES stays the judge: a guess whose FP64 true residual is
  * not below ||b|| is dropped and that direction starts cold.  used_guess[3] (may be NULL) reports
  * the choice. */
 int fftperm_solve_tensor_warm(fftperm_ctx* ctx, fftperm_unet* net, double tol, uint64_t maxit, double K[9],
