@@ -1,7 +1,8 @@
 // unet.hip -- learned warm start: a 4-level 3D U-Net (16 base channels, BF16 weights and
 // activations, FP32 accumulation) that maps a periodic binary medium to the interface force
-// density of the x-direction variational problem, with its own training kernels (backward
-// data / weight passes, Adam) and the C ABI of include/fftperm.h.
+// densities of the three unit load directions of the variational problem (one forward pass for
+// the whole tensor solve), with its own training kernels (backward data / weight passes, Adam)
+// and the C ABI of include/fftperm.h.
 //
 //   features x0 [n^3][8]: solid indicator, interface-support indicator, 6 zero channels
 //   level 0 (n)   : conv 8->16, conv 16->16                 -> e0 (skip)
@@ -9,10 +10,11 @@
 //   level 2 (n/4) : avgpool, conv 32->64, conv 64->64        -> e2 (skip)
 //   level 3 (n/8) : avgpool, conv 64->128, conv 128->128
 //   decoder       : nearest upsample, concat skip, conv (C+skip)->C_l, conv C_l->C_l per level
-//   head          : 1x1x1 conv 16->16 (channels 0..2 = force density, FP32)
+//   head          : 1x1x1 conv 16->16 (channels 3d..3d+2 = force density of load direction d, FP32)
 // Every 3x3x3 convolution is periodic and followed by ReLU.  The loss is the mean squared error
-// over the support voxels (channel 1 of the features) of the three predicted components.
+// over the support voxels (channel 1 of the features) of the nine predicted components.
 #include "kernels.hpp"
+#include "unet.hpp"
 #include "unet_kernels.hpp"
 
 #include <cmath>
@@ -133,9 +135,9 @@ __global__ void k_split(const uint16_t* __restrict__ d, long V, int C0, int C1, 
     }
 }
 
-// masked MSE over the support (feature channel 1) of pred channels 0..2 against target [V][3];
-// dpred [V][16] BF16 = d loss / d pred (0 off the support and on channels 3..15); partial sums
-// of (squared error, support count) per block.
+// masked MSE over the support (feature channel 1) of pred channels 0..8 (direction d's three
+// components at 3d..3d+2) against target [V][9]; dpred [V][16] BF16 = d loss / d pred (0 off the
+// support and on channels 9..15); partial sums of (squared error, support count) per block.
 __global__ void k_loss(const float* __restrict__ pred, const float* __restrict__ target,
                        const uint16_t* __restrict__ feat, long V, float inv_count, uint16_t* __restrict__ dpred,
                        double* __restrict__ part) {
@@ -148,8 +150,8 @@ __global__ void k_loss(const float* __restrict__ pred, const float* __restrict__
         if (sup) {
             acc[1] += 1.0;
 #pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                const float d = pred[v * 16 + c] - target[v * 3 + c];
+            for (int c = 0; c < kOutputs; ++c) {
+                const float d = pred[v * 16 + c] - target[v * kOutputs + c];
                 acc[0] += (double)d * d;
                 if (inv_count > 0.f) dp[c] = f2bf(2.f * d * inv_count);
             }
@@ -215,53 +217,22 @@ __global__ void k_to_bf16(const float* __restrict__ x, long count, uint16_t* __r
         y[e] = f2bf(x[e]);
 }
 
-// axis swap of a cubic uint8 mask: axis 1 swaps x and y, axis 2 swaps x and z (involutions)
-__global__ void k_swap_mask(const uint8_t* __restrict__ in, int n, int axis, uint8_t* __restrict__ out) {
-    const long V = (long)n * n * n;
-    for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < V; v += (long)gridDim.x * blockDim.x) {
-        const int i = (int)(v % n), j = (int)((v / n) % n), k = (int)(v / ((long)n * n));
-        long src;
-        if (axis == 1)
-            src = ((long)k * n + i) * n + j;
-        else if (axis == 2)
-            src = ((long)i * n + j) * n + k;
-        else
-            src = v;
-        out[v] = in[src];
-    }
-}
-
-// Force density of the original problem from the prediction on the axis-swapped medium: the
-// field is swapped back and its components x <-> axis exchanged.  Output field [3][V] (SoA) or
-// compact (support gather through `active`) in double.
-__global__ void k_unswap_gather(const float* __restrict__ pred, int n, int axis, const int64_t* __restrict__ active,
-                                long nb, double* __restrict__ x0) {
+// The predicted force density of load direction `dir` (channels 3 dir .. 3 dir + 2) gathered on
+// the support (through `active`) into the compact AoS vector, in double.
+__global__ void k_gather_direction(const float* __restrict__ pred, int dir, const int64_t* __restrict__ active,
+                                   long nb, double* __restrict__ x0) {
     for (long l = (long)blockIdx.x * blockDim.x + threadIdx.x; l < nb; l += (long)gridDim.x * blockDim.x) {
-        const long v = active[l];
-        const int i = (int)(v % n), j = (int)((v / n) % n), k = (int)(v / ((long)n * n));
-        long src;
-        if (axis == 1)
-            src = ((long)k * n + i) * n + j;
-        else if (axis == 2)
-            src = ((long)i * n + j) * n + k;
-        else
-            src = v;
-        float f[3] = {pred[src * 16 + 0], pred[src * 16 + 1], pred[src * 16 + 2]};
-        if (axis != 0) {
-            const float t = f[0];
-            f[0] = f[axis];
-            f[axis] = t;
-        }
+        const float* f = pred + active[l] * 16 + 3 * dir;
         x0[3 * l + 0] = f[0];
         x0[3 * l + 1] = f[1];
         x0[3 * l + 2] = f[2];
     }
 }
 
-__global__ void k_pred_field(const float* __restrict__ pred, long V, float* __restrict__ out3) {
+__global__ void k_pred_field(const float* __restrict__ pred, long V, float* __restrict__ out9) {
     for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < V; v += (long)gridDim.x * blockDim.x)
 #pragma unroll
-        for (int c = 0; c < 3; ++c) out3[v * 3 + c] = pred[v * 16 + c];
+        for (int c = 0; c < kOutputs; ++c) out9[v * kOutputs + c] = pred[v * 16 + c];
 }
 
 // ------------------------------------------------------------------ launchers
@@ -338,7 +309,7 @@ struct Acts {
     // per tensor: BF16 [V_level][C]
     std::vector<uint16_t*> t;
     float* pred = nullptr;   // [V][16]
-    float* tgt = nullptr;    // [V][3]
+    float* tgt = nullptr;    // [V][kOutputs]
     uint8_t* mask = nullptr; // swapped medium
     double* part = nullptr;
     uint16_t* g1 = nullptr;  // gradient scratch, largest tensor
@@ -460,7 +431,7 @@ struct Net {
         a.g2 = dmalloc<uint16_t>(gmax);
         a.g3 = dmalloc<uint16_t>(gmax);
         a.pred = dmalloc<float>(vox(n, 0) * 16);
-        a.tgt = dmalloc<float>(vox(n, 0) * 3);
+        a.tgt = dmalloc<float>(vox(n, 0) * kOutputs);
         a.mask = dmalloc<uint8_t>(vox(n, 0));
         a.dpred = dmalloc<uint16_t>(vox(n, 0) * 16);
         a.part = dmalloc<double>(2 * 1024);
@@ -612,13 +583,13 @@ struct Net {
         check(hipStreamSynchronize(s));
         double cnt = 0.0;
         for (unsigned i = 0; i < blocks; ++i) cnt += part[i];
-        const float inv = (grad && cnt > 0) ? (float)(1.0 / (3.0 * cnt)) : 0.f;
+        const float inv = (grad && cnt > 0) ? (float)(1.0 / ((double)kOutputs * cnt)) : 0.f;
         hipLaunchKernelGGL(k_loss, dim3(blocks), dim3(kEw), 0, s, a.pred, a.tgt, T(X0), V, inv, a.dpred, a.part);
         check(hipMemcpyAsync(part.data(), a.part, 2 * blocks * sizeof(double), hipMemcpyDeviceToHost, s));
         check(hipStreamSynchronize(s));
         double se = 0.0;
         for (unsigned i = 0; i < blocks; ++i) se += part[2 * i];
-        return cnt > 0 ? se / (3.0 * cnt) : 0.0;
+        return cnt > 0 ? se / ((double)kOutputs * cnt) : 0.0;
     }
 
     // gradient of layer li given dY (pre-ReLU-masked inside): accumulates gw, gb; writes dX to
@@ -724,24 +695,23 @@ static void check_edge(int n) {
     if (n < 8 || n % 8 != 0) throw std::invalid_argument("U-Net grid edge must be a positive multiple of 8");
 }
 
-void predict(Net* net, const uint8_t* mask_dev, int n, int axis) {
+void predict(Net* net, const uint8_t* mask_dev, int n) {
     check_edge(n);
     net->ensure_acts(n);
-    hipLaunchKernelGGL(k_swap_mask, dim3(ew_grid(Net::vox(n, 0))), dim3(kEw), 0, net->s, mask_dev, n, axis,
-                       net->a.mask);
+    Net::check(hipMemcpyAsync(net->a.mask, mask_dev, Net::vox(n, 0), hipMemcpyDeviceToDevice, net->s));
     net->forward(n);
     Net::check(hipGetLastError());
 }
 
-void pred_field(Net* net, float* out3) {
+void pred_field(Net* net, float* out9) {
     const long V = Net::vox(net->a.n, 0);
-    hipLaunchKernelGGL(k_pred_field, dim3(ew_grid(V)), dim3(kEw), 0, net->s, net->a.pred, V, out3);
+    hipLaunchKernelGGL(k_pred_field, dim3(ew_grid(V)), dim3(kEw), 0, net->s, net->a.pred, V, out9);
     Net::check(hipStreamSynchronize(net->s));
 }
 
-void gather_guess(Net* net, int axis, const int64_t* active, long nb, double* x0) {
-    hipLaunchKernelGGL(k_unswap_gather, dim3(ew_grid(nb)), dim3(kEw), 0, net->s, net->a.pred, net->a.n, axis, active,
-                       nb, x0);
+void gather_guess(Net* net, int dir, const int64_t* active, long nb, double* x0) {
+    hipLaunchKernelGGL(k_gather_direction, dim3(ew_grid(nb)), dim3(kEw), 0, net->s, net->a.pred, dir, active, nb,
+                       x0);
     Net::check(hipStreamSynchronize(net->s));
 }
 
@@ -750,7 +720,7 @@ double loss_grad(Net* net, const uint8_t* mask_dev, const float* target_dev, int
     net->ensure_acts(n);
     const long V = Net::vox(n, 0);
     Net::check(hipMemcpyAsync(net->a.mask, mask_dev, V, hipMemcpyDeviceToDevice, net->s));
-    Net::check(hipMemcpyAsync(net->a.tgt, target_dev, V * 3 * sizeof(float), hipMemcpyDeviceToDevice, net->s));
+    Net::check(hipMemcpyAsync(net->a.tgt, target_dev, V * kOutputs * sizeof(float), hipMemcpyDeviceToDevice, net->s));
     net->forward(n);
     const double l = net->loss(n, true);
     net->zero_grads();

@@ -17,15 +17,19 @@ long num_params(Net* n);
 void get_params(Net* n, float* out);
 void set_params(Net* n, const float* in);
 void get_grads(Net* n, float* out);
-// forward on the fluid mask (device, n^3, fluid = 1, n a multiple of 8) swapped across axis
-// (0: none, 1: x <-> y, 2: x <-> z); the prediction stays in the network
-void predict(Net* n, const uint8_t* mask_dev, int edge, int axis);
-// last prediction as [V][3] FP32 (device), synchronous
-void pred_field(Net* n, float* out3_dev);
-// last prediction (made with `axis`) swapped back, components exchanged, gathered on the support
-// voxels `active` (device, nb) into x0 (device, 3 nb, double), synchronous
-void gather_guess(Net* n, int axis, const int64_t* active, long nb, double* x0);
-// forward + masked MSE + backward (+ Adam step when update), synchronous; returns the loss
+// Output channels that carry the prediction: the force densities of the three unit load
+// directions, direction d's x, y, z components at channels 3d, 3d + 1, 3d + 2.
+constexpr int kOutputs = 9;
+// forward on the fluid mask (device, n^3, fluid = 1, n a multiple of 8); the prediction stays in
+// the network
+void predict(Net* n, const uint8_t* mask_dev, int edge);
+// last prediction as [V][kOutputs] FP32 (device), synchronous
+void pred_field(Net* n, float* out9_dev);
+// direction dir of the last prediction gathered on the support voxels `active` (device, nb) into
+// x0 (device, 3 nb, double, AoS), synchronous
+void gather_guess(Net* n, int dir, const int64_t* active, long nb, double* x0);
+// forward + masked MSE against target_dev ([V][kOutputs] FP32) + backward (+ Adam step when
+// update), synchronous; returns the loss
 double loss_grad(Net* n, const uint8_t* mask_dev, const float* target_dev, int edge, bool update, float lr);
 hipStream_t stream(Net* n);
 

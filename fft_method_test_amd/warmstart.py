@@ -1,8 +1,8 @@
 """Learned warm start for the Bignonnet MINRES solves (no counterpart in the reference).
 
 A 4-level 3D U-Net (16 base channels, BF16 weights/activations, FP32 accumulation; HIP kernels in
-csrc/unet*.hip) maps a periodic fluid mask to the interface force density of the x-direction
-variational problem.  It is trained on solver-generated (medium, force density) pairs by its own
+csrc/unet*.hip) maps a periodic fluid mask to the interface force densities of the three unit load
+directions of the variational problem in one forward pass (output channels 3d..3d+2 for direction d).  It is trained on solver-generated (medium, force density) pairs by its own
 forward / backward / Adam kernels, and its prediction seeds MINRES; MINRES stays the judge of
 the error (fftperm_solve_tensor_warm drops a guess that does not lower the FP64 true residual).
 """
@@ -16,6 +16,8 @@ import numpy as np
 from . import _lib, geometry
 from ._lib import check
 
+# predicted channels: three load directions x three components
+OUTPUTS = 9
 # the flat parameter layout of the network, per layer (taps, cin0, cin1, cout)
 LAYERS = ((27, 8, 0, 16), (27, 16, 0, 16), (27, 16, 0, 32), (27, 32, 0, 32), (27, 32, 0, 64), (27, 64, 0, 64),
           (27, 64, 0, 128), (27, 128, 0, 128), (27, 128, 64, 64), (27, 64, 0, 64), (27, 64, 32, 32),
@@ -66,14 +68,17 @@ class WarmStartNet:
         p = self.get_params()
         if bf16:
             bits = (p.view(np.uint32) + 0x7FFF + ((p.view(np.uint32) >> 16) & 1)) >> 16
-            np.savez(path, params_bf16=bits.astype(np.uint16), layers=np.asarray(LAYERS, dtype=np.int32))
+            np.savez(path, params_bf16=bits.astype(np.uint16), layers=np.asarray(LAYERS, dtype=np.int32),
+                     outputs=np.int32(OUTPUTS))
         else:
-            np.savez(path, params=p, layers=np.asarray(LAYERS, dtype=np.int32))
+            np.savez(path, params=p, layers=np.asarray(LAYERS, dtype=np.int32), outputs=np.int32(OUTPUTS))
 
     def load(self, path):
         with np.load(path, allow_pickle=False) as z:
             if not np.array_equal(z["layers"], np.asarray(LAYERS, dtype=np.int32)):
                 raise ValueError("saved network has a different architecture")
+            if "outputs" not in z or int(z["outputs"]) != OUTPUTS:
+                raise ValueError("saved network predicts a different set of outputs")
             if "params_bf16" in z:
                 p = (z["params_bf16"].astype(np.uint32) << 16).view(np.float32)
             else:
@@ -81,7 +86,8 @@ class WarmStartNet:
             self.set_params(p)
 
     def loss_grad(self, mask_dev, target_dev, n, lr=0.0, want_grad=False):
-        """mask_dev: torch uint8 (n^3, fluid = 1) on the GPU; target_dev: torch float32 (n^3, 3).
+        """mask_dev: torch uint8 (n^3, fluid = 1) on the GPU; target_dev: torch float32 (n^3, 9): the
+        force densities of load directions x, y, z (three components each).
         Returns (loss, grads or None); lr > 0 also takes one Adam step."""
         import torch
 
@@ -97,7 +103,7 @@ class WarmStartNet:
         import torch
 
         torch.cuda.synchronize()
-        out = torch.empty((n ** 3, 3), dtype=torch.float32, device=mask_dev.device)
+        out = torch.empty((n ** 3, OUTPUTS), dtype=torch.float32, device=mask_dev.device)
         check(_lib.load().fftperm_unet_predict(self._h, C.c_void_p(mask_dev.data_ptr()), int(n),
                                                C.c_void_p(out.data_ptr())))
         return out
@@ -124,122 +130,131 @@ def random_medium(n, rng):
     return geometry.boolean_ellipsoids(n, (12.0, 4.0, 4.0), phi, seed)
 
 
-def force_density_x(green, support, tol, maxit):
-    """Direction-x force density of the variational problem on the grid (n^3, 3) float32, from
-    the GPU solver (RHS + MINRES, FP64)."""
+def medium_for_seed(n, seed):
+    """The medium of one training / held-out seed (flat uint8 n^3, fluid = 1)."""
+    return np.ascontiguousarray(random_medium(n, np.random.default_rng(seed)).reshape(-1), dtype=np.uint8)
+
+
+def _media_worker(args):
+    n, seed = args
+    return seed, medium_for_seed(n, seed)
+
+
+def media(n, seeds, workers=1):
+    """Yields (seed, mask) in seed order; workers > 1 generates them in spawned processes (the
+    ellipsoid packs take ~1 s each at 128^3 on one core)."""
+    if workers <= 1:
+        for sd in seeds:
+            yield sd, medium_for_seed(n, sd)
+        return
+    import multiprocessing as mp
+
+    with mp.get_context("spawn").Pool(workers) as pool:
+        for out in pool.imap(_media_worker, [(n, sd) for sd in seeds], chunksize=2):
+            yield out
+
+
+def density_targets(green, support, tol, maxit):
+    """The three load directions' force densities on the support (NB x 9 float32: direction d's
+    components in columns 3d..3d+2), each scaled to unit RMS over the support (the network learns
+    the shape; the solve fixes the scale by the minimal-residual step along it), from the GPU
+    solver (RHS + MinresSolver of the context's precision).  Returns (densities, iterations[3])."""
     from . import permeability as P
 
-    b = P.VariationalRHS(support, green, None).build([1.0, 0.0, 0.0])
-    res = P.MinresSolver(tol, maxit).solve(P.VariationalOperator(support, green), b)
-    N = green.grid().total_size()
-    field = np.zeros((N, 3), dtype=np.float32)
-    field[support.active_voxels().astype(np.int64)] = res.solution.reshape(-1, 3)
-    return field, res
+    op = P.VariationalOperator(support, green)
+    out = np.zeros((support.num_active_voxels(), OUTPUTS), dtype=np.float32)
+    its = []
+    for d in range(3):
+        g = [0.0, 0.0, 0.0]
+        g[d] = 1.0
+        b = P.VariationalRHS(support, green, None).build(g)
+        res = P.MinresSolver(tol, maxit).solve(op, b)
+        x = res.solution.reshape(-1, 3)
+        rms = float(np.sqrt(np.mean(x * x)))
+        out[:, 3 * d:3 * d + 3] = x / rms if rms > 0 else x
+        its.append(int(res.iterations))
+    return out, its
 
 
-def generate_pairs(n, count, seed, tol=1e-6, maxit=20000, log=None):
-    """`count` (fluid mask, force density normalised to unit RMS on the support) pairs at n^3
-    from the FP64 GPU solver."""
+class TrainingSet:
+    """Solver-generated (medium, densities) pairs held on the GPU: per pair the mask (n^3 uint8),
+    the support voxels (int64) and the densities on them (NB x 9 float32)."""
+
+    def __init__(self, n):
+        self.n = n
+        self.items = []
+
+    def __len__(self):
+        return len(self.items)
+
+    def add(self, mask, active, dens):
+        import torch
+
+        dev = torch.device("cuda:0")
+        self.items.append((torch.from_numpy(mask).to(dev), torch.from_numpy(active.astype(np.int64)).to(dev),
+                           torch.from_numpy(dens).to(dev)))
+
+    def target(self, i, buf):
+        """Writes pair i's target field ([n^3][9], zero off the support) into buf."""
+        _, act, dens = self.items[i]
+        buf.zero_()
+        buf[act] = dens
+        return buf
+
+
+def generate(n, seeds, tol=1e-5, maxit=20000, workers=1, precision=None, log=None, into=None):
+    """Training pairs for the given seeds at n^3: media (spawned workers), then the GPU solves."""
     from . import permeability as P
 
-    rng = np.random.default_rng(seed)
+    prec = _lib.PRECISION_FP64 if precision is None else precision
+    data = TrainingSet(n) if into is None else into
     g = P.Grid3D(n, n, n)
-    green = P.GreenOperator(g)
-    pairs = []
+    green = P.GreenOperator(g, 1.0, 2, prec)
     t0 = time.perf_counter()
-    for i in range(count):
-        mask = random_medium(n, rng).reshape(-1)
+    seeds = list(seeds)
+    for k, (sd, mask) in enumerate(media(n, seeds, workers)):
         sup = P.ForceSupport(P.BinaryMedium(g, mask), P.SUPPORT_INTERFACE, green)
         if sup.num_active_voxels() == 0:
             continue
-        field, res = force_density_x(green, sup, tol, maxit)
-        # unit RMS over the support: the network learns the shape, the solve fixes the scale
-        # (minimal-residual step along the prediction)
-        rms = float(np.sqrt(np.mean(field[sup.active_voxels().astype(np.int64)] ** 2)))
-        if rms > 0:
-            field /= rms
-        pairs.append((mask.astype(np.uint8), field))
-        if log and (i + 1) % max(1, count // 10) == 0:
-            log(f"[warmstart] pairs {i + 1}/{count} at {n}^3 ({time.perf_counter() - t0:.1f} s, last "
-                f"{res.iterations} iterations)")
+        dens, its = density_targets(green, sup, tol, maxit)
+        data.add(mask, sup.active_voxels().astype(np.int64), dens)
+        if log and ((k + 1) % 25 == 0 or k + 1 == len(seeds)):
+            log(f"[warmstart] pairs {k + 1}/{len(seeds)} at {n}^3 ({time.perf_counter() - t0:.1f} s, last seed {sd} "
+                f"iterations {its})")
     green.close()
-    return pairs
+    return data
 
 
-def train(net: WarmStartNet, pairs, n, epochs, lr=1e-3, seed=0, log=None):
-    """Adam over the pairs (one sample per step, shuffled each epoch); returns the loss history."""
+def train(net: WarmStartNet, data: TrainingSet, epochs, lr=1e-3, lr_min=2e-5, seed=0, log=None, deadline=None):
+    """Adam over the pairs, one pair per step, shuffled each epoch; the learning rate follows a
+    cosine from lr to lr_min over all steps.  Stops early (after finishing a step) past `deadline`
+    (time.perf_counter() value).  Returns the per-epoch mean loss."""
+    import math
+
     import torch
 
     rng = np.random.default_rng(seed)
-    dev = torch.device("cuda:0")
-    masks = [torch.from_numpy(m).to(dev) for m, _ in pairs]
-    targets = [torch.from_numpy(np.ascontiguousarray(f)).to(dev) for _, f in pairs]
+    n = data.n
+    buf = torch.zeros((n ** 3, OUTPUTS), dtype=torch.float32, device="cuda:0")
+    total = max(1, epochs * len(data))
+    step = 0
     hist = []
+    t0 = time.perf_counter()
     for ep in range(epochs):
-        order = rng.permutation(len(pairs))
-        tot = 0.0
-        for i in order:
-            l, _ = net.loss_grad(masks[i], targets[i], n, lr=lr)
+        tot, cnt = 0.0, 0
+        for i in rng.permutation(len(data)):
+            a = lr_min + 0.5 * (lr - lr_min) * (1.0 + math.cos(math.pi * step / total))
+            l, _ = net.loss_grad(data.items[i][0], data.target(int(i), buf), n, lr=a)
             tot += l
-        hist.append(tot / max(1, len(pairs)))
+            cnt += 1
+            step += 1
+            if deadline is not None and time.perf_counter() > deadline:
+                break
+        hist.append(tot / max(1, cnt))
         if log:
-            log(f"[warmstart] epoch {ep + 1}/{epochs} mean loss {hist[-1]:.4e}")
+            log(f"[warmstart] epoch {ep + 1}/{epochs} mean loss {hist[-1]:.4e} ({time.perf_counter() - t0:.0f} s)")
+        if deadline is not None and time.perf_counter() > deadline:
+            if log:
+                log("[warmstart] time budget reached")
+            break
     return hist
-
-
-def train_residual(net: WarmStartNet, masks, n, epochs, lr=2e-4, seed=0, log=None):
-    """Train on the quantity MINRES sees: the relative residual of the scaled guess,
-    loss = min_alpha ||b - alpha A p||^2 / ||b||^2 (p = the prediction gathered on the support,
-    b the direction-x RHS, A the FP64 variational operator).  At the optimal alpha its gradient
-    with respect to p is -2 alpha A r / ||b||^2, r = b - alpha A p (A is symmetric).  The network's
-    squared-error kernel delivers a prescribed output gradient g when trained against the target
-    pred - (3 count / 2) g, count = support voxels, which is how the step is taken.  Returns the
-    per-epoch mean relative residual (the square root of the loss)."""
-    import torch
-
-    from . import permeability as P
-
-    rng = np.random.default_rng(seed)
-    dev = torch.device("cuda:0")
-    g = P.Grid3D(n, n, n)
-    green = P.GreenOperator(g)
-    samples = []
-    for m in masks:
-        sup = P.ForceSupport(P.BinaryMedium(g, m), P.SUPPORT_INTERFACE, green)
-        if sup.num_active_voxels() == 0:
-            continue
-        b = P.VariationalRHS(sup, green, None).build([1.0, 0.0, 0.0])
-        samples.append((np.ascontiguousarray(m, dtype=np.uint8), sup.active_voxels().astype(np.int64), b))
-    hist = []
-    for ep in range(epochs):
-        tot = 0.0
-        for i in rng.permutation(len(samples)):
-            m, act, b = samples[i]
-            nb_ = np.zeros(1, dtype=np.uint64)
-            check(_lib.load().fftperm_set_medium(green._h, m, _lib.SUPPORT_INTERFACE,
-                                                 nb_.ctypes.data_as(C.POINTER(C.c_uint64))))
-            mdev = torch.from_numpy(m).to(dev)
-            pred = net.predict(mdev, n).cpu().numpy()
-            p = pred[act].astype(np.float64).reshape(-1)
-            Ap = _apply(green, p)
-            bb = float(b @ b)
-            apap = float(Ap @ Ap)
-            alpha = float(b @ Ap) / apap if apap > 0 else 0.0
-            r = b - alpha * Ap
-            tot += np.sqrt(float(r @ r) / bb)
-            grad_p = (-2.0 * alpha / bb) * _apply(green, r)
-            gfield = np.zeros_like(pred)
-            gfield[act] = grad_p.reshape(-1, 3)
-            target = (pred - 1.5 * act.size * gfield).astype(np.float32)
-            net.loss_grad(mdev, torch.from_numpy(target).to(dev), n, lr=lr)
-        hist.append(tot / max(1, len(samples)))
-        if log:
-            log(f"[warmstart] residual epoch {ep + 1}/{epochs} mean relative residual of the guess {hist[-1]:.4f}")
-    green.close()
-    return hist
-
-
-def _apply(green, x):
-    y = np.zeros_like(x)
-    check(_lib.load().fftperm_operator_apply(green._h, np.ascontiguousarray(x, dtype=np.float64), y))
-    return y

@@ -62,7 +62,7 @@ def test_unet_gradients_match_torch(fp):
     net, p = _net(seed=5)
     mask = random_mask((n, n, n), 0.55, seed=11)
     rng = np.random.default_rng(2)
-    target = rng.standard_normal((n ** 3, 3)).astype(np.float32) * 0.1
+    target = rng.standard_normal((n ** 3, 9)).astype(np.float32) * 0.1
     loss, g = net.loss_grad(torch.from_numpy(mask).to("cuda:0"), torch.from_numpy(target).to("cuda:0"), n,
                             want_grad=True)
     layers = R.unflatten(p)
@@ -96,9 +96,10 @@ def test_unet_training_reduces_loss(fp):
     # x-difference of the fluid indicator, a constant
     f = mask.reshape(n, n, n).astype(np.float32)
     face = sum(np.roll(f, s, axis=a) for a in range(3) for s in (-1, 1)) / 6.0
-    dx = 0.5 * (np.roll(f, -1, axis=2) - np.roll(f, 1, axis=2))
-    t = np.stack([face, dx, np.full_like(f, 0.5)], -1)
-    target = torch.from_numpy(t.reshape(-1, 3).astype(np.float32)).to("cuda:0")
+    d = [0.5 * (np.roll(f, -1, axis=a) - np.roll(f, 1, axis=a)) for a in (2, 1, 0)]
+    # direction k: (face, d_k, 0.5) on its three channels
+    t = np.stack([c for k in range(3) for c in (face, d[k], np.full_like(f, 0.5))], -1)
+    target = torch.from_numpy(t.reshape(-1, 9).astype(np.float32)).to("cuda:0")
     m = torch.from_numpy(mask).to("cuda:0")
     first, _ = net.loss_grad(m, target, n, lr=2e-3)
     for _ in range(60):

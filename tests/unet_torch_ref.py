@@ -74,19 +74,20 @@ def forward(layers, x, bf16_acts=False):
     d2 = conv(9, conv(8, torch.cat([up(b3), e2], 1)))
     d1 = conv(11, conv(10, torch.cat([up(d2), e1], 1)))
     d0 = conv(13, conv(12, torch.cat([up(d1), e0], 1)))
-    return conv(14, d0, relu=False)[:, :3]
+    return conv(14, d0, relu=False)[:, :9]
 
 
 def loss(pred, target, support):
-    """Mean squared error over the support voxels of the 3 components; target [1, 3, n, n, n]."""
+    """Mean squared error over the support voxels of the 9 predicted components (three load
+    directions x three components); target [1, 9, n, n, n]."""
     m = support.to(pred.dtype)
-    return ((pred - target) ** 2 * m).sum() / (3.0 * m.sum())
+    return ((pred - target) ** 2 * m).sum() / (9.0 * m.sum())
 
 
 def field_to_ncdhw(field, n):
-    """(n^3, 3) voxel-major -> [1, 3, n, n, n]."""
-    return torch.as_tensor(np.asarray(field, dtype=np.float32)).reshape(n, n, n, 3).permute(3, 0, 1, 2)[None]
+    """(n^3, 9) voxel-major -> [1, 9, n, n, n]."""
+    return torch.as_tensor(np.asarray(field, dtype=np.float32)).reshape(n, n, n, 9).permute(3, 0, 1, 2)[None]
 
 
 def ncdhw_to_field(t):
-    return t[0].permute(1, 2, 3, 0).reshape(-1, 3).detach().numpy()
+    return t[0].permute(1, 2, 3, 0).reshape(-1, 9).detach().numpy()
