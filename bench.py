@@ -119,11 +119,36 @@ def run_precision(args, P, _lib, mask_dev, precision, tol=None, steps=None, warm
     return res
 
 
+# HBM traffic per launch of the fused z pass from two rocprofv3 --pmc runs (FETCH_SIZE x 2.0,
+# WRITE_SIZE x 1.0: the factors a 2 GiB copy kernel reads for the same counters) on this workload,
+# keyed by (precision, tile code as fftperm_profile_read reports it); the summary with the commands
+# is profiles/passes_256_hbm_traffic.md.  A tile the autotuner picks that was not profiled reports null.
+Z_TRAFFIC_MB = {
+    ("fp64", 41): (856.7, 436.4),  # k_zgreen_s<double, 256, 4, 1>
+    ("fp64", 81): (856.3, 427.8),  # k_zgreen_s<double, 256, 8, 1>
+    ("fp32", 82): (428.2, 216.4),  # k_zgreen_s<float, 256, 8, 2>
+    ("fp32", 161): (428.1, 213.9),  # k_zgreen_s<float, 256, 16, 1>
+    ("fp32", 41): (575.8, 303.7),  # k_zgreen_s<float, 256, 4, 1>
+}
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(args, mask, applies_per_solve, min_seconds=10.0):
     """Oracle (FP64 restatement of the reference path) on the host cores: repeated operator applies
     at the same size (a bounded sample of about min_seconds), scaled by the applies one tensor solve
-    needs."""
+    needs; plus one fully timed oracle tensor solve on the 32^3 sphere cell (tol 1e-10)."""
     import oracle
+    from fft_method_test_amd import geometry
 
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     n = args.n
@@ -139,6 +164,12 @@ def cpu_baseline(args, mask, applies_per_solve, min_seconds=10.0):
             break
     t_apply = (time.perf_counter() - t0) / reps
     o.close()
+    c1 = oracle.Oracle((32, 32, 32))
+    c1.set_medium(geometry.sphere_cell(32).reshape(-1))
+    t0 = time.perf_counter()
+    K1, infos1 = c1.solve_tensor(1e-10, 10000)
+    c1_s = time.perf_counter() - t0
+    c1.close()
     return {
         "value": 1.0 / (t_apply * applies_per_solve),
         "unit": "k-tensor solves/s",
@@ -148,7 +179,29 @@ def cpu_baseline(args, mask, applies_per_solve, min_seconds=10.0):
                    f"FFTs + Green multiply, {threads} OpenMP threads), scaled by the {applies_per_solve} applies of "
                    f"one FP64 tensor solve at tol {args.tol:g} (RHS + MINRES iterations + recovery, 3 directions); "
                    f"Green-tensor setup excluded"),
+        "fft_provider": ("the oracle's own mixed-radix Stockham C2C FFT (oracle/oracle.c; FFTW3 is not installed, "
+                         "the reference's FFTW backend cannot be built)"),
+        "cpu_model": _cpu_model(),
+        "apply_s_at_n": t_apply,
+        "c1_sphere32_tensor_solve": {"seconds": c1_s, "tol": 1e-10,
+                                     "iterations": [int(i.iterations) for i in infos1],
+                                     "k_xx": float(K1[0, 0]), "threads": threads},
     }
+
+
+def oracle_fixture_column(args, mask):
+    """The oracle's FP64 x column of K for this medium at tol 1e-12, minted in the build container
+    (tests/golden/make_k_fixtures.py), when the medium is the one it was minted on."""
+    import hashlib
+
+    path = os.path.join(ROOT, "tests", "golden", "k_fixtures.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        rec = json.load(f).get("c256")
+    if not rec or hashlib.sha256(mask.reshape(-1).tobytes()).hexdigest() != rec["mask_sha256"]:
+        return None
+    return np.asarray(rec["Kcol_x_tol1e-12"], dtype=np.float64)
 
 
 def main():
@@ -201,15 +254,26 @@ def main():
             from fft_method_test_amd import warmstart as W
 
             net = W.WarmStartNet(0)
-            net.load(args.warmstart)
-            runs["mixed_learned"] = run_precision(args, P, _lib, mask_dev, _lib.PRECISION_MIXED,
-                                                  label=f"mixed+learned tol={args.tol:g}", net=net)
+            try:
+                net.load(args.warmstart)
+            except ValueError as e:
+                log(f"[bench] warm-start network not usable ({e}); no mixed+learned run")
+            else:
+                runs["mixed_learned"] = run_precision(args, P, _lib, mask_dev, _lib.PRECISION_MIXED,
+                                                      label=f"mixed+learned tol={args.tol:g}", net=net)
             net.close()
 
     def k_err(r):
         if ref is None:
             return None
         return float(np.linalg.norm(r["K"] - ref["K"]) / np.linalg.norm(ref["K"]))
+
+    col_o = oracle_fixture_column(args, mask)
+
+    def col_err(r):
+        if col_o is None:
+            return None
+        return float(np.linalg.norm(r["K"][:, 0] - col_o) / np.linalg.norm(col_o))
 
     errs = {k: k_err(r) for k, r in runs.items()}
     meets = {k: (e is not None and e <= K_ERR_BAR) for k, e in errs.items()}
@@ -240,6 +304,9 @@ def main():
     z_bytes = sides_per_launch * 2 * 3 * Nh * sc + 6 * Nh * sr
     z_ms = pr["z_ms"][pi] / z_n
     achieved = z_bytes / (z_ms * 1e-3) / 1e9
+    z_tile = pr["tiles"]["zgreen"][pi]
+    tr = Z_TRAFFIC_MB.get(("fp32" if pi == 1 else "fp64", z_tile)) if sides_per_launch == 1 else None
+    traffic = None if tr is None else (tr[0] + tr[1]) * 1e6
     L = 3 * head["nb"]
     a_bytes = bytes_apply(N, Nh, L, sc, sr)
     a_ms = pr["a_ms"][pi] / max(1, pr["a_rhs"][pi])  # per right-hand side
@@ -276,7 +343,10 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": (None if tr is None else
+                               f"rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of z tile {z_tile}: {tr[0]} MB read + "
+                               f"{tr[1]} MB written per launch (profiles/passes_256_hbm_traffic.md)"),
             "algorithmic_bytes_per_launch": z_bytes,
             "sides_per_launch": sides_per_launch,
             "avg_launch_ms": z_ms,
@@ -293,7 +363,8 @@ def main():
         "runs": {
             k: {
                 "solves_per_s": r["solves_per_s"], "ms_per_step": r["ms_per_step"], "tol": r["tol"],
-                "k_rel_error": errs[k], "meets_k_error_bar": meets[k], "iterations": r["iterations"],
+                "k_rel_error": errs[k], "meets_k_error_bar": meets[k],
+                "k_col_x_rel_error_vs_oracle": col_err(r), "iterations": r["iterations"],
                 "fp64_residual_checks": r["sweeps"], "converged": r["converged"], "final_rel_residual": r["final_rel"],
                 "setup_s": r["setup_s"], "variants": r["prof"]["tiles"], "K": r["K"].tolist(),
                 "used_guess": r["used_guess"],
@@ -301,8 +372,11 @@ def main():
         },
         "reference": None if ref is None else {
             "tol": ref["tol"], "solves_per_s": ref["solves_per_s"], "iterations": ref["iterations"],
-            "K": ref["K"].tolist(),
+            "K": ref["K"].tolist(), "k_col_x_rel_error_vs_oracle": col_err(ref),
         },
+        "oracle_fixture": (None if col_o is None else
+                           {"what": "CPU oracle FP64 x column of K at tol 1e-12 (tests/golden/k_fixtures.json)",
+                            "Kcol_x": col_o.tolist()}),
         "speedup_mixed_vs_fp64": (runs["mixed"]["solves_per_s"] / runs["fp64"]["solves_per_s"]
                                   if "fp64" in runs and "mixed" in runs else None),
         "speedup_head_vs_fp64": (head["solves_per_s"] / runs["fp64"]["solves_per_s"] if "fp64" in runs else None),

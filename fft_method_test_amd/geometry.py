@@ -72,16 +72,32 @@ def _random_rotation(rng) -> np.ndarray:
 
 
 def boolean_ellipsoids(n: int, semi_axes=(12.0, 4.0, 4.0), porosity: float = 0.5, seed: int = 0) -> np.ndarray:
-    """Boolean model of prolate ellipsoids with uniformly random orientations."""
+    """Boolean model of prolate ellipsoids with uniformly random orientations.  Per ellipsoid the
+    generator draws the integer centre, then the rotation; the voxel sets are evaluated in
+    batches (the union does not depend on the order)."""
     rng = np.random.Generator(np.random.PCG64(seed))
     vol = 4.0 / 3.0 * math.pi * semi_axes[0] * semi_axes[1] * semi_axes[2]
     lam = -math.log(porosity) / vol
     count = rng.poisson(lam * n ** 3)
+    centres = np.empty((count, 3), dtype=np.int64)
+    rots = np.empty((count, 3, 3))
+    for e in range(count):
+        centres[e] = rng.integers(0, n, size=(1, 3))[0]
+        rots[e] = _random_rotation(rng)
     solid = np.zeros((n, n, n), dtype=np.uint8)
-    for _ in range(count):
-        c = rng.integers(0, n, size=(1, 3))
-        off = _ellipsoid_offsets(semi_axes, _random_rotation(rng))
-        _stamp(solid, c, off)
+    flat = solid.reshape(-1)
+    R = int(math.ceil(max(semi_axes)))
+    o = np.arange(-R, R + 1)
+    dz, dy, dx = np.meshgrid(o, o, o, indexing="ij")
+    p = np.stack([dx.ravel(), dy.ravel(), dz.ravel()], axis=1).astype(np.float64)
+    pzyx = np.stack([dz.ravel(), dy.ravel(), dx.ravel()], axis=1).astype(np.int64)
+    sa = np.asarray(semi_axes, dtype=np.float64)
+    for s0 in range(0, count, 32):
+        rb = rots[s0:s0 + 32]
+        q = np.matmul(p[None], rb)  # body coordinates, p @ rot per ellipsoid
+        keep = ((q / sa) ** 2).sum(axis=2) <= 1.0
+        pts = (centres[s0:s0 + 32, None, :] + pzyx[None])[keep] % n
+        flat[(pts[:, 0] * n + pts[:, 1]) * n + pts[:, 2]] = 1
     return (1 - solid).astype(np.uint8)
 
 
