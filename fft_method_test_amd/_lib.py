@@ -26,7 +26,8 @@ ZGREEN_CODES = (11, 12, 21, 22, 41, 42, 81, 82, 161, 162, 44, 84, 164, 324, 45, 
 
 EXPORTED_SYMBOLS = (
     "fftperm_last_error", "fftperm_device_info", "fftperm_create", "fftperm_destroy",
-    "fftperm_set_medium", "fftperm_set_medium_dev", "fftperm_support_noalloc", "fftperm_porosity",
+    "fftperm_set_medium", "fftperm_set_medium_dev", "fftperm_support_noalloc", "fftperm_solve_batch",
+    "fftperm_porosity",
     "fftperm_get_support",
     "fftperm_green_tensor", "fftperm_fft_forward", "fftperm_fft_inverse", "fftperm_spectral_green",
     "fftperm_green_apply", "fftperm_background_force", "fftperm_rhs", "fftperm_operator_apply",
@@ -90,6 +91,8 @@ def load():
     L.fftperm_set_medium_dev.argtypes = [_vp, _vp, C.c_int, C.POINTER(C.c_uint64)]
     L.fftperm_support_noalloc.argtypes = [C.c_uint64] * 3 + [_vp, C.c_int, _vp, _vp, C.c_uint64, _vp, _vp,
                                                              C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+    L.fftperm_solve_batch.argtypes = [_vp, C.c_uint32, C.POINTER(C.c_void_p), C.c_int, _dp, C.c_double, C.c_uint64,
+                                      _dp, _vp]
     L.fftperm_porosity.argtypes = [_vp, C.POINTER(C.c_double)]
     L.fftperm_get_support.argtypes = [_vp, _vp, _vp]
     L.fftperm_green_tensor.argtypes = [_vp, _dp]

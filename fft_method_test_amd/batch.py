@@ -1,12 +1,14 @@
 """Batched permeability solves of many microstructures of uneven sizes on one GPU.
 
-The batch is dealt largest-first (n^3 as the work estimate) to `workers` host threads.  Each
-worker owns one fftperm context per grid size on its own HIP stream, so the problems of
-different workers run concurrently on the GPU; inside a context every MINRES iteration pair
-replays a captured hipGraph, which keeps the small problems from being bound by kernel launch
-latency.  ctypes releases the GIL during each library call, so the workers' host loops overlap.
-The per-sample solve is the one of `fftperm_solve_direction` (support detection, RHS, MINRES,
-recovery, average), so a sample's result does not depend on the batch it ran in.
+Media of the same grid size form a size class.  A size class is solved by fftperm_solve_batch:
+every Krylov iteration advances all of its media, the y and z FFT passes of an iteration are one
+launch for the class, the vector and scalar kernels one launch each, and the iterations replay
+captured hipGraphs (MIXED precision, CR on the FP32 operator with reliable FP64 updates; each
+medium keeps its own support map, state and stopping test, so its result is the one its own
+solve_one_direction gives).  Size classes are dealt largest-first to `workers` host threads, each
+owning one context per size on its own HIP stream, so classes run concurrently on the GPU.
+`mode="per_sample"` solves the media one at a time per worker instead (fftperm_solve_direction; the
+only mode for FP64).
 """
 from __future__ import annotations
 
@@ -22,23 +24,37 @@ from . import _lib, geometry
 from ._lib import check
 
 
-def c4_batch(count=128, seed0=4000, sizes=(64, 80, 96, 112, 128, 144, 160)):
+def c4_sample(i, seed0=4000, sizes=(64, 80, 96, 112, 128, 144, 160)):
+    """Sample i of the batch alone: (n, fluid mask uint8 n^3, kind, porosity)."""
+    return _c4_sample((i, seed0, tuple(sizes)))
+
+
+def _c4_sample(args):
+    i, seed0, sizes = args
+    rng = np.random.default_rng(seed0 + i)
+    n = int(rng.choice(sizes))
+    phi = float(rng.uniform(0.3, 0.8))
+    kind = "spheres" if rng.uniform() < 0.5 else "ellipsoids"
+    s = int(rng.integers(0, 2 ** 31 - 1))
+    if kind == "spheres":
+        mask = geometry.boolean_spheres(n, 8.0, phi, s)
+    else:
+        mask = geometry.boolean_ellipsoids(n, (12.0, 4.0, 4.0), phi, s)
+    return n, mask.reshape(-1).astype(np.uint8), kind, phi
+
+
+def c4_batch(count=128, seed0=4000, sizes=(64, 80, 96, 112, 128, 144, 160), workers=1):
     """The 128-sample batch: cubic n drawn uniformly from `sizes`, porosity U[0.3, 0.8], half
     Boolean spheres of radius 8, half prolate ellipsoids (semi-axes 12, 4, 4, random orientation);
-    sample i uses seed seed0 + i.  Returns a list of (n, fluid mask uint8 n^3, kind, porosity)."""
-    out = []
-    for i in range(count):
-        rng = np.random.default_rng(seed0 + i)
-        n = int(rng.choice(sizes))
-        phi = float(rng.uniform(0.3, 0.8))
-        kind = "spheres" if rng.uniform() < 0.5 else "ellipsoids"
-        s = int(rng.integers(0, 2 ** 31 - 1))
-        if kind == "spheres":
-            mask = geometry.boolean_spheres(n, 8.0, phi, s)
-        else:
-            mask = geometry.boolean_ellipsoids(n, (12.0, 4.0, 4.0), phi, s)
-        out.append((n, mask.reshape(-1).astype(np.uint8), kind, phi))
-    return out
+    sample i uses seed seed0 + i.  Returns a list of (n, fluid mask uint8 n^3, kind, porosity).
+    workers > 1 generates the media in spawned processes."""
+    jobs = [(i, seed0, tuple(sizes)) for i in range(count)]
+    if workers <= 1:
+        return [_c4_sample(j) for j in jobs]
+    import multiprocessing as mp
+
+    with mp.get_context("spawn").Pool(workers) as pool:
+        return pool.map(_c4_sample, jobs, chunksize=1)
 
 
 class _Worker:
@@ -61,14 +77,35 @@ class _Worker:
         self.ctx.clear()
 
 
+BATCH_MAX = 64  # FFTPERM_BATCH_MAX
+
+
 def solve_batch(samples, tol=1e-6, maxit=20000, precision=_lib.PRECISION_MIXED, workers=4, grad=(1.0, 0.0, 0.0),
-                mu=1.0, p_radius=2, prepare=True):
+                mu=1.0, p_radius=2, prepare=True, mode="batched", max_batch=BATCH_MAX):
     """Solve one load direction for every (n, mask, ...) sample.  Returns (results, seconds) where
     results[i] = dict(n, k (the column of K: +mu <v>), iterations, converged, seconds).  With
     prepare=True every context (and its autotuning) is created before the clock starts."""
+    if mode not in ("batched", "per_sample"):
+        raise ValueError("mode must be 'batched' or 'per_sample'")
+    if mode == "batched" and precision != _lib.PRECISION_MIXED:
+        raise ValueError("batched solves run in MIXED precision")
     L = _lib.load()
     ws = [_Worker(precision, mu, p_radius) for _ in range(workers)]
-    order = sorted(range(len(samples)), key=lambda i: -samples[i][0] ** 3)
+    g = np.asarray(grad, dtype=np.float64)
+    results = [None] * len(samples)
+    errors = []
+    q = queue.Queue()
+    if mode == "batched":
+        groups = {}
+        for i, s_ in enumerate(samples):
+            groups.setdefault(s_[0], []).append(i)
+        items = []
+        for n, idx in groups.items():
+            for s0 in range(0, len(idx), max_batch):
+                items.append((n, idx[s0:s0 + max_batch]))
+        items.sort(key=lambda it: -(it[0] ** 3) * len(it[1]))
+    else:
+        items = [(samples[i][0], [i]) for i in sorted(range(len(samples)), key=lambda i: -samples[i][0] ** 3)]
     if prepare:
         # contexts (FFT plans, Green tensor, z / y autotuning) and the x-pass autotuning, which
         # runs on the first medium of a grid shape, happen here, one at a time, off the clock
@@ -80,37 +117,53 @@ def solve_batch(samples, tol=1e-6, maxit=20000, precision=_lib.PRECISION_MIXED, 
             for n, mask in sorted(first.items()):
                 check(L.fftperm_set_medium(w.context(n), np.ascontiguousarray(mask, dtype=np.uint8),
                                            _lib.SUPPORT_INTERFACE, C.byref(nb)))
-    q = queue.Queue()
-    for i in order:
-        q.put(i)
-    results = [None] * len(samples)
-    errors = []
-    g = np.asarray(grad, dtype=np.float64)
+    for it in items:
+        q.put(it)
+
+    def run_batch(w, n, idx):
+        h = w.context(n)
+        masks = [np.ascontiguousarray(samples[i][1], dtype=np.uint8) for i in idx]
+        ptrs = (C.c_void_p * len(idx))(*[m.ctypes.data for m in masks])
+        k = np.zeros(3 * len(idx))
+        infos = (_lib.SolveInfo * len(idx))()
+        t0 = time.perf_counter()
+        check(L.fftperm_solve_batch(h, len(idx), ptrs, _lib.SUPPORT_INTERFACE, g, tol, maxit, k,
+                                    C.cast(infos, C.c_void_p)))
+        dt = time.perf_counter() - t0
+        for j, i in enumerate(idx):
+            results[i] = dict(n=n, k=k[3 * j:3 * j + 3].tolist(), iterations=int(infos[j].iterations),
+                              converged=bool(infos[j].converged), seconds=dt, batch=len(idx))
+
+    def run_one(w, n, i):
+        mask = samples[i][1]
+        h = w.context(n)
+        t0 = time.perf_counter()
+        nb = C.c_uint64()
+        check(L.fftperm_set_medium(h, np.ascontiguousarray(mask, dtype=np.uint8), _lib.SUPPORT_INTERFACE,
+                                   C.byref(nb)))
+        avg = np.zeros(3)
+        info = _lib.SolveInfo()
+        if nb.value == 0:
+            results[i] = dict(n=n, k=[0.0, 0.0, 0.0], iterations=0, converged=False, seconds=0.0,
+                              note="empty support")
+            return
+        check(L.fftperm_solve_direction(h, g, tol, maxit, None, avg, None, C.byref(info)))
+        results[i] = dict(n=n, k=(mu * avg).tolist(), iterations=int(info.iterations),
+                          converged=bool(info.converged), seconds=time.perf_counter() - t0)
 
     def run(w):
         while True:
             try:
-                i = q.get_nowait()
+                n, idx = q.get_nowait()
             except queue.Empty:
                 return
-            n, mask = samples[i][0], samples[i][1]
             try:
-                h = w.context(n)
-                t0 = time.perf_counter()
-                nb = C.c_uint64()
-                check(L.fftperm_set_medium(h, np.ascontiguousarray(mask, dtype=np.uint8), _lib.SUPPORT_INTERFACE,
-                                           C.byref(nb)))
-                avg = np.zeros(3)
-                info = _lib.SolveInfo()
-                if nb.value == 0:
-                    results[i] = dict(n=n, k=[0.0, 0.0, 0.0], iterations=0, converged=False, seconds=0.0,
-                                      note="empty support")
-                    continue
-                check(L.fftperm_solve_direction(h, g, tol, maxit, None, avg, None, C.byref(info)))
-                results[i] = dict(n=n, k=(mu * avg).tolist(), iterations=int(info.iterations),
-                                  converged=bool(info.converged), seconds=time.perf_counter() - t0)
+                if mode == "batched":
+                    run_batch(w, n, idx)
+                else:
+                    run_one(w, n, idx[0])
             except Exception as e:  # reported after the batch
-                errors.append((i, repr(e)))
+                errors.append((idx, repr(e)))
 
     t0 = time.perf_counter()
     threads = [threading.Thread(target=run, args=(w,)) for w in ws]
@@ -122,7 +175,7 @@ def solve_batch(samples, tol=1e-6, maxit=20000, precision=_lib.PRECISION_MIXED, 
     for w in ws:
         w.close()
     if errors:
-        raise RuntimeError(f"{len(errors)} batch samples failed, first: {errors[0]}")
+        raise RuntimeError(f"{len(errors)} batch items failed, first: {errors[0]}")
     return results, elapsed
 
 
@@ -142,15 +195,17 @@ def main():
     ap.add_argument("--workers", type=int, default=4)
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="mixed")
+    ap.add_argument("--mode", choices=["batched", "per_sample"], default="batched")
+    ap.add_argument("--gen-workers", type=int, default=1, help="processes generating the media")
     args = ap.parse_args()
     t0 = time.perf_counter()
-    samples = c4_batch(args.count, args.seed0)
+    samples = c4_batch(args.count, args.seed0, workers=args.gen_workers)
     print(f"[batch] generated {len(samples)} media in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     prec = _lib.PRECISION_MIXED if args.precision == "mixed" else _lib.PRECISION_FP64
-    res, sec = solve_batch(samples, tol=args.tol, precision=prec, workers=args.workers)
+    res, sec = solve_batch(samples, tol=args.tol, precision=prec, workers=args.workers, mode=args.mode)
     its = [r["iterations"] for r in res]
     out = dict(samples=len(res), seconds=sec, samples_per_s=len(res) / sec, workers=args.workers, tol=args.tol,
-               precision=args.precision, iterations_total=int(sum(its)), converged=int(sum(r["converged"] for r in res)),
+               precision=args.precision, mode=args.mode, iterations_total=int(sum(its)), converged=int(sum(r["converged"] for r in res)),
                voxels_total=int(sum(r["n"] ** 3 for r in res)),
                k_xx=[r["k"][0] for r in res])
     print(json.dumps(out), flush=True)

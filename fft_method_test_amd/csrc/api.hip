@@ -1650,6 +1650,304 @@ void set_medium_impl(fftperm_ctx* c, const uint8_t* mask, bool host, int mode, u
     }
 }
 
+// ------------------------------------------------------------ batches of media of one shape
+// The uneven-size batch solves many media of a grid shape together: every Krylov iteration
+// advances all of them, the y and z FFT passes of an iteration are one launch for all media (the
+// z pass reads the Green tensor once per tile for all of them), the vector and scalar kernels are
+// one launch each, and the iterations replay captured hipGraphs.  Each medium keeps its own support
+// map, compact vectors, CR state, reliable updates and stopping test, so its result is the one its
+// own solve_one_direction gives.
+
+// A medium's support map, owned by the batch (the context's medium fields alias it while the
+// batch works on that medium).
+struct MediumSlot {
+    uint8_t* mask = nullptr;
+    int32_t* code = nullptr;
+    int64_t* active = nullptr;
+    uint32_t* lbits = nullptr;
+    int32_t* lstart = nullptr;
+    unsigned long long* lcount = nullptr;
+    uint64_t nb = 0, nf = 0;
+};
+
+struct BatchBufs {
+    int sides = 0;
+    long vs = 0;       // FP32 vector slot stride (>= the largest 3 NB, 256-byte aligned)
+    long Lmax = 0;     // FP64 vector slot stride
+    long spec = 0;     // spectrum slot stride (3 Nc)
+    float2* S = nullptr;
+    float *x32 = nullptr, *r32 = nullptr, *p32 = nullptr, *ap32 = nullptr, *q32 = nullptr;
+    double *b = nullptr, *xacc = nullptr;
+    double *part_x = nullptr, *part_z = nullptr, *part = nullptr;
+    CrState* st = nullptr;
+    long xstride = 0, zstride = 0, pstride = 0;
+    ~BatchBufs() {
+        dfree(S); dfree(x32); dfree(r32); dfree(p32); dfree(ap32); dfree(q32); dfree(b); dfree(xacc);
+        dfree(part_x); dfree(part_z); dfree(part); dfree(st);
+    }
+};
+
+// The context's medium fields point at slot m (non-owning).
+void select_medium(fftperm_ctx* c, const MediumSlot& m) {
+    c->mask = m.mask;
+    c->code = m.code;
+    c->active = m.active;
+    c->lbits = m.lbits;
+    c->lstart = m.lstart;
+    c->lcount = m.lcount;
+    c->nb = m.nb;
+    c->nf = m.nf;
+    c->has_medium = true;
+}
+
+void release_media(fftperm_ctx* c, std::vector<MediumSlot>& slots) {
+    // the context's medium fields alias a slot: clear them before the slots go
+    c->mask = nullptr; c->code = nullptr; c->active = nullptr;
+    c->lbits = nullptr; c->lstart = nullptr; c->lcount = nullptr;
+    c->nb = c->nf = 0;
+    c->has_medium = false;
+    for (MediumSlot& m : slots) {
+        dfree(m.mask); dfree(m.code); dfree(m.active); dfree(m.lbits); dfree(m.lstart); dfree(m.lcount);
+    }
+    slots.clear();
+}
+
+// One lockstep CR iteration over the batch (the steps of cr32_batch_iteration with per-medium
+// support maps).
+void cr32_multi_iteration(fftperm_ctx* c, BatchBufs& B, const std::vector<MediumSlot>& slots) {
+    const Geometry3& g = c->g;
+    const int n = B.sides;
+    for (int b = 0; b < n; ++b) {
+        const MediumSlot& m = slots[b];
+        XSource sr;
+        std::memset(&sr, 0, sizeof(sr));
+        sr.code = m.code;
+        sr.lines = LineBits{m.lbits, m.lstart, (long)m.nb};
+        sr.compact32 = B.r32 + b * B.vs;
+        sr.mean = B.st[b].mean_in;
+        launch_xfwd<float>(sr, B.S + b * B.spec, g, c->t32, &B.st[b].done, c->s);
+    }
+    launch_ypass_n<float>(B.S, -1, g, c->t32, n, &B.st[0].done, kCrDoneStride, c->s);
+    const long zparts = launch_zgreen_batch<float>(B.S, c->G32, g, c->t32, B.part_z, B.zstride, n, &B.st[0].done,
+                                                   kCrDoneStride, c->s);
+    launch_ypass_n<float>(B.S, +1, g, c->t32, n, &B.st[0].done, kCrDoneStride, c->s);
+    long xparts = 0;
+    for (int b = 0; b < n; ++b) {
+        const MediumSlot& m = slots[b];
+        XSink sk;
+        std::memset(&sk, 0, sizeof(sk));
+        sk.code = m.code;
+        sk.lines = LineBits{m.lbits, m.lstart, (long)m.nb};
+        sk.compact32 = B.q32 + b * B.vs;
+        sk.partials = B.part_x + b * B.xstride;
+        xparts = launch_xinv<float>(B.S + b * B.spec, sk, g, c->t32, &B.st[b].done, c->s);
+    }
+    const long nbmax = B.vs / 3;
+    launch_cr_beta_n(B.part_x, (int)xparts, B.xstride, B.part_z, (int)zparts, B.zstride, B.st, n, 0.0,
+                     1.0 / (double)g.N, c->s);
+    launch_cr32_direction_n(B.p32, B.ap32, B.r32, B.q32, B.st, nbmax, B.vs, B.part, B.pstride, n, c->s);
+    launch_cr_alpha_n(B.part, B.pstride, B.st, n, c->s);
+    launch_cr32_update_n(B.x32, B.r32, B.p32, B.ap32, B.st, nbmax, B.vs, B.part, B.pstride, n, c->s);
+    launch_cr_check_n(B.part, B.pstride, B.st, nullptr, 0, n, 0.0, c->s);
+}
+
+// device_cr_reliable<float, true> for every medium of the batch at once (from zero guesses).
+void device_cr32_multi(fftperm_ctx* c, BatchBufs& B, const std::vector<MediumSlot>& slots, double tol,
+                       uint64_t maxit, std::vector<MinresOut>& outs) {
+    const int n = B.sides;
+    std::vector<CrState> h(n);
+    std::vector<double> bnorm(n, 0.0);
+    std::vector<char> active(n, 0);
+    outs.assign(n, MinresOut{});
+    auto L_of = [&](int b) { return 3L * (long)slots[b].nb; };
+    auto true_residual = [&](int b, bool xzero) {
+        select_medium(c, slots[b]);
+        const long L = L_of(b);
+        double* xb = B.xacc + b * B.Lmax;
+        double* bb = B.b + b * B.Lmax;
+        if (!xzero) {
+            device_apply<double>(c, xb, c->tmp);
+            launch_axpby(1.0, bb, -1.0, c->tmp, c->r, L, c->s);
+        } else {
+            HIPCHK(hipMemcpyAsync(c->r, bb, L * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+        }
+        return device_norm(c, c->r, L) / bnorm[b];
+    };
+    auto load_residual = [&](int b) {
+        const long nb = (long)slots[b].nb;
+        launch_compact_sums(c->r, nb, c->part, nullptr, c->s);
+        launch_finish_sum(c->part, kRedBlocks, 3, 1.0 / (double)nb, B.st[b].mean_in, nullptr, c->s);
+        launch_to_f32(c->r, B.r32 + b * B.vs, 3 * nb, c->s);
+    };
+    for (int b = 0; b < n; ++b) {
+        std::memset(&h[b], 0, sizeof(CrState));
+        h[b].done = 1;
+        h[b].nb = (long long)slots[b].nb;
+        HIPCHK(hipMemcpyAsync(&B.st[b], &h[b], sizeof(CrState), hipMemcpyHostToDevice, c->s));
+        if (slots[b].nb == 0) {
+            outs[b].final_rel = 1.0;
+            continue;
+        }
+        const long L = L_of(b);
+        HIPCHK(hipMemsetAsync(B.xacc + b * B.Lmax, 0, L * sizeof(double), c->s));
+        HIPCHK(hipMemsetAsync(B.x32 + b * B.vs, 0, B.vs * sizeof(float), c->s));
+        bnorm[b] = device_norm(c, B.b + b * B.Lmax, L);
+        if (bnorm[b] == 0.0) {
+            outs[b].final_rel = 0.0;
+            outs[b].converged = 1;
+            continue;
+        }
+        const double rel = true_residual(b, true);
+        if (rel <= tol || maxit == 0) {
+            outs[b].final_rel = rel;
+            outs[b].converged = rel <= tol ? 1 : 0;
+            continue;
+        }
+        h[b].done = 0;
+        h[b].rnorm_max = rel * bnorm[b];
+        h[b].rhs_norm = bnorm[b];
+        h[b].tol = tol;
+        h[b].delta = c->reliable_delta;
+        h[b].maxit = (long long)maxit;
+        h[b].first = 1;
+        outs[b].final_rel = rel;
+        HIPCHK(hipMemcpyAsync(&B.st[b], &h[b], sizeof(CrState), hipMemcpyHostToDevice, c->s));
+        load_residual(b);
+        active[b] = 1;
+    }
+    // two iterations per graph replay; the capture bakes in this batch's buffers and support maps
+    hipGraphExec_t exec = nullptr;
+    if (c->use_graphs) {
+        hipGraph_t graph = nullptr;
+        HIPCHK(hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal));
+        for (int i = 0; i < 2; ++i) cr32_multi_iteration(c, B, slots);
+        HIPCHK(hipStreamEndCapture(c->s, &graph));
+        const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        HIPCHK(e);
+    }
+    struct GraphGuard {
+        hipGraphExec_t e;
+        ~GraphGuard() {
+            if (e) (void)hipGraphExecDestroy(e);
+        }
+    } guard{exec};
+    const uint64_t chunk = 8;
+    for (;;) {
+        bool any = false;
+        for (int b = 0; b < n; ++b) any = any || active[b];
+        if (!any) break;
+        bool stop = false;
+        while (!stop) {
+            if (exec) {
+                for (uint64_t i = 0; i < chunk; i += 2) HIPCHK(hipGraphLaunch(exec, c->s));
+            } else {
+                for (uint64_t i = 0; i < chunk; ++i) cr32_multi_iteration(c, B, slots);
+            }
+            HIPCHK(hipMemcpyAsync(h.data(), B.st, n * sizeof(CrState), hipMemcpyDeviceToHost, c->s));
+            c->sync();
+            for (int b = 0; b < n; ++b) stop = stop || (active[b] && h[b].done);
+        }
+        for (int b = 0; b < n; ++b) {
+            if (!active[b] || !h[b].done) continue;
+            const long L = L_of(b);
+            launch_accumulate_f32(B.xacc + b * B.Lmax, B.x32 + b * B.vs, L, c->s);
+            const double rel = true_residual(b, false);
+            outs[b].final_rel = rel;
+            if (rel <= tol || !h[b].pause || h[b].breakdown || h[b].iterations >= h[b].maxit) {
+                active[b] = 0;
+                outs[b].iterations = (uint64_t)h[b].iterations;
+                outs[b].converged = rel <= tol ? 1 : 0;
+                c->fp32_iters += (uint64_t)h[b].iterations;
+                continue;
+            }
+            const double rn = rel * bnorm[b];
+            h[b].rr = rn * rn;
+            h[b].rnorm_max = rn;
+            h[b].done = 0;
+            h[b].pause = 0;
+            HIPCHK(hipMemcpyAsync(&B.st[b], &h[b], sizeof(CrState), hipMemcpyHostToDevice, c->s));
+            load_residual(b);
+        }
+    }
+    c->sync();
+}
+
+void solve_batch_impl(fftperm_ctx* c, uint32_t count, const uint8_t* const* masks, int support_mode,
+                      const double grad[3], double tol, uint64_t maxit, double* k_out, fftperm_solve_info* info) {
+    const Geometry3& g = c->g;
+    std::vector<MediumSlot> slots;
+    struct Release {
+        fftperm_ctx* c;
+        std::vector<MediumSlot>& s;
+        ~Release() { release_media(c, s); }
+    } release{c, slots};
+    // 1. the support map of every medium, owned by its slot
+    c->has_medium = false;
+    dfree(c->mask); dfree(c->code); dfree(c->active); dfree(c->lbits); dfree(c->lstart); dfree(c->lcount);
+    uint64_t nbmax = 0;
+    for (uint32_t b = 0; b < count; ++b) {
+        if (!masks[b]) throw Err(FFTPERM_EINVAL, "null mask");
+        uint64_t nb = 0;
+        set_medium_impl(c, masks[b], true, support_mode, &nb);
+        MediumSlot m;
+        m.mask = c->mask; m.code = c->code; m.active = c->active;
+        m.lbits = c->lbits; m.lstart = c->lstart; m.lcount = c->lcount;
+        m.nb = c->nb; m.nf = c->nf;
+        c->mask = nullptr; c->code = nullptr; c->active = nullptr;
+        c->lbits = nullptr; c->lstart = nullptr; c->lcount = nullptr;
+        slots.push_back(m);
+        nbmax = std::max(nbmax, nb);
+    }
+    // 2. batch buffers
+    BatchBufs B;
+    B.sides = (int)count;
+    const long Lm = std::max(1L, 3L * (long)nbmax);
+    B.vs = (Lm + 63) / 64 * 64;
+    B.Lmax = B.vs;
+    B.spec = 3 * g.Nc;
+    B.S = dalloc<float2>((size_t)count * (size_t)B.spec);
+    HIPCHK(hipMemsetAsync(B.S, 0, (size_t)count * (size_t)B.spec * sizeof(float2), c->s));
+    const size_t nv = (size_t)count * (size_t)B.vs;
+    B.x32 = dalloc<float>(nv); B.r32 = dalloc<float>(nv); B.p32 = dalloc<float>(nv); B.ap32 = dalloc<float>(nv);
+    B.q32 = dalloc<float>(nv);
+    // lanes past a medium's 3 NB stay zero: the vector kernels stop at each medium's length
+    for (float* v : {B.x32, B.r32, B.p32, B.ap32, B.q32}) HIPCHK(hipMemsetAsync(v, 0, nv * sizeof(float), c->s));
+    B.b = dalloc<double>((size_t)count * (size_t)B.Lmax);
+    B.xacc = dalloc<double>((size_t)count * (size_t)B.Lmax);
+    B.xstride = (long)kXinvParts * xinv_max_blocks(g);
+    B.zstride = zgreen_max_blocks(g);
+    B.pstride = 4L * kRedBlocks;
+    B.part_x = dalloc<double>((size_t)count * (size_t)B.xstride);
+    B.part_z = dalloc<double>((size_t)count * (size_t)B.zstride);
+    B.part = dalloc<double>((size_t)count * (size_t)B.pstride);
+    B.st = dalloc<CrState>(count);
+    // 3. right-hand sides
+    for (uint32_t b = 0; b < count; ++b) {
+        if (slots[b].nb == 0) continue;
+        select_medium(c, slots[b]);
+        c->ensure_vectors();
+        device_rhs(c, grad, B.b + b * B.Lmax);
+    }
+    // 4. the lockstep solve
+    std::vector<MinresOut> outs;
+    device_cr32_multi(c, B, slots, tol, maxit, outs);
+    // 5. velocity recovery and domain average per medium: K column = +mu <v>
+    for (uint32_t b = 0; b < count; ++b) {
+        double avg[3] = {0.0, 0.0, 0.0};
+        if (slots[b].nb > 0) {
+            select_medium(c, slots[b]);
+            device_recover(c, grad, B.xacc + b * B.Lmax, avg);
+        }
+        for (int r = 0; r < 3; ++r) k_out[3 * b + r] = c->mu * avg[r];
+        if (info) {
+            fill_info(&info[b], outs[b], 0);
+            if (slots[b].nb == 0) info[b].converged = 0;
+        }
+    }
+    c->sync();
+}
+
 }  // namespace
 
 // ============================================================== C ABI
@@ -2124,6 +2422,21 @@ int fftperm_solve_tensor(fftperm_ctx* ctx, double tol, uint64_t maxit, double K[
                 K[r * 3 + col] = -colK;
             }
         }
+    });
+}
+
+int fftperm_solve_batch(fftperm_ctx* ctx, uint32_t count, const uint8_t* const* fluid_masks, int support_mode,
+                        const double grad[3], double tol, uint64_t maxit, double* k_out, fftperm_solve_info* info) {
+    return guarded([&] {
+        check_ctx(ctx);
+        if (!fluid_masks || !grad || !k_out) throw Err(FFTPERM_EINVAL, "null argument");
+        if (count == 0 || count > FFTPERM_BATCH_MAX) throw Err(FFTPERM_EINVAL, "batch size out of range");
+        if (!(tol > 0.0)) throw Err(FFTPERM_EINVAL, "MinresSolver tolerance must be positive");
+        if (support_mode != FFTPERM_SUPPORT_INTERFACE && support_mode != FFTPERM_SUPPORT_FULLSOLID)
+            throw Err(FFTPERM_EINVAL, "unknown support mode");
+        if (ctx->precision != FFTPERM_PRECISION_MIXED || ctx->mixed_solver != FFTPERM_MIXED_CR_FP32_VECTORS)
+            throw Err(FFTPERM_EINVAL, "batched solves need a MIXED context with the FP32-vector CR solver");
+        solve_batch_impl(ctx, count, fluid_masks, support_mode, grad, tol, maxit, k_out, info);
     });
 }
 

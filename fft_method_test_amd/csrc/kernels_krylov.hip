@@ -523,7 +523,9 @@ constexpr int kFinishThreads = 1024;
 // ------------------------------------------------------------ CR with reliable updates
 // Batched launches serve nrhs independent right-hand sides at once: RHS b owns st[b] and the
 // partial rows at b * (its stride); the finish kernels take one workgroup per RHS, the vector
-// kernels one grid row (blockIdx.y) per RHS.  With nrhs = 1 they are the single-RHS kernels.
+// kernels one grid row (blockIdx.y) per RHS.  With nrhs = 1 they are the single-RHS kernels.  A
+// side whose state carries its own support size (st->nb > 0: sides on different media) uses it
+// for its vector length and support means instead of the launch's.
 __global__ void k_cr_beta(const double* __restrict__ xpart, int nxparts, long xstride,
                           const double* __restrict__ zpart, int nzparts, long zstride, CrState* st, double inv_nb,
                           double inv_n) {
@@ -531,6 +533,7 @@ __global__ void k_cr_beta(const double* __restrict__ xpart, int nxparts, long xs
     xpart += blockIdx.x * xstride;
     zpart += blockIdx.x * zstride;
     if (st->done) return;
+    if (st->nb > 0) inv_nb = 1.0 / (double)st->nb;
     double ax[3] = {0, 0, 0}, az[1] = {0};
     sum_rows<3, 4>(xpart, kXinvParts, nxparts, ax);
     sum_rows<1, 4>(zpart, 1, nzparts, az);
@@ -683,6 +686,7 @@ __global__ void k_cr_check(const double* __restrict__ partials, long pstride, Cr
                            double* __restrict__ history, long hstride, double inv_nb) {
     st += blockIdx.x;
     if (st->done) return;
+    if (st->nb > 0) inv_nb = 1.0 / (double)st->nb;
     cr_check_finish(partials + blockIdx.x * pstride, kRedBlocks, st,
                     history != nullptr ? history + blockIdx.x * hstride : nullptr, inv_nb);
 }
@@ -715,6 +719,7 @@ __global__ __launch_bounds__(kVecThreads) void k_cr32_direction(float* __restric
     st += blockIdx.y;
     partials += blockIdx.y * pstride;
     if (st->done) return;
+    if (st->nb > 0) n = 3 * st->nb;
     const double b = st->beta;
     const double m3[3] = {st->mean_out[0], st->mean_out[1], st->mean_out[2]};
     const bool fresh = (b == 0.0);  // first iteration: set p and Ap (see k_cr_direction)
@@ -775,6 +780,7 @@ __global__ __launch_bounds__(kVecThreads) void k_cr32_update(float* __restrict__
     st += blockIdx.y;
     partials += blockIdx.y * pstride;
     if (st->done) return;
+    if (st->nb > 0) n = 3 * st->nb;
     const double a = st->alpha;
     double rr = 0.0, sum[3] = {0.0, 0.0, 0.0};
     const long n4 = n / 4;

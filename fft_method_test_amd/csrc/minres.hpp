@@ -100,6 +100,7 @@ struct CrState {
     double rel_res;      // ||r|| / ||b|| of the recursive residual
     long long iterations;
     long long maxit;
+    long long nb;        // support voxels of this side when the sides of a launch differ (0: the launch's)
     int done;            // every CR kernel returns at once while set
     int pause;           // 1: the host owes a reliable update / convergence check
     int breakdown;       // r.A r <= 0 or Ap.Ap <= 0

@@ -212,6 +212,22 @@ int fftperm_solve_tensor(fftperm_ctx* ctx, double tol, uint64_t maxit, double K[
  * precision modes and solvers ignore it.  mode outside 0 .. 2 -> FFTPERM_EINVAL. */
 int fftperm_set_tensor_batch(fftperm_ctx* ctx, int mode);
 
+/* Batched solves of many media of one grid shape (no counterpart in the reference, which solves
+ * one medium per run; the building block of the uneven-size batch: one call per size class).  The
+ * count media (fluid_masks: count host pointers to N bytes each) are solved for one load direction
+ * grad together: every Krylov iteration advances all of them, the y and z FFT passes of an iteration
+ * are one launch for all media (the fused z pass reads the Green tensor once per column tile for
+ * all of them), the vector and scalar kernels one launch each, and the iterations replay captured
+ * hipGraphs.  Each medium keeps its own support map, compact vectors, CR state, reliable FP64
+ * updates and stopping test (FFTPERM_MIXED_CR_FP32_VECTORS on a MIXED context, which is required;
+ * FFTPERM_EINVAL otherwise), so its result is the one fftperm_solve_direction gives for it.
+ * k_out (count x 3) receives each medium's column +mu <v> (K column of grad), info (count, may be
+ * NULL) each solve.  A medium with an empty support gets k = 0 and converged = 0.  The context is
+ * left without a medium.  count in 1 .. FFTPERM_BATCH_MAX. */
+#define FFTPERM_BATCH_MAX 64
+int fftperm_solve_batch(fftperm_ctx* ctx, uint32_t count, const uint8_t* const* fluid_masks, int support_mode,
+                        const double grad[3], double tol, uint64_t maxit, double* k_out, fftperm_solve_info* info);
+
 /* Diagnostics::residual_norm / relative_symmetry_gap (src/post/Diagnostics.cpp:26-64), FP64. */
 int fftperm_residual_norm(fftperm_ctx* ctx, const double* x, const double* b, double* out);
 int fftperm_symmetry_gap(fftperm_ctx* ctx, const double* x, const double* y, double* out);

@@ -202,14 +202,13 @@ def test_c4_batch_subset_against_oracle_fixtures(fp):
     recs = fx["samples"]
     if len({r["n"] for r in recs}) < 5 or len({r["kind"] for r in recs}) < 2:
         pytest.skip("c4 fixtures incomplete")
-    samples = batch.c4_batch()
     chosen = []
     for r in recs:
-        n, mask, kind, phi = samples[r["index"]]
+        n, mask, kind, phi = batch.c4_sample(r["index"])
         assert n == r["n"] and kind == r["kind"]
         assert hashlib.sha256(mask.tobytes()).hexdigest() == r["mask_sha256"], "generator drifted from the fixture"
         chosen.append((n, mask, kind, phi))
-    res, sec = batch.solve_batch(chosen, tol=1e-8, precision=_lib.PRECISION_MIXED, workers=2)
+    res, sec = batch.solve_batch(chosen, tol=1e-8, precision=_lib.PRECISION_MIXED, workers=2, mode="batched")
     worst = 0.0
     for r, out in zip(recs, res):
         err = abs(out["k"][0] - r["Kcol_x"][0]) / abs(r["Kcol_x"][0])
