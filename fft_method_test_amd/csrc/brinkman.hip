@@ -8,8 +8,9 @@
 // the residual stay FP64.
 //
 // Layout of the work: the first sweep (from phi = 0) is fused into the divergence kernel, the
-// other 34 run two at a time on LDS-staged bricks (phi with a two-voxel halo, the intermediate
-// sweep with a one-voxel halo), so a pair of sweeps reads phi and div once and writes phi once.
+// other 34 run two at a time in one pass that marches through z with phi, div and the
+// intermediate sweep held in LDS plane rings, so a pair of sweeps reads phi and div once and
+// writes phi once.
 // Every sweep keeps the reference's per-voxel arithmetic, (sum of the six neighbours in the order
 // -x, +x, -y, +y, -z, +z  minus div) / 6, so the result is independent of the blocking.  The host
 // does not wait for each iteration: a device-side check after each projection records the
@@ -100,15 +101,18 @@ __global__ void k_divergence(const double* __restrict__ sx, const double* __rest
     block_sum<1>(acc, partials + blockIdx.x);
 }
 
-// Two Jacobi sweeps (:402-433) on a brick of BX x BY x BZ voxels: phi (two-voxel halo) and div
-// (one-voxel halo) are staged in LDS, the first sweep fills the one-voxel-halo region, the second
-// the brick.  Every LDS slot holds the periodic value of its wrapped voxel, so bricks that stick
-// out of small grids stay exact (their outside voxels are not written).
-template <int BX, int BY, int BZ>
-struct JBrick {
-    static constexpr int AX = BX + 4, AY = BY + 4, AZ = BZ + 4;  // phi with the two-voxel halo
-    static constexpr int CX = BX + 2, CY = BY + 2, CZ = BZ + 2;  // first sweep and div
-    static constexpr int NA = AX * AY * AZ, NC = CX * CY * CZ, NB = BX * BY * BZ;
+// Two Jacobi sweeps (:402-433) in one pass.  A workgroup owns a TX x TY column of the x-y plane
+// and ZC consecutive planes, and marches through z keeping three planes of phi (with a two-voxel
+// x-y halo), three of div and three of the intermediate sweep (one-voxel halo) in LDS rings: at
+// step q it loads phi plane q + 1 and div plane q, computes the first sweep on plane q and the
+// second on plane q - 1, which it writes.  Every LDS slot holds the periodic value of its wrapped
+// voxel, so columns that stick out of small grids stay exact (their outside voxels are not
+// written).
+template <int TX, int TY>
+struct JCol {
+    static constexpr int AX = TX + 4, AY = TY + 4;  // phi plane with the two-voxel halo
+    static constexpr int CX = TX + 2, CY = TY + 2;  // first sweep and div plane with the one-voxel halo
+    static constexpr int NA = AX * AY, NC = CX * CY, NT = TX * TY;
 };
 
 __device__ __forceinline__ int wrapm(int v, int n) {
@@ -116,47 +120,101 @@ __device__ __forceinline__ int wrapm(int v, int n) {
     return v < 0 ? v + n : v;
 }
 
-template <typename P, int BX, int BY, int BZ>
+template <typename P, int TX, int TY, int ZC>
 __global__ __launch_bounds__(256) void k_jacobi2(const P* __restrict__ phi, const P* __restrict__ div,
                                                  P* __restrict__ phin, int nx, int ny, int nz, int tiles_x,
                                                  int tiles_y, const int* __restrict__ done) {
     if (*done) return;
-    using J = JBrick<BX, BY, BZ>;
-    __shared__ P s0[J::NA];
-    __shared__ P s1[J::NC];
-    __shared__ P sd[J::NC];
+    using J = JCol<TX, TY>;
+    constexpr int T = 256;
+    constexpr int KA = (J::NA + T - 1) / T, KC = (J::NC + T - 1) / T;
+    __shared__ P f0[3][J::NA];
+    __shared__ P f1[3][J::NC];
+    __shared__ P dv[3][J::NC];
+    __shared__ int xt[J::AX], yt[J::AY];
     const int bx = (int)(blockIdx.x % tiles_x);
     const int by = (int)((blockIdx.x / tiles_x) % tiles_y);
     const int bz = (int)(blockIdx.x / ((unsigned)tiles_x * tiles_y));
-    const int x0 = bx * BX, y0 = by * BY, z0 = bz * BZ;
+    const int x0 = bx * TX, y0 = by * TY, z0 = bz * ZC;
+    const int z1 = min(z0 + ZC, nz);  // planes [z0, z1) are written
     const long plane = (long)nx * ny;
-    for (int t = threadIdx.x; t < J::NA; t += blockDim.x) {
-        const int lx = t % J::AX, ly = (t / J::AX) % J::AY, lz = t / (J::AX * J::AY);
-        const int gx = wrapm(x0 - 2 + lx, nx), gy = wrapm(y0 - 2 + ly, ny), gz = wrapm(z0 - 2 + lz, nz);
-        s0[t] = phi[(long)gz * plane + (long)gy * nx + gx];
-    }
-    for (int t = threadIdx.x; t < J::NC; t += blockDim.x) {
-        const int lx = t % J::CX, ly = (t / J::CX) % J::CY, lz = t / (J::CX * J::CY);
-        const int gx = wrapm(x0 - 1 + lx, nx), gy = wrapm(y0 - 1 + ly, ny), gz = wrapm(z0 - 1 + lz, nz);
-        sd[t] = div[(long)gz * plane + (long)gy * nx + gx];
-    }
+    for (int t = threadIdx.x; t < J::AX; t += T) xt[t] = wrapm(x0 - 2 + t, nx);
+    for (int t = threadIdx.x; t < J::AY; t += T) yt[t] = wrapm(y0 - 2 + t, ny) * nx;
     __syncthreads();
-    for (int t = threadIdx.x; t < J::NC; t += blockDim.x) {
-        const int lx = t % J::CX, ly = (t / J::CX) % J::CY, lz = t / (J::CX * J::CY);
-        const int a = ((lz + 1) * J::AY + (ly + 1)) * J::AX + (lx + 1);
-        const P nb = s0[a - 1] + s0[a + 1] + s0[a - J::AX] + s0[a + J::AX] + s0[a - J::AX * J::AY] +
-                     s0[a + J::AX * J::AY];
-        s1[t] = (nb - sd[t]) / (P)6.0;
-    }
-    __syncthreads();
-    for (int t = threadIdx.x; t < J::NB; t += blockDim.x) {
-        const int lx = t % BX, ly = (t / BX) % BY, lz = t / (BX * BY);
-        const int gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
-        if (gx >= nx || gy >= ny || gz >= nz) continue;
-        const int c = ((lz + 1) * J::CY + (ly + 1)) * J::CX + (lx + 1);
-        const P nb = s1[c - 1] + s1[c + 1] + s1[c - J::CX] + s1[c + J::CX] + s1[c - J::CX * J::CY] +
-                     s1[c + J::CX * J::CY];
-        phin[(long)gz * plane + (long)gy * nx + gx] = (nb - sd[c]) / (P)6.0;
+    // ring slot of plane q (q >= z0 - 2)
+    auto slot = [&](int q) { return (q - z0 + 3) % 3; };
+    // each thread's share of a phi plane (two-voxel halo) and of a div plane (one-voxel halo),
+    // fetched into registers one step ahead so that the loads overlap the sweeps of the step
+    P ra[KA], rc[KC];
+    auto fetch = [&](int qa, int qc) {
+        const P* sa = phi + (long)wrapm(qa, nz) * plane;
+        const P* sc = div + (long)wrapm(qc, nz) * plane;
+#pragma unroll
+        for (int k = 0; k < KA; ++k) {
+            const int t = threadIdx.x + k * T;
+            if (t < J::NA) ra[k] = sa[yt[t / J::AX] + xt[t % J::AX]];
+        }
+#pragma unroll
+        for (int k = 0; k < KC; ++k) {
+            const int t = threadIdx.x + k * T;
+            if (t < J::NC) rc[k] = sc[yt[t / J::CX + 1] + xt[t % J::CX + 1]];
+        }
+    };
+    auto store = [&](int qa, int qc) {
+        P* da = f0[slot(qa)];
+        P* dc = dv[slot(qc)];
+#pragma unroll
+        for (int k = 0; k < KA; ++k) {
+            const int t = threadIdx.x + k * T;
+            if (t < J::NA) da[t] = ra[k];
+        }
+#pragma unroll
+        for (int k = 0; k < KC; ++k) {
+            const int t = threadIdx.x + k * T;
+            if (t < J::NC) dc[t] = rc[k];
+        }
+    };
+    // phi planes z0 - 2 and z0 - 1 (the div plane staged with the first is not used)
+    fetch(z0 - 2, z0 - 2);
+    store(z0 - 2, z0 - 2);
+    fetch(z0 - 1, z0 - 2);
+    store(z0 - 1, z0 - 2);
+    fetch(z0, z0 - 1);
+    for (int q = z0 - 1; q <= z1; ++q) {
+        store(q + 1, q);  // phi plane q + 1, div plane q
+        __syncthreads();
+        if (q < z1) fetch(q + 2, q + 1);
+        {
+            // first sweep on plane q, over the one-voxel-halo region
+            const P* pm = f0[slot(q - 1)];
+            const P* pc = f0[slot(q)];
+            const P* pp = f0[slot(q + 1)];
+            const P* d = dv[slot(q)];
+            P* o = f1[slot(q)];
+            for (int t = threadIdx.x; t < J::NC; t += T) {
+                const int lx = t % J::CX, ly = t / J::CX;
+                const int a = (ly + 1) * J::AX + (lx + 1);
+                const P nb = pc[a - 1] + pc[a + 1] + pc[a - J::AX] + pc[a + J::AX] + pm[a] + pp[a];
+                o[t] = (nb - d[t]) / (P)6.0;
+            }
+        }
+        __syncthreads();
+        const int zo = q - 1;  // second sweep on plane q - 1
+        if (zo >= z0) {
+            const P* pm = f1[slot(zo - 1)];
+            const P* pc = f1[slot(zo)];
+            const P* pp = f1[slot(zo + 1)];
+            const P* d = dv[slot(zo)];
+            P* out = phin + (long)zo * plane;
+            for (int t = threadIdx.x; t < J::NT; t += T) {
+                const int lx = t % TX, ly = t / TX;
+                const int gx = x0 + lx, gy = y0 + ly;
+                if (gx >= nx || gy >= ny) continue;
+                const int c = (ly + 1) * J::CX + (lx + 1);
+                const P nb = pc[c - 1] + pc[c + 1] + pc[c - J::CX] + pc[c + J::CX] + pm[c] + pp[c];
+                out[(long)gy * nx + gx] = (nb - d[c]) / (P)6.0;
+            }
+        }
     }
 }
 
@@ -248,7 +306,7 @@ struct DevBuf {
     }
 };
 
-constexpr int kJBX = 32, kJBY = 8, kJBZ = 4;  // brick of the fused sweep pairs
+constexpr int kJTX = 32, kJTY = 8, kJZC = 16;  // column of the fused sweep pairs: x-y tile, planes
 constexpr int kChunk = 8;                     // iterations per captured graph / host check
 
 template <typename P>
@@ -293,7 +351,7 @@ int run(int nx, int ny, int nz, const double* alpha_h, const double* visc_h, dou
         if ((err = hipMemsetAsync(a, 0, N * sizeof(double), s)) != hipSuccess) return fail(err);
     if ((err = hipMemsetAsync(bst, 0, sizeof(BState), s)) != hipSuccess) return fail(err);
     const unsigned grid = (unsigned)std::min<long>((N + kBThreads - 1) / kBThreads, 65536);
-    const int tiles_x = (nx + kJBX - 1) / kJBX, tiles_y = (ny + kJBY - 1) / kJBY, tiles_z = (nz + kJBZ - 1) / kJBZ;
+    const int tiles_x = (nx + kJTX - 1) / kJTX, tiles_y = (ny + kJTY - 1) / kJTY, tiles_z = (nz + kJZC - 1) / kJZC;
     const unsigned jgrid = (unsigned)((long)tiles_x * tiles_y * tiles_z);
     const int pressureIterationsPerStep = 35;  // the first one fused into the divergence kernel
     const int pairs = (pressureIterationsPerStep - 1) / 2, single = (pressureIterationsPerStep - 1) % 2;
@@ -310,7 +368,7 @@ int run(int nx, int ny, int nz, const double* alpha_h, const double* visc_h, dou
         P* a = phi;
         P* b = phin;
         for (int k = 0; k < pairs; ++k) {
-            hipLaunchKernelGGL((k_jacobi2<P, kJBX, kJBY, kJBZ>), dim3(jgrid), dim3(256), 0, s, a, divsrc, b, nx, ny, nz,
+            hipLaunchKernelGGL((k_jacobi2<P, kJTX, kJTY, kJZC>), dim3(jgrid), dim3(256), 0, s, a, divsrc, b, nx, ny, nz,
                                tiles_x, tiles_y, done);
             std::swap(a, b);
         }
