@@ -33,25 +33,46 @@ struct Nbr {
     long id, xm, xp, ym, yp, zm, zp;
 };
 
-__device__ __forceinline__ Nbr neighbours(long i, int nx, int ny, int nz) {
-    const int x = (int)(i % nx);
-    const long r = i / nx;
-    const int y = (int)(r % ny);
-    const int z = (int)(r / ny);
-    const int xm = x == 0 ? nx - 1 : x - 1, xp = x + 1 == nx ? 0 : x + 1;
-    const int ym = y == 0 ? ny - 1 : y - 1, yp = y + 1 == ny ? 0 : y + 1;
-    const int zm = z == 0 ? nz - 1 : z - 1, zp = z + 1 == nz ? 0 : z + 1;
-    const long pl = (long)nx * ny;
-    const long row = (long)y * nx, pln = (long)z * pl;
-    Nbr n;
-    n.id = i;
-    n.xm = pln + row + xm;
-    n.xp = pln + row + xp;
-    n.ym = pln + (long)ym * nx + x;
-    n.yp = pln + (long)yp * nx + x;
-    n.zm = (long)zm * pl + row + x;
-    n.zp = (long)zp * pl + row + x;
-    return n;
+// Workgroups are dealt to the 8 XCDs round-robin and each XCD has its own L2: remap the block
+// index so that each XCD takes one contiguous range of the work (consecutive rows or tiles), whose
+// neighbour rows and halo planes then stay in its own L2.
+__device__ __forceinline__ long xcd_block(long b, long nblocks) {
+    constexpr int kXcd = 8;
+    const long per = nblocks / kXcd;
+    if (per == 0 || b >= per * kXcd) return b;
+    return (b % kXcd) * per + b / kXcd;
+}
+
+// Walks the voxels row by row: workgroup b (XCD-remapped) takes a contiguous range of x rows
+// (y, z) (the row coordinates are uniform across the workgroup), its threads the x positions of a
+// row; the six periodic neighbours (:28-33) come from the row coordinates, with no per-voxel
+// division.
+template <typename F>
+__device__ __forceinline__ void for_each_voxel(int nx, int ny, int nz, F&& f) {
+    const long rows = (long)ny * nz;
+    const long per = (rows + gridDim.x - 1) / gridDim.x;
+    const long r0 = xcd_block(blockIdx.x, gridDim.x) * per;
+    const long r1 = r0 + per < rows ? r0 + per : rows;
+    for (long r = r0; r < r1; ++r) {
+        const int y = (int)(r % ny), z = (int)(r / ny);
+        const int ym = y == 0 ? ny - 1 : y - 1, yp = y + 1 == ny ? 0 : y + 1;
+        const int zm = z == 0 ? nz - 1 : z - 1, zp = z + 1 == nz ? 0 : z + 1;
+        const long base = r * nx;
+        const long oym = ((long)z * ny + ym) * nx, oyp = ((long)z * ny + yp) * nx;
+        const long ozm = ((long)zm * ny + y) * nx, ozp = ((long)zp * ny + y) * nx;
+        for (int x = threadIdx.x; x < nx; x += blockDim.x) {
+            const int xm = x == 0 ? nx - 1 : x - 1, xp = x + 1 == nx ? 0 : x + 1;
+            Nbr n;
+            n.id = base + x;
+            n.xm = base + xm;
+            n.xp = base + xp;
+            n.ym = oym + x;
+            n.yp = oyp + x;
+            n.zm = ozm + x;
+            n.zp = ozp + x;
+            f(n);
+        }
+    }
 }
 
 // :313-369
@@ -59,10 +80,10 @@ __global__ void k_momentum(const double* __restrict__ ux, const double* __restri
                            const double* __restrict__ uz, const double* __restrict__ p,
                            const double* __restrict__ alpha, const double* __restrict__ visc, double* __restrict__ sx,
                            double* __restrict__ sy, double* __restrict__ sz, double f0, double f1, double f2,
-                           double relax, int nx, int ny, int nz, long N, const int* __restrict__ done) {
+                           double relax, int nx, int ny, int nz, const int* __restrict__ done) {
     if (*done) return;
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
-        const Nbr n = neighbours(i, nx, ny, nz);
+    for_each_voxel(nx, ny, nz, [&](const Nbr& n) {
+        const long i = n.id;
         const double mu = visc[i];
         const double al = alpha[i];
         const double lapUx = ux[n.xm] + ux[n.xp] + ux[n.ym] + ux[n.yp] + ux[n.zm] + ux[n.zp];
@@ -78,26 +99,26 @@ __global__ void k_momentum(const double* __restrict__ ux, const double* __restri
         sx[i] = relax * jx + (1.0 - relax) * ux[i];
         sy[i] = relax * jy + (1.0 - relax) * uy[i];
         sz[i] = relax * jz + (1.0 - relax) * uz[i];
-    }
+    });
 }
 
 // :371-400, plus the first pressure sweep (:402-433 from phi = 0): phi1 = (0 - div) / 6
 template <typename P>
 __global__ void k_divergence(const double* __restrict__ sx, const double* __restrict__ sy,
                              const double* __restrict__ sz, double* __restrict__ div, P* __restrict__ divp,
-                             P* __restrict__ phi1, int nx, int ny, int nz, long N, double* __restrict__ partials,
+                             P* __restrict__ phi1, int nx, int ny, int nz, double* __restrict__ partials,
                              const int* __restrict__ done) {
     if (*done) return;
     double acc[1] = {0.0};
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
-        const Nbr n = neighbours(i, nx, ny, nz);
+    for_each_voxel(nx, ny, nz, [&](const Nbr& n) {
+        const long i = n.id;
         const double d = 0.5 * (sx[n.xp] - sx[n.xm]) + 0.5 * (sy[n.yp] - sy[n.ym]) + 0.5 * (sz[n.zp] - sz[n.zm]);
         div[i] = d;
         const P dp = (P)d;
         if (divp != nullptr) divp[i] = dp;
         phi1[i] = ((P)0 - dp) / (P)6.0;
         acc[0] += d * d;
-    }
+    });
     block_sum<1>(acc, partials + blockIdx.x);
 }
 
@@ -132,9 +153,10 @@ __global__ __launch_bounds__(256) void k_jacobi2(const P* __restrict__ phi, cons
     __shared__ P f1[3][J::NC];
     __shared__ P dv[3][J::NC];
     __shared__ int xt[J::AX], yt[J::AY];
-    const int bx = (int)(blockIdx.x % tiles_x);
-    const int by = (int)((blockIdx.x / tiles_x) % tiles_y);
-    const int bz = (int)(blockIdx.x / ((unsigned)tiles_x * tiles_y));
+    const long blk = xcd_block(blockIdx.x, gridDim.x);
+    const int bx = (int)(blk % tiles_x);
+    const int by = (int)((blk / tiles_x) % tiles_y);
+    const int bz = (int)(blk / ((long)tiles_x * tiles_y));
     const int x0 = bx * TX, y0 = by * TY, z0 = bz * ZC;
     const int z1 = min(z0 + ZC, nz);  // planes [z0, z1) are written
     const long plane = (long)nx * ny;
@@ -221,13 +243,12 @@ __global__ __launch_bounds__(256) void k_jacobi2(const P* __restrict__ phi, cons
 // One Jacobi sweep (:402-433), for an odd count of sweeps after the fused first one.
 template <typename P>
 __global__ void k_jacobi(const P* __restrict__ phi, const P* __restrict__ div, P* __restrict__ phin, int nx, int ny,
-                         int nz, long N, const int* __restrict__ done) {
+                         int nz, const int* __restrict__ done) {
     if (*done) return;
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
-        const Nbr n = neighbours(i, nx, ny, nz);
+    for_each_voxel(nx, ny, nz, [&](const Nbr& n) {
         const P nb = phi[n.xm] + phi[n.xp] + phi[n.ym] + phi[n.yp] + phi[n.zm] + phi[n.zp];
-        phin[i] = (nb - div[i]) / (P)6.0;
-    }
+        phin[n.id] = (nb - div[n.id]) / (P)6.0;
+    });
 }
 
 // :435-466
@@ -235,10 +256,10 @@ template <typename P>
 __global__ void k_project(const double* __restrict__ sx, const double* __restrict__ sy,
                           const double* __restrict__ sz, const P* __restrict__ phi, double* __restrict__ ux,
                           double* __restrict__ uy, double* __restrict__ uz, double* __restrict__ p, int nx, int ny,
-                          int nz, long N, const int* __restrict__ done) {
+                          int nz, const int* __restrict__ done) {
     if (*done) return;
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
-        const Nbr n = neighbours(i, nx, ny, nz);
+    for_each_voxel(nx, ny, nz, [&](const Nbr& n) {
+        const long i = n.id;
         const double dx = 0.5 * ((double)phi[n.xp] - (double)phi[n.xm]);
         const double dy = 0.5 * ((double)phi[n.yp] - (double)phi[n.ym]);
         const double dz = 0.5 * ((double)phi[n.zp] - (double)phi[n.zm]);
@@ -246,7 +267,7 @@ __global__ void k_project(const double* __restrict__ sx, const double* __restric
         uy[i] = sy[i] - dy;
         uz[i] = sz[i] - dz;
         p[i] += (double)phi[i];
-    }
+    });
 }
 
 // Device-side stop rule of one iteration (:468-499): after the projection, the residual
@@ -350,7 +371,7 @@ int run(int nx, int ny, int nz, const double* alpha_h, const double* visc_h, dou
     for (double* a : {ux, uy, uz, p})
         if ((err = hipMemsetAsync(a, 0, N * sizeof(double), s)) != hipSuccess) return fail(err);
     if ((err = hipMemsetAsync(bst, 0, sizeof(BState), s)) != hipSuccess) return fail(err);
-    const unsigned grid = (unsigned)std::min<long>((N + kBThreads - 1) / kBThreads, 65536);
+    const unsigned grid = (unsigned)std::min<long>((long)ny * nz, 65536);  // row-walking kernels
     const int tiles_x = (nx + kJTX - 1) / kJTX, tiles_y = (ny + kJTY - 1) / kJTY, tiles_z = (nz + kJZC - 1) / kJZC;
     const unsigned jgrid = (unsigned)((long)tiles_x * tiles_y * tiles_z);
     const int pressureIterationsPerStep = 35;  // the first one fused into the divergence kernel
@@ -361,9 +382,9 @@ int run(int nx, int ny, int nz, const double* alpha_h, const double* visc_h, dou
     // chunk replay unchanged
     auto iteration = [&]() {
         hipLaunchKernelGGL(k_momentum, dim3(grid), dim3(kBThreads), 0, s, ux, uy, uz, p, alpha, visc, sx, sy, sz,
-                           force[0], force[1], force[2], relax, nx, ny, nz, N, done);
+                           force[0], force[1], force[2], relax, nx, ny, nz, done);
         hipLaunchKernelGGL(k_divergence<P>, dim3(kBRedBlocks), dim3(kBThreads), 0, s, sx, sy, sz, div,
-                           mixed ? divp : (P*)nullptr, phi, nx, ny, nz, N, partials, done);
+                           mixed ? divp : (P*)nullptr, phi, nx, ny, nz, partials, done);
         hipLaunchKernelGGL(k_bfinish, dim3(1), dim3(kBThreads), 0, s, partials, kBRedBlocks, 1, scal, done);
         P* a = phi;
         P* b = phin;
@@ -373,11 +394,11 @@ int run(int nx, int ny, int nz, const double* alpha_h, const double* visc_h, dou
             std::swap(a, b);
         }
         if (single) {
-            hipLaunchKernelGGL(k_jacobi<P>, dim3(grid), dim3(kBThreads), 0, s, a, divsrc, b, nx, ny, nz, N, done);
+            hipLaunchKernelGGL(k_jacobi<P>, dim3(grid), dim3(kBThreads), 0, s, a, divsrc, b, nx, ny, nz, done);
             std::swap(a, b);
         }
         hipLaunchKernelGGL(k_project<P>, dim3(grid), dim3(kBThreads), 0, s, sx, sy, sz, a, ux, uy, uz, p, nx, ny,
-                           nz, N, done);
+                           nz, done);
         hipLaunchKernelGGL(k_bcheck, dim3(1), dim3(64), 0, s, scal, bst, tol, 1.0 / (double)N);
     };
     BState hs{0, 0, 0.0};
