@@ -13,9 +13,8 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libfftperm.so")
 ARCH = os.environ.get("FFTPERM_ARCH", "gfx950")
 
-SOURCES = ["api.hip", "kernels_fft.hip", "passes_x.hip", "passes_col.hip", "kernels_setup.hip", "kernels_krylov.hip",
-           "unet.hip",
-           "brinkman.hip", "noalloc.hip"]
+SOURCES = ["kernels_fft.hip", "passes_x.hip", "passes_x_mixed.hip", "zgreen_mixed.hip", "api.hip", "passes_col.hip",
+           "kernels_setup.hip", "kernels_krylov.hip", "unet.hip", "brinkman.hip", "noalloc.hip"]
 
 HIPCC_FLAGS = [
     "-O3",

@@ -128,6 +128,150 @@ __device__ __forceinline__ void dft_pq(cx<R>* v) {
     }
 }
 
+// cos and sin of 2 pi e / D for the odd and mixed radices (D = 3, 5, 6, 7, 9, 10; e <= D / 2; the
+// others follow from cos(2 pi (D - e) / D) = cos(2 pi e / D), sin(2 pi (D - e) / D) = -sin(...)).
+// Decimal expansions to 24 places, rounded once to the working precision.
+__host__ __device__ constexpr double root_cos(int d, int e) {
+    switch (d * 16 + e) {
+        case 3 * 16 + 1: return -0.5;
+        case 5 * 16 + 1: return 0.309016994374947424102293;
+        case 5 * 16 + 2: return -0.809016994374947424102293;
+        case 6 * 16 + 1: return 0.5;
+        case 6 * 16 + 2: return -0.5;
+        case 7 * 16 + 1: return 0.623489801858733530525005;
+        case 7 * 16 + 2: return -0.222520933956314404288903;
+        case 7 * 16 + 3: return -0.900968867902419126236102;
+        case 9 * 16 + 1: return 0.766044443118978035202393;
+        case 9 * 16 + 2: return 0.173648177666930348851717;
+        case 9 * 16 + 3: return -0.5;
+        case 9 * 16 + 4: return -0.939692620785908384054109;
+        case 10 * 16 + 1: return 0.809016994374947424102293;
+        case 10 * 16 + 2: return 0.309016994374947424102293;
+        case 10 * 16 + 3: return -0.309016994374947424102293;
+        case 10 * 16 + 4: return -0.809016994374947424102293;
+        case 3 * 16: case 5 * 16: case 6 * 16: case 7 * 16: case 9 * 16: case 10 * 16: return 1.0;
+        default: return __builtin_nan("");  // not tabulated
+    }
+}
+__host__ __device__ constexpr double root_sin(int d, int e) {
+    switch (d * 16 + e) {
+        case 3 * 16 + 1: return 0.866025403784438646763723;
+        case 5 * 16 + 1: return 0.951056516295153572116439;
+        case 5 * 16 + 2: return 0.587785252292473129168706;
+        case 6 * 16 + 1: return 0.866025403784438646763723;
+        case 6 * 16 + 2: return 0.866025403784438646763723;
+        case 7 * 16 + 1: return 0.781831482468029808708445;
+        case 7 * 16 + 2: return 0.974927912181823607018132;
+        case 7 * 16 + 3: return 0.433883739117558120475768;
+        case 9 * 16 + 1: return 0.642787609686539326322643;
+        case 9 * 16 + 2: return 0.984807753012208059366743;
+        case 9 * 16 + 3: return 0.866025403784438646763723;
+        case 9 * 16 + 4: return 0.342020143325668733044100;
+        case 10 * 16 + 1: return 0.587785252292473129168706;
+        case 10 * 16 + 2: return 0.951056516295153572116439;
+        case 10 * 16 + 3: return 0.951056516295153572116439;
+        case 10 * 16 + 4: return 0.587785252292473129168706;
+        case 3 * 16: case 5 * 16: case 6 * 16: case 7 * 16: case 9 * 16: case 10 * 16: return 0.0;
+        default: return __builtin_nan("");  // not tabulated
+    }
+}
+// cos / sin of 2 pi e / D for any e (reduced to 0 <= e <= D / 2 by the symmetries above)
+__host__ __device__ constexpr double root_cos_any(int d, int e) {
+    e %= d;
+    return 2 * e == d ? -1.0 : (2 * e < d ? root_cos(d, e) : root_cos(d, d - e));
+}
+__host__ __device__ constexpr double root_sin_any(int d, int e) {
+    e %= d;
+    return 2 * e == d ? 0.0 : (2 * e < d ? root_sin(d, e) : -root_sin(d, d - e));
+}
+
+// v * W_D^E with W_D = exp(SIGN * 2 pi i / D); exact for E = 0 and 2 E = D.
+template <typename R, int SIGN, int D, int E>
+__device__ __forceinline__ cx<R> twd(cx<R> v) {
+    constexpr int e = E % D;
+    if constexpr (e == 0) {
+        return v;
+    } else if constexpr (2 * e == D) {
+        return mkc<R>(-v.x, -v.y);
+    } else {
+        constexpr double c = root_cos_any(D, e);
+        constexpr double s = SIGN < 0 ? -root_sin_any(D, e) : root_sin_any(D, e);
+        static_assert(c == c && s == s, "root of unity not tabulated");
+        return cmul<R>(v, mkc<R>((R)c, (R)s));
+    }
+}
+
+// Odd prime radix P (3, 5, 7) with the pairs a_m = v_m + v_{P-m}, b_m = v_m - v_{P-m}:
+// X_k = v_0 + sum_m cos(2 pi m k / P) a_m + i SIGN sum_m sin(2 pi m k / P) b_m and X_{P-k} the same
+// with the sine sum negated.  Products fused in a fixed order (fmar), contraction off otherwise.
+template <typename R, int SIGN, int P>
+__device__ __forceinline__ void dft_prime(cx<R>* v) {
+#pragma clang fp contract(off)
+    constexpr int H = (P - 1) / 2;
+    cx<R> a[H + 1], b[H + 1];
+#pragma unroll
+    for (int m = 1; m <= H; ++m) {
+        a[m] = cadd<R>(v[m], v[P - m]);
+        b[m] = csub<R>(v[m], v[P - m]);
+    }
+    cx<R> x0 = v[0];
+#pragma unroll
+    for (int m = 1; m <= H; ++m) x0 = cadd<R>(x0, a[m]);
+    cx<R> o[P];
+    o[0] = x0;
+#pragma unroll
+    for (int k = 1; k <= H; ++k) {
+        cx<R> c = v[0];
+        cx<R> s = mkc<R>((R)0, (R)0);
+#pragma unroll
+        for (int m = 1; m <= H; ++m) {
+            const R cw = (R)root_cos_any(P, m * k);
+            const R sw = (R)root_sin_any(P, m * k);
+            c = mkc<R>(fmar<R>(cw, a[m].x, c.x), fmar<R>(cw, a[m].y, c.y));
+            s = (m == 1) ? mkc<R>(sw * b[m].x, sw * b[m].y)
+                         : mkc<R>(fmar<R>(sw, b[m].x, s.x), fmar<R>(sw, b[m].y, s.y));
+        }
+        // i SIGN s
+        const cx<R> is = SIGN < 0 ? mkc<R>(s.y, -s.x) : mkc<R>(-s.y, s.x);
+        o[k] = cadd<R>(c, is);
+        o[P - k] = csub<R>(c, is);
+    }
+#pragma unroll
+    for (int k = 0; k < P; ++k) v[k] = o[k];
+}
+
+// Mixed radix P * Q (6 = 3 x 2, 9 = 3 x 3, 10 = 5 x 2) by the same split as dft_pq, with the
+// inner twiddles W_{PQ}^{n2 k1} from the root tables.
+template <typename R, int SIGN, int P, int Q>
+__device__ __forceinline__ void dft_pq_mixed(cx<R>* v) {
+    cx<R> y[P * Q];
+#pragma unroll
+    for (int n2 = 0; n2 < Q; ++n2) {
+        cx<R> t[P];
+#pragma unroll
+        for (int n1 = 0; n1 < P; ++n1) t[n1] = v[Q * n1 + n2];
+        dft_small<R, SIGN, P>(t);
+#pragma unroll
+        for (int k1 = 0; k1 < P; ++k1) y[n2 * P + k1] = t[k1];
+    }
+#define FFTPERM_TWM(N2, K1)                                                                  \
+    if constexpr ((N2) < Q && (K1) < P)                                                      \
+        y[(N2) * P + (K1)] = twd<R, SIGN, P * Q, (N2) * (K1)>(y[(N2) * P + (K1)]);
+    FFTPERM_TWM(1, 1) FFTPERM_TWM(1, 2) FFTPERM_TWM(1, 3) FFTPERM_TWM(1, 4)
+    FFTPERM_TWM(2, 1) FFTPERM_TWM(2, 2) FFTPERM_TWM(2, 3) FFTPERM_TWM(2, 4)
+#undef FFTPERM_TWM
+    static_assert(Q <= 3 && P <= 5, "inner twiddles enumerated for n2 < 3, k1 < 5");
+#pragma unroll
+    for (int k1 = 0; k1 < P; ++k1) {
+        cx<R> t[Q];
+#pragma unroll
+        for (int n2 = 0; n2 < Q; ++n2) t[n2] = y[n2 * P + k1];
+        dft_small<R, SIGN, Q>(t);
+#pragma unroll
+        for (int k2 = 0; k2 < Q; ++k2) v[k1 + P * k2] = t[k2];
+    }
+}
+
 template <typename R, int SIGN, int RAD>
 __device__ __forceinline__ void dft_small(cx<R>* v) {
     if constexpr (RAD == 1) {
@@ -139,6 +283,16 @@ __device__ __forceinline__ void dft_small(cx<R>* v) {
         dft_pq<R, SIGN, 4, 2>(v);
     } else if constexpr (RAD == 16) {
         dft_pq<R, SIGN, 4, 4>(v);
+    } else if constexpr (RAD == 3 || RAD == 5 || RAD == 7) {
+        dft_prime<R, SIGN, RAD>(v);
+    } else if constexpr (RAD == 6) {
+        dft_pq_mixed<R, SIGN, 3, 2>(v);
+    } else if constexpr (RAD == 9) {
+        dft_pq_mixed<R, SIGN, 3, 3>(v);
+    } else if constexpr (RAD == 10) {
+        dft_pq_mixed<R, SIGN, 5, 2>(v);
+    } else {
+        static_assert(RAD == 1, "radix without an in-register DFT");
     }
 }
 

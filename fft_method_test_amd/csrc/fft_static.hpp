@@ -1,4 +1,5 @@
-// fft_static.hpp -- compile-time specialised line transforms for the hot grid sizes.
+// fft_static.hpp -- compile-time specialised line transforms for the hot grid sizes (the powers of
+// two 32 ... 512 and the mixed-radix edges 80, 96, 112, 144, 160 of the uneven-size batch).
 //
 // Same Stockham algorithm as fft_engine.hpp, with the line length N, the radix sequence,
 // the number of lines per workgroup (LINES) and the workgroup size (THREADS) known at
@@ -25,6 +26,14 @@ template <> struct StaticPlan<64> { using type = Radices<8, 8>; };
 template <> struct StaticPlan<128> { using type = Radices<16, 8>; };
 template <> struct StaticPlan<256> { using type = Radices<16, 16>; };
 template <> struct StaticPlan<512> { using type = Radices<8, 8, 8>; };
+// Mixed-radix lengths (the uneven-size batch): a radix-16 stage, then the odd / mixed radix.
+template <> struct StaticPlan<80> { using type = Radices<16, 5>; };
+template <> struct StaticPlan<96> { using type = Radices<16, 6>; };
+template <> struct StaticPlan<112> { using type = Radices<16, 7>; };
+template <> struct StaticPlan<144> { using type = Radices<16, 9>; };
+template <> struct StaticPlan<160> { using type = Radices<16, 10>; };
+
+constexpr bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
 
 template <typename P>
 struct PlanInfo;
@@ -40,16 +49,31 @@ struct PlanInfo<Radices<Rs...>> {
         for (int r : {Rs...}) m = r > m ? r : m;
         return m;
     }
+    // Threads per line for length n: the smallest stage's butterfly count when every stage's count
+    // is a multiple of it (power-of-two plans: each thread owns M_s / T butterflies of stage s),
+    // else the largest count (mixed plans: one butterfly per thread, idle threads in the stages
+    // with fewer butterflies).
+    static constexpr int threads(int n) {
+        int lo = 1 << 30, hi = 0;
+        for (int r : {Rs...}) {
+            lo = n / r < lo ? n / r : lo;
+            hi = n / r > hi ? n / r : hi;
+        }
+        for (int r : {Rs...})
+            if ((n / r) % lo != 0) return hi;
+        return lo;
+    }
 };
 
-// One stage: each thread owns K = (N/RAD)/T butterflies of one line.  Values are stored in LDS
+// One stage: each thread owns K = ceil((N/RAD)/T) butterflies of one line (the last ones idle when
+// T does not divide the stage's butterfly count).  Values are stored in LDS
 // as cx<R> and computed in registers as cx<RC> with twiddles from a cx<RT> table; RC = double over
 // R = float keeps the FP32 storage but rounds once per stage instead of once per butterfly level.
 template <typename R, int SIGN, int N, int RAD, int NS, int LINES, int T, int LD, typename RC, typename RT>
 __device__ __forceinline__ void sstage(cx<R>* __restrict__ buf, const cx<RT>* __restrict__ tw, int nlines) {
     constexpr int M = N / RAD;
-    static_assert(M % T == 0, "butterflies per line must split evenly over the line's threads");
-    constexpr int K = M / T;
+    constexpr int K = (M + T - 1) / T;
+    constexpr bool EVEN = M % T == 0;
     constexpr int TS = N / (NS * RAD);
     const int ci = threadIdx.x % LINES;
     const int jt = threadIdx.x / LINES;
@@ -60,6 +84,7 @@ __device__ __forceinline__ void sstage(cx<R>* __restrict__ buf, const cx<RT>* __
 #pragma unroll
         for (int q = 0; q < K; ++q) {
             const int j = jt + q * T;
+            if (!EVEN && j >= M) continue;
             const int k = j % NS;
 #pragma unroll
             for (int r = 0; r < RAD; ++r) v[q][r] = cvtc<RC>(L[j + r * M]);
@@ -79,6 +104,7 @@ __device__ __forceinline__ void sstage(cx<R>* __restrict__ buf, const cx<RT>* __
 #pragma unroll
         for (int q = 0; q < K; ++q) {
             const int j = jt + q * T;
+            if (!EVEN && j >= M) continue;
             const int k = j % NS;
             const int base = (j / NS) * NS * RAD + k;
 #pragma unroll
@@ -121,11 +147,12 @@ template <> struct SmallRadixPlan<128> { using type = Radices<8, 4, 4>; };
 template <> struct SmallRadixPlan<256> { using type = Radices<8, 8, 4>; };
 template <> struct SmallRadixPlan<512> { using type = Radices<8, 8, 8>; };
 
-// Threads per line: the butterfly count of the largest-radix stage, so that every stage's
-// butterflies split evenly over them (radices are powers of two here).
+// Threads per line (PlanInfo::threads): for the power-of-two plans the butterfly count of the
+// largest-radix stage, so that every stage's butterflies split evenly over them; 16 for the
+// mixed-radix plans.
 template <int N>
 constexpr int threads_per_line() {
-    return N / PlanInfo<typename StaticPlan<N>::type>::max_radix();
+    return PlanInfo<typename StaticPlan<N>::type>::threads(N);
 }
 
 // Tile variants of the x and column passes.  Variant v in [0, kPassVariants) puts LINES lines in
@@ -152,9 +179,10 @@ struct PassShape {
     static constexpr int LD = N + 1;
     static constexpr int THREADS = OS * LINES * T;
     static constexpr size_t LDS = (size_t)(N + LINES * LD) * sizeof(cx<R>);
-    // every thread owns KE values of each line element set
+    // every thread owns ceil(LINES N / THREADS) elements of the tile (the last ones idle when the
+    // tile does not split evenly: mixed-radix lengths)
     static constexpr bool valid = LINES >= 2 && LINES % 2 == 0 && THREADS >= 64 && THREADS <= 1024 &&
-                                  (LINES * N) % THREADS == 0 && LDS <= 100u * 1024u;
+                                  LDS <= 100u * 1024u;
 };
 
 // Merged x-pass variants v in [kPassVariants, kXPassVariants): the two two-for-one rounds of a tile

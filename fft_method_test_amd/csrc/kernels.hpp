@@ -112,6 +112,8 @@ void set_lds_limit(size_t bytes);
 // compute units and LDS per compute unit of the device (sizes the persistent z-pass grid)
 void set_device_shape(int cus, size_t lds_per_cu);
 size_t lds_limit();
+int device_cus();
+size_t lds_per_cu();
 
 template <typename R>
 void launch_xfwd(const XSource& src, cx<R>* S, const Geometry3& g, const FFTTables<R>& t,
@@ -154,6 +156,29 @@ struct KernelEvents {
     hipEvent_t start = nullptr, stop = nullptr;
 };
 KernelEvents& zpass_events();
+// Right-hand sides of one fused z-pass launch: side b's spectrum starts at b * rhs_stride, its
+// quadratic-form partials at b * qstride, and skip[b * skip_stride] set skips it.
+struct ZBatch {
+    int nrhs;
+    int skip_stride;
+    long rhs_stride;
+    long qstride;
+};
+constexpr ZBatch kOneRhs{1, 0, 0, 0};
+// The compile-time specialised passes of the mixed-radix lengths (80, 96, 112, 144, 160), built in
+// their own translation units; each returns -1 when the variant does not exist at that length.
+bool mixed_length(int n);
+template <typename R>
+long static_zgreen_mixed(int code, cx<R>* S, const R* G6, long ncols, const Geometry3& g, const cx<R>* tw,
+                         double* qpart, const int* skip, hipStream_t st, ZBatch zb);
+int zgreen_default_code_mixed(int n, int prec);
+template <typename R>
+long static_xfwd_mixed(int v, const XSource& src, cx<R>* S, const Geometry3& g, const cx<R>* tw, const int* skip,
+                       hipStream_t st);
+template <typename R>
+long static_xinv_mixed(int v, const cx<R>* S, const XSink& sink, const Geometry3& g, const cx<R>* tw, const int* skip,
+                       hipStream_t st);
+int pass_variant_code_mixed(int n, int v);
 // Upper bounds of the workgroup counts of launch_xinv / launch_zgreen over every variant.
 long xinv_max_blocks(const Geometry3& g);
 long zgreen_max_blocks(const Geometry3& g);

@@ -1,6 +1,6 @@
 // passes_col.hip -- compile-time specialised column passes (the y transform of all three
-// components; the z transform of the standalone FFT hooks) for column lengths 32 ... 512, in
-// kPassVariants tile shapes each.
+// components; the z transform of the standalone FFT hooks) for the column lengths 32 ... 512 and
+// 80, 96, 112, 144, 160, in kColVariants tile shapes each.
 //
 // Column col has base (col / inner) * outer_stride + col % inner and element e at
 // base + e * estride; a workgroup takes LINES consecutive columns, so each row of its tile is
@@ -37,24 +37,28 @@ __global__ __launch_bounds__((PassShape<R, N, LINES, OS>::THREADS)) void k_col_s
     const unsigned cu = (unsigned)col, iu = (unsigned)inner;
     const long base = (long)(cu / iu) * outer_stride + (long)(cu % iu);
     const bool own = ci < ncl;
-    // all loads of the thread in flight at once, then one LDS fill
-    constexpr int KL = N / EST;
-    static_assert(N % EST == 0, "column elements split evenly over the load threads");
+    // all loads of the thread in flight at once, then one LDS fill; a thread's last element slot
+    // idles when EST does not divide the column length (mixed-radix lengths)
+    constexpr int KL = (N + EST - 1) / EST;
+    constexpr bool EXACT = N % EST == 0;
     cx<R> ld[KL];
 #pragma unroll
     for (int k = 0; k < KL; ++k) {
+        const int e = threadIdx.x / LINES + k * EST;
         ld[k] = mkc<R>((R)0, (R)0);
-        if (own) ld[k] = Sc[base + (long)(threadIdx.x / LINES + k * EST) * estride];
+        if (own && (EXACT || e < N)) ld[k] = Sc[base + (long)e * estride];
     }
 #pragma unroll
-    for (int k = 0; k < KL; ++k)
-        if (own) buf[ci * LD + threadIdx.x / LINES + k * EST] = ld[k];
+    for (int k = 0; k < KL; ++k) {
+        const int e = threadIdx.x / LINES + k * EST;
+        if (own && (EXACT || e < N)) buf[ci * LD + e] = ld[k];
+    }
     __syncthreads();
     sfft_lines<R, SIGN, N, LINES, SH::T, LD>(buf, tw, ncl);
 #pragma unroll
     for (int k = 0; k < KL; ++k) {
         const int e = threadIdx.x / LINES + k * EST;
-        if (own) Sc[base + (long)e * estride] = buf[ci * LD + e];
+        if (own && (EXACT || e < N)) Sc[base + (long)e * estride] = buf[ci * LD + e];
     }
 }
 
@@ -129,6 +133,21 @@ long static_col(int v, int n, int sign, cx<R>* S, long ncols, long inner, long o
         case 512:
             return col_n<R, 512>(v, sign, S, ncols, inner, outer, estride, comp_stride, tw, skip, ncomp, skip_stride,
                                     st);
+        case 80:
+            return col_n<R, 80>(v, sign, S, ncols, inner, outer, estride, comp_stride, tw, skip, ncomp, skip_stride,
+                                st);
+        case 96:
+            return col_n<R, 96>(v, sign, S, ncols, inner, outer, estride, comp_stride, tw, skip, ncomp, skip_stride,
+                                st);
+        case 112:
+            return col_n<R, 112>(v, sign, S, ncols, inner, outer, estride, comp_stride, tw, skip, ncomp, skip_stride,
+                                 st);
+        case 144:
+            return col_n<R, 144>(v, sign, S, ncols, inner, outer, estride, comp_stride, tw, skip, ncomp, skip_stride,
+                                 st);
+        case 160:
+            return col_n<R, 160>(v, sign, S, ncols, inner, outer, estride, comp_stride, tw, skip, ncomp, skip_stride,
+                                 st);
         default: return -1;
     }
 }
@@ -146,6 +165,11 @@ int col_variant_code(int n, int v) {
         case 128: return col_code_of<128>(v);
         case 256: return col_code_of<256>(v);
         case 512: return col_code_of<512>(v);
+        case 80: return col_code_of<80>(v);
+        case 96: return col_code_of<96>(v);
+        case 112: return col_code_of<112>(v);
+        case 144: return col_code_of<144>(v);
+        case 160: return col_code_of<160>(v);
         default: return 0;
     }
 }

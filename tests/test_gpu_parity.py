@@ -179,9 +179,13 @@ def test_symmetry_gap_fp64(fp, oracle_mod):
     assert abs(gap - o.symmetry_gap(x, y)) < 1e-13
 
 
-# every compile-time specialised line length on every axis (x R2C/C2R, y columns, fused z)
+# every compile-time specialised line length on every axis (x R2C/C2R, y columns, fused z): the
+# powers of two and the mixed-radix lengths 80 = 16 x 5, 96 = 16 x 6, 112 = 16 x 7, 144 = 16 x 9,
+# 160 = 16 x 10 of the uneven-size batch
 STATIC_SHAPES = [(512, 4, 6), (6, 512, 4), (4, 6, 512), (256, 10, 6), (10, 256, 6), (6, 10, 256),
-                 (128, 32, 64), (64, 128, 32), (32, 64, 128), (64, 64, 64)]
+                 (128, 32, 64), (64, 128, 32), (32, 64, 128), (64, 64, 64),
+                 (80, 96, 12), (112, 8, 80), (10, 144, 96), (160, 12, 112), (8, 112, 144), (144, 160, 6),
+                 (96, 10, 160), (80, 80, 80)]
 
 
 @pytest.mark.parametrize("shape", STATIC_SHAPES)

@@ -16,20 +16,26 @@ from conftest import random_mask, rel
 
 pytestmark = pytest.mark.gpu
 
-# largest radix of the compile-time plan per length (fft_static.hpp StaticPlan)
-_MAX_RADIX = {32: 8, 64: 8, 128: 16, 256: 16, 512: 8}
+# threads per line of the compile-time plan per length (fft_static.hpp StaticPlan / PlanInfo::threads):
+# the largest radix's butterfly count for the power-of-two plans (32 = 8 x 4, 64 = 8 x 8, 128 = 16 x 8,
+# 256 = 16 x 16, 512 = 8 x 8 x 8), 16 for the mixed plans 16 x r (one butterfly per thread)
+_T = {32: 4, 64: 8, 128: 8, 256: 16, 512: 64, 80: 16, 96: 16, 112: 16, 144: 16, 160: 16}
+POW2 = [32, 64, 128, 256, 512]
+MIXED = [80, 96, 112, 144, 160]
 
 
-def pass_codes(n):
+def pass_codes(n, precision=0):
     """Codes (10 * lines per workgroup + threads per butterfly thread) of the x / column pass
     variants at length n, as fft_static.hpp enumerates them."""
-    t = n // _MAX_RADIX[n]
+    t = _T[n]
     base = max(2, 256 // t)
+    cx_bytes = 16 if precision == 0 else 8
     out = []
     for v in range(5):
         lines = base if v < 2 else (base // 2 if v < 4 else base // 4)
         os_ = 2 if v in (0, 2) else 4
-        if lines >= 2 and lines % 2 == 0 and 64 <= os_ * lines * t <= 1024:
+        lds = (n + lines * (n + 1)) * cx_bytes
+        if lines >= 2 and lines % 2 == 0 and 64 <= os_ * lines * t <= 1024 and lds <= 100 * 1024:
             out.append(10 * lines + os_)
     return out
 
@@ -38,9 +44,9 @@ def xpass_codes(n, precision=0):
     """pass_codes(n) plus the merged-round x-pass variants valid at this precision (one transform of
     3 lines / 2 complex lines per tile; code 10 * lines + threads per butterfly thread + 5) and the
     wave-per-line-pair variant (code 88) at the lengths it serves."""
-    t = n // _MAX_RADIX[n]
+    t = _T[n]
     base = max(2, 256 // t)
-    out = pass_codes(n)
+    out = pass_codes(n, precision)
     cx_bytes = 16 if precision == 0 else 8
     for j in range(3):
         lines, os_ = base >> j, 1 << j
@@ -57,15 +63,15 @@ def xpass_codes(n, precision=0):
 def col_codes(n, precision=0):
     """pass_codes(n) plus the wide column-pass variants (twice the base line count, OS 1 and 2)
     that fit the LDS budget at this precision."""
-    t = n // _MAX_RADIX[n]
+    t = _T[n]
     base = max(2, 256 // t)
-    out = pass_codes(n)
+    out = pass_codes(n, precision)
     cx_bytes = 16 if precision == 0 else 8
     lines = 2 * base
     for os_ in (1, 2):
         threads = os_ * lines * t
         lds = (n + lines * (n + 1)) * cx_bytes
-        if 64 <= threads <= 1024 and (lines * n) % threads == 0 and lds <= 100 * 1024:
+        if 64 <= threads <= 1024 and lds <= 100 * 1024:
             out.append(10 * lines + os_)
     return out
 
@@ -104,12 +110,12 @@ def _check_apply(green, sup, o, shape, seed):
 
 
 # one shape per axis and length: the varied axis carries the specialised length
-X_SHAPES = {n: (n, 6, 10) for n in (32, 64, 128, 256, 512)}
-Y_SHAPES = {n: (6, n, 10) for n in (32, 64, 128, 256, 512)}
-Z_SHAPES = {n: (6, 10, n) for n in (32, 64, 128, 256, 512)}
+X_SHAPES = {n: (n, 6, 10) for n in POW2 + MIXED}
+Y_SHAPES = {n: (6, n, 10) for n in POW2 + MIXED}
+Z_SHAPES = {n: (6, 10, n) for n in POW2 + MIXED}
 
 
-@pytest.mark.parametrize("n", [32, 64, 128, 256, 512])
+@pytest.mark.parametrize("n", POW2 + MIXED)
 def test_x_pass_variants_fp64(fp, oracle_mod, n):
     from fft_method_test_amd import _lib
 
@@ -194,7 +200,7 @@ def test_x_wave_staged_and_direct_runs(fp, oracle_mod, precision, n, ny, nz):
     green.close()
 
 
-@pytest.mark.parametrize("n", [32, 64, 128, 256, 512])
+@pytest.mark.parametrize("n", POW2 + MIXED)
 def test_y_pass_variants_fp64(fp, oracle_mod, n):
     from fft_method_test_amd import _lib
 
@@ -205,7 +211,7 @@ def test_y_pass_variants_fp64(fp, oracle_mod, n):
             _check_apply(green, sup, o, shape, seed=code)
 
 
-@pytest.mark.parametrize("n", [32, 64, 128, 256, 512])
+@pytest.mark.parametrize("n", POW2 + MIXED)
 def test_zgreen_variants_fp64(fp, oracle_mod, n):
     from fft_method_test_amd import _lib
 
@@ -257,7 +263,7 @@ def test_fp32_variants_mixed_k_sphere32(fp, oracle_mod):
 FP32_APPLY_BOUND = 1e-6
 
 
-@pytest.mark.parametrize("n", [32, 64, 128, 256, 512])
+@pytest.mark.parametrize("n", POW2 + MIXED)
 def test_fp32_operator_variants(fp, oracle_mod, n):
     """Every FP32 tile variant (x forward / inverse, column pass, z pass) pinned in turn: the FP32
     variational operator of the mixed sweeps within FP32_APPLY_BOUND of the FP64 oracle."""
