@@ -48,16 +48,18 @@ def fp64_bytes_zgreen(Nh, s_c=16, s_r=8):
     return 2 * 3 * Nh * s_c + 6 * Nh * s_r
 
 
-def bytes_apply(N, Nh, L, s_c=16, s_r=8, nlines=None):
+def bytes_apply(N, Nh, L, s_c=16, s_r=8, nlines=None, s_v=8, plane=False):
     # x-fwd: line bitmask of the support (1 bit per voxel, one int32 start per x line) + compact
-    # vector in, 3 half spectra out; y fwd/inv and z: in + out; z adds the Green tensor; x-inv:
-    # spectra + bitmask in, compact vector out
+    # vector in (s_v bytes per value: FP64, or FP32 for the mixed solver's vectors), 3 half spectra
+    # out; y fwd/inv and z: in + out; z adds the Green tensor; x-inv: spectra + bitmask in, compact
+    # vector out.  plane: the x and y passes fused over z planes, so the half spectra cross HBM
+    # only on their way into and out of the z pass.
     nl = nlines if nlines is not None else round(N ** (2.0 / 3.0))
     bits = N // 8 + 4 * nl
-    xf = bits + 8 * L + 3 * Nh * s_c
-    yp = 2 * 3 * Nh * s_c
+    xf = bits + s_v * L + 3 * Nh * s_c
+    yp = 0 if plane else 2 * 3 * Nh * s_c
     zg = fp64_bytes_zgreen(Nh, s_c, s_r)
-    xi = 3 * Nh * s_c + bits + 8 * L
+    xi = 3 * Nh * s_c + bits + s_v * L
     return xf + 2 * yp + zg + xi
 
 
@@ -308,7 +310,8 @@ def main():
     tr = Z_TRAFFIC_MB.get(("fp32" if pi == 1 else "fp64", z_tile)) if sides_per_launch == 1 else None
     traffic = None if tr is None else (tr[0] + tr[1]) * 1e6
     L = 3 * head["nb"]
-    a_bytes = bytes_apply(N, Nh, L, sc, sr)
+    plane = pi == 1 and pr["tiles"]["xfwd"][pi] == 1000
+    a_bytes = bytes_apply(N, Nh, L, sc, sr, s_v=4 if pi == 1 else 8, plane=plane)
     a_ms = pr["a_ms"][pi] / max(1, pr["a_rhs"][pi])  # per right-hand side
 
     out = {

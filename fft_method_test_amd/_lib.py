@@ -19,7 +19,7 @@ LIB_PATH = os.path.join(HERE, "libfftperm.so")
 OK, EINVAL, ERANGE, ERUNTIME, EDEVICE, EUNSUPPORTED = range(6)
 SUPPORT_FULLSOLID, SUPPORT_INTERFACE = 0, 1
 PRECISION_FP64, PRECISION_MIXED = 0, 1
-PASS_XFWD, PASS_XINV, PASS_YPASS, PASS_ZGREEN = 0, 1, 2, 3
+PASS_XFWD, PASS_XINV, PASS_YPASS, PASS_ZGREEN, PASS_PLANE = 0, 1, 2, 3, 4
 MIXED_CG_RELIABLE, MIXED_MINRES_REFINED, MIXED_CR_RELIABLE, MIXED_CR_FP32_VECTORS = 0, 1, 2, 3
 # tile variants the specialised passes offer at a length (codes as in fftperm_profile_stats)
 ZGREEN_CODES = (11, 12, 21, 22, 41, 42, 81, 82, 161, 162, 44, 84, 164, 324, 45, 85, 165, 86, 87, 88, 166, 167, 168, 326, 327)
@@ -31,7 +31,7 @@ EXPORTED_SYMBOLS = (
     "fftperm_get_support",
     "fftperm_green_tensor", "fftperm_fft_forward", "fftperm_fft_inverse", "fftperm_spectral_green",
     "fftperm_green_apply", "fftperm_background_force", "fftperm_rhs", "fftperm_operator_apply",
-    "fftperm_operator_apply_fp32",
+    "fftperm_operator_apply_fp32", "fftperm_operator_apply_fp32v", "fftperm_check_edges",
     "fftperm_minres", "fftperm_set_refinement", "fftperm_set_mixed_solver", "fftperm_pin_variant", "fftperm_recover",
     "fftperm_solve_direction", "fftperm_solve_tensor", "fftperm_residual_norm", "fftperm_symmetry_gap",
     "fftperm_profile", "fftperm_profile_read", "fftperm_set_graphs", "fftperm_unet_create", "fftperm_unet_destroy",
@@ -104,6 +104,8 @@ def load():
     L.fftperm_rhs.argtypes = [_vp, _dp, _dp]
     L.fftperm_operator_apply.argtypes = [_vp, _dp, _dp]
     L.fftperm_operator_apply_fp32.argtypes = [_vp, _dp, _dp]
+    L.fftperm_operator_apply_fp32v.argtypes = [_vp, _dp, _dp]
+    L.fftperm_check_edges.argtypes = [C.c_int, C.c_int, C.c_int]
     L.fftperm_minres.argtypes = [_vp, _dp, _vp, C.c_double, C.c_uint64, _dp, _vp, C.POINTER(SolveInfo)]
     L.fftperm_set_refinement.argtypes = [_vp, C.c_double, C.c_double, C.c_uint32]
     L.fftperm_set_mixed_solver.argtypes = [_vp, C.c_int, C.c_double]

@@ -407,6 +407,14 @@ struct fftperm_ctx {
     }
 };
 
+// Waves per workgroup of the FP32 x and y passes fused over z planes when they are on for this
+// grid shape (autotuner registry), else 0.
+static int plane_on(const fftperm_ctx* c) {
+    const Geometry3& g = c->g;
+    const int v = tuned_variant(kKindPlane, 1, g.nx, (long)g.ny * g.nz);
+    return (v == 1 || v == 2) && plane_pass_shape_ok(g) ? 8 * v : 0;
+}
+
 template <typename R, typename S_, typename G_, typename T_>
 static void run_chain(fftperm_ctx* c, S_* S, const G_* G, const T_& t, const XSource& src, const XSink& sink,
                       const int* skip, double* qpart, int fp32) {
@@ -420,13 +428,24 @@ static void run_chain(fftperm_ctx* c, S_* S, const G_* G, const T_& t, const XSo
         rec.fp32 = fp32;
         HIPCHK(hipEventRecord(rec.a, c->s));
     }
-    launch_xfwd<R>(src, S, c->g, t, skip, c->s);
-    launch_ypass<R>(S, -1, c->g, t, skip, c->s);
+    const int plane = fp32 && plane_src_ok(src) && plane_sink_ok(sink) ? plane_on(c) : 0;
+    if (plane)
+        launch_xyfwd_plane(src, reinterpret_cast<cx<float>*>(S), c->g, reinterpret_cast<const cx<float>*>(t.twx), plane,
+                           skip, c->s);
+    else {
+        launch_xfwd<R>(src, S, c->g, t, skip, c->s);
+        launch_ypass<R>(S, -1, c->g, t, skip, c->s);
+    }
     // the z-pass launch records rec.b / rec.c at its own start and end
     if (c->prof) zpass_events() = KernelEvents{rec.b, rec.c};
     c->zparts = launch_zgreen<R>(S, G, c->g, t, qpart, skip, c->s);
-    launch_ypass<R>(S, +1, c->g, t, skip, c->s);
-    c->xparts = launch_xinv<R>(S, sink, c->g, t, skip, c->s);
+    if (plane)
+        c->xparts = launch_yxinv_plane(reinterpret_cast<const cx<float>*>(S), sink, c->g,
+                                       reinterpret_cast<const cx<float>*>(t.twx), plane, skip, c->s);
+    else {
+        launch_ypass<R>(S, +1, c->g, t, skip, c->s);
+        c->xparts = launch_xinv<R>(S, sink, c->g, t, skip, c->s);
+    }
     if (c->prof) {
         HIPCHK(hipEventRecord(rec.d, c->s));
         c->prof_pending.push_back(rec);
@@ -1223,12 +1242,21 @@ void chain32_batch(fftperm_ctx* c, int nrhs) {
             cx<float>* S = c->S32 + b * spec;
             fftperm_ctx::ProfRec rec{};
             if (c->prof) rec = prof_begin(0, b);
-            launch_xfwd<float>(source(b), S, g, c->t32, skip, c->s);
-            launch_ypass<float>(S, -1, g, c->t32, skip, c->s);
+            const int plane = plane_src_ok(source(b)) && plane_sink_ok(sink(b)) ? plane_on(c) : 0;
+            if (plane)
+                launch_xyfwd_plane(source(b), S, g, c->t32.twx, plane, skip, c->s);
+            else {
+                launch_xfwd<float>(source(b), S, g, c->t32, skip, c->s);
+                launch_ypass<float>(S, -1, g, c->t32, skip, c->s);
+            }
             if (c->prof) zpass_events() = KernelEvents{rec.b, rec.c};
             c->zparts = launch_zgreen<float>(S, c->G32, g, c->t32, c->part_z + b * cr_zstride(c), skip, c->s);
-            launch_ypass<float>(S, +1, g, c->t32, skip, c->s);
-            c->xparts = launch_xinv<float>(S, sink(b), g, c->t32, skip, c->s);
+            if (plane)
+                c->xparts = launch_yxinv_plane(S, sink(b), g, c->t32.twx, plane, skip, c->s);
+            else {
+                launch_ypass<float>(S, +1, g, c->t32, skip, c->s);
+                c->xparts = launch_xinv<float>(S, sink(b), g, c->t32, skip, c->s);
+            }
             if (c->prof) {
                 HIPCHK(hipEventRecord(rec.d, c->s));
                 c->prof_pending.push_back(rec);
@@ -1605,6 +1633,21 @@ void tune_chain_prec(fftperm_ctx* c) {
                     best = i;
         }
         set_tuned_variant(k.kind, prec, k.n, k.count, cands[best]);
+    }
+    if (prec == 1 && plane_pass_shape_ok(g)) {
+        // FP32 x and y passes fused over z planes against the separate passes just tuned
+        const long pcount = (long)g.ny * g.nz;
+        float best_of[3] = {1e30f, 1e30f, 1e30f};
+        for (int round = 0; round < kRounds; ++round) {
+            for (int v = 0; v < 3; ++v) {
+                set_tuned_variant(kKindPlane, prec, g.nx, pcount, v);
+                best_of[v] = std::min(best_of[v], time_chain<R>(c, kReps));
+            }
+        }
+        int best = 0;
+        for (int v = 1; v < 3; ++v)
+            if (best_of[v] < best_of[best]) best = v;
+        set_tuned_variant(kKindPlane, prec, g.nx, pcount, best);
     }
     c->sync();
 }
@@ -2293,6 +2336,49 @@ int fftperm_operator_apply_fp32(fftperm_ctx* ctx, const double* x, double* y) {
     });
 }
 
+int fftperm_check_edges(int nx, int ny, int nz) {
+    return guarded([&] {
+        for (int n : {nx, ny, nz}) {
+            if (n <= 0) throw Err(FFTPERM_EINVAL, "Grid dimensions must be positive");
+            (void)make_plan(n);
+        }
+    });
+}
+
+int fftperm_operator_apply_fp32v(fftperm_ctx* ctx, const double* x, double* y) {
+    return guarded([&] {
+        check_ctx(ctx);
+        ctx->require_support();
+        if (!x || !y) throw Err(FFTPERM_EINVAL, "null argument");
+        if (ctx->precision != FFTPERM_PRECISION_MIXED)
+            throw Err(FFTPERM_EINVAL, "the FP32 operator exists in MIXED contexts only");
+        ctx->ensure_vectors();
+        ctx->ensure_vectors32();
+        const long nb = (long)ctx->nb;
+        const long L = ctx->L3();
+        const double inv_nb = 1.0 / (double)nb;
+        HIPCHK(hipMemcpyAsync(ctx->r, x, L * sizeof(double), hipMemcpyHostToDevice, ctx->s));
+        // the support mean of x in FP64, x itself rounded to an FP32 compact vector
+        launch_compact_sums(ctx->r, nb, ctx->part, nullptr, ctx->s);
+        launch_finish_sum(ctx->part, kRedBlocks, 3, inv_nb, ctx->small, nullptr, ctx->s);
+        launch_to_f32(ctx->r, ctx->x32, L, ctx->s);
+        XSource sr = src_none(ctx);
+        sr.compact32 = ctx->x32;
+        sr.mean = ctx->small;
+        XSink sk = sink_none(ctx);
+        sk.compact32 = ctx->q32;
+        sk.partials = ctx->part_x;
+        ctx->chain<float>(sr, sk, nullptr);
+        launch_finish_sum(ctx->part_x, (int)ctx->xparts, kXinvParts, inv_nb, ctx->small + 4, nullptr, ctx->s);
+        // y = the FP32 output widened, minus its support mean
+        HIPCHK(hipMemsetAsync(ctx->tmp, 0, L * sizeof(double), ctx->s));
+        launch_accumulate_f32(ctx->tmp, ctx->q32, L, ctx->s);
+        launch_sub_mean(ctx->tmp, ctx->small + 4, ctx->tmp, nb, 1.0, nullptr, ctx->s);
+        HIPCHK(hipMemcpyAsync(y, ctx->tmp, L * sizeof(double), hipMemcpyDeviceToHost, ctx->s));
+        ctx->sync();
+    });
+}
+
 int fftperm_minres(fftperm_ctx* ctx, const double* b, const double* x0, double tol, uint64_t maxit, double* x,
                    double* history, fftperm_solve_info* info) {
     return guarded([&] {
@@ -2495,11 +2581,23 @@ int fftperm_symmetry_gap(fftperm_ctx* ctx, const double* x, const double* y, dou
 int fftperm_pin_variant(fftperm_ctx* ctx, int kind, int precision, int code) {
     return guarded([&] {
         check_ctx(ctx);
-        if (kind < kKindXfwd || kind > kKindZgreen) throw Err(FFTPERM_EINVAL, "unknown pass kind");
+        if (kind < kKindXfwd || kind > kKindPlane) throw Err(FFTPERM_EINVAL, "unknown pass kind");
         if (precision != FFTPERM_PRECISION_FP64 && precision != FFTPERM_PRECISION_MIXED)
             throw Err(FFTPERM_EINVAL, "unknown precision mode");
         const Geometry3& g = ctx->g;
         const int prec = precision == FFTPERM_PRECISION_MIXED ? 1 : 0;
+        if (kind == kKindPlane) {
+            const long pcount = (long)g.ny * g.nz;
+            if (code == 0)
+                clear_tuned_variant(kind, prec, g.nx, pcount);
+            else if (code < 1 || code > 3)
+                throw Err(FFTPERM_EINVAL, "unknown plane-pass code " + std::to_string(code));
+            else if (prec != 1)
+                throw Err(FFTPERM_EINVAL, "the fused plane passes are FP32 (precision MIXED)");
+            else
+                set_tuned_variant(kind, prec, g.nx, pcount, code - 1);
+            return;
+        }
         int n = 0;
         long count = 0;
         switch (kind) {
@@ -2583,6 +2681,7 @@ int fftperm_profile_read(fftperm_ctx* ctx, fftperm_profile_stats* out) {
             out->xinv_tile[i] = variant_code(kKindXinv, i, g.nx, (long)g.ny * g.nz);
             out->ypass_tile[i] = variant_code(kKindCol, i, g.ny, (long)g.nz * g.nxh);
         }
+        if (plane_on(ctx)) out->xfwd_tile[1] = out->xinv_tile[1] = out->ypass_tile[1] = 1000;
     });
 }
 

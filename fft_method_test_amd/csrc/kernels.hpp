@@ -179,6 +179,19 @@ template <typename R>
 long static_xinv_mixed(int v, const cx<R>* S, const XSink& sink, const Geometry3& g, const cx<R>* tw, const int* skip,
                        hipStream_t st);
 int pass_variant_code_mixed(int n, int v);
+// FP32 x and y transforms fused over whole z planes (nx = ny = 256; passes_plane.hip): the
+// forward reads the compact FP32 source over the line bitmask and writes the y-transformed half
+// spectrum, the inverse reads it and writes the compact FP32 sink and one partial row per z plane
+// (component c's support sum at partials[3 z + c]).  Each returns its workgroup count (the inverse:
+// the partial rows), or -1 when the shape, source or sink is not one it serves.
+bool plane_pass_shape_ok(const Geometry3& g);
+bool plane_src_ok(const XSource& src);
+bool plane_sink_ok(const XSink& sink);
+// waves: 8 or 16 waves per workgroup (one workgroup per CU either way).
+long launch_xyfwd_plane(const XSource& src, cx<float>* S, const Geometry3& g, const cx<float>* twx, int waves,
+                        const int* skip, hipStream_t st);
+long launch_yxinv_plane(const cx<float>* S, const XSink& sink, const Geometry3& g, const cx<float>* twx, int waves,
+                        const int* skip, hipStream_t st);
 // Upper bounds of the workgroup counts of launch_xinv / launch_zgreen over every variant.
 long xinv_max_blocks(const Geometry3& g);
 long zgreen_max_blocks(const Geometry3& g);
@@ -203,7 +216,9 @@ int col_variant_code(int n, int v);   // column passes (kColVariants)
 // Autotuner registry, keyed by (pass kind, precision 0 = FP64 / 1 = FP32, transform length,
 // line or column count).  Values: pass variant index for the x and column passes, variant code
 // (10 * columns per workgroup + threads per butterfly thread) for the fused z pass; -1 untuned.
-enum PassKind { kKindXfwd = 0, kKindXinv = 1, kKindCol = 2, kKindZgreen = 3 };
+// kKindPlane (keyed by nx and ny * nz): the FP32 x and y passes fused over z planes with 8 (1) or
+// 16 (2) waves per workgroup; 0 or untuned = separate passes.
+enum PassKind { kKindXfwd = 0, kKindXinv = 1, kKindCol = 2, kKindZgreen = 3, kKindPlane = 4 };
 int tuned_variant(int kind, int prec, int n, long count);
 void set_tuned_variant(int kind, int prec, int n, long count, int v);
 void clear_tuned_variant(int kind, int prec, int n, long count);

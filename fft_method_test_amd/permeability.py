@@ -121,6 +121,8 @@ class GreenOperator:
         self.precision = precision
         h = C.c_void_p()
         g = grid._c()
+        # a grid edge the FFT plans cannot take is reported before any device work
+        check(L.fftperm_check_edges(*grid.shape))
         check(L.fftperm_create(C.byref(g), float(viscosity), int(p_radius), int(precision), C.byref(h)))
         self._h = h
         self._tensor = None
@@ -318,6 +320,16 @@ class VariationalOperator:
             raise ValueError("VariationalOperator::apply input size mismatch")
         y = np.zeros(self.size())
         check(_lib.load().fftperm_operator_apply_fp32(self._green._h, x, y))
+        return y
+
+    def apply_fp32v(self, x):
+        """The FP32 apply on FP32 compact vectors that the default MIXED solver iterates (MIXED
+        contexts only; no reference counterpart)."""
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
+        if x.size != self.size():
+            raise ValueError("VariationalOperator::apply input size mismatch")
+        y = np.zeros(self.size())
+        check(_lib.load().fftperm_operator_apply_fp32v(self._green._h, x, y))
         return y
 
 

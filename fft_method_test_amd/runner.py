@@ -539,6 +539,7 @@ class PermeabilityRunner:
         nz, ny, nx = mask.shape
         h = cfg.voxel_size
         grid = P.Grid3D(nx, ny, nz, h, h, h)
+        _lib.check(_lib.load().fftperm_check_edges(nx, ny, nz))
         medium = P.BinaryMedium(grid, mask)
         prec = _lib.PRECISION_MIXED if cfg.precision == "mixed" else _lib.PRECISION_FP64
         green = P.GreenOperator(grid, cfg.mu, cfg.p_radius, prec)

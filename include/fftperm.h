@@ -161,6 +161,18 @@ int fftperm_operator_apply(fftperm_ctx* ctx, const double* x, double* y);
  * hook to measure the FP32 operator's error against fftperm_operator_apply.  MIXED contexts only
  * (FFTPERM_EINVAL otherwise). */
 int fftperm_operator_apply_fp32(fftperm_ctx* ctx, const double* x, double* y);
+/* The FP32 apply as the default MIXED solver (conjugate residuals on FP32 vectors) iterates it: x
+ * rounded to an FP32 compact vector (its support mean taken in FP64), the FP32 sweeps, the FP32
+ * compact output widened to y and centred.  On 256 x 256 x nz grids this is the apply whose x and
+ * y passes may run fused over z planes (FFTPERM_PASS_PLANE).  No counterpart in the reference;
+ * MIXED contexts only (FFTPERM_EINVAL otherwise). */
+int fftperm_operator_apply_fp32v(fftperm_ctx* ctx, const double* x, double* y);
+/* Host-only check of a grid shape against the FFT plans (touches no device): every edge must be a
+ * product of the factors 2, 3, 5, 7, 11, 13 within eight transform stages, else
+ * FFTPERM_EUNSUPPORTED with a message naming the edge (the reference's FFTW backend plans any length,
+ * src/fft/FFTWBackend.cpp:176-192); a non-positive edge -> FFTPERM_EINVAL.  The context creation
+ * applies the same rule plus the device's LDS limits. */
+int fftperm_check_edges(int nx, int ny, int nz);
 
 /* MinresSolver(tol, maxit).solve(op, b, x0) (src/solver/MinresSolver.cpp:46-218) on the variational
  * operator.  x0 may be NULL (the reference's empty guess).  history (maxit + 1 entries) may be NULL.
@@ -248,7 +260,8 @@ typedef struct {
     uint64_t apply_launches[2];
     uint64_t N, Nh, nb;          /* grid size, half-spectrum size, support voxels */
     int zgreen_tile[2];          /* fused z-pass variant in use: 10*columns + threads per butterfly [FP64, FP32]; 0 = runtime kernel */
-    int xfwd_tile[2];            /* x-forward variant in use: 10*lines + threads per butterfly [FP64, FP32]; 0 = runtime kernel */
+    int xfwd_tile[2];            /* x-forward variant in use: 10*lines + threads per butterfly [FP64, FP32]; 0 = runtime kernel;
+                                    1000 (FP32) = x and y passes fused over z planes (the x, x-inverse and y codes all read 1000) */
     int xinv_tile[2];            /* x-inverse variant in use, same code */
     int ypass_tile[2];           /* y-pass variant in use, same code */
     uint64_t zgreen_rhs[2];      /* right-hand sides the counted z-pass launches transformed (a multi-RHS launch counts each side it served) */
@@ -260,8 +273,12 @@ typedef struct {
  * _ZGREEN; precision: FFTPERM_PRECISION_FP64 or _MIXED (the FP32 transforms); code as reported in
  * fftperm_profile_stats (10 * lines or columns per workgroup + threads per butterfly thread, or + 4
  * for the register-resident z pass); code 0 returns the choice to the autotuner.  An unknown code
- * -> FFTPERM_EINVAL; a known code that does not fit the device runs the generic kernel. */
-enum { FFTPERM_PASS_XFWD = 0, FFTPERM_PASS_XINV = 1, FFTPERM_PASS_YPASS = 2, FFTPERM_PASS_ZGREEN = 3 };
+ * -> FFTPERM_EINVAL; a known code that does not fit the device runs the generic kernel.
+ * kind FFTPERM_PASS_PLANE (precision _MIXED only): code 1 = separate x and y passes, code 2 / 3 = the
+ * FP32 x and y passes of the solver's applies fused over z planes with 8 / 16 waves per workgroup
+ * (grids with nx = ny = 256; other shapes, sources and sinks keep the separate passes). */
+enum { FFTPERM_PASS_XFWD = 0, FFTPERM_PASS_XINV = 1, FFTPERM_PASS_YPASS = 2, FFTPERM_PASS_ZGREEN = 3,
+       FFTPERM_PASS_PLANE = 4 };
 int fftperm_pin_variant(fftperm_ctx* ctx, int kind, int precision, int code);
 
 /* MINRES iterations replay hipGraphs of two captured iterations (default on; no counterpart in the

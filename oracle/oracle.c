@@ -167,8 +167,10 @@ static void fft_axis(const fft1d* p, double* data, size_t nx, size_t ny, size_t 
     (void)stride;
     /* Lines are grouped in blocks of up to ORC_COLBLOCK consecutive x positions (for y/z axes). */
     const size_t nblocks = (axis == 0) ? nlines : ((nx + ORC_COLBLOCK - 1) / ORC_COLBLOCK) * (axis == 1 ? nz : ny);
+    /* Lines are independent, so the thread count never changes the result; grids below 32^3 run
+     * serially, where a parallel region costs more than the lines it splits. */
 #ifdef _OPENMP
-#pragma omp parallel
+#pragma omp parallel if (nx * ny * nz >= 32768)
 #endif
     {
         double* line = (double*)malloc(2 * n * ORC_COLBLOCK * sizeof(double));
