@@ -188,10 +188,13 @@ __global__ __launch_bounds__(PlaneShape<W>::THREADS) void k_xyfwd_plane(XSource 
     const long line0 = (long)z * kPN;
     const uint32_t blo = 1u << n, bhi = 1u << (n + 16);
     const float* __restrict__ src32 = src.compact32;
+    const unsigned last = (unsigned)(src.lines.nb - 1);
 
-    // gathers of the rows of round t (line pair G + GR t) into gv[t & 1]: every lane loads
-    // (off-support voxels read compact entry 0, which exists since the support is not empty), so
-    // the loads issue back to back; 32-bit byte offsets (12 NB < 2^32) keep each address in one VGPR
+    // gathers of the rows of round t (line pair G + GR t) into gv[t & 1]: every lane loads, so the
+    // loads issue back to back.  An off-support voxel reads the entry of the next support voxel of
+    // its line (clamped to the last one): a line the neighbouring lanes fetch anyway, where a fixed
+    // dummy entry would be one cache line that every CU hits.  32-bit byte offsets (12 NB < 2^32)
+    // keep each address in one VGPR.
     float gv[2][2][16];
     auto gather = [&](int t) {
         const int m = G + GR * t;
@@ -204,7 +207,7 @@ __global__ __launch_bounds__(PlaneShape<W>::THREADS) void k_xyfwd_plane(XSource 
                 const uint32_t w = lb.word[j >> 1];
                 const uint32_t bit = (j & 1) ? bhi : blo;
                 const bool on = (w & bit) != 0u;
-                const unsigned loc = (unsigned)(lb.pre[j >> 1] + __popc(w & (bit - 1u))) & (0u - (unsigned)on);
+                const unsigned loc = min((unsigned)(lb.pre[j >> 1] + __popc(w & (bit - 1u))), last);
                 const float raw = *reinterpret_cast<const float*>(reinterpret_cast<const char*>(src32) +
                                                                   4u * (3u * loc + (unsigned)c));
                 gv[t & 1][p][j] = on ? raw - mc : 0.f;
