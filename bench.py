@@ -191,6 +191,27 @@ def cpu_baseline(args, mask, applies_per_solve, min_seconds=10.0):
     }
 
 
+def hbm_probes(args, P, _lib, reps=20):
+    """Copy probes over a MIXED context's three padded FP32 half spectra at this grid
+    (fftperm_probe_copy): the contiguous stream (what HBM delivers here, beside the 8 TB/s nominal
+    peak) and the y / z passes' column access patterns, in GB/s of bytes read plus written."""
+    import ctypes as C
+
+    green = P.GreenOperator(P.Grid3D(args.n, args.n, args.n), 1.0, 2, _lib.PRECISION_MIXED)
+    out = {}
+    try:
+        rate = C.c_double()
+        for pattern, key in ((0, "stream_copy_GBs"), (1, "ypass_pattern_copy_GBs"), (2, "zpass_pattern_copy_GBs")):
+            _lib.check(_lib.load().fftperm_probe_copy(green._h, pattern, reps, C.byref(rate)))
+            out[key] = rate.value
+    finally:
+        green.close()
+    out["what"] = (f"{reps} copies each of the three padded FP32 half spectra of a {args.n}^3 context (spectrum slot "
+                   f"0 -> 1): contiguous 16-byte accesses; 256-byte row segments down the y columns; 64-byte row "
+                   f"segments down the z columns (the y and z passes' access patterns)")
+    return out
+
+
 def oracle_fixture_column(args, mask):
     """The oracle's FP64 x column of K for this medium at tol 1e-12, minted in the build container
     (tests/golden/make_k_fixtures.py), when the medium is the one it was minted on."""
@@ -277,6 +298,8 @@ def main():
             return None
         return float(np.linalg.norm(r["K"][:, 0] - col_o) / np.linalg.norm(col_o))
 
+    probes = hbm_probes(args, P, _lib)
+    log(f"[bench] HBM copy probes: {probes}")
     errs = {k: k_err(r) for k, r in runs.items()}
     meets = {k: (e is not None and e <= K_ERR_BAR) for k, e in errs.items()}
     # headline: mixed precision with the learned warm start (the metric's configuration) when it
@@ -356,6 +379,7 @@ def main():
             "launches": pr["z_n"][pi],
             "precision": "fp32" if pi == 1 else "fp64",
         },
+        "hbm_probe": probes,
         "apply": {
             "avg_ms": a_ms,
             "algorithmic_bytes": a_bytes,

@@ -32,6 +32,7 @@ EXPORTED_SYMBOLS = (
     "fftperm_green_tensor", "fftperm_fft_forward", "fftperm_fft_inverse", "fftperm_spectral_green",
     "fftperm_green_apply", "fftperm_background_force", "fftperm_rhs", "fftperm_operator_apply",
     "fftperm_operator_apply_fp32", "fftperm_operator_apply_fp32v", "fftperm_check_edges",
+    "fftperm_probe_copy",
     "fftperm_minres", "fftperm_set_refinement", "fftperm_set_mixed_solver", "fftperm_pin_variant", "fftperm_recover",
     "fftperm_solve_direction", "fftperm_solve_tensor", "fftperm_residual_norm", "fftperm_symmetry_gap",
     "fftperm_profile", "fftperm_profile_read", "fftperm_set_graphs", "fftperm_unet_create", "fftperm_unet_destroy",
@@ -106,6 +107,7 @@ def load():
     L.fftperm_operator_apply_fp32.argtypes = [_vp, _dp, _dp]
     L.fftperm_operator_apply_fp32v.argtypes = [_vp, _dp, _dp]
     L.fftperm_check_edges.argtypes = [C.c_int, C.c_int, C.c_int]
+    L.fftperm_probe_copy.argtypes = [_vp, C.c_int, C.c_int, C.POINTER(C.c_double)]
     L.fftperm_minres.argtypes = [_vp, _dp, _vp, C.c_double, C.c_uint64, _dp, _vp, C.POINTER(SolveInfo)]
     L.fftperm_set_refinement.argtypes = [_vp, C.c_double, C.c_double, C.c_uint32]
     L.fftperm_set_mixed_solver.argtypes = [_vp, C.c_int, C.c_double]

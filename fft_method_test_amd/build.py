@@ -13,7 +13,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libfftperm.so")
 ARCH = os.environ.get("FFTPERM_ARCH", "gfx950")
 
-SOURCES = ["kernels_fft.hip", "passes_x.hip", "passes_x_mixed.hip", "zgreen_mixed.hip", "api.hip", "passes_col.hip", "passes_plane.hip",
+SOURCES = ["kernels_fft.hip", "passes_x.hip", "passes_x_mixed.hip", "zgreen_mixed.hip", "api.hip", "passes_col.hip", "passes_plane.hip", "probe.hip",
            "kernels_setup.hip", "kernels_krylov.hip", "unet.hip", "brinkman.hip", "noalloc.hip"]
 
 HIPCC_FLAGS = [

@@ -2336,6 +2336,31 @@ int fftperm_operator_apply_fp32(fftperm_ctx* ctx, const double* x, double* y) {
     });
 }
 
+int fftperm_probe_copy(fftperm_ctx* ctx, int pattern, int reps, double* gbps) {
+    return guarded([&] {
+        check_ctx(ctx);
+        if (!gbps) throw Err(FFTPERM_EINVAL, "null output");
+        if (reps < 1) throw Err(FFTPERM_EINVAL, "reps must be >= 1");
+        if (ctx->precision != FFTPERM_PRECISION_MIXED)
+            throw Err(FFTPERM_EINVAL, "the copy probes use the FP32 spectrum slots of a MIXED context");
+        const long bytes = probe_bytes(pattern, ctx->g);
+        if (bytes < 0) throw Err(FFTPERM_EINVAL, "unknown probe pattern " + std::to_string(pattern));
+        const float2* src = ctx->S32;
+        float2* dst = ctx->S32 + 3 * ctx->g.Nc;
+        launch_probe(pattern, src, dst, ctx->g, ctx->s);  // untimed first launch
+        hipEvent_t a = ctx->take_event(), b = ctx->take_event();
+        HIPCHK(hipEventRecord(a, ctx->s));
+        for (int r = 0; r < reps; ++r) launch_probe(pattern, src, dst, ctx->g, ctx->s);
+        HIPCHK(hipEventRecord(b, ctx->s));
+        HIPCHK(hipEventSynchronize(b));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, a, b));
+        ctx->ev_free.push_back(a);
+        ctx->ev_free.push_back(b);
+        *gbps = (double)bytes / ((double)ms * 1e-3 / reps) / 1e9;
+    });
+}
+
 int fftperm_check_edges(int nx, int ny, int nz) {
     return guarded([&] {
         for (int n : {nx, ny, nz}) {

@@ -192,6 +192,11 @@ long launch_xyfwd_plane(const XSource& src, cx<float>* S, const Geometry3& g, co
                         const int* skip, hipStream_t st);
 long launch_yxinv_plane(const cx<float>* S, const XSink& sink, const Geometry3& g, const cx<float>* twx, int waves,
                         const int* skip, hipStream_t st);
+// HBM copy probes over the three padded FP32 half spectra (probe.hip): pattern 0 contiguous, 1 the
+// y passes' 32-element column tiles, 2 the z pass's 8-element column tiles; src -> dst, both 3 Nc
+// complex.  probe_bytes: bytes one launch reads plus writes (-1: unknown pattern).
+long probe_bytes(int pattern, const Geometry3& g);
+void launch_probe(int pattern, const float2* src, float2* dst, const Geometry3& g, hipStream_t st);
 // Upper bounds of the workgroup counts of launch_xinv / launch_zgreen over every variant.
 long xinv_max_blocks(const Geometry3& g);
 long zgreen_max_blocks(const Geometry3& g);

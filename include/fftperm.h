@@ -173,6 +173,14 @@ int fftperm_operator_apply_fp32v(fftperm_ctx* ctx, const double* x, double* y);
  * src/fft/FFTWBackend.cpp:176-192); a non-positive edge -> FFTPERM_EINVAL.  The context creation
  * applies the same rule plus the device's LDS limits. */
 int fftperm_check_edges(int nx, int ny, int nz);
+/* HBM copy probe (a measurement hook, no reference counterpart): reps copies of the context's three
+ * padded FP32 half spectra (spectrum slot 0 -> slot 1) in an access pattern: 0 contiguous 16-byte
+ * loads and stores (the bandwidth the device reaches), 1 the y passes' pattern (32-element row
+ * segments down the columns), 2 the z pass's pattern (8-element row segments down the z columns).
+ * *gbps = (bytes read + written per copy) / (HIP-event time / reps).  Overwrites the spectrum
+ * slots (no solver state lives there between calls).  MIXED contexts only; an unknown pattern or
+ * reps < 1 -> FFTPERM_EINVAL. */
+int fftperm_probe_copy(fftperm_ctx* ctx, int pattern, int reps, double* gbps);
 
 /* MinresSolver(tol, maxit).solve(op, b, x0) (src/solver/MinresSolver.cpp:46-218) on the variational
  * operator.  x0 may be NULL (the reference's empty guess).  history (maxit + 1 entries) may be NULL.
