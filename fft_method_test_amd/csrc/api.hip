@@ -103,18 +103,35 @@ LinePlan make_plan(int n) {
             m /= f;
         }
     }
-    if (m != 1) throw Err(FFTPERM_EUNSUPPORTED, "grid edge " + std::to_string(n) +
-                                                    " has a prime factor above 13 (unsupported FFT length)");
+    // any other prime factor is a stage of its own, computed as direct DFT sums with one thread
+    // per output (FFTW, the reference's backend, plans every length: src/fft/FFTWBackend.cpp:176-192)
+    for (int f = 17; m > 1; f += 2) {
+        if ((long)f * f > m) f = m;
+        while (m % f == 0) {
+            rad.push_back(f);
+            m /= f;
+        }
+    }
     if (rad.size() > 8) throw Err(FFTPERM_EUNSUPPORTED, "grid edge " + std::to_string(n) + " needs too many stages");
     p.nst = (int)rad.size();
     p.maxbf = 1;
     for (int s = 0; s < p.nst; ++s) {
         p.rad[s] = rad[s];
-        p.maxbf = std::max(p.maxbf, n / rad[s]);
+        // threads a line needs in this stage: one per butterfly, one per output for a direct-sum prime
+        p.maxbf = std::max(p.maxbf, rad[s] > 16 ? n : n / rad[s]);
     }
     if (p.maxbf > 1024)
-        throw Err(FFTPERM_EUNSUPPORTED, "grid edge " + std::to_string(n) + " needs more than 1024 butterflies per stage");
+        throw Err(FFTPERM_EUNSUPPORTED, "grid edge " + std::to_string(n) +
+                                            " needs more than 1024 threads per line in one FFT stage");
     return p;
+}
+
+// The runtime fused z pass transforms three components of a column in one workgroup: its plan
+// must fit three lines' threads into 1024.
+void check_z_threads(const LinePlan& pz) {
+    if (3 * pz.maxbf > 1024)
+        throw Err(FFTPERM_EUNSUPPORTED, "grid edge " + std::to_string(pz.n) +
+                                            " along z needs more than 1024 threads for the fused z pass");
 }
 
 template <typename R>
@@ -2056,6 +2073,7 @@ int fftperm_create(const fftperm_grid* grid, double mu, int p_radius, int precis
             if (pc.lds > lds_limit())
                 throw Err(FFTPERM_EUNSUPPORTED, "grid edge " + std::to_string(lp->n) + " needs more LDS than the device has");
         }
+        check_z_threads(c->t64.pz);
         c->t64.twx = make_twiddles<double>(g.nx);
         c->t64.twy = make_twiddles<double>(g.ny);
         c->t64.twz = make_twiddles<double>(g.nz);
@@ -2363,10 +2381,11 @@ int fftperm_probe_copy(fftperm_ctx* ctx, int pattern, int reps, double* gbps) {
 
 int fftperm_check_edges(int nx, int ny, int nz) {
     return guarded([&] {
-        for (int n : {nx, ny, nz}) {
+        for (int n : {nx, ny, nz})
             if (n <= 0) throw Err(FFTPERM_EINVAL, "Grid dimensions must be positive");
-            (void)make_plan(n);
-        }
+        (void)make_plan(nx);
+        (void)make_plan(ny);
+        check_z_threads(make_plan(nz));
     });
 }
 

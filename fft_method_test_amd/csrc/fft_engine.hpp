@@ -296,7 +296,7 @@ __device__ __forceinline__ void dft_small(cx<R>* v) {
     }
 }
 
-// Odd prime radices: direct O(RAD^2) DFT with roots from the line's twiddle table
+// Odd prime radices up to 13: direct O(RAD^2) DFT with roots from the line's twiddle table
 // (n is a multiple of RAD, so W_RAD^m = tw[m * n / RAD]).
 template <typename R, int SIGN, int RAD>
 __device__ __forceinline__ void dft_odd(cx<R>* v, const cx<R>* tw, int n) {
@@ -358,12 +358,52 @@ __device__ __forceinline__ void stockham_stage(cx<R>* buf, int ld, int nlines, i
     __syncthreads();
 }
 
+// A stage of prime radix RAD > 16: thread t computes output q of butterfly j of its line as the
+// direct sum o_q = sum_r W_RAD^{r q} (W^{r k} x_{j + r m}) over the RAD twiddled inputs (one thread
+// per output element: nlines * n threads).
+template <typename R, int SIGN>
+__device__ void stockham_stage_prime(cx<R>* buf, int ld, int nlines, int n, int Ns, int RAD, const cx<R>* tw) {
+    const int m = n / RAD;
+    const int t = threadIdx.x;
+    const bool act = t < nlines * n;
+    int line = 0, j = 0, k = 0, q = 0;
+    cx<R> acc = mkc<R>((R)0, (R)0);
+    if (act) {
+        line = t / n;
+        const int e = t - line * n;
+        j = e / RAD;
+        q = e - j * RAD;
+        k = j % Ns;
+        const cx<R>* L = buf + line * ld;
+        const int tstride = n / (Ns * RAD);
+        for (int r = 0; r < RAD; ++r) {
+            cx<R> x = L[j + r * m];
+            if (k != 0 && r != 0) {
+                cx<R> w = tw[r * k * tstride];
+                if (SIGN > 0) w.y = -w.y;
+                x = cmul<R>(x, w);
+            }
+            cx<R> wq = tw[((r * q) % RAD) * m];
+            if (SIGN > 0) wq.y = -wq.y;
+            acc = cadd<R>(acc, cmul<R>(x, wq));
+        }
+    }
+    __syncthreads();
+    if (act) buf[line * ld + (j / Ns) * Ns * RAD + k + q * Ns] = acc;
+    __syncthreads();
+}
+
 // Transform nlines lines in LDS.  Caller has synchronised after filling buf and tw.
 template <typename R, int SIGN>
 __device__ void fft_lines(cx<R>* buf, int ld, int nlines, const LinePlan& pl, const cx<R>* tw) {
     int Ns = 1;
     for (int s = 0; s < pl.nst; ++s) {
         const int rad = pl.rad[s];
+        if (rad > 16) {
+            stockham_stage_prime<R, SIGN>(buf, ld, nlines, pl.n, Ns, rad, tw);
+            Ns *= rad;
+            continue;
+        }
         switch (rad) {
             case 2: stockham_stage<R, SIGN, 2>(buf, ld, nlines, pl.n, Ns, tw); break;
             case 3: stockham_stage<R, SIGN, 3>(buf, ld, nlines, pl.n, Ns, tw); break;

@@ -167,11 +167,14 @@ int fftperm_operator_apply_fp32(fftperm_ctx* ctx, const double* x, double* y);
  * y passes may run fused over z planes (FFTPERM_PASS_PLANE).  No counterpart in the reference;
  * MIXED contexts only (FFTPERM_EINVAL otherwise). */
 int fftperm_operator_apply_fp32v(fftperm_ctx* ctx, const double* x, double* y);
-/* Host-only check of a grid shape against the FFT plans (touches no device): every edge must be a
- * product of the factors 2, 3, 5, 7, 11, 13 within eight transform stages, else
- * FFTPERM_EUNSUPPORTED with a message naming the edge (the reference's FFTW backend plans any length,
- * src/fft/FFTWBackend.cpp:176-192); a non-positive edge -> FFTPERM_EINVAL.  The context creation
- * applies the same rule plus the device's LDS limits. */
+/* Host-only check of a grid shape against the FFT plans (touches no device).  Every length is
+ * planned, as by FFTW, the reference's backend (src/fft/FFTWBackend.cpp:176-192): factors 2..16 as
+ * Stockham radices, any other prime as a stage of direct DFT sums with one thread per output.  A plan
+ * needs at most eight stages and at most 1024 threads per line in a stage (so an edge with a prime
+ * factor above 16 is at most 1024), and the z edge's plan three times that (the fused z pass
+ * transforms three components together: an edge with a prime factor above 16 is at most 341 along
+ * z); otherwise FFTPERM_EUNSUPPORTED with a message naming the edge.  A non-positive edge ->
+ * FFTPERM_EINVAL.  The context creation applies the same rules plus the device's LDS limits. */
 int fftperm_check_edges(int nx, int ny, int nz);
 /* HBM copy probe (a measurement hook, no reference counterpart): reps copies of the context's three
  * padded FP32 half spectra (spectrum slot 0 -> slot 1) in an access pattern: 0 contiguous 16-byte

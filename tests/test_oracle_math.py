@@ -17,7 +17,7 @@ from conftest import random_mask, rel
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "reference_vectors.npz")
 
 
-@pytest.mark.parametrize("shape", [(4, 4, 4), (2, 3, 5), (7, 11, 13), (16, 8, 12), (17, 3, 2), (1, 1, 6), (32, 32, 32)])
+@pytest.mark.parametrize("shape", [(4, 4, 4), (2, 3, 5), (7, 11, 13), (16, 8, 12), (17, 3, 2), (1, 1, 6), (32, 32, 32), (101, 3, 34)])
 def test_fft3d_matches_numpy(oracle_mod, shape):
     nx, ny, nz = shape
     rng = np.random.default_rng(nx * 100 + ny)

@@ -152,25 +152,32 @@ def test_permeability_file_roundtrip(tmp_path):
     assert np.allclose(K2, K, rtol=5e-7, atol=0)
 
 
-@pytest.mark.parametrize("shape,edge", [((17, 17, 17), 17), ((32, 32, 101), 101), ((64, 34, 16), 34)])
-def test_unsupported_grid_edge_is_a_clear_error(tmp_path, shape, edge):
-    """An edge with a prime factor above 13 (FFTW, the reference's backend, plans any length) is
-    reported by name before any device work: by the C ABI's host-only check and by the runner."""
+@pytest.mark.parametrize("shape,edge,msg", [
+    ((1031, 4, 4), 1031, "needs more than 1024 threads per line in one FFT stage"),
+    ((16, 16, 353), 353, "along z needs more than 1024 threads for the fused z pass"),
+    ((16, 3 ** 9, 16), 3 ** 9, "needs too many stages"),
+])
+def test_unsupported_grid_edge_is_a_clear_error(tmp_path, shape, edge, msg):
+    """Edges with any prime factor are planned (as FFTW, the reference's backend, does); an edge whose
+    plan does not fit a workgroup is reported by name before any device work, by the C ABI's
+    host-only check and by the runner."""
     from fft_method_test_amd import _lib, voxelio
 
     L = _lib.load()
+    for ok in ((17, 17, 17), (32, 32, 101), (64, 34, 16), (1021, 4, 4), (8, 8, 337)):
+        assert L.fftperm_check_edges(*ok) == _lib.OK, ok
     rc = L.fftperm_check_edges(*shape)
     assert rc == _lib.EUNSUPPORTED
-    msg = L.fftperm_last_error().decode()
-    assert f"grid edge {edge} " in msg and "prime factor above 13" in msg
-    assert L.fftperm_check_edges(256, 160, 48) == _lib.OK
+    text = L.fftperm_last_error().decode()
+    assert f"grid edge {edge} " in text and msg in text
     assert L.fftperm_check_edges(0, 8, 8) == _lib.EINVAL
-    nx, ny, nz = shape
-    geo = str(tmp_path / "odd.fgeo")
-    voxelio.write_fgeo(geo, np.ones((nz, ny, nx), dtype=np.uint16))
-    cfg = R().from_cli(["--input", geo, "--output", str(tmp_path / "o.txt")])
-    with pytest.raises(RuntimeError, match=f"grid edge {edge} has a prime factor above 13"):
-        R().PermeabilityRunner(cfg).run()
+    if np.prod(shape) <= 1 << 20:
+        nx, ny, nz = shape
+        geo = str(tmp_path / "odd.fgeo")
+        voxelio.write_fgeo(geo, np.ones((nz, ny, nx), dtype=np.uint16))
+        cfg = R().from_cli(["--input", geo, "--output", str(tmp_path / "o.txt")])
+        with pytest.raises(RuntimeError, match=f"grid edge {edge} "):
+            R().PermeabilityRunner(cfg).run()
     assert L.fftperm_check_edges(8, 8, 8) == _lib.OK  # a successful call clears the error text
     assert L.fftperm_last_error() == b""
 
