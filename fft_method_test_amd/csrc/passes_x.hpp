@@ -627,7 +627,17 @@ __device__ __forceinline__ void wave_fft(cx<R> (&v)[N / 64], cx<R>* L, const cx<
 }
 
 // Support words of x line `line` (uniform across the wave) and their in-line prefix counts;
-// loc(r) = support index of voxel lane + 64 r, or -1.
+// loc(r) = support index of voxel lane + 64 r, or -1.  Voxel lane + 64 r is bit `lane` of the 64-bit
+// word pair (2 r, 2 r + 1), so with the lane's own masks (LaneBits64, computed once per kernel) the
+// index is two masked popcounts of uniform words: no per-lane choice between the two words.
+struct LaneBits64 {
+    uint32_t bit_lo, bit_hi, below_lo, below_hi;
+};
+__device__ __forceinline__ LaneBits64 lane_bits64(int lane) {
+    const uint64_t bit = 1ull << lane, below = bit - 1ull;
+    return LaneBits64{(uint32_t)bit, (uint32_t)(bit >> 32), (uint32_t)below, (uint32_t)(below >> 32)};
+}
+
 template <int N>
 struct WaveLineBits {
     static constexpr int W = N / 32;
@@ -646,21 +656,11 @@ struct WaveLineBits {
             acc += __popc(word[u]);
         }
     }
-    __device__ __forceinline__ int loc(int lane, int r) const {
-        // voxel lane + 64 r lies in word 2 r (lanes 0..31) or 2 r + 1 (lanes 32..63)
-        const bool hi = lane >= 32;
-        const int b = lane & 31;
-        uint32_t w = 0u;
-        int p = 0;
-#pragma unroll
-        for (int u = 0; u < W; u += 2) {
-            if (u == 2 * r) {
-                w = hi ? word[u + 1] : word[u];
-                p = hi ? pre[u + 1] : pre[u];
-            }
-        }
-        if (((w >> b) & 1u) == 0u) return -1;
-        return start + p + __popc(w & ((1u << b) - 1u));
+    __device__ __forceinline__ int loc(const LaneBits64& m, int r) const {
+        const uint32_t wl = word[2 * r], wh = word[2 * r + 1];
+        const bool on = ((wl & m.bit_lo) | (wh & m.bit_hi)) != 0u;
+        const int cnt = start + pre[2 * r] + __popc(wl & m.below_lo) + __popc(wh & m.below_hi);
+        return on ? cnt : -1;
     }
 };
 
@@ -708,6 +708,7 @@ __global__ __launch_bounds__(256) void k_xfwd_w(XSource src, cx<R>* __restrict__
     cx<R>* tw = reinterpret_cast<cx<R>*>(smem_raw);
     // the wave index is uniform: the line numbers, support words and run bounds below are too
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
+    const LaneBits64 lmask = lane_bits64(lane);
     cx<R>* L = tw + lpad_size<N>() + wave * lpad_size<N>();
     for (int i = threadIdx.x; i < N; i += 256) tw[lpad(i)] = twg[i];
     __syncthreads();
@@ -745,7 +746,7 @@ __global__ __launch_bounds__(256) void k_xfwd_w(XSource src, cx<R>* __restrict__
         for (int a = 0; a < 2; ++a) {
 #pragma unroll
             for (int r = 0; r < RAD; ++r) {
-                const int l = (a == 0 || two) ? wb[a].loc(lane, r) : -1;
+                const int l = (a == 0 || two) ? wb[a].loc(lmask, r) : -1;
                 const int e = l >= 0 ? 3 * (l - (int)cs) : 0;
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
@@ -766,7 +767,7 @@ __global__ __launch_bounds__(256) void k_xfwd_w(XSource src, cx<R>* __restrict__
         for (int a = 0; a < 2; ++a) {
 #pragma unroll
             for (int r = 0; r < RAD; ++r) {
-                const int l = (a == 0 || two) ? wb[a].loc(lane, r) : -1;
+                const int l = (a == 0 || two) ? wb[a].loc(lmask, r) : -1;
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
                     if (src.compact32 != nullptr) {
@@ -843,6 +844,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(xinv_w_min_
     cx<R>* tw = reinterpret_cast<cx<R>*>(smem_raw);
     // the wave index is uniform: the line numbers, support words and run bounds below are too
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
+    const LaneBits64 lmask = lane_bits64(lane);
     cx<R>* L = tw + lpad_size<N>() + wave * lpad_size<N>();
     R* ost = reinterpret_cast<R*>(reinterpret_cast<unsigned char*>(tw + 5 * lpad_size<N>()) +
                                   wave * wave_ostage_bytes<R, N>());
@@ -868,7 +870,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(xinv_w_min_
             if (ok) wb.load(sink.lines, line0 + a);
             if (a == 0) cs = wb.start;
 #pragma unroll
-            for (int r = 0; r < RAD; ++r) loc[a][r] = ok ? wb.loc(lane, r) : -1;
+            for (int r = 0; r < RAD; ++r) loc[a][r] = ok ? wb.loc(lmask, r) : -1;
         }
         // The pair's support voxels are one contiguous run [cs, ce) of the compact vector.  A sink of
         // the transform's precision whose run fits the wave's output stage takes the values there
