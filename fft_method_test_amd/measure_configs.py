@@ -3,6 +3,8 @@
     python -m fft_method_test_amd.measure_configs c1     # 32^3 sphere cell, full K at tol 1e-10, FP64 and mixed
     python -m fft_method_test_amd.measure_configs c2p    # 512^3 Boolean spheres r 8, porosity 0.35 (seed 2002)
     python -m fft_method_test_amd.measure_configs c3     # 256^3 Brinkman, spheres r 6, porosity 0.5 (seed 3001)
+    python -m fft_method_test_amd.measure_configs apply  # FP32 apply of the bench medium, separate vs fused x+y
+    python -m fft_method_test_amd.measure_configs probe  # HBM copy probes at 256^3 and 512^3
 
 c2p solves direction x to tol 1e-6 (at most --maxit iterations) in FP64 and mixed precision and
 reports seconds per iteration and the five-pass apply's HBM rate.  c3 runs the Brinkman iteration
@@ -141,9 +143,55 @@ def c3(args):
     print(json.dumps(out), flush=True)
 
 
+def apply(args):
+    """The FP32 apply of the 256^3 bench medium (radius-8 spheres, porosity 0.6, seed 2001) timed by
+    fftperm_time_apply with the x and y passes pinned to each form (FFTPERM_PASS_PLANE codes: 1
+    separate passes, 2 / 3 fused over z planes with 8 / 16 waves per workgroup), three rounds."""
+    from . import _lib, geometry, permeability as P
+
+    L = _lib.load()
+    n = args.n
+    mask = geometry.boolean_spheres(n, 8.0, 0.6, 2001).reshape(-1)
+    g = P.Grid3D(n, n, n)
+    green = P.GreenOperator(g, 1.0, 2, _lib.PRECISION_MIXED)
+    P.ForceSupport(P.BinaryMedium(g, mask), P.SUPPORT_INTERFACE, green)
+    codes = (1, 2, 3) if args.plane == 0 else (args.plane,)
+    ms = C.c_double()
+    out = dict(config="apply", n=n, reps=args.reps, us_per_apply={str(c): [] for c in codes})
+    for _ in range(3):
+        for code in codes:
+            _lib.check(L.fftperm_pin_variant(green._h, _lib.PASS_PLANE, _lib.PRECISION_MIXED, code))
+            _lib.check(L.fftperm_time_apply(green._h, _lib.PRECISION_MIXED, args.reps, C.byref(ms)))
+            out["us_per_apply"][str(code)].append(ms.value * 1e3)
+    _lib.check(L.fftperm_pin_variant(green._h, _lib.PASS_PLANE, _lib.PRECISION_MIXED, 0))
+    green.close()
+    print(json.dumps(out), flush=True)
+
+
+def probe(args):
+    """fftperm_probe_copy at 256^3 and 512^3 (contiguous, y-pass and z-pass patterns)."""
+    from . import _lib, permeability as P
+
+    L = _lib.load()
+    out = dict(config="probe", runs=[])
+    for n, reps in ((256, 20), (512, 5)):
+        green = P.GreenOperator(P.Grid3D(n, n, n), 1.0, 2, _lib.PRECISION_MIXED)
+        rate = C.c_double()
+        r = dict(n=n, reps=reps)
+        for pattern, key in ((0, "stream_copy_GBs"), (1, "ypass_pattern_copy_GBs"), (2, "zpass_pattern_copy_GBs")):
+            _lib.check(L.fftperm_probe_copy(green._h, pattern, reps, C.byref(rate)))
+            r[key] = rate.value
+        green.close()
+        out["runs"].append(r)
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
-    ap.add_argument("config", choices=["c1", "c2p", "c3"])
+    ap.add_argument("config", choices=["c1", "c2p", "c3", "apply", "probe"])
+    ap.add_argument("--n", type=int, default=256, help="apply: grid edge")
+    ap.add_argument("--plane", type=int, default=0, help="apply: only this FFTPERM_PASS_PLANE code (0: 1, 2 and 3)")
+    ap.add_argument("--reps", type=int, default=20, help="apply: applies per timing")
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--maxit", type=int, default=3000)
     ap.add_argument("--warm", action="store_true", help="c3: an untimed solve first")
@@ -151,7 +199,7 @@ def main():
     from . import _lib
 
     _lib.load()
-    {"c1": c1, "c2p": c2p, "c3": c3}[args.config](args)
+    {"c1": c1, "c2p": c2p, "c3": c3, "apply": apply, "probe": probe}[args.config](args)
 
 
 if __name__ == "__main__":
