@@ -1211,8 +1211,9 @@ MinresOut device_cr_reliable(fftperm_ctx* c, const double* bdev, const double* x
 // ------------------------------------------------------------------ multi-RHS CR on FP32 vectors
 // The three load directions of a tensor iterate in lockstep (src/app/PermeabilityRunner.cpp:186-198
 // solves them one after the other): side b owns CR state rst[b], FP32 vector slot b (at b * vs32),
-// spectrum slot b (components 3b .. 3b + 2 of S32) and partial rows b; every kernel skips a side
-// whose state is done, so each side runs exactly the single-RHS recurrence.
+// partial rows b and, when the passes serve all sides per launch, spectrum slot b (components
+// 3b .. 3b + 2 of S32); every kernel skips a side whose state is done, so each side runs exactly
+// the single-RHS recurrence.
 
 // int stride between the done flags of consecutive CR states
 constexpr int kCrDoneStride = (int)(sizeof(CrState) / sizeof(int));
@@ -1254,9 +1255,12 @@ void chain32_batch(fftperm_ctx* c, int nrhs) {
         return rec;
     };
     if (c->tensor_batch != 2) {
+        // The sides' chains run one after the other and a chain consumes its spectra before the
+        // next starts, so every side uses the first side's spectrum buffer: it stays in the
+        // last-level cache from one chain to the next (FP32 at 256^3: 214 MB).
         for (int b = 0; b < nrhs; ++b) {
             const int* skip = &st[b].done;
-            cx<float>* S = c->S32 + b * spec;
+            cx<float>* S = c->S32;
             fftperm_ctx::ProfRec rec{};
             if (c->prof) rec = prof_begin(0, b);
             const int plane = plane_src_ok(source(b)) && plane_sink_ok(sink(b)) ? plane_on(c) : 0;

@@ -82,6 +82,17 @@ struct Geometry3 {
 inline int spec_row_pitch(int nxh) { return (nxh + 7) / 8 * 8; }
 inline long spec_plane_pitch(int ny, int rp) { return (long)ny * rp + 16; }
 
+// Last-level cache (Infinity Cache) of the device: 256 MiB.  The fused z pass reads the Green
+// tensor with non-temporal loads when a context's three half spectra fit in it but do not fit
+// next to the tensor, so the spectra the pass writes are still cached when the y-inverse pass
+// reads them (FP32 at 256^3: 214 MB of spectra beside 214 MB of tensor; y-inverse 71 -> 58 us).
+constexpr long kLastLevelCacheBytes = 256L << 20;
+template <typename R>
+inline bool zpass_streams_green(const Geometry3& g) {
+    const long spectra = 3 * g.Nc * 2 * (long)sizeof(R), green = 6 * g.Nc * (long)sizeof(R);
+    return spectra <= kLastLevelCacheBytes && spectra + green > kLastLevelCacheBytes;
+}
+
 // The spectrum layout as the x passes see it: x line l = kz * ny + ky starts at kz * pz + ky * rp.
 struct SpecLayout {
     long nc;  // component stride

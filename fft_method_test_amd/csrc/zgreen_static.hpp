@@ -89,7 +89,9 @@ constexpr int zgreen_threads() {
     return OS * 3 * ZL * ZShape<R, N>::T;
 }
 
-template <typename R, int N, int ZL, int OS>
+// STREAM: the Green tensor is read with non-temporal loads, so that it does not displace the
+// spectra from the last-level cache (zpass_streams_green).
+template <typename R, int N, int ZL, int OS, bool STREAM>
 __global__ __launch_bounds__((zgreen_threads<R, N, ZL, OS>())) void k_zgreen_s(cx<R>* __restrict__ S,
                                                                                  const R* __restrict__ G6, long ncols,
                                                                                  long pitch, long Nc, int nx, int rp,
@@ -129,16 +131,21 @@ __global__ __launch_bounds__((zgreen_threads<R, N, ZL, OS>())) void k_zgreen_s(c
     const bool mown = mci < ncl;
     R gt[KM][6];
     {
-        const R* Gc = G6 + 6L * (col0 + mci);
+        const long gc = 6L * (col0 + mci);
 #pragma unroll
         for (int m = 0; m < KM; ++m) {
             const int e = threadIdx.x / ZL + m * MT;
 #pragma unroll
             for (int q = 0; q < 6; ++q) gt[m][q] = (R)0;
             if (mown && e < N) {
-                const R* gp = Gc + 6L * e * pitch;
+                const long o = gc + 6L * e * pitch;
+                if constexpr (STREAM) {
 #pragma unroll
-                for (int q = 0; q < 6; ++q) gt[m][q] = gp[q];
+                    for (int q = 0; q < 6; ++q) gt[m][q] = __builtin_nontemporal_load(G6 + o + q);
+                } else {
+#pragma unroll
+                    for (int q = 0; q < 6; ++q) gt[m][q] = G6[o + q];
+                }
             }
         }
     }
@@ -654,8 +661,13 @@ long launch_zgreen_zl(cx<R>* S, const R* G6, long ncols, const Geometry3& g, con
         const long blocks = (ncols + ZL - 1) / ZL;
         constexpr int threads = zgreen_threads<R, N, ZL, OS>();
         if (zb.nrhs == 1) {
-            zlaunch((k_zgreen_s<R, N, ZL, OS>), dim3((unsigned)blocks), dim3(threads), lds, st, S, G6,
-                               ncols, g.pz, g.Nc, g.nx, g.rp, tw, qpart, skip);
+            // the FP32 operator's tensor is streamed where the spectra fit the last-level cache alone
+            if (sizeof(R) == 4 && zpass_streams_green<R>(g))
+                zlaunch((k_zgreen_s<R, N, ZL, OS, sizeof(R) == 4>), dim3((unsigned)blocks), dim3(threads), lds, st,
+                        S, G6, ncols, g.pz, g.Nc, g.nx, g.rp, tw, qpart, skip);
+            else
+                zlaunch((k_zgreen_s<R, N, ZL, OS, false>), dim3((unsigned)blocks), dim3(threads), lds, st, S, G6,
+                        ncols, g.pz, g.Nc, g.nx, g.rp, tw, qpart, skip);
         } else if constexpr (sizeof(R) == 4) {  // the multi-RHS solve runs the FP32 operator
             zlaunch((k_zgreen_sb<R, N, ZL, OS>), dim3((unsigned)blocks), dim3(threads), lds, st, S, G6,
                                ncols, g.pz, g.Nc, g.nx, g.rp, tw, qpart, skip, zb);

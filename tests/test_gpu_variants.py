@@ -287,6 +287,33 @@ def test_fp32_operator_variants(fp, oracle_mod, n):
     print("fp32 apply errors", n, max(c[2] for c in cases))
 
 
+def test_fp32_operator_streamed_green_tensor(fp, oracle_mod):
+    """A 256 x 256 x 192 grid: its FP32 half spectra (160 MB) fit the 256 MiB last-level cache but
+    not beside the FP32 Green tensor (160 MB), so the single-side z kernel reads the tensor with
+    non-temporal loads.  Both of its tile shapes pinned: the FP32 operator within FP32_APPLY_BOUND
+    of the oracle, and the same values from both tiles."""
+    from fft_method_test_amd import _lib, permeability as P
+
+    shape = (256, 256, 192)
+    green, sup, o = _pair(oracle_mod, shape, seed=19, precision=_lib.PRECISION_MIXED)
+    try:
+        op = P.VariationalOperator(sup, green)
+        x = np.random.default_rng(23).standard_normal(sup.size())
+        ref = o.apply(x)
+        ys = []
+        for code in (82, 41):  # k_zgreen_s: 8 columns x 2 threads, 4 columns x 1 thread
+            with pinned(green, _lib.PASS_ZGREEN, _lib.PRECISION_MIXED, code):
+                y = op.apply_fp32(x)
+            err = rel(y, ref)
+            print(f"[stream-green] {shape} z code {code}: FP32 operator rel err {err:.2e}", flush=True)
+            assert err < FP32_APPLY_BOUND
+            ys.append(y)
+        assert rel(ys[0], ys[1]) < 1e-6
+    finally:
+        green.close()
+        o.close()
+
+
 def test_fp32_operator_rejected_in_fp64_context(fp, oracle_mod):
     from fft_method_test_amd import permeability as P
 
