@@ -43,12 +43,20 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def fp64_bytes_zgreen(Nh, s_c=16, s_r=8):
-    # fused z pass: read + write 3 complex components, read the 6 real Green components
-    return 2 * 3 * Nh * s_c + 6 * Nh * s_r
+def bytes_zgreen(Nh, s_c=16, s_r=8, gfrac=1.0):
+    # fused z pass: read + write 3 complex components, read the 6 real Green components of the
+    # fraction gfrac of the kz planes (FP32 all-in-LDS z kernels read planes kz <= n/2 only and
+    # take the others by mirror symmetry)
+    return 2 * 3 * Nh * s_c + 6 * Nh * s_r * gfrac
 
 
-def bytes_apply(N, Nh, L, s_c=16, s_r=8, nlines=None, s_v=8, plane=False):
+def green_plane_fraction(n, fp32, z_tile):
+    """Fraction of the Green tensor's kz planes a z launch reads: (n // 2 + 1) / n for the FP32
+    all-in-LDS z kernels (tile codes ending in 1 or 2), 1 otherwise."""
+    return (n // 2 + 1) / n if fp32 and z_tile % 10 in (1, 2) else 1.0
+
+
+def bytes_apply(N, Nh, L, s_c=16, s_r=8, nlines=None, s_v=8, plane=False, gfrac=1.0):
     # x-fwd: line bitmask of the support (1 bit per voxel, one int32 start per x line) + compact
     # vector in (s_v bytes per value: FP64, or FP32 for the mixed solver's vectors), 3 half spectra
     # out; y fwd/inv and z: in + out; z adds the Green tensor; x-inv: spectra + bitmask in, compact
@@ -58,7 +66,7 @@ def bytes_apply(N, Nh, L, s_c=16, s_r=8, nlines=None, s_v=8, plane=False):
     bits = N // 8 + 4 * nl
     xf = bits + s_v * L + 3 * Nh * s_c
     yp = 0 if plane else 2 * 3 * Nh * s_c
-    zg = fp64_bytes_zgreen(Nh, s_c, s_r)
+    zg = bytes_zgreen(Nh, s_c, s_r, gfrac)
     xi = 3 * Nh * s_c + bits + s_v * L
     return xf + 2 * yp + zg + xi
 
@@ -126,11 +134,13 @@ def run_precision(args, P, _lib, mask_dev, precision, tol=None, steps=None, warm
 # keyed by (precision, tile code as fftperm_profile_read reports it); the summary with the commands
 # is profiles/passes_256_hbm_traffic.md.  A tile the autotuner picks that was not profiled reports null.
 Z_TRAFFIC_MB = {
-    ("fp64", 41): (856.7, 436.4),  # k_zgreen_s<double, 256, 4, 1>
-    ("fp64", 81): (856.3, 427.8),  # k_zgreen_s<double, 256, 8, 1>
-    ("fp32", 82): (428.2, 216.4),  # k_zgreen_s<float, 256, 8, 2>
-    ("fp32", 161): (428.1, 213.9),  # k_zgreen_s<float, 256, 16, 1>
-    ("fp32", 41): (575.8, 303.7),  # k_zgreen_s<float, 256, 4, 1>
+    ("fp64", 41): (856.6, 436.3),  # k_zgreen_s<double, 256, 4, 1, false>
+    ("fp64", 81): (856.2, 427.8),  # k_zgreen_s<double, 256, 8, 1, false>
+    ("fp64", 44): (856.4, 451.2),  # k_zgreen_r<double, 256, 4>
+    ("fp32", 82): (329.5, 217.3),  # k_zgreen_s<float, 256, 8, 2, true>
+    ("fp32", 81): (353.4, 243.2),  # k_zgreen_s<float, 256, 8, 1, true>
+    ("fp32", 161): (322.0, 213.9),  # k_zgreen_s<float, 256, 16, 1, true>
+    ("fp32", 41): (566.7, 266.7),  # k_zgreen_s<float, 256, 4, 1, true>
 }
 
 
@@ -329,15 +339,16 @@ def main():
     # reads the Green tensor once
     z_n = max(1, pr["z_n"][pi])
     sides_per_launch = pr["z_rhs"][pi] / z_n
-    z_bytes = sides_per_launch * 2 * 3 * Nh * sc + 6 * Nh * sr
+    z_tile = pr["tiles"]["zgreen"][pi]
+    gfrac = green_plane_fraction(args.n, pi == 1, z_tile)
+    z_bytes = sides_per_launch * 2 * 3 * Nh * sc + 6 * Nh * sr * gfrac
     z_ms = pr["z_ms"][pi] / z_n
     achieved = z_bytes / (z_ms * 1e-3) / 1e9
-    z_tile = pr["tiles"]["zgreen"][pi]
     tr = Z_TRAFFIC_MB.get(("fp32" if pi == 1 else "fp64", z_tile)) if sides_per_launch == 1 else None
     traffic = None if tr is None else (tr[0] + tr[1]) * 1e6
     L = 3 * head["nb"]
     plane = pi == 1 and pr["tiles"]["xfwd"][pi] == 1000
-    a_bytes = bytes_apply(N, Nh, L, sc, sr, s_v=4 if pi == 1 else 8, plane=plane)
+    a_bytes = bytes_apply(N, Nh, L, sc, sr, s_v=4 if pi == 1 else 8, plane=plane, gfrac=gfrac)
     a_ms = pr["a_ms"][pi] / max(1, pr["a_rhs"][pi])  # per right-hand side
 
     out = {

@@ -1553,7 +1553,7 @@ void tune_x_passes(fftperm_ctx* c) {
     const bool need32 = c->precision == FFTPERM_PRECISION_MIXED && untuned(1);
     if (!need64 && !need32) return;
     c->ensure_vectors();
-    HIPCHK(hipMemsetAsync(c->x, 0, c->L3() * sizeof(double), c->s));
+    launch_fill_pattern(c->x, c->L3(), c->s);
     XSource sr = src_none(c);  // f0 = 0: the source of every MINRES apply
     sr.compact = c->x;
     XSink sk = sink_none(c);
@@ -1567,6 +1567,7 @@ void tune_x_passes(fftperm_ctx* c) {
         auto t = autotune_x<float>(sr, sk, c->S32, g, c->t32, c->s);
         c->tune32.insert(c->tune32.end(), t.begin(), t.end());
     }
+    HIPCHK(hipMemsetAsync(c->x, 0, c->L3() * sizeof(double), c->s));
     c->sync();
 }
 
@@ -1580,8 +1581,8 @@ std::mutex g_chain_mu;
 std::set<std::tuple<int, int, int, int>> g_chain_tuned;
 
 // Milliseconds per five-sweep apply of the solver's form (FP32 compact source and sink for the FP32
-// operator, FP64 for FP64; Parseval partials on) over reps back-to-back applies on a zero input,
-// after one untimed apply.
+// operator, FP64 for FP64; Parseval partials on) over reps back-to-back applies of a patterned
+// input (launch_fill_pattern), after one untimed apply.
 template <typename R>
 float time_chain(fftperm_ctx* c, int reps) {
     const long L = c->L3();
@@ -1591,11 +1592,11 @@ float time_chain(fftperm_ctx* c, int reps) {
     sk.partials = c->part_x;
     if (sizeof(R) == 4) {
         c->ensure_vectors32();
-        HIPCHK(hipMemsetAsync(c->x32, 0, L * sizeof(float), c->s));
+        launch_fill_pattern(c->x32, L, c->s);
         sr.compact32 = c->x32;
         sk.compact32 = c->q32;
     } else {
-        HIPCHK(hipMemsetAsync(c->x, 0, L * sizeof(double), c->s));
+        launch_fill_pattern(c->x, L, c->s);
         sr.compact = c->x;
         sk.compact = c->tmp;
     }
@@ -1604,6 +1605,11 @@ float time_chain(fftperm_ctx* c, int reps) {
     HIPCHK(hipEventRecord(a, c->s));
     for (int rep = 0; rep < reps; ++rep) c->chain<R>(sr, sk, nullptr, c->part_z);
     HIPCHK(hipEventRecord(b, c->s));
+    // the solvers start from a zero iterate
+    if (sizeof(R) == 4)
+        HIPCHK(hipMemsetAsync(c->x32, 0, L * sizeof(float), c->s));
+    else
+        HIPCHK(hipMemsetAsync(c->x, 0, L * sizeof(double), c->s));
     HIPCHK(hipEventSynchronize(b));
     float ms = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, a, b));

@@ -84,12 +84,14 @@ inline long spec_plane_pitch(int ny, int rp) { return (long)ny * rp + 16; }
 
 // Last-level cache (Infinity Cache) of the device: 256 MiB.  The fused z pass reads the Green
 // tensor with non-temporal loads when a context's three half spectra fit in it but do not fit
-// next to the tensor, so the spectra the pass writes are still cached when the y-inverse pass
-// reads them (FP32 at 256^3: 214 MB of spectra beside 214 MB of tensor; y-inverse 71 -> 58 us).
+// next to the tensor it reads (FP32: half of the planes, the others by mirror symmetry), so that
+// the spectra the pass writes are still cached when the y-inverse pass reads them (FP32 at 256^3:
+// 214 MB of spectra beside 107 MB of tensor).
 constexpr long kLastLevelCacheBytes = 256L << 20;
 template <typename R>
 inline bool zpass_streams_green(const Geometry3& g) {
-    const long spectra = 3 * g.Nc * 2 * (long)sizeof(R), green = 6 * g.Nc * (long)sizeof(R);
+    const long spectra = 3 * g.Nc * 2 * (long)sizeof(R);
+    const long green = (sizeof(R) == 4 ? 3 : 6) * g.Nc * (long)sizeof(R);
     return spectra <= kLastLevelCacheBytes && spectra + green > kLastLevelCacheBytes;
 }
 

@@ -853,4 +853,28 @@ void launch_to_f32(const double* a, float* out, long n, hipStream_t s) {
     hipLaunchKernelGGL(k_to_f32, dim3(red_grid(n)), dim3(kRedThreads), 0, s, a, out, n);
 }
 
+// Values in [-1, 1) from a 32-bit hash of the index (the MurmurHash3 finaliser): a deterministic
+// input for the timing applies.  A zero input would time the passes on zero spectra, which run
+// faster than the solver's data (at 256^3, ~10 us per column pass).
+template <typename T>
+__global__ void k_fill_pattern(T* __restrict__ p, long n) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        uint32_t h = (uint32_t)i * 0x9E3779B1u;
+        h ^= h >> 16;
+        h *= 0x85EBCA6Bu;
+        h ^= h >> 13;
+        h *= 0xC2B2AE35u;
+        h ^= h >> 16;
+        p[i] = (T)((double)h * (2.0 / 4294967296.0) - 1.0);
+    }
+}
+
+void launch_fill_pattern(float* p, long n, hipStream_t s) {
+    hipLaunchKernelGGL(k_fill_pattern<float>, dim3(red_grid(n)), dim3(kRedThreads), 0, s, p, n);
+}
+
+void launch_fill_pattern(double* p, long n, hipStream_t s) {
+    hipLaunchKernelGGL(k_fill_pattern<double>, dim3(red_grid(n)), dim3(kRedThreads), 0, s, p, n);
+}
+
 }  // namespace fftperm

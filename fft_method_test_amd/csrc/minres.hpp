@@ -144,6 +144,9 @@ void launch_cr32_update_n(float* x, float* r, const float* p, const float* ap, C
 void launch_accumulate_f32(double* xacc, float* x32, long n, hipStream_t s);
 // out = (float) a
 void launch_to_f32(const double* a, float* out, long n, hipStream_t s);
+// deterministic values in [-1, 1) (the input of the timing applies)
+void launch_fill_pattern(float* p, long n, hipStream_t s);
+void launch_fill_pattern(double* p, long n, hipStream_t s);
 
 // Generic helpers
 void launch_dot_partials(const double* a, const double* b, long n, double* partials, hipStream_t s);

@@ -89,6 +89,69 @@ constexpr int zgreen_threads() {
     return OS * 3 * ZL * ZShape<R, N>::T;
 }
 
+// Green planes a z column reads: FP32 tensors are read for the planes kz <= N / 2 only, and plane
+// N - kz takes plane kz with the xz and yz components negated (G(kx, ky, -kz) = Rz G(kx, ky, kz) Rz),
+// so a launch moves half of the tensor and each value is read once.  The stored G_sym obeys that
+// mirror to 2e-8 .. 8e-8 of max |G| (the truncated image sum is symmetric under the full inversion
+// only), below the FP32 rounding of the stored values; FP64 reads every plane.
+template <typename R, int N>
+constexpr bool zgreen_mirror() {
+    return sizeof(R) == 4;
+}
+template <typename R, int N>
+constexpr int zgreen_planes() {
+    return zgreen_mirror<R, N>() ? N / 2 + 1 : N;
+}
+
+// The six Green components of plane p of a z column whose first tensor element is at gc.
+// STREAM: non-temporal loads.
+template <typename R, bool STREAM>
+__device__ __forceinline__ void load_green_plane(const R* __restrict__ G6, long gc, int p, long pitch, R (&g)[6]) {
+    const R* gp = G6 + gc + 6L * p * pitch;
+    if constexpr (STREAM) {
+#pragma unroll
+        for (int q = 0; q < 6; ++q) g[q] = __builtin_nontemporal_load(gp + q);
+    } else {
+#pragma unroll
+        for (int q = 0; q < 6; ++q) g[q] = gp[q];
+    }
+}
+
+// Symmetric Green multiply (src/bignonnet/GreenOperator.cpp:84-92 with G_sym) of frequency e of
+// the column whose components are at bx, by, bz (LDS), with the tensor t (xz and yz negated when
+// flip: the mirrored plane), and the quadratic form conj(f) . G f of the forward spectrum added to
+// q when asked for.
+template <typename R>
+__device__ __forceinline__ void green_step(cx<R>* bx, cx<R>* by, cx<R>* bz, int e, const R (&t)[6], bool flip,
+                                           bool want_q, double& q) {
+    const R txz = flip ? -t[2] : t[2], tyz = flip ? -t[4] : t[4];
+    const cx<R> fx = bx[e], fy = by[e], fz = bz[e];
+    cx<R> gx, gy, gz;
+    green_mul<R>(t[0], t[1], txz, t[3], tyz, t[5], fx, fy, fz, gx, gy, gz);
+    bx[e] = gx;
+    by[e] = gy;
+    bz[e] = gz;
+    if (want_q) green_quad_add<R>(q, 1.0, fx, fy, fz, gx, gy, gz);
+}
+
+// The Green step of a thread's planes p = threadIdx.x / ZL + m * MT (m < KP) of column mci: the
+// frequency p and, with the mirror, N - p.
+template <typename R, int N, int ZL, int MT, int KP>
+__device__ __forceinline__ void green_tile_step(cx<R>* buf, int LD, int mci, bool mown, const R (&gt)[KP][6],
+                                                bool want_q, double& q) {
+    constexpr int NP = zgreen_planes<R, N>();
+    cx<R>* bx = buf + (0 * ZL + mci) * LD;
+    cx<R>* by = buf + (1 * ZL + mci) * LD;
+    cx<R>* bz = buf + (2 * ZL + mci) * LD;
+#pragma unroll
+    for (int m = 0; m < KP; ++m) {
+        const int p = threadIdx.x / ZL + m * MT;
+        if (!mown || p >= NP) continue;
+        green_step<R>(bx, by, bz, p, gt[m], false, want_q, q);
+        if (zgreen_mirror<R, N>() && p != 0 && 2 * p != N) green_step<R>(bx, by, bz, N - p, gt[m], true, want_q, q);
+    }
+}
+
 // STREAM: the Green tensor is read with non-temporal loads, so that it does not displace the
 // spectra from the last-level cache (zpass_streams_green).
 template <typename R, int N, int ZL, int OS, bool STREAM>
@@ -126,27 +189,19 @@ __global__ __launch_bounds__((zgreen_threads<R, N, ZL, OS>())) void k_zgreen_s(c
     // the Green tensor of the frequencies this thread multiplies, fetched now so that its latency
     // overlaps the column loads and the forward transform
     constexpr int MT = THREADS / ZL;
-    constexpr int KM = (N + MT - 1) / MT;
+    constexpr int NP = zgreen_planes<R, N>();
+    constexpr int KP = (NP + MT - 1) / MT;
     const int mci = threadIdx.x % ZL;
     const bool mown = mci < ncl;
-    R gt[KM][6];
+    R gt[KP][6];
     {
         const long gc = 6L * (col0 + mci);
 #pragma unroll
-        for (int m = 0; m < KM; ++m) {
-            const int e = threadIdx.x / ZL + m * MT;
+        for (int m = 0; m < KP; ++m) {
+            const int p = threadIdx.x / ZL + m * MT;
 #pragma unroll
             for (int q = 0; q < 6; ++q) gt[m][q] = (R)0;
-            if (mown && e < N) {
-                const long o = gc + 6L * e * pitch;
-                if constexpr (STREAM) {
-#pragma unroll
-                    for (int q = 0; q < 6; ++q) gt[m][q] = __builtin_nontemporal_load(G6 + o + q);
-                } else {
-#pragma unroll
-                    for (int q = 0; q < 6; ++q) gt[m][q] = G6[o + q];
-                }
-            }
+            if (mown && p < NP) load_green_plane<R, STREAM>(G6, gc, p, pitch, gt[m]);
         }
     }
 #pragma unroll
@@ -156,28 +211,10 @@ __global__ __launch_bounds__((zgreen_threads<R, N, ZL, OS>())) void k_zgreen_s(c
     }
     __syncthreads();
     sfft_lines<R, -1, N, L3, SH::T, LD>(buf, tw, L3);
-    // symmetric Green multiply (src/bignonnet/GreenOperator.cpp:84-92 with G_sym), and the
-    // quadratic form conj(f) . G f of the forward spectrum when asked for
+    // symmetric Green multiply and the quadratic form of the forward spectrum when asked for
     {
         double q[1] = {0.0};
-#pragma unroll
-        for (int m = 0; m < KM; ++m) {
-            const int e = threadIdx.x / ZL + m * MT;
-            if (!mown || e >= N) continue;
-            const R txx = gt[m][0], txy = gt[m][1], txz = gt[m][2];
-            const R tyy = gt[m][3], tyz = gt[m][4], tzz = gt[m][5];
-            cx<R>* px = buf + (0 * ZL + mci) * LD + e;
-            cx<R>* py = buf + (1 * ZL + mci) * LD + e;
-            cx<R>* pz = buf + (2 * ZL + mci) * LD + e;
-            const cx<R> fx = *px, fy = *py, fz = *pz;
-            cx<R> gx, gy, gz;
-            green_mul<R>(txx, txy, txz, tyy, tyz, tzz, fx, fy, fz, gx, gy, gz);
-            *px = gx;
-            *py = gy;
-            *pz = gz;
-            if (qpart != nullptr)
-                green_quad_add<R>(q[0], 1.0, fx, fy, fz, gx, gy, gz);
-        }
+        green_tile_step<R, N, ZL, MT, KP>(buf, LD, mci, mown, gt, qpart != nullptr, q[0]);
         if (qpart != nullptr) {
             q[0] *= herm_weight((int)((col0 + mci) % rp), nx);
             block_sum<1>(q, qpart + blockIdx.x);
@@ -220,13 +257,14 @@ __global__ __launch_bounds__((zgreen_threads<R, N, ZL, OS>())) void k_zgreen_sb(
     constexpr int KL = (N + EST - 1) / EST;  // a thread's last slot idles when EST does not divide N
     constexpr bool EXACT = N % EST == 0;
     constexpr int MT = THREADS / ZL;
-    constexpr int KM = (N + MT - 1) / MT;
+    constexpr int NP = zgreen_planes<R, N>();
+    constexpr int KP = (NP + MT - 1) / MT;
     const int mci = threadIdx.x % ZL;
     const bool mown = mci < ncl;
     // the Green tensor of the frequencies this thread multiplies stays in registers for every
     // right-hand side of the launch: one read of G per launch
     bool gloaded = false;
-    R gt[KM][6];
+    R gt[KP][6];
     for (int b = 0; b < zb.nrhs; ++b) {
         if (skip != nullptr && skip[b * zb.skip_stride]) continue;
         cx<R>* Sb = S + b * zb.rhs_stride;
@@ -241,17 +279,13 @@ __global__ __launch_bounds__((zgreen_threads<R, N, ZL, OS>())) void k_zgreen_sb(
         if (!gloaded) {
             // fetched after the first side's column loads are issued, so that its latency overlaps
             // them and the forward transform; the twiddles go to LDS meanwhile
-            const R* Gc = G6 + 6L * (col0 + mci);
+            const long gc = 6L * (col0 + mci);
 #pragma unroll
-            for (int m = 0; m < KM; ++m) {
-                const int e = threadIdx.x / ZL + m * MT;
+            for (int m = 0; m < KP; ++m) {
+                const int p = threadIdx.x / ZL + m * MT;
 #pragma unroll
                 for (int q = 0; q < 6; ++q) gt[m][q] = (R)0;
-                if (mown && e < N) {
-                    const R* gp = Gc + 6L * e * pitch;
-#pragma unroll
-                    for (int q = 0; q < 6; ++q) gt[m][q] = gp[q];
-                }
+                if (mown && p < NP) load_green_plane<R, false>(G6, gc, p, pitch, gt[m]);
             }
             for (int i = threadIdx.x; i < N; i += THREADS) tw[i] = twg[i];
             gloaded = true;
@@ -265,28 +299,10 @@ __global__ __launch_bounds__((zgreen_threads<R, N, ZL, OS>())) void k_zgreen_sb(
         }
         __syncthreads();
         sfft_lines<R, -1, N, L3, SH::T, LD>(buf, tw, L3);
-        // symmetric Green multiply (src/bignonnet/GreenOperator.cpp:84-92 with G_sym), and the
-        // quadratic form conj(f) . G f of the forward spectrum when asked for
+        // symmetric Green multiply and the quadratic form of the forward spectrum when asked for
         {
             double q[1] = {0.0};
-#pragma unroll
-            for (int m = 0; m < KM; ++m) {
-                const int e = threadIdx.x / ZL + m * MT;
-                if (!mown || e >= N) continue;
-                const R txx = gt[m][0], txy = gt[m][1], txz = gt[m][2];
-                const R tyy = gt[m][3], tyz = gt[m][4], tzz = gt[m][5];
-                cx<R>* px = buf + (0 * ZL + mci) * LD + e;
-                cx<R>* py = buf + (1 * ZL + mci) * LD + e;
-                cx<R>* pz = buf + (2 * ZL + mci) * LD + e;
-                const cx<R> fx = *px, fy = *py, fz = *pz;
-                cx<R> gx, gy, gz;
-                green_mul<R>(txx, txy, txz, tyy, tyz, tzz, fx, fy, fz, gx, gy, gz);
-                *px = gx;
-                *py = gy;
-                *pz = gz;
-                if (qpart != nullptr)
-                    green_quad_add<R>(q[0], 1.0, fx, fy, fz, gx, gy, gz);
-            }
+            green_tile_step<R, N, ZL, MT, KP>(buf, LD, mci, mown, gt, qpart != nullptr, q[0]);
             if (qpart != nullptr) {
                 q[0] *= herm_weight((int)((col0 + mci) % rp), nx);
                 block_sum<1>(q, qpart + b * zb.qstride + blockIdx.x);

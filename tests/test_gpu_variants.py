@@ -288,13 +288,14 @@ def test_fp32_operator_variants(fp, oracle_mod, n):
 
 
 def test_fp32_operator_streamed_green_tensor(fp, oracle_mod):
-    """A 256 x 256 x 192 grid: its FP32 half spectra (160 MB) fit the 256 MiB last-level cache but
-    not beside the FP32 Green tensor (160 MB), so the single-side z kernel reads the tensor with
-    non-temporal loads.  Both of its tile shapes pinned: the FP32 operator within FP32_APPLY_BOUND
-    of the oracle, and the same values from both tiles."""
+    """A 256 x 256 x 224 grid: its FP32 half spectra (187 MB) fit the 256 MiB last-level cache but
+    not beside the half of the FP32 Green tensor that the z pass reads (94 MB; the other planes by
+    mirror symmetry), so the single-side z kernel reads the tensor with non-temporal loads.  Both
+    of its tile shapes pinned: the FP32 operator within FP32_APPLY_BOUND of the oracle, and the
+    same values from both tiles."""
     from fft_method_test_amd import _lib, permeability as P
 
-    shape = (256, 256, 192)
+    shape = (256, 256, 224)
     green, sup, o = _pair(oracle_mod, shape, seed=19, precision=_lib.PRECISION_MIXED)
     try:
         op = P.VariationalOperator(sup, green)
