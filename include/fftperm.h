@@ -298,10 +298,11 @@ int fftperm_pin_variant(fftperm_ctx* ctx, int kind, int precision, int code);
 int fftperm_set_graphs(fftperm_ctx* ctx, int enable);
 
 /* Device time of the operator apply (no counterpart in the reference; a measurement hook): reps
- * back-to-back five-sweep applies of the solver's form on a zero input held on the device (FP32
- * compact vectors for precision FFTPERM_PRECISION_MIXED, which needs a MIXED context; FP64
- * otherwise), after one untimed apply; *ms_per_apply = HIP-event time / reps.  Needs a medium with
- * a non-empty support; reps < 1 -> FFTPERM_EINVAL. */
+ * back-to-back five-sweep applies of the solver's form on a patterned input held on the device
+ * (values in [-1, 1) from a hash of the index; FP32 compact vectors for precision
+ * FFTPERM_PRECISION_MIXED, which needs a MIXED context; FP64 otherwise), after one untimed apply;
+ * *ms_per_apply = HIP-event time / reps.  The context's iterate is zero again afterwards.  Needs a
+ * medium with a non-empty support; reps < 1 -> FFTPERM_EINVAL. */
 int fftperm_time_apply(fftperm_ctx* ctx, int precision, int reps, double* ms_per_apply);
 
 int fftperm_profile(fftperm_ctx* ctx, int enable);
