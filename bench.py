@@ -388,6 +388,10 @@ def main():
                                f"rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of z tile {z_tile}: {tr[0]} MB read + "
                                f"{tr[1]} MB written per launch (profiles/passes_256_hbm_traffic.md)"),
             "algorithmic_bytes_per_launch": z_bytes,
+            "green_planes_fraction": gfrac,
+            # the rate the same launch time would give with every Green plane read (the measure of
+            # the pass before the FP32 kernels read the mirrored planes once): context, not `achieved`
+            "full_tensor_equivalent_GBs": (sides_per_launch * 2 * 3 * Nh * sc + 6 * Nh * sr) / (z_ms * 1e-3) / 1e9,
             "sides_per_launch": sides_per_launch,
             "avg_launch_ms": z_ms,
             "launches": pr["z_n"][pi],
