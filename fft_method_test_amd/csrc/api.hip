@@ -1040,9 +1040,10 @@ void cr32_iteration(fftperm_ctx* c) {
     sk.partials = c->part_x;
     c->chain<R>(sr, sk, &st->done, c->part_z);
     launch_cr_beta(c->part_x, (int)c->xparts, c->part_z, (int)c->zparts, st, inv_nb, 1.0 / (double)c->g.N, c->s);
-    launch_cr32_direction(c->p32, c->ap32, c->r32, c->q32, st, nb, c->part, c->s);
+    const bool stream = cr32_streams(nb, 1, 3 * c->g.Nc * (long)sizeof(float2));
+    launch_cr32_direction(c->p32, c->ap32, c->r32, c->q32, st, nb, c->part, c->s, stream);
     launch_cr_alpha(c->part, st, c->s);
-    launch_cr32_update(c->x32, c->r32, c->p32, c->ap32, st, nb, c->part, c->s);
+    launch_cr32_update(c->x32, c->r32, c->p32, c->ap32, st, nb, c->part, c->s, stream);
     launch_cr_check(c->part, st, c->hist, inv_nb, c->s);
 }
 
@@ -1311,9 +1312,13 @@ void cr32_batch_iteration(fftperm_ctx* c, int nrhs) {
     chain32_batch(c, nrhs);
     launch_cr_beta_n(c->part_x, (int)c->xparts, cr_xstride(c), c->part_z, (int)c->zparts, cr_zstride(c), c->rst, nrhs,
                      inv_nb, 1.0 / (double)c->g.N, c->s);
-    launch_cr32_direction_n(c->p32, c->ap32, c->r32, c->q32, c->rst, nb, c->vs32, c->part, pstride, nrhs, c->s);
+    // tensor-batch mode 2 keeps a spectrum per side
+    const long spectra = 3 * c->g.Nc * (long)sizeof(float2) * (c->tensor_batch == 2 ? nrhs : 1);
+    const bool stream = cr32_streams(nb, nrhs, spectra);
+    launch_cr32_direction_n(c->p32, c->ap32, c->r32, c->q32, c->rst, nb, c->vs32, c->part, pstride, nrhs, c->s,
+                            stream);
     launch_cr_alpha_n(c->part, pstride, c->rst, nrhs, c->s);
-    launch_cr32_update_n(c->x32, c->r32, c->p32, c->ap32, c->rst, nb, c->vs32, c->part, pstride, nrhs, c->s);
+    launch_cr32_update_n(c->x32, c->r32, c->p32, c->ap32, c->rst, nb, c->vs32, c->part, pstride, nrhs, c->s, stream);
     launch_cr_check_n(c->part, pstride, c->rst, nullptr, 0, nrhs, inv_nb, c->s);
 }
 

@@ -8,6 +8,10 @@
 
 namespace fftperm {
 
+// Last-level cache (Infinity Cache) of the device: 256 MiB.  Passes whose data would not stay
+// cached beside the spectra of the next pass read or write it with non-temporal accesses.
+constexpr long kLastLevelCacheBytes = 256L << 20;
+
 // Complex values are stored interleaved (re, im) as HIP's double2 / float2.
 template <typename R> struct CxOf;
 template <> struct CxOf<double> { using T = double2; };

@@ -82,12 +82,10 @@ struct Geometry3 {
 inline int spec_row_pitch(int nxh) { return (nxh + 7) / 8 * 8; }
 inline long spec_plane_pitch(int ny, int rp) { return (long)ny * rp + 16; }
 
-// Last-level cache (Infinity Cache) of the device: 256 MiB.  The fused z pass reads the Green
-// tensor with non-temporal loads when a context's three half spectra fit in it but do not fit
-// next to the tensor it reads (FP32: half of the planes, the others by mirror symmetry), so that
-// the spectra the pass writes are still cached when the y-inverse pass reads them (FP32 at 256^3:
-// 214 MB of spectra beside 107 MB of tensor).
-constexpr long kLastLevelCacheBytes = 256L << 20;
+// The fused z pass reads the Green tensor with non-temporal loads when a context's three half
+// spectra fit the last-level cache but do not fit next to the tensor it reads (FP32: half of the
+// planes, the others by mirror symmetry), so that the spectra the pass writes are still cached
+// when the y-inverse pass reads them (FP32 at 256^3: 214 MB of spectra beside 107 MB of tensor).
 template <typename R>
 inline bool zpass_streams_green(const Geometry3& g) {
     const long spectra = 3 * g.Nc * 2 * (long)sizeof(R);

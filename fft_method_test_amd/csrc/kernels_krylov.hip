@@ -703,9 +703,32 @@ void launch_cr_check_n(const double* partials, long pstride, CrState* st, double
 
 // ------------------------------------------------------------ CR on FP32 vectors
 // FP32 vector kernels move four consecutive elements per thread and step (one 16-byte load per
-// array); element i has component i % 3.  n4 = n / 4 full quads, then the n % 4 tail.
+// array); element i has component i % 3.  n4 = n / 4 full quads, then the n % 4 tail.  NT: the
+// quads are loaded and stored with non-temporal accesses (the vectors then pass through the
+// last-level cache without displacing the spectra of the next apply).
 constexpr int kVecThreads = 256;
 
+typedef float f32x4v __attribute__((ext_vector_type(4)));
+template <bool NT>
+__device__ __forceinline__ float4 ld_quad(const float* a, long v) {
+    if constexpr (NT) {
+        const f32x4v t = __builtin_nontemporal_load(reinterpret_cast<const f32x4v*>(a) + v);
+        return make_float4(t.x, t.y, t.z, t.w);
+    } else {
+        return reinterpret_cast<const float4*>(a)[v];
+    }
+}
+template <bool NT>
+__device__ __forceinline__ void st_quad(float* a, long v, float4 q) {
+    if constexpr (NT) {
+        const f32x4v t = {q.x, q.y, q.z, q.w};
+        __builtin_nontemporal_store(t, reinterpret_cast<f32x4v*>(a) + v);
+    } else {
+        reinterpret_cast<float4*>(a)[v] = q;
+    }
+}
+
+template <bool NT>
 __global__ __launch_bounds__(kVecThreads) void k_cr32_direction(float* __restrict__ p, float* __restrict__ ap,
                                                                 const float* __restrict__ r,
                                                                 const float* __restrict__ q, CrState* st, long n,
@@ -727,10 +750,10 @@ __global__ __launch_bounds__(kVecThreads) void k_cr32_direction(float* __restric
     const long n4 = n / 4;
     const long stride = (long)gridDim.x * blockDim.x;
     for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < n4; v += stride) {
-        const float4 rv = reinterpret_cast<const float4*>(r)[v];
-        const float4 qv = reinterpret_cast<const float4*>(q)[v];
-        float4 pv = reinterpret_cast<float4*>(p)[v];
-        float4 av = reinterpret_cast<float4*>(ap)[v];
+        const float4 rv = ld_quad<NT>(r, v);
+        const float4 qv = ld_quad<NT>(q, v);
+        float4 pv = ld_quad<NT>(p, v);
+        float4 av = ld_quad<NT>(ap, v);
         const int c0 = (int)((4 * v) % 3);
         float* pe = &pv.x;
         float* ae = &av.x;
@@ -743,8 +766,8 @@ __global__ __launch_bounds__(kVecThreads) void k_cr32_direction(float* __restric
             ae[j] = fresh ? (float)((double)qe[j] - m) : (float)(((double)qe[j] - m) + b * (double)ae[j]);
             acc[0] += (double)ae[j] * (double)ae[j];
         }
-        reinterpret_cast<float4*>(p)[v] = pv;
-        reinterpret_cast<float4*>(ap)[v] = av;
+        st_quad<NT>(p, v, pv);
+        st_quad<NT>(ap, v, av);
     }
     for (long i = 4 * n4 + (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         const double m = m3[i % 3];
@@ -757,16 +780,21 @@ __global__ __launch_bounds__(kVecThreads) void k_cr32_direction(float* __restric
 }
 
 void launch_cr32_direction(float* p, float* ap, const float* r, const float* q, CrState* st, long nb,
-                           double* partials, hipStream_t s) {
-    launch_cr32_direction_n(p, ap, r, q, st, nb, 0, partials, 0, 1, s);
+                           double* partials, hipStream_t s, bool stream) {
+    launch_cr32_direction_n(p, ap, r, q, st, nb, 0, partials, 0, 1, s, stream);
 }
 
 void launch_cr32_direction_n(float* p, float* ap, const float* r, const float* q, CrState* st, long nb, long vstride,
-                             double* partials, long pstride, int nrhs, hipStream_t s) {
-    hipLaunchKernelGGL(k_cr32_direction, dim3(kRedBlocks, nrhs), dim3(kVecThreads), 0, s, p, ap, r, q, st, 3 * nb,
-                       vstride, partials, pstride);
+                             double* partials, long pstride, int nrhs, hipStream_t s, bool stream) {
+    if (stream)
+        hipLaunchKernelGGL(k_cr32_direction<true>, dim3(kRedBlocks, nrhs), dim3(kVecThreads), 0, s, p, ap, r, q, st,
+                           3 * nb, vstride, partials, pstride);
+    else
+        hipLaunchKernelGGL(k_cr32_direction<false>, dim3(kRedBlocks, nrhs), dim3(kVecThreads), 0, s, p, ap, r, q, st,
+                           3 * nb, vstride, partials, pstride);
 }
 
+template <bool NT>
 __global__ __launch_bounds__(kVecThreads) void k_cr32_update(float* __restrict__ x, float* __restrict__ r,
                                                              const float* __restrict__ p,
                                                              const float* __restrict__ ap, CrState* st, long n,
@@ -786,10 +814,10 @@ __global__ __launch_bounds__(kVecThreads) void k_cr32_update(float* __restrict__
     const long n4 = n / 4;
     const long stride = (long)gridDim.x * blockDim.x;
     for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < n4; v += stride) {
-        float4 xv = reinterpret_cast<float4*>(x)[v];
-        float4 rv = reinterpret_cast<float4*>(r)[v];
-        const float4 pv = reinterpret_cast<const float4*>(p)[v];
-        const float4 av = reinterpret_cast<const float4*>(ap)[v];
+        float4 xv = ld_quad<NT>(x, v);
+        float4 rv = ld_quad<NT>(r, v);
+        const float4 pv = ld_quad<NT>(p, v);
+        const float4 av = ld_quad<NT>(ap, v);
         const int c0 = (int)((4 * v) % 3);
         float* xe = &xv.x;
         float* re = &rv.x;
@@ -805,8 +833,8 @@ __global__ __launch_bounds__(kVecThreads) void k_cr32_update(float* __restrict__
             sum[1] += c == 1 ? (double)re[j] : 0.0;
             sum[2] += c == 2 ? (double)re[j] : 0.0;
         }
-        reinterpret_cast<float4*>(x)[v] = xv;
-        reinterpret_cast<float4*>(r)[v] = rv;
+        st_quad<NT>(x, v, xv);
+        st_quad<NT>(r, v, rv);
     }
     for (long i = 4 * n4 + (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         x[i] = (float)((double)x[i] + a * (double)p[i]);
@@ -823,14 +851,18 @@ __global__ __launch_bounds__(kVecThreads) void k_cr32_update(float* __restrict__
 }
 
 void launch_cr32_update(float* x, float* r, const float* p, const float* ap, CrState* st, long nb, double* partials,
-                        hipStream_t s) {
-    launch_cr32_update_n(x, r, p, ap, st, nb, 0, partials, 0, 1, s);
+                        hipStream_t s, bool stream) {
+    launch_cr32_update_n(x, r, p, ap, st, nb, 0, partials, 0, 1, s, stream);
 }
 
 void launch_cr32_update_n(float* x, float* r, const float* p, const float* ap, CrState* st, long nb, long vstride,
-                          double* partials, long pstride, int nrhs, hipStream_t s) {
-    hipLaunchKernelGGL(k_cr32_update, dim3(kRedBlocks, nrhs), dim3(kVecThreads), 0, s, x, r, p, ap, st, 3 * nb,
-                       vstride, partials, pstride);
+                          double* partials, long pstride, int nrhs, hipStream_t s, bool stream) {
+    if (stream)
+        hipLaunchKernelGGL(k_cr32_update<true>, dim3(kRedBlocks, nrhs), dim3(kVecThreads), 0, s, x, r, p, ap, st,
+                           3 * nb, vstride, partials, pstride);
+    else
+        hipLaunchKernelGGL(k_cr32_update<false>, dim3(kRedBlocks, nrhs), dim3(kVecThreads), 0, s, x, r, p, ap, st,
+                           3 * nb, vstride, partials, pstride);
 }
 
 __global__ void k_accumulate_f32(double* __restrict__ xacc, float* __restrict__ x32, long n) {

@@ -123,11 +123,12 @@ void launch_cr_update(double* x, double* r, const double* p, const double* ap, C
 void launch_cr_check(const double* partials, CrState* st, double* history, double inv_nb, hipStream_t s);
 
 // The same CR steps on FP32 Krylov vectors (x, r, p, Ap and the apply output in FP32; scalars,
-// reductions and the arithmetic of each element in FP64).
+// reductions and the arithmetic of each element in FP64).  stream: non-temporal vector accesses
+// (cr32_streams).
 void launch_cr32_direction(float* p, float* ap, const float* r, const float* q, CrState* st, long nb,
-                           double* partials, hipStream_t s);
+                           double* partials, hipStream_t s, bool stream = false);
 void launch_cr32_update(float* x, float* r, const float* p, const float* ap, CrState* st, long nb, double* partials,
-                        hipStream_t s);
+                        hipStream_t s, bool stream = false);
 // Batched forms over nrhs right-hand sides: RHS b uses st[b], the vectors at b * vstride (a
 // multiple of 4 elements), the partial rows at b * pstride / xstride / zstride and the history at
 // b * hstride.  The single-RHS launchers above are the nrhs = 1 case.
@@ -137,9 +138,17 @@ void launch_cr_alpha_n(const double* partials, long pstride, CrState* st, int nr
 void launch_cr_check_n(const double* partials, long pstride, CrState* st, double* history, long hstride, int nrhs,
                        double inv_nb, hipStream_t s);
 void launch_cr32_direction_n(float* p, float* ap, const float* r, const float* q, CrState* st, long nb, long vstride,
-                             double* partials, long pstride, int nrhs, hipStream_t s);
+                             double* partials, long pstride, int nrhs, hipStream_t s, bool stream = false);
 void launch_cr32_update_n(float* x, float* r, const float* p, const float* ap, CrState* st, long nb, long vstride,
-                          double* partials, long pstride, int nrhs, hipStream_t s);
+                          double* partials, long pstride, int nrhs, hipStream_t s, bool stream = false);
+// Whether the FP32 CR vector kernels stream their vectors: when the vectors one launch touches
+// (five arrays per side) and the spectra of the passes do not fit the last-level cache together,
+// non-temporal vector accesses keep the spectra cached for the next apply (256^3, three sides:
+// 466 MB of vectors, 214 MB of spectra; the first side's y-forward pass 83 -> 66 us).
+inline bool cr32_streams(long nb, int nrhs, long spectra_bytes) {
+    const long vectors = 5L * nrhs * 3 * nb * (long)sizeof(float);
+    return vectors + spectra_bytes > kLastLevelCacheBytes;
+}
 // xacc += x32, x32 = 0
 void launch_accumulate_f32(double* xacc, float* x32, long n, hipStream_t s);
 // out = (float) a
