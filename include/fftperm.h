@@ -227,8 +227,9 @@ int fftperm_solve_tensor(fftperm_ctx* ctx, double tol, uint64_t maxit, double K[
 /* Multi-RHS tensor solves (no counterpart in the reference).  With the MIXED solver
  * FFTPERM_MIXED_CR_FP32_VECTORS, fftperm_solve_tensor and fftperm_solve_tensor_warm run the three
  * directions' CR iterations in lockstep: one launch of each vector and scalar kernel serves the
- * three.  mode 1 (the default): each direction's five FFT passes run back to back (its spectrum
- * stays in the last-level cache from one pass to the next); mode 2: one launch per pass for the
+ * three.  mode 1 (the default): each direction's five FFT passes run back to back on one spectrum
+ * buffer that the three share (it stays in the last-level cache from one pass and one direction to
+ * the next); mode 2: one launch per pass for the
  * three (one y launch for the nine spectrum components, one fused z launch reading the Green tensor
  * once); mode 0: one direction after the other.  Every direction keeps its own recurrence, reliable
  * updates and stopping test, so iterations, residuals and K are bit-identical in all modes.  Other
