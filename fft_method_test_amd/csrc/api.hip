@@ -1656,9 +1656,7 @@ void tune_chain_prec(fftperm_ctx* c) {
         if (k.kind == kKindCol) {
             // column tiles within 1% of the fastest are a tie here (the timing noise of back-to-back
             // applies); the widest of them reads the longest contiguous row segments, which is what
-            // counts in the solver, where the spectrum comes from HBM more often than between
-            // back-to-back applies (at 256^3 the 32-column tile is ~5 us per pass faster there while
-            // the chain timing cannot tell it from the 16-column one)
+            // counts when the spectrum comes from HBM rather than the last-level cache
             const float bound = best_of[best] * 1.01f;
             for (size_t i = 0; i < cands.size(); ++i)
                 if (best_of[i] <= bound && col_variant_code(k.n, cands[i]) / 10 > col_variant_code(k.n, cands[best]) / 10)
